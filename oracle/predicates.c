@@ -1,0 +1,388 @@
+/*
+ * predicates.c - CPU restatement of del3d's filtered exact predicates.
+ *
+ * TEST INFRASTRUCTURE ONLY (see oracle.h): the checker the GPU path is
+ * compared against, never part of the product.
+ *
+ * Restates, in C, the semantics of
+ *   src/predicates.cpp:22-27   first-stage error-bound constants
+ *   src/predicates.cpp:37-39   flip / sign_of
+ *   src/predicates.cpp:46-63   orient3d_exact
+ *   src/predicates.cpp:76-113  in_sphere_exact
+ *   src/predicates.cpp:117-121 lexico_less
+ *   src/predicates.cpp:125-147 orient3d (filter, then exact)
+ *   src/predicates.cpp:149-204 in_sphere_unperturbed (filter, then exact)
+ *   src/predicates.cpp:206-232 in_sphere (symbolic perturbation)
+ *   src/expansion.h:33-178     error-free transforms and expansion arithmetic
+ * Must be compiled with -ffp-contract=off (the reference's own flag,
+ * CMakeLists.txt:81) so that no product is fused into an FMA.
+ */
+#include "oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+/* ---------------------------------------------------------------------- */
+/* Scratch arena: expansions are short-lived within one predicate call.    */
+/* ---------------------------------------------------------------------- */
+typedef struct ArenaBlock {
+    struct ArenaBlock* next;
+    size_t used, cap;
+    double data[];
+} ArenaBlock;
+
+static _Thread_local ArenaBlock* g_arena = NULL;
+
+static double* arena_alloc(size_t n) {
+    if (n == 0) n = 1;
+    ArenaBlock* b = g_arena;
+    if (!b || b->used + n > b->cap) {
+        size_t cap = n > 65536 ? n : 65536;
+        ArenaBlock* nb = (ArenaBlock*)malloc(sizeof(ArenaBlock) + cap * sizeof(double));
+        if (!nb) abort();
+        nb->next = b;
+        nb->used = 0;
+        nb->cap = cap;
+        g_arena = nb;
+        b = nb;
+    }
+    double* p = b->data + b->used;
+    b->used += n;
+    return p;
+}
+
+static void arena_reset(void) {
+    /* keep the newest block, free the rest */
+    ArenaBlock* b = g_arena;
+    if (!b) return;
+    ArenaBlock* rest = b->next;
+    b->next = NULL;
+    b->used = 0;
+    while (rest) { ArenaBlock* n = rest->next; free(rest); rest = n; }
+}
+
+/* ---------------------------------------------------------------------- */
+/* Error-free transformations (src/expansion.h:33-67).                      */
+/* ---------------------------------------------------------------------- */
+static inline void eft_sum(double a, double b, double* x, double* err) {
+    double s = a + b;
+    double bb = s - a;          /* the part of b that made it into s */
+    double aa = s - bb;         /* the part of a that made it into s */
+    *x = s;
+    *err = (a - aa) + (b - bb);
+}
+
+static inline void eft_diff(double a, double b, double* x, double* err) {
+    eft_sum(a, -b, x, err);     /* negation is exact */
+}
+
+static inline void dekker_split(double a, double* hi, double* lo) {
+    double t = 134217729.0 * a;     /* 2^27 + 1 */
+    double big = t - a;
+    *hi = t - big;
+    *lo = a - *hi;
+}
+
+static inline void eft_product(double a, double b, double* x, double* err) {
+    double p = a * b;
+    double ah, al, bh, bl;
+    dekker_split(a, &ah, &al);
+    dekker_split(b, &bh, &bl);
+    double r1 = p - (ah * bh);
+    double r2 = r1 - (al * bh);
+    double r3 = r2 - (ah * bl);
+    *x = p;
+    *err = (al * bl) - r3;
+}
+
+/* ---------------------------------------------------------------------- */
+/* Expansions: nonoverlapping components, increasing magnitude, no zeros.   */
+/* ---------------------------------------------------------------------- */
+typedef struct { int n; double* c; } Exp;
+
+static Exp exp_diff2(double a, double b) {
+    double x, e;
+    eft_diff(a, b, &x, &e);
+    Exp r;
+    r.c = arena_alloc(2);
+    r.n = 0;
+    if (e != 0.0) r.c[r.n++] = e;
+    if (x != 0.0) r.c[r.n++] = x;
+    return r;
+}
+
+/* Add one double into an expansion (the Grow step of src/expansion.h:137-150). */
+static Exp exp_grow(Exp x, double b) {
+    if (b == 0.0) return x;
+    Exp r;
+    r.c = arena_alloc((size_t)x.n + 1);
+    r.n = 0;
+    double q = b;
+    for (int i = 0; i < x.n; ++i) {
+        double s, e;
+        eft_sum(q, x.c[i], &s, &e);
+        if (e != 0.0) r.c[r.n++] = e;
+        q = s;
+    }
+    if (q != 0.0) r.c[r.n++] = q;
+    return r;
+}
+
+static Exp exp_add(Exp x, Exp y) {
+    Exp r = x;
+    for (int i = 0; i < y.n; ++i) r = exp_grow(r, y.c[i]);
+    return r;
+}
+
+static Exp exp_neg(Exp x) {
+    Exp r;
+    r.c = arena_alloc((size_t)x.n);
+    r.n = x.n;
+    for (int i = 0; i < x.n; ++i) r.c[i] = -x.c[i];
+    return r;
+}
+
+static Exp exp_sub(Exp x, Exp y) { return exp_add(x, exp_neg(y)); }
+
+/* Scale an expansion by one double (src/expansion.h:155-175). */
+static Exp exp_scale(Exp x, double b) {
+    Exp r;
+    r.n = 0;
+    r.c = NULL;
+    if (b == 0.0 || x.n == 0) return r;
+    r.c = arena_alloc(2 * (size_t)x.n);
+    double q, lo;
+    eft_product(x.c[0], b, &q, &lo);
+    if (lo != 0.0) r.c[r.n++] = lo;
+    for (int i = 1; i < x.n; ++i) {
+        double ph, pl;
+        eft_product(x.c[i], b, &ph, &pl);
+        double s1, e1;
+        eft_sum(q, pl, &s1, &e1);
+        if (e1 != 0.0) r.c[r.n++] = e1;
+        double s2, e2;
+        eft_sum(ph, s1, &s2, &e2);
+        if (e2 != 0.0) r.c[r.n++] = e2;
+        q = s2;
+    }
+    if (q != 0.0) r.c[r.n++] = q;
+    return r;
+}
+
+/* Product: accumulate x scaled by each component of y (src/expansion.h:127-131). */
+static Exp exp_mul(Exp x, Exp y) {
+    Exp acc;
+    acc.n = 0;
+    acc.c = NULL;
+    for (int i = 0; i < y.n; ++i) acc = exp_add(acc, exp_scale(x, y.c[i]));
+    return acc;
+}
+
+static int exp_sign(Exp x) {
+    if (x.n == 0) return 0;
+    return x.c[x.n - 1] > 0.0 ? 1 : -1;
+}
+
+/* ---------------------------------------------------------------------- */
+/* Constants and sign helpers (src/predicates.cpp:22-39).                   */
+/* ---------------------------------------------------------------------- */
+static const double kEps = 1.1102230246251565e-16;                 /* 2^-53 */
+#define ORIENT_BOUND ((7.0 + 56.0 * kEps) * kEps)
+#define INSPHERE_BOUND ((16.0 + 224.0 * kEps) * kEps)
+
+static inline int sgn(double d) { return d > 0.0 ? 1 : (d < 0.0 ? -1 : 0); }
+
+static _Thread_local int64_t g_stage[3];
+
+/* |a-d, b-d, c-d| exactly; returns its sign (before the interface flip). */
+static int orient3d_exact_raw(const double* a, const double* b, const double* c,
+                              const double* d) {
+    Exp adx = exp_diff2(a[0], d[0]), ady = exp_diff2(a[1], d[1]), adz = exp_diff2(a[2], d[2]);
+    Exp bdx = exp_diff2(b[0], d[0]), bdy = exp_diff2(b[1], d[1]), bdz = exp_diff2(b[2], d[2]);
+    Exp cdx = exp_diff2(c[0], d[0]), cdy = exp_diff2(c[1], d[1]), cdz = exp_diff2(c[2], d[2]);
+    Exp m1 = exp_sub(exp_mul(bdx, cdy), exp_mul(cdx, bdy));
+    Exp m2 = exp_sub(exp_mul(cdx, ady), exp_mul(adx, cdy));
+    Exp m3 = exp_sub(exp_mul(adx, bdy), exp_mul(bdx, ady));
+    Exp det = exp_add(exp_add(exp_mul(adz, m1), exp_mul(bdz, m2)), exp_mul(cdz, m3));
+    return exp_sign(det);
+}
+
+static int in_sphere_exact_raw(const double* a, const double* b, const double* c,
+                               const double* d, const double* e) {
+    Exp ax = exp_diff2(a[0], e[0]), ay = exp_diff2(a[1], e[1]), az = exp_diff2(a[2], e[2]);
+    Exp bx = exp_diff2(b[0], e[0]), by = exp_diff2(b[1], e[1]), bz = exp_diff2(b[2], e[2]);
+    Exp cx = exp_diff2(c[0], e[0]), cy = exp_diff2(c[1], e[1]), cz = exp_diff2(c[2], e[2]);
+    Exp dx = exp_diff2(d[0], e[0]), dy = exp_diff2(d[1], e[1]), dz = exp_diff2(d[2], e[2]);
+
+    Exp la = exp_add(exp_add(exp_mul(ax, ax), exp_mul(ay, ay)), exp_mul(az, az));
+    Exp lb = exp_add(exp_add(exp_mul(bx, bx), exp_mul(by, by)), exp_mul(bz, bz));
+    Exp lc = exp_add(exp_add(exp_mul(cx, cx), exp_mul(cy, cy)), exp_mul(cz, cz));
+    Exp ld = exp_add(exp_add(exp_mul(dx, dx), exp_mul(dy, dy)), exp_mul(dz, dz));
+
+    Exp ab = exp_sub(exp_mul(ax, by), exp_mul(bx, ay));
+    Exp bc = exp_sub(exp_mul(bx, cy), exp_mul(cx, by));
+    Exp cd = exp_sub(exp_mul(cx, dy), exp_mul(dx, cy));
+    Exp da = exp_sub(exp_mul(dx, ay), exp_mul(ax, dy));
+    Exp ac = exp_sub(exp_mul(ax, cy), exp_mul(cx, ay));
+    Exp bd = exp_sub(exp_mul(bx, dy), exp_mul(dx, by));
+
+    Exp abc = exp_add(exp_sub(exp_mul(az, bc), exp_mul(bz, ac)), exp_mul(cz, ab));
+    Exp bcd = exp_add(exp_sub(exp_mul(bz, cd), exp_mul(cz, bd)), exp_mul(dz, bc));
+    Exp cda = exp_add(exp_add(exp_mul(cz, da), exp_mul(dz, ac)), exp_mul(az, cd));
+    Exp dab = exp_add(exp_add(exp_mul(dz, ab), exp_mul(az, bd)), exp_mul(bz, da));
+
+    Exp det = exp_add(exp_sub(exp_mul(ld, abc), exp_mul(lc, dab)),
+                      exp_sub(exp_mul(lb, cda), exp_mul(la, bcd)));
+    return exp_sign(det);
+}
+
+/* Public orient3d: sign of |b-a, c-a, d-a| (the flip of the computed form). */
+static int orient3d_impl(const double* a, const double* b, const double* c, const double* d) {
+    const double adx = a[0] - d[0], bdx = b[0] - d[0], cdx = c[0] - d[0];
+    const double ady = a[1] - d[1], bdy = b[1] - d[1], cdy = c[1] - d[1];
+    const double adz = a[2] - d[2], bdz = b[2] - d[2], cdz = c[2] - d[2];
+
+    const double bdxcdy = bdx * cdy, cdxbdy = cdx * bdy;
+    const double cdxady = cdx * ady, adxcdy = adx * cdy;
+    const double adxbdy = adx * bdy, bdxady = bdx * ady;
+
+    const double det = adz * (bdxcdy - cdxbdy) + bdz * (cdxady - adxcdy) + cdz * (adxbdy - bdxady);
+    const double perm = (fabs(bdxcdy) + fabs(cdxbdy)) * fabs(adz)
+                      + (fabs(cdxady) + fabs(adxcdy)) * fabs(bdz)
+                      + (fabs(adxbdy) + fabs(bdxady)) * fabs(cdz);
+    const double bound = ORIENT_BOUND * perm;
+    if (det > bound || -det > bound) { g_stage[0]++; return -sgn(det); }
+    g_stage[1]++;
+    return -orient3d_exact_raw(a, b, c, d);
+}
+
+static int in_sphere_raw_impl(const double* a, const double* b, const double* c,
+                              const double* d, const double* e) {
+    const double aex = a[0] - e[0], aey = a[1] - e[1], aez = a[2] - e[2];
+    const double bex = b[0] - e[0], bey = b[1] - e[1], bez = b[2] - e[2];
+    const double cex = c[0] - e[0], cey = c[1] - e[1], cez = c[2] - e[2];
+    const double dex = d[0] - e[0], dey = d[1] - e[1], dez = d[2] - e[2];
+
+    const double ab = aex * bey - bex * aey;
+    const double bc = bex * cey - cex * bey;
+    const double cd = cex * dey - dex * cey;
+    const double da = dex * aey - aex * dey;
+    const double ac = aex * cey - cex * aey;
+    const double bd = bex * dey - dex * bey;
+
+    const double abc = aez * bc - bez * ac + cez * ab;
+    const double bcd = bez * cd - cez * bd + dez * bc;
+    const double cda = cez * da + dez * ac + aez * cd;
+    const double dab = dez * ab + aez * bd + bez * da;
+
+    const double alift = aex * aex + aey * aey + aez * aez;
+    const double blift = bex * bex + bey * bey + bez * bez;
+    const double clift = cex * cex + cey * cey + cez * cez;
+    const double dlift = dex * dex + dey * dey + dez * dez;
+
+    const double det = (dlift * abc - clift * dab) + (blift * cda - alift * bcd);
+
+    const double az_ = fabs(aez), bz_ = fabs(bez), cz_ = fabs(cez), dz_ = fabs(dez);
+    const double aexbey = fabs(aex * bey), bexaey = fabs(bex * aey);
+    const double bexcey = fabs(bex * cey), cexbey = fabs(cex * bey);
+    const double cexdey = fabs(cex * dey), dexcey = fabs(dex * cey);
+    const double dexaey = fabs(dex * aey), aexdey = fabs(aex * dey);
+    const double aexcey = fabs(aex * cey), cexaey = fabs(cex * aey);
+    const double bexdey = fabs(bex * dey), dexbey = fabs(dex * bey);
+
+    const double perm =
+        ((cexdey + dexcey) * bz_ + (dexbey + bexdey) * cz_ + (bexcey + cexbey) * dz_) * alift +
+        ((dexaey + aexdey) * cz_ + (aexcey + cexaey) * dz_ + (cexdey + dexcey) * az_) * blift +
+        ((aexbey + bexaey) * dz_ + (bexdey + dexbey) * az_ + (dexaey + aexdey) * bz_) * clift +
+        ((bexcey + cexbey) * az_ + (cexaey + aexcey) * bz_ + (aexbey + bexaey) * cz_) * dlift;
+
+    const double bound = INSPHERE_BOUND * perm;
+    if (det > bound || -det > bound) { g_stage[0]++; return -sgn(det); }
+    g_stage[1]++;
+    return -in_sphere_exact_raw(a, b, c, d, e);
+}
+
+static int lex_less(const double* p, const double* q) {
+    if (p[0] != q[0]) return p[0] < q[0];
+    if (p[1] != q[1]) return p[1] < q[1];
+    return p[2] < q[2];
+}
+
+/* Stable insertion sort of 5 pointers by lex_less: what libstdc++'s std::sort
+ * does for ranges of 16 or fewer elements (a plain insertion sort). */
+static void sort5_stable(const double* p[5]) {
+    for (int i = 1; i < 5; ++i) {
+        const double* v = p[i];
+        int j = i;
+        while (j > 0 && lex_less(v, p[j - 1])) { p[j] = p[j - 1]; --j; }
+        p[j] = v;
+    }
+}
+
+static int in_sphere_impl(const double* a, const double* b, const double* c,
+                          const double* d, const double* e) {
+    int s = in_sphere_raw_impl(a, b, c, d, e);
+    if (s != 0) return s;
+    g_stage[2]++;
+    const double* p[5] = {a, b, c, d, e};
+    sort5_stable(p);
+    for (int i = 4; i > 2; --i) {
+        if (p[i] == e) return -1;
+        int o;
+        if (p[i] == d && (o = orient3d_impl(a, b, c, e)) != 0) return o;
+        if (p[i] == c && (o = orient3d_impl(a, b, e, d)) != 0) return o;
+        if (p[i] == b && (o = orient3d_impl(a, e, c, d)) != 0) return o;
+        if (p[i] == a && (o = orient3d_impl(e, b, c, d)) != 0) return o;
+    }
+    return -1;
+}
+
+int orc_orient3d(const double* a, const double* b, const double* c, const double* d) {
+    int r = orient3d_impl(a, b, c, d);
+    arena_reset();
+    return r;
+}
+
+int orc_in_sphere_unperturbed(const double* a, const double* b, const double* c,
+                              const double* d, const double* e) {
+    int r = in_sphere_raw_impl(a, b, c, d, e);
+    arena_reset();
+    return r;
+}
+
+int orc_in_sphere(const double* a, const double* b, const double* c,
+                  const double* d, const double* e) {
+    int r = in_sphere_impl(a, b, c, d, e);
+    arena_reset();
+    return r;
+}
+
+void orc_pred_batch(const double* xyz, int64_t n_pts, const uint8_t* op,
+                    const int32_t* idx5, int64_t n_q, int8_t* sign_out) {
+    (void)n_pts;
+    g_stage[0] = g_stage[1] = g_stage[2] = 0;
+    for (int64_t q = 0; q < n_q; ++q) {
+        const int32_t* r = idx5 + 5 * q;
+        const double* a = xyz + 3 * (int64_t)r[0];
+        const double* b = xyz + 3 * (int64_t)r[1];
+        const double* c = xyz + 3 * (int64_t)r[2];
+        const double* d = xyz + 3 * (int64_t)r[3];
+        int s;
+        if (op[q] == ORC_OP_ORIENT3D) {
+            s = orient3d_impl(a, b, c, d);
+        } else {
+            const double* e = xyz + 3 * (int64_t)r[4];
+            s = op[q] == ORC_OP_INSPHERE ? in_sphere_impl(a, b, c, d, e)
+                                         : in_sphere_raw_impl(a, b, c, d, e);
+        }
+        sign_out[q] = (int8_t)s;
+        arena_reset();
+    }
+}
+
+void orc_last_stage_counts(int64_t out[3]) {
+    out[0] = g_stage[0];
+    out[1] = g_stage[1];
+    out[2] = g_stage[2];
+}
