@@ -1,0 +1,167 @@
+"""del3d_amd - del3d's exact predicates and spatial sort on MI355X (gfx950).
+
+Python access to libdel3d_amd.so (include/del3d_amd.h). The names mirror the
+reference's interface:
+
+  orient3d(a, b, c, d)                 include/del3d/predicates.h:26
+  in_sphere(a, b, c, d, e)             include/del3d/predicates.h:49-50
+  in_sphere_unperturbed(a, b, c, d, e) include/del3d/predicates.h:54-55
+  spatial_sort(points, order)          include/del3d/spatial_sort.h:37
+
+with the same sign convention (NEGATIVE=-1, ZERO=0, POSITIVE=1) and the same
+results, plus batched forms over device buffers (``predicates``,
+``spatial_sort_device``, ``spatial_sort_sets``). Every call runs on the GPU;
+there is no CPU fallback, and importing the package fails if the library has
+not been built.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+from . import _hip
+from ._hip import DeviceArray, Event, Stream, device_count, synchronize
+
+NEGATIVE, ZERO, POSITIVE = -1, 0, 1
+ORIENT3D, IN_SPHERE, IN_SPHERE_UNPERTURBED = 0, 1, 2
+
+LIB_PATH = Path(__file__).resolve().parent / "libdel3d_amd.so"
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    """The loaded C library. Raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise RuntimeError(f"{LIB_PATH} is missing: run __graft_entry__.build() first")
+        _hip._lib()  # the HIP runtime the library links, loaded first and shared
+        L = C.CDLL(str(LIB_PATH))
+        vp, i64 = C.c_void_p, C.c_int64
+        dp = C.POINTER(C.c_double)
+        L.del3d_last_error.restype = C.c_char_p
+        L.del3d_version.restype = C.c_char_p
+        L.del3d_predicates.argtypes = [vp, i64, vp, vp, i64, vp, vp]
+        L.del3d_predicates_stats.argtypes = [vp, i64, vp, vp, i64, vp, vp, C.POINTER(C.c_int64)]
+        L.del3d_predicates_host.argtypes = [vp, i64, vp, vp, i64, vp]
+        L.del3d_spatial_sort.argtypes = [vp, i64, vp, i64, vp]
+        L.del3d_spatial_sort_sets.argtypes = [vp, C.POINTER(C.c_int64), i64, vp, vp]
+        L.del3d_spatial_sort_host.argtypes = [vp, i64, vp, i64]
+        L.del3d_orient3d.argtypes = [dp] * 4
+        L.del3d_in_sphere.argtypes = [dp] * 5
+        L.del3d_in_sphere_unperturbed.argtypes = [dp] * 5
+        _lib = L
+    return _lib
+
+
+class Del3dError(RuntimeError):
+    pass
+
+
+def _check(rc: int, what: str) -> None:
+    if rc != 0:
+        raise Del3dError(f"{what}: {lib().del3d_last_error().decode()} (code {rc})")
+
+
+def _stream_handle(stream):
+    return stream.handle if stream is not None else None
+
+
+# ---- batched, device-resident ------------------------------------------------
+
+def predicates(xyz: DeviceArray, op: DeviceArray, idx: DeviceArray, out: DeviceArray,
+               stream: Stream | None = None) -> None:
+    """Signs of a mixed batch: op[q] in {ORIENT3D, IN_SPHERE, IN_SPHERE_UNPERTURBED},
+    idx[q] = 5 point indices (orient3d ignores the fifth). Enqueued on `stream`."""
+    n_q = int(op.shape[0])
+    _check(lib().del3d_predicates(xyz.ptr, int(xyz.shape[0]), op.ptr, idx.ptr, n_q, out.ptr,
+                                  _stream_handle(stream)), "del3d_predicates")
+
+
+def predicates_stats(xyz: DeviceArray, op: DeviceArray, idx: DeviceArray, out: DeviceArray,
+                     stream: Stream | None = None) -> dict:
+    st = (C.c_int64 * 5)()
+    _check(lib().del3d_predicates_stats(xyz.ptr, int(xyz.shape[0]), op.ptr, idx.ptr,
+                                        int(op.shape[0]), out.ptr, _stream_handle(stream), st),
+           "del3d_predicates_stats")
+    return {"filter": st[0], "certified": st[1], "exact_regs": st[2], "exact_wide": st[3],
+            "perturbed": st[4]}
+
+
+def spatial_sort_device(xyz: DeviceArray, order: DeviceArray, stream: Stream | None = None) -> None:
+    _check(lib().del3d_spatial_sort(xyz.ptr, int(xyz.shape[0]), order.ptr, int(order.shape[0]),
+                                    _stream_handle(stream)), "del3d_spatial_sort")
+
+
+def spatial_sort_sets(xyz: DeviceArray, offsets: np.ndarray, order: DeviceArray,
+                      stream: Stream | None = None) -> None:
+    off = np.ascontiguousarray(offsets, dtype=np.int64)
+    _check(lib().del3d_spatial_sort_sets(xyz.ptr, off.ctypes.data_as(C.POINTER(C.c_int64)),
+                                         len(off) - 1, order.ptr, _stream_handle(stream)),
+           "del3d_spatial_sort_sets")
+
+
+# ---- host-memory conveniences ------------------------------------------------
+
+def predicates_np(points, op, idx) -> np.ndarray:
+    xyz = np.ascontiguousarray(np.asarray(points, dtype=np.float64).reshape(-1, 3))
+    op = np.ascontiguousarray(op, dtype=np.uint8)
+    idx = np.ascontiguousarray(np.asarray(idx, dtype=np.int32).reshape(-1, 5))
+    out = np.zeros(len(op), dtype=np.int8)
+    _check(lib().del3d_predicates_host(xyz.ctypes.data, len(xyz), op.ctypes.data, idx.ctypes.data,
+                                       len(op), out.ctypes.data), "del3d_predicates_host")
+    return out
+
+
+def spatial_sort(points, order=None) -> np.ndarray:
+    """The reference's spatial_sort(points, order) on host arrays; returns the
+    permuted order (a new array)."""
+    xyz = np.ascontiguousarray(np.asarray(points, dtype=np.float64).reshape(-1, 3))
+    o = (np.arange(len(xyz), dtype=np.int32) if order is None
+         else np.array(order, dtype=np.int32, copy=True))
+    _check(lib().del3d_spatial_sort_host(xyz.ctypes.data, len(xyz), o.ctypes.data, len(o)),
+           "del3d_spatial_sort_host")
+    return o
+
+
+def _pts(*args):
+    """One double[3] per argument; the same Python object passed twice becomes
+    the same pointer, which is what the reference's perturbation compares."""
+    seen = {}
+    keep, ptrs = [], []
+    for p in args:
+        if id(p) not in seen:
+            a = np.ascontiguousarray(p, dtype=np.float64).reshape(3)
+            keep.append(a)
+            seen[id(p)] = a.ctypes.data_as(C.POINTER(C.c_double))
+        ptrs.append(seen[id(p)])
+    return keep, ptrs
+
+
+def orient3d(a, b, c, d) -> int:
+    """Sign of |b-a, c-a, d-a| (one GPU query)."""
+    keep, p = _pts(a, b, c, d)
+    return int(lib().del3d_orient3d(*p))
+
+
+def in_sphere(a, b, c, d, e) -> int:
+    """POSITIVE iff e is inside the sphere through a, b, c, d (positively
+    oriented); never ZERO (symbolic perturbation). One GPU query."""
+    keep, p = _pts(a, b, c, d, e)
+    return int(lib().del3d_in_sphere(*p))
+
+
+def in_sphere_unperturbed(a, b, c, d, e) -> int:
+    keep, p = _pts(a, b, c, d, e)
+    return int(lib().del3d_in_sphere_unperturbed(*p))
+
+
+__all__ = [
+    "NEGATIVE", "ZERO", "POSITIVE", "ORIENT3D", "IN_SPHERE", "IN_SPHERE_UNPERTURBED",
+    "DeviceArray", "Stream", "Event", "device_count", "synchronize", "lib", "Del3dError",
+    "predicates", "predicates_stats", "spatial_sort_device", "spatial_sort_sets",
+    "predicates_np", "spatial_sort", "orient3d", "in_sphere", "in_sphere_unperturbed",
+]

@@ -1,0 +1,88 @@
+// cpu_check.cpp - host build of the per-query tier logic in pred_core.h.
+//
+// Not part of the product library: built into del3d_amd/libpredcore_host.so
+// so the tests can run every tier of the GPU's arithmetic (filter, certified
+// small grid, 1-2 limb and any-width exact integers, perturbation walk) on
+// the CPU and compare it with the oracle, with a switch that forces a chosen
+// tier on every query, which the GPU kernels only reach on rare inputs.
+#include <cstdint>
+#include <vector>
+
+#include "pred_core.h"
+
+using namespace d3;
+
+namespace {
+
+// mode 0: the kernels' tier order; 1: skip T0/T1 (force exact integers,
+// 1-2 limbs where possible); 2: force the any-width tier.
+int eval_one(uint8_t o, const P3 p_in[5], const int32_t id[5], int mode, int* tier_out) {
+    P3 p[5];
+    const int np = o == 0 ? 4 : 5;
+    for (int k = 0; k < 5; ++k) p[k] = p_in[k < np ? k : 3];
+    std::vector<uint32_t> ws(64 * 67);
+    const Grid g = grid_of(p, np);
+    const int L = limbs_for(g);
+    auto exact_orient = [&](const P3* t) -> int {
+        if (mode != 2 && L == 1) return orient_public_int<1>(t, g.q);
+        if (mode != 2 && L == 2) return orient_public_int<2>(t, g.q);
+        return -rt_orient_raw(t[0], t[1], t[2], t[3], g.q, L, ws.data());
+    };
+    if (o == 0) {
+        if (mode == 0) {
+            const Filt f = orient_filter(p[0], p[1], p[2], p[3]);
+            if (filter_decides(f, kOrientBound)) { *tier_out = 0; return -sgn(f.det); }
+            if (t1_certify(p, 4, f.maxd, 15, -358)) { *tier_out = 1; return -sgn(f.det); }
+        }
+        *tier_out = (mode != 2 && L <= 2) ? 2 : 3;
+        return exact_orient(p);
+    }
+    int raw;
+    bool cert = false;
+    const Filt f = insphere_filter(p[0], p[1], p[2], p[3], p[4]);
+    if (mode == 0 && filter_decides(f, kInSphereBound)) {
+        *tier_out = 0;
+        return -sgn(f.det);
+    }
+    if (mode == 0 && t1_certify(p, 5, f.maxd, 8, -214)) {
+        *tier_out = 1;
+        cert = true;
+        raw = -sgn(f.det);
+    } else if (mode != 2 && L <= 2) {
+        *tier_out = 2;
+        raw = L == 1 ? -int_insphere_raw<1>(to_lp3<1>(p[0], g.q), to_lp3<1>(p[1], g.q), to_lp3<1>(p[2], g.q),
+                                           to_lp3<1>(p[3], g.q), to_lp3<1>(p[4], g.q))
+                     : -int_insphere_raw<2>(to_lp3<2>(p[0], g.q), to_lp3<2>(p[1], g.q), to_lp3<2>(p[2], g.q),
+                                           to_lp3<2>(p[3], g.q), to_lp3<2>(p[4], g.q));
+    } else {
+        *tier_out = 3;
+        raw = -rt_insphere_raw(p[0], p[1], p[2], p[3], p[4], g.q, L, ws.data());
+    }
+    if (raw != 0 || o == 2) return raw;
+    return perturbation_walk(p, id, [&](int k) {
+        P3 t[4];
+        subst_tet(p, k, t);
+        const Filt ft = orient_filter(t[0], t[1], t[2], t[3]);
+        if (cert) return -sgn(ft.det);
+        if (mode == 0 && filter_decides(ft, kOrientBound)) return -sgn(ft.det);
+        return exact_orient(t);
+    });
+}
+
+}  // namespace
+
+extern "C" void predcore_host_batch(const double* xyz, int64_t n_pts, const uint8_t* op,
+                                    const int32_t* idx, int64_t n_q, int8_t* out, int mode,
+                                    int8_t* tier) {
+    (void)n_pts;
+    for (int64_t q = 0; q < n_q; ++q) {
+        int32_t id[5];
+        P3 p[5];
+        for (int k = 0; k < 5; ++k) id[k] = idx[5 * q + k];
+        const int np = op[q] == 0 ? 4 : 5;
+        for (int k = 0; k < 5; ++k) p[k] = load_p3(xyz, id[k < np ? k : 3]);
+        int t = 0;
+        out[q] = (int8_t)eval_one(op[q], p, id, mode, &t);
+        if (tier) tier[q] = (int8_t)t;
+    }
+}

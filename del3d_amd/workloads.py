@@ -1,0 +1,171 @@
+"""Synthetic inputs of the measured configurations (deterministic, seeded).
+
+C1  10,000 points, coordinates = consecutive rand48 states (srand48(1)
+    layout) times 2^-48, consumed x, y, z per point.
+C2  the 32x32x32 integer lattice (index (i*32+j)*32+k, point (i,j,k)) and
+    2^24 queries, half orient3d, half in_sphere, in random order:
+      orient3d: 4 distinct points of one lattice plane through a 3x3x3 block
+                (axis and diagonal planes), or 4 distinct corners of a unit cube;
+      in_sphere: 5 distinct corners of a unit cube (always cospherical), or 5
+                distinct lattice points on a sphere of radius^2 1, 2, 3 or 5
+                around a lattice point (always cospherical); a and b are
+                swapped when orient3d(a,b,c,d) < 0 and coplanar (a,b,c,d) are
+                redrawn, so every in_sphere query meets its precondition.
+C3  10^7 points: even indices N(0,1)^3, odd indices uniform [-1,1)^3, and
+    random 5-point in_sphere / 4-point orient3d queries over them.
+Everything is numpy, seeded with numpy's PCG64 except C1's rand48 stream.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+MASK48 = (1 << 48) - 1
+
+
+def rand48_states(seed: int, count: int) -> np.ndarray:
+    """The rand48 states after 1..count steps from srand48(seed), as uint64."""
+    a, c = 0x5DEECE66D, 0xB
+    x = ((seed << 16) | 0x330E) & MASK48
+    out = np.empty(count, dtype=np.uint64)
+    for i in range(count):
+        x = (a * x + c) & MASK48
+        out[i] = x
+    return out
+
+
+def c1_points(n: int = 10_000, seed: int = 1) -> np.ndarray:
+    st = rand48_states(seed, 3 * n)
+    return (st.astype(np.float64) * (1.0 / 281474976710656.0)).reshape(n, 3)
+
+
+def lattice(nx: int, ny: int, nz: int) -> np.ndarray:
+    i, j, k = np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz), indexing="ij")
+    return np.stack([i.ravel(), j.ravel(), k.ravel()], axis=1).astype(np.float64)
+
+
+def _distinct_rows(rng, sizes: np.ndarray, k: int) -> np.ndarray:
+    """Per row, k distinct integers in [0, sizes[row])."""
+    n = len(sizes)
+    out = np.empty((n, k), dtype=np.int64)
+    todo = np.arange(n)
+    while len(todo):
+        draw = (rng.random((len(todo), k)) * sizes[todo, None]).astype(np.int64)
+        s = np.sort(draw, axis=1)
+        ok = np.all(s[:, 1:] != s[:, :-1], axis=1)
+        out[todo[ok]] = draw[ok]
+        todo = todo[~ok]
+    return out
+
+
+_CUBE = np.array([[i, j, k] for i in (0, 1) for j in (0, 1) for k in (0, 1)], dtype=np.int64)
+
+
+def _sphere_shells():
+    shells = {}
+    for r2 in (1, 2, 3, 5):
+        pts = [(x, y, z) for x in range(-2, 3) for y in range(-2, 3) for z in range(-2, 3)
+               if x * x + y * y + z * z == r2]
+        shells[r2] = np.array(pts, dtype=np.int64)
+    return shells
+
+
+def _plane_patches():
+    """Lattice planes through the 27 points of a 3x3x3 block with >= 4 points."""
+    local = np.array([[i, j, k] for i in range(3) for j in range(3) for k in range(3)])
+    normals = [(1, 0, 0), (0, 1, 0), (0, 0, 1), (1, 1, 0), (1, -1, 0), (1, 0, 1), (1, 0, -1),
+               (0, 1, 1), (0, 1, -1), (1, 1, 1), (1, 1, -1), (1, -1, 1), (-1, 1, 1)]
+    patches = []
+    for nrm in normals:
+        dots = local @ np.array(nrm)
+        for c in np.unique(dots):
+            members = local[dots == c]
+            if len(members) >= 4:
+                patches.append(members)
+    size = np.array([len(p) for p in patches])
+    table = np.zeros((len(patches), 9, 3), dtype=np.int64)
+    for i, p in enumerate(patches):
+        table[i, :len(p)] = p
+    return table, size
+
+
+def _orient_sign_int(a, b, c, d) -> np.ndarray:
+    """sign |b-a, c-a, d-a| for small integer coordinates (exact in int64)."""
+    u, v, w = b - a, c - a, d - a
+    det = (u[:, 0] * (v[:, 1] * w[:, 2] - v[:, 2] * w[:, 1])
+           - u[:, 1] * (v[:, 0] * w[:, 2] - v[:, 2] * w[:, 0])
+           + u[:, 2] * (v[:, 0] * w[:, 1] - v[:, 1] * w[:, 0]))
+    return np.sign(det)
+
+
+def c2(n_q: int = 1 << 24, seed: int = 2, side: int = 32):
+    """(points, op, idx) of the degenerate-lattice configuration."""
+    rng = np.random.default_rng(seed)
+    pts = lattice(side, side, side)
+
+    def lin(p):
+        return (p[..., 0] * side + p[..., 1]) * side + p[..., 2]
+
+    op = rng.integers(0, 2, size=n_q).astype(np.uint8)  # 0 orient3d, 1 in_sphere
+    idx = np.zeros((n_q, 5), dtype=np.int64)
+
+    # orient3d
+    o_rows = np.nonzero(op == 0)[0]
+    half = rng.random(len(o_rows)) < 0.5
+    patch_tab, patch_size = _plane_patches()
+    r = o_rows[half]
+    pid = rng.integers(0, len(patch_tab), size=len(r))
+    base = rng.integers(0, side - 2, size=(len(r), 3))
+    pick = _distinct_rows(rng, patch_size[pid], 4)
+    pp = patch_tab[pid[:, None], pick] + base[:, None, :]
+    idx[r, :4] = lin(pp)
+    r = o_rows[~half]
+    base = rng.integers(0, side - 1, size=(len(r), 3))
+    pick = _distinct_rows(rng, np.full(len(r), 8), 4)
+    idx[r, :4] = lin(_CUBE[pick] + base[:, None, :])
+    idx[o_rows, 4] = idx[o_rows, 3]
+
+    # in_sphere, redrawn until (a,b,c,d) is not coplanar, then oriented
+    s_rows = np.nonzero(op == 1)[0]
+    shells = _sphere_shells()
+    todo = s_rows
+    while len(todo):
+        kind = rng.random(len(todo)) < 0.5
+        P = np.zeros((len(todo), 5, 3), dtype=np.int64)
+        r = np.nonzero(kind)[0]
+        base = rng.integers(0, side - 1, size=(len(r), 3))
+        pick = _distinct_rows(rng, np.full(len(r), 8), 5)
+        P[r] = _CUBE[pick] + base[:, None, :]
+        r = np.nonzero(~kind)[0]
+        r2 = rng.choice(np.array([1, 2, 3, 5]), size=len(r))
+        center = rng.integers(2, side - 2, size=(len(r), 3))
+        for val, shell in shells.items():
+            m = r[r2 == val]
+            if len(m) == 0:
+                continue
+            pick = _distinct_rows(rng, np.full(len(m), len(shell)), 5)
+            P[m] = shell[pick] + center[r2 == val][:, None, :]
+        s = _orient_sign_int(P[:, 0], P[:, 1], P[:, 2], P[:, 3])
+        neg = s < 0
+        P[neg, 0], P[neg, 1] = P[neg, 1].copy(), P[neg, 0].copy()
+        ok = s != 0
+        idx[todo[ok]] = lin(P[ok])
+        todo = todo[~ok]
+    return pts, op, idx.astype(np.int32)
+
+
+def c3_points(n: int = 10_000_000, seed: int = 3) -> np.ndarray:
+    rng = np.random.default_rng(seed)
+    pts = np.empty((n, 3), dtype=np.float64)
+    ne = (n + 1) // 2
+    pts[0::2] = rng.standard_normal((ne, 3))
+    pts[1::2] = 2.0 * rng.random((n - ne, 3)) - 1.0
+    return pts
+
+
+def random_queries(n_pts: int, n_q: int, seed: int):
+    """Random mixed queries (half orient3d, half in_sphere) over n_pts points,
+    indices drawn independently (no precondition)."""
+    rng = np.random.default_rng(seed)
+    op = rng.integers(0, 2, size=n_q).astype(np.uint8)
+    idx = rng.integers(0, n_pts, size=(n_q, 5), dtype=np.int64).astype(np.int32)
+    return op, idx

@@ -1,0 +1,129 @@
+/*
+ * del3d_amd.h - C ABI of the MI355X (gfx950) implementation of del3d's hot path:
+ * the exact filtered predicates orient3d / in_sphere (with symbolic
+ * perturbation) and the rand48-shuffled multiscale Hilbert spatial sort.
+ *
+ * Every result is bit-exact with the reference on the same input: predicate
+ * signs are the signs the reference returns, sort permutations are the
+ * permutations it produces.
+ *
+ * Conventions
+ *   - Plain pointers and sizes; no framework types. "Device pointer" means
+ *     memory the HIP runtime can address from the current device (hipMalloc,
+ *     hipMallocAsync, or a framework's device tensor).
+ *   - `stream` is a hipStream_t passed as void*; NULL is the null stream.
+ *     Calls are stream-ordered and return once the work is enqueued, except
+ *     where stated. Scratch memory is taken with hipMallocAsync on `stream`
+ *     and released on it, so concurrent calls on different streams are safe.
+ *   - Return value: DEL3D_OK (0) or a DEL3D_ERR_* code; del3d_last_error()
+ *     gives the message of the calling thread's last failure.
+ *   - Points are the reference's Point3 (include/del3d/spatial_sort.h:31):
+ *     x, y, z as three consecutive doubles, 24 bytes per point.
+ *   - Inputs must be finite (include/del3d/predicates.h:25 assumes the same).
+ */
+#ifndef DEL3D_AMD_H
+#define DEL3D_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DEL3D_OK 0
+#define DEL3D_ERR_INVALID 1     /* bad argument (size, pointer, op code) */
+#define DEL3D_ERR_DEVICE 2      /* a HIP call failed */
+#define DEL3D_ERR_NOMEM 3       /* device allocation failed */
+
+/* Predicate op codes, one byte per query. */
+#define DEL3D_OP_ORIENT3D 0              /* orient3d(r0, r1, r2, r3) */
+#define DEL3D_OP_IN_SPHERE 1             /* in_sphere(r0, r1, r2, r3, r4) */
+#define DEL3D_OP_IN_SPHERE_UNPERTURBED 2 /* in_sphere_unperturbed(r0..r4) */
+
+/* Message of the calling thread's last failed call ("" if none). */
+const char* del3d_last_error(void);
+
+/* Library version string. */
+const char* del3d_version(void);
+
+/*
+ * Batched predicates over device-resident data.
+ *
+ * Replaces, per query q, one call of
+ *   orient3d(a,b,c,d)               include/del3d/predicates.h:26 (src/predicates.cpp:125)
+ *   in_sphere(a,b,c,d,e)            include/del3d/predicates.h:49-50 (src/predicates.cpp:206)
+ *   in_sphere_unperturbed(a,..,e)   include/del3d/predicates.h:54-55 (src/predicates.cpp:149)
+ * with a = xyz + 3*idx[5q+0], ..., e = xyz + 3*idx[5q+4]. Equal indices are
+ * the reference's equal pointers (the perturbation compares pointer
+ * identity, src/predicates.cpp:220-227); distinct indices of equal
+ * coordinates are distinct pointers. idx[5q+4] is ignored for orient3d.
+ *
+ *   xyz      device, 3*n_pts doubles
+ *   op       device, n_q bytes (DEL3D_OP_*)
+ *   idx      device, 5*n_q int32, each in [0, n_pts)
+ *   sign_out device, n_q int8, written with -1 / 0 / +1
+ * Indices are not range-checked on the device; out-of-range indices are
+ * undefined behaviour, as out-of-range pointers are in the reference.
+ */
+int del3d_predicates(const double* xyz, int64_t n_pts, const uint8_t* op, const int32_t* idx,
+                     int64_t n_q, int8_t* sign_out, void* stream);
+
+/*
+ * As del3d_predicates, and additionally returns, after synchronizing the
+ * stream, how the queries were decided (host array of 5 int64):
+ *   [0] by the FP64 semi-static filter
+ *   [1] by the certified small-integer re-evaluation in registers
+ *   [2] by exact integer arithmetic in registers (1-2 limbs per difference)
+ *   [3] by exact integer arithmetic in LDS (wide exponent spans)
+ *   [4] in_sphere queries that needed the symbolic perturbation
+ */
+int del3d_predicates_stats(const double* xyz, int64_t n_pts, const uint8_t* op,
+                           const int32_t* idx, int64_t n_q, int8_t* sign_out, void* stream,
+                           int64_t* stats_host);
+
+/*
+ * Spatial sort of device-resident data. Replaces
+ *   void spatial_sort(const std::vector<Point3>& points, std::vector<int>& order)
+ *   include/del3d/spatial_sort.h:37 (src/spatial_sort.cpp:141-151)
+ * `order` (device, n_order int32 indices into the n_pts points, normally
+ * 0..n-1) is permuted in place into the reference's order; `xyz` (device,
+ * 3*n_pts doubles) is not modified. Indices must lie in [0, n_pts).
+ * Synchronizes `stream` several times per call (the recursion plan is driven
+ * from the host).
+ */
+int del3d_spatial_sort(const double* xyz, int64_t n_pts, int32_t* order, int64_t n_order,
+                       void* stream);
+
+/*
+ * Many independent spatial sorts in one call. Set s owns points
+ * xyz[3*off[s] .. 3*off[s+1]) and order[off[s] .. off[s+1]), whose entries
+ * are indices local to the set (in [0, off[s+1]-off[s])). Each set is sorted
+ * exactly as del3d_spatial_sort would sort it alone.
+ *   off: host array of n_sets+1 int64, non-decreasing, off[0] = 0.
+ */
+int del3d_spatial_sort_sets(const double* xyz, const int64_t* off, int64_t n_sets,
+                            int32_t* order, void* stream);
+
+/* Host-memory forms: copy in, compute on the current device, copy out,
+ * synchronously. For callers without device buffers. */
+int del3d_predicates_host(const double* xyz, int64_t n_pts, const uint8_t* op,
+                          const int32_t* idx, int64_t n_q, int8_t* sign_out);
+int del3d_spatial_sort_host(const double* xyz, int64_t n_pts, int32_t* order, int64_t n_order);
+
+/*
+ * Scalar predicates for the C++ drop-in (see del3d_amd_dropin.hpp): one
+ * query evaluated on the GPU, synchronously, on a per-thread stream. They
+ * return the sign; on a device failure they print del3d_last_error() and
+ * abort, since the reference's signatures have no error channel.
+ */
+int del3d_orient3d(const double* a, const double* b, const double* c, const double* d);
+int del3d_in_sphere(const double* a, const double* b, const double* c, const double* d,
+                    const double* e);
+int del3d_in_sphere_unperturbed(const double* a, const double* b, const double* c,
+                                const double* d, const double* e);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DEL3D_AMD_H */

@@ -1,0 +1,32 @@
+// del3d_amd_dropin.hpp - the reference's C++ entry points, served by the GPU.
+//
+// Declares, with the reference's exact signatures (so that code compiled
+// against the reference's headers links against libdel3d_amd.so unchanged):
+//   Sign orient3d(...)               include/del3d/predicates.h:26
+//   Sign in_sphere(...)              include/del3d/predicates.h:49-50
+//   Sign in_sphere_unperturbed(...)  include/del3d/predicates.h:54-55
+//   void spatial_sort(points, order) include/del3d/spatial_sort.h:37
+// The enum and struct layouts are the reference's (predicates.h:21,
+// spatial_sort.h:31), which the mangled symbols and the data layout depend on.
+#ifndef DEL3D_AMD_DROPIN_HPP
+#define DEL3D_AMD_DROPIN_HPP
+
+#include <vector>
+
+namespace del3d {
+
+enum Sign { NEGATIVE = -1, ZERO = 0, POSITIVE = 1 };
+
+struct Point3 {
+    double x, y, z;
+};
+
+Sign orient3d(const double* a, const double* b, const double* c, const double* d);
+Sign in_sphere(const double* a, const double* b, const double* c, const double* d, const double* e);
+Sign in_sphere_unperturbed(const double* a, const double* b, const double* c, const double* d,
+                           const double* e);
+void spatial_sort(const std::vector<Point3>& points, std::vector<int>& order);
+
+}  // namespace del3d
+
+#endif

@@ -136,9 +136,10 @@ def stage_counts() -> tuple[int, int, int]:
 
 
 def spatial_sort(pts, order=None, which: str = "oracle") -> np.ndarray:
+    """order (default 0..n-1) holds indices into pts; it may be shorter."""
     xyz = _xyz(pts)
-    n = len(xyz)
-    o = np.arange(n, dtype=np.int32) if order is None else np.array(order, dtype=np.int32)
+    o = np.arange(len(xyz), dtype=np.int32) if order is None else np.array(order, dtype=np.int32)
+    n = len(o)
     if which == "oracle":
         oracle().orc_spatial_sort(_ptr(xyz, _dp), n, _ptr(o, _i32p))
     elif which == "ref":

@@ -1,0 +1,69 @@
+"""CPU checks of the C ABI: the library loads, exports every declared entry
+point (C and the C++ drop-in), and rejects bad arguments before touching a
+device."""
+import ctypes
+import re
+import subprocess
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+ROOT = Path(__file__).resolve().parents[1]
+LIB = ROOT / "del3d_amd" / "libdel3d_amd.so"
+
+DROPIN_SYMBOLS = [
+    "_ZN5del3d8orient3dEPKdS1_S1_S1_",
+    "_ZN5del3d9in_sphereEPKdS1_S1_S1_S1_",
+    "_ZN5del3d21in_sphere_unperturbedEPKdS1_S1_S1_S1_",
+    "_ZN5del3d12spatial_sortERKSt6vectorINS_6Point3ESaIS1_EERS0_IiSaIiEE",
+]
+
+
+def declared_functions():
+    text = (ROOT / "include" / "del3d_amd.h").read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(del3d_[a-z0-9_]+)\s*\(", text)))
+
+
+@pytest.fixture(scope="module")
+def lib():
+    if not LIB.exists():
+        pytest.fail(f"{LIB} not built")
+    return ctypes.CDLL(str(LIB))
+
+
+def test_exports_every_declared_function(lib):
+    names = declared_functions()
+    assert "del3d_predicates" in names and "del3d_spatial_sort" in names
+    for n in names:
+        assert hasattr(lib, n), n
+
+
+def test_exports_reference_cpp_symbols():
+    out = subprocess.run(["nm", "-D", "--defined-only", str(LIB)], capture_output=True,
+                         text=True, check=True).stdout
+    for sym in DROPIN_SYMBOLS:
+        assert re.search(rf"\bT {re.escape(sym)}$", out, flags=re.M), sym
+
+
+def test_argument_errors_without_device(lib):
+    lib.del3d_last_error.restype = ctypes.c_char_p
+    vp = ctypes.c_void_p
+    lib.del3d_predicates.argtypes = [vp, ctypes.c_int64, vp, vp, ctypes.c_int64, vp, vp]
+    assert lib.del3d_predicates(None, 0, None, None, 0, None, None) == 0   # nothing to do
+    assert lib.del3d_predicates(None, 0, None, None, -1, None, None) == 1
+    assert b"negative" in lib.del3d_last_error()
+    assert lib.del3d_predicates(None, 0, None, None, 5, None, None) == 1
+    lib.del3d_spatial_sort.argtypes = [vp, ctypes.c_int64, vp, ctypes.c_int64, vp]
+    assert lib.del3d_spatial_sort(None, 0, None, 1, None) == 0           # n < 2: no-op
+    assert lib.del3d_spatial_sort(None, 0, None, 5, None) == 1
+    off = (ctypes.c_int64 * 3)(0, 5, 3)
+    lib.del3d_spatial_sort_sets.argtypes = [vp, ctypes.POINTER(ctypes.c_int64), ctypes.c_int64, vp, vp]
+    assert lib.del3d_spatial_sort_sets(None, off, 2, None, None) == 1
+    assert b"decrease" in lib.del3d_last_error()
+
+
+def test_version(lib):
+    lib.del3d_version.restype = ctypes.c_char_p
+    assert b"gfx950" in lib.del3d_version()

@@ -1,0 +1,67 @@
+"""The GPU's per-query arithmetic (del3d_amd/csrc/pred_core.h), built for the
+host, against the big-integer oracle: every tier, including tiers forced on
+queries the kernels would decide earlier."""
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+
+LIB = Path(__file__).resolve().parents[1] / "del3d_amd" / "libpredcore_host.so"
+
+
+def run(pts, op, idx, mode):
+    lib = C.CDLL(str(LIB))
+    pts = np.ascontiguousarray(pts, dtype=np.float64)
+    op = np.ascontiguousarray(op, np.uint8)
+    idx = np.ascontiguousarray(idx, np.int32)
+    out = np.zeros(len(op), np.int8)
+    tier = np.zeros(len(op), np.int8)
+    lib.predcore_host_batch(pts.ctypes.data_as(C.c_void_p), C.c_int64(len(pts)),
+                            op.ctypes.data_as(C.c_void_p), idx.ctypes.data_as(C.c_void_p),
+                            C.c_int64(len(op)), out.ctypes.data_as(C.c_void_p), C.c_int(mode),
+                            tier.ctypes.data_as(C.c_void_p))
+    return out, tier
+
+
+def point_sets():
+    rng = np.random.default_rng(3)
+    return {
+        "lattice": O.lattice(4, 4, 4),
+        "duplicates": np.vstack([O.lattice(3, 3, 3)] * 2),
+        "sphere_q8": O.sphere(400, 8),
+        "box_shell": O.box_shell(6),
+        "cube": O.random_cube(300, 4),
+        "wide_exponents": rng.normal(size=(200, 3)) * np.exp2(rng.integers(-80, 80, size=(200, 3))),
+        "tiny": rng.normal(size=(100, 3)) * 1e-100,
+        "mixed_scales": np.vstack([rng.normal(size=(100, 3)), rng.normal(size=(100, 3)) * 2.0 ** -40]),
+        "grid_2^-48": rng.integers(0, 2 ** 48, size=(300, 3)) * 2.0 ** -48,
+        "signed_zero": np.array([[0.0, -0.0, 0.0], [-0.0, 0.0, 1.0], [1.0, 0.0, -0.0],
+                                 [0.0, 1.0, 0.0], [0.0, 0.0, -0.0], [1.0, 1.0, 0.0]]),
+        "denormals": np.array([[5e-324, 0, 0], [0, 5e-324, 0], [0, 0, 5e-324], [1e-310, 1e-310, 0],
+                               [0, 0, 0], [2.5e-323, 1e-323, 0]]),
+    }
+
+
+@pytest.mark.parametrize("name", list(point_sets().keys()))
+@pytest.mark.parametrize("mode", [0, 1, 2])
+def test_tiers_match_int_oracle(name, mode):
+    if not LIB.exists():
+        pytest.fail(f"{LIB} not built")
+    pts = point_sets()[name]
+    rng = np.random.default_rng(hash(name) % 2 ** 32)
+    n = 3000
+    idx = rng.integers(0, len(pts), size=(n, 5)).astype(np.int32)
+    idx[:400, 4] = idx[:400, 0]      # e aliases a
+    idx[400:600, 2] = idx[400:600, 1]
+    op = rng.integers(0, 3, size=n).astype(np.uint8)
+    want = O.pred_batch(pts, op, idx, "int")
+    got, tier = run(pts, op, idx, mode)
+    bad = np.nonzero(got != want)[0]
+    assert len(bad) == 0, (name, mode, bad[:5], op[bad[:5]], got[bad[:5]], want[bad[:5]])
+    if mode == 1:
+        assert set(np.unique(tier)) <= {2, 3}
+    if mode == 2:
+        assert set(np.unique(tier)) == {3}
