@@ -155,8 +155,7 @@ int del3d_predicates_stats(const double* xyz, int64_t n_pts, const uint8_t* op, 
         D3_HIP(hipMemsetAsync(st.as<void>(), 0, 5 * sizeof(unsigned long long), s));
         launch_predicates(xyz, op, idx, n_q, sign_out, s, st.as<unsigned long long>());
         unsigned long long h[5];
-        D3_HIP(hipMemcpyAsync(h, st.as<void>(), sizeof h, hipMemcpyDeviceToHost, s));
-        D3_HIP(hipStreamSynchronize(s));
+        copy_d2h(h, st.as<void>(), sizeof h, s);
         for (int k = 0; k < 5; ++k) stats_host[k] = (int64_t)h[k];
         return DEL3D_OK;
     });
@@ -171,13 +170,12 @@ int del3d_predicates_host(const double* xyz, int64_t n_pts, const uint8_t* op, c
         hipStream_t s = thread_stream();
         DevBuf dx((size_t)n_pts * 24, s), dop((size_t)n_q, s), did((size_t)n_q * 20, s),
             dout((size_t)n_q, s);
-        D3_HIP(hipMemcpyAsync(dx.as<void>(), xyz, (size_t)n_pts * 24, hipMemcpyHostToDevice, s));
-        D3_HIP(hipMemcpyAsync(dop.as<void>(), op, (size_t)n_q, hipMemcpyHostToDevice, s));
-        D3_HIP(hipMemcpyAsync(did.as<void>(), idx, (size_t)n_q * 20, hipMemcpyHostToDevice, s));
+        copy_h2d(dx.as<void>(), xyz, (size_t)n_pts * 24, s);
+        copy_h2d(dop.as<void>(), op, (size_t)n_q, s);
+        copy_h2d(did.as<void>(), idx, (size_t)n_q * 20, s);
         launch_predicates(dx.as<double>(), dop.as<uint8_t>(), did.as<int32_t>(), n_q,
                           dout.as<int8_t>(), s, nullptr);
-        D3_HIP(hipMemcpyAsync(sign_out, dout.as<void>(), (size_t)n_q, hipMemcpyDeviceToHost, s));
-        D3_HIP(hipStreamSynchronize(s));
+        copy_d2h(sign_out, dout.as<void>(), (size_t)n_q, s);
         return DEL3D_OK;
     });
 }
@@ -218,11 +216,10 @@ int del3d_spatial_sort_host(const double* xyz, int64_t n_pts, int32_t* order, in
         if (!xyz || !order || n_pts == 0) return set_error(DEL3D_ERR_INVALID, "null pointer or no points");
         hipStream_t s = thread_stream();
         DevBuf dx((size_t)n_pts * 24, s), dord((size_t)n_order * 4, s);
-        D3_HIP(hipMemcpyAsync(dx.as<void>(), xyz, (size_t)n_pts * 24, hipMemcpyHostToDevice, s));
-        D3_HIP(hipMemcpyAsync(dord.as<void>(), order, (size_t)n_order * 4, hipMemcpyHostToDevice, s));
+        copy_h2d(dx.as<void>(), xyz, (size_t)n_pts * 24, s);
+        copy_h2d(dord.as<void>(), order, (size_t)n_order * 4, s);
         spatial_sort_device(dx.as<double>(), n_pts, dord.as<int32_t>(), n_order, s);
-        D3_HIP(hipMemcpyAsync(order, dord.as<void>(), (size_t)n_order * 4, hipMemcpyDeviceToHost, s));
-        D3_HIP(hipStreamSynchronize(s));
+        copy_d2h(order, dord.as<void>(), (size_t)n_order * 4, s);
         return DEL3D_OK;
     });
 }

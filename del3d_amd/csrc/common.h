@@ -7,6 +7,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <string>
+#include <vector>
 
 #include "../../include/del3d_amd.h"
 
@@ -74,6 +75,43 @@ public:
 private:
     void* p_ = nullptr;
     hipStream_t s_ = nullptr;
+};
+
+// Copies between pageable host memory and device memory. The runtime performs
+// such copies synchronously, without a promise that queued work on `s` has
+// finished (or has not started), so the stream is drained first: a queued
+// kernel can neither read what the copy is about to overwrite nor produce,
+// after the copy, data the host was meant to read.
+inline void copy_h2d(void* dst, const void* src, size_t bytes, hipStream_t s) {
+    if (!bytes) return;
+    D3_HIP(hipStreamSynchronize(s));
+    D3_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s));
+    D3_HIP(hipStreamSynchronize(s));
+}
+
+inline void copy_d2h(void* dst, const void* src, size_t bytes, hipStream_t s) {
+    if (!bytes) return;
+    D3_HIP(hipStreamSynchronize(s));
+    D3_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, s));
+    D3_HIP(hipStreamSynchronize(s));
+}
+
+// Scratch for one library call: every allocation lives until the arena is
+// destroyed, so no buffer is freed and handed out again while work queued
+// earlier in the call may still use it. Destroy it only after the stream has
+// been synchronized (or accept that the frees are stream-ordered).
+class Scratch {
+public:
+    explicit Scratch(hipStream_t s) : s_(s) {}
+    template <class T>
+    T* get(size_t count) {
+        bufs_.emplace_back(count * sizeof(T) + 16, s_);
+        return bufs_.back().as<T>();
+    }
+
+private:
+    hipStream_t s_;
+    std::vector<DevBuf> bufs_;
 };
 
 // Number of compute units of the current device (cached per device).

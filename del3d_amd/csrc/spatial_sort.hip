@@ -714,10 +714,10 @@ static int grid_for(uint64_t work, int per_block, int cus) {
 
 // Universal shuffle tables for steps 1..K-1: j, F, G (device arrays of K).
 struct ShuffleTables {
-    DevBuf j, F, G;
+    uint32_t *j = nullptr, *F = nullptr, *G = nullptr;
 };
 
-static void build_shuffle_tables(uint64_t K, hipStream_t s, ShuffleTables& T) {
+static ShuffleTables build_shuffle_tables(uint64_t K, hipStream_t s, Scratch& sc) {
     const int cus = device_cu_count();
     const uint32_t last_step = (uint32_t)(K - 1);
     // expected rejections ~ K^2 / 2^33; generous margin, re-run if exceeded
@@ -725,65 +725,58 @@ static void build_shuffle_tables(uint64_t K, hipStream_t s, ShuffleTables& T) {
     uint64_t D = K - 1 + (uint64_t)(er + 12.0 * std::sqrt(er + 1.0)) + 256;
     const uint64_t x0 = ((1ull << 16) | 0x330Eull) & kMask48;  // seed 1
     for (;;) {
-        DevBuf u(D * 4, s);
+        uint32_t* u = sc.get<uint32_t>(D);
         const uint64_t nthreads = (D + kDrawsPerThread - 1) / kDrawsPerThread;
-        k_draws<<<(unsigned)((nthreads + 255) / 256), 256, 0, s>>>(u.as<uint32_t>(), D, x0);
+        k_draws<<<(unsigned)((nthreads + 255) / 256), 256, 0, s>>>(u, D, x0);
         D3_LAUNCH_CHECK();
         const uint32_t thr = 0x80000000u - last_step;  // 2^31 - (K-1)
         const uint64_t nblk = (D + kScanTile - 1) / kScanTile;
-        DevBuf blk(nblk * 4 + 16, s), meta(16, s);
-        uint32_t* blkp = blk.as<uint32_t>();
-        uint32_t* metap = meta.as<uint32_t>();  // [0] ncand, [1] nrej
-        k_cand_count<<<(unsigned)nblk, kScanBlock, 0, s>>>(u.as<uint32_t>(), D, thr, blkp);
+        uint32_t* blkp = sc.get<uint32_t>(nblk);
+        uint32_t* metap = sc.get<uint32_t>(4);  // [0] ncand, [1] nrej, [2] group total
+        k_cand_count<<<(unsigned)nblk, kScanBlock, 0, s>>>(u, D, thr, blkp);
         D3_LAUNCH_CHECK();
         k_scan_small<<<1, 1024, 0, s>>>(blkp, (uint32_t)nblk, metap);
         D3_LAUNCH_CHECK();
         uint32_t ncand_h = 0;
-        D3_HIP(hipMemcpyAsync(&ncand_h, metap, 4, hipMemcpyDeviceToHost, s));
-        D3_HIP(hipStreamSynchronize(s));
-        DevBuf cand((uint64_t)ncand_h * 4 + 4, s), rej((uint64_t)ncand_h * 4 + 4, s);
-        k_cand_write<<<(unsigned)nblk, kScanBlock, 0, s>>>(u.as<uint32_t>(), D, thr, blkp,
-                                                            cand.as<uint32_t>());
+        copy_d2h(&ncand_h, metap, 4, s);
+        uint32_t* cand = sc.get<uint32_t>((uint64_t)ncand_h + 1);
+        uint32_t* rej = sc.get<uint32_t>((uint64_t)ncand_h + 1);
+        k_cand_write<<<(unsigned)nblk, kScanBlock, 0, s>>>(u, D, thr, blkp, cand);
         D3_LAUNCH_CHECK();
-        k_resolve<<<1, 64, 0, s>>>(u.as<uint32_t>(), cand.as<uint32_t>(), metap, last_step,
-                                   rej.as<uint32_t>(), metap + 1);
+        k_resolve<<<1, 64, 0, s>>>(u, cand, metap, last_step, rej, metap + 1);
         D3_LAUNCH_CHECK();
         uint32_t nrej_h = 0;
-        D3_HIP(hipMemcpyAsync(&nrej_h, metap + 1, 4, hipMemcpyDeviceToHost, s));
-        D3_HIP(hipStreamSynchronize(s));
+        copy_d2h(&nrej_h, metap + 1, 4, s);
         if (D < (uint64_t)last_step + nrej_h) {  // not enough draws: redo with more
             D = (uint64_t)last_step + nrej_h + 4096 + D / 64;
             continue;
         }
-        T.j = DevBuf(K * 4, s);
-        T.F = DevBuf(K * 4, s);
-        T.G = DevBuf(K * 4, s);
-        DevBuf gcount(K * 4, s), gstart(K * 4, s), gfill(K * 4, s), grp(K * 4, s);
-        D3_HIP(hipMemsetAsync(gcount.as<void>(), 0, K * 4, s));
-        D3_HIP(hipMemsetAsync(gfill.as<void>(), 0, K * 4, s));
-        D3_HIP(hipMemsetAsync(T.G.as<void>(), 0xFF, K * 4, s));
-        k_steps<<<grid_for(D, 256, cus), 256, 0, s>>>(u.as<uint32_t>(), D, rej.as<uint32_t>(),
-                                                      metap + 1, last_step, T.j.as<uint32_t>(),
-                                                      gcount.as<uint32_t>());
+        ShuffleTables T;
+        T.j = sc.get<uint32_t>(K);
+        T.F = sc.get<uint32_t>(K);
+        T.G = sc.get<uint32_t>(K);
+        uint32_t* gcount = sc.get<uint32_t>(K);
+        uint32_t* gstart = sc.get<uint32_t>(K);
+        uint32_t* gfill = sc.get<uint32_t>(K);
+        uint32_t* grp = sc.get<uint32_t>(K);
+        D3_HIP(hipMemsetAsync(gcount, 0, K * 4, s));
+        D3_HIP(hipMemsetAsync(gfill, 0, K * 4, s));
+        D3_HIP(hipMemsetAsync(T.G, 0xFF, K * 4, s));
+        k_steps<<<grid_for(D, 256, cus), 256, 0, s>>>(u, D, rej, metap + 1, last_step, T.j, gcount);
         D3_LAUNCH_CHECK();
         const uint64_t kb = (K + kScanTile - 1) / kScanTile;
-        DevBuf kblk(kb * 4 + 16, s);
-        k_tile_sum<<<(unsigned)kb, kScanBlock, 0, s>>>(gcount.as<uint32_t>(), K, kblk.as<uint32_t>());
+        uint32_t* kblk = sc.get<uint32_t>(kb);
+        k_tile_sum<<<(unsigned)kb, kScanBlock, 0, s>>>(gcount, K, kblk);
         D3_LAUNCH_CHECK();
-        k_scan_small<<<1, 1024, 0, s>>>(kblk.as<uint32_t>(), (uint32_t)kb, metap + 2);
+        k_scan_small<<<1, 1024, 0, s>>>(kblk, (uint32_t)kb, metap + 2);
         D3_LAUNCH_CHECK();
-        k_tile_scan<<<(unsigned)kb, kScanBlock, 0, s>>>(gcount.as<uint32_t>(), K, kblk.as<uint32_t>(),
-                                                         gstart.as<uint32_t>());
+        k_tile_scan<<<(unsigned)kb, kScanBlock, 0, s>>>(gcount, K, kblk, gstart);
         D3_LAUNCH_CHECK();
-        k_group_scatter<<<grid_for(K, 256, cus), 256, 0, s>>>(T.j.as<uint32_t>(), last_step,
-                                                              gstart.as<uint32_t>(),
-                                                              gfill.as<uint32_t>(), grp.as<uint32_t>());
+        k_group_scatter<<<grid_for(K, 256, cus), 256, 0, s>>>(T.j, last_step, gstart, gfill, grp);
         D3_LAUNCH_CHECK();
-        k_group_links<<<grid_for(K, 256, cus), 256, 0, s>>>((uint32_t)K, gstart.as<uint32_t>(),
-                                                            gcount.as<uint32_t>(), grp.as<uint32_t>(),
-                                                            T.F.as<uint32_t>(), T.G.as<uint32_t>());
+        k_group_links<<<grid_for(K, 256, cus), 256, 0, s>>>((uint32_t)K, gstart, gcount, grp, T.F, T.G);
         D3_LAUNCH_CHECK();
-        return;
+        return T;
     }
 }
 
@@ -795,7 +788,7 @@ struct SplitReq {
 };
 
 static void run_splits(Rec* A, Rec* B, int64_t* Lpos, const std::vector<SplitReq>& reqs,
-                       hipStream_t s) {
+                       hipStream_t s, Scratch& sc) {
     const int cus = device_cu_count();
     std::vector<SelTask> big;
     std::vector<int64_t> big_nth;
@@ -824,32 +817,28 @@ static void run_splits(Rec* A, Rec* B, int64_t* Lpos, const std::vector<SplitReq
             }
         }
         tile0.push_back((int64_t)tiles.size());
-        DevBuf dt(big.size() * sizeof(SelTask), s), dtl(tiles.size() * sizeof(SelTile), s),
-            dt0(tile0.size() * 8, s), dcnt(tiles.size() * 4, s), doff(tiles.size() * 4, s);
-        D3_HIP(hipMemcpyAsync(dt.as<void>(), big.data(), big.size() * sizeof(SelTask),
-                              hipMemcpyHostToDevice, s));
-        D3_HIP(hipMemcpyAsync(dtl.as<void>(), tiles.data(), tiles.size() * sizeof(SelTile),
-                              hipMemcpyHostToDevice, s));
-        D3_HIP(hipMemcpyAsync(dt0.as<void>(), tile0.data(), tile0.size() * 8, hipMemcpyHostToDevice, s));
-        D3_HIP(hipStreamSynchronize(s));  // host vectors above
+        SelTask* dt = sc.get<SelTask>(big.size());
+        SelTile* dtl = sc.get<SelTile>(tiles.size());
+        int64_t* dt0 = sc.get<int64_t>(tile0.size());
+        uint32_t* dcnt = sc.get<uint32_t>(tiles.size());
+        uint32_t* doff = sc.get<uint32_t>(tiles.size());
+        copy_h2d(dt, big.data(), big.size() * sizeof(SelTask), s);
+        copy_h2d(dtl, tiles.data(), tiles.size() * sizeof(SelTile), s);
+        copy_h2d(dt0, tile0.data(), tile0.size() * 8, s);
         const unsigned nt = (unsigned)tiles.size();
-        k_sel_count<<<nt, kTileThreads, 0, s>>>(A, dt.as<SelTask>(), dtl.as<SelTile>(), dcnt.as<uint32_t>());
+        k_sel_count<<<nt, kTileThreads, 0, s>>>(A, dt, dtl, dcnt);
         D3_LAUNCH_CHECK();
-        k_sel_offsets<<<grid_for(big.size(), 256, cus), 256, 0, s>>>(
-            dt.as<SelTask>(), (uint32_t)big.size(), dt0.as<int64_t>(), dcnt.as<uint32_t>(),
-            doff.as<uint32_t>());
+        k_sel_offsets<<<grid_for(big.size(), 256, cus), 256, 0, s>>>(dt, (uint32_t)big.size(), dt0,
+                                                                     dcnt, doff);
         D3_LAUNCH_CHECK();
-        k_sel_flagged<<<nt, kTileThreads, 0, s>>>(A, dt.as<SelTask>(), dtl.as<SelTile>(),
-                                                  doff.as<uint32_t>(), Lpos, B);
+        k_sel_flagged<<<nt, kTileThreads, 0, s>>>(A, dt, dtl, doff, Lpos, B);
         D3_LAUNCH_CHECK();
-        k_sel_unflagged<<<nt, kTileThreads, 0, s>>>(A, dt.as<SelTask>(), dtl.as<SelTile>(), Lpos, B);
+        k_sel_unflagged<<<nt, kTileThreads, 0, s>>>(A, dt, dtl, Lpos, B);
         D3_LAUNCH_CHECK();
-        k_sel_copy<<<nt, kTileThreads, 0, s>>>(A, dt.as<SelTask>(), dtl.as<SelTile>(), B);
+        k_sel_copy<<<nt, kTileThreads, 0, s>>>(A, dt, dtl, B);
         D3_LAUNCH_CHECK();
         std::vector<SelTask> back(big.size());
-        D3_HIP(hipMemcpyAsync(back.data(), dt.as<void>(), big.size() * sizeof(SelTask),
-                              hipMemcpyDeviceToHost, s));
-        D3_HIP(hipStreamSynchronize(s));
+        copy_d2h(back.data(), dt, big.size() * sizeof(SelTask), s);
         std::vector<SelTask> next;
         std::vector<int64_t> next_nth;
         for (size_t i = 0; i < big.size(); ++i) {
@@ -870,12 +859,10 @@ static void run_splits(Rec* A, Rec* B, int64_t* Lpos, const std::vector<SplitReq
         big_nth.swap(next_nth);
     }
     if (!small.empty()) {
-        DevBuf dw(small.size() * sizeof(WaveTask), s);
-        D3_HIP(hipMemcpyAsync(dw.as<void>(), small.data(), small.size() * sizeof(WaveTask),
-                              hipMemcpyHostToDevice, s));
-        D3_HIP(hipStreamSynchronize(s));  // `small` is a host vector
+        WaveTask* dw = sc.get<WaveTask>(small.size());
+        copy_h2d(dw, small.data(), small.size() * sizeof(WaveTask), s);
         const unsigned g = (unsigned)std::min<size_t>(small.size(), (size_t)cus * 8);
-        k_wave_select<<<g, 64, 0, s>>>(A, B, dw.as<WaveTask>(), (uint32_t)small.size());
+        k_wave_select<<<g, 64, 0, s>>>(A, B, dw, (uint32_t)small.size());
         D3_LAUNCH_CHECK();
     }
 }
@@ -890,34 +877,32 @@ static void sort_sets(const double* xyz, const int64_t* poff, const int64_t* off
     uint64_t K = 2;
     for (int64_t i = 0; i < n_sets; ++i) K = std::max<uint64_t>(K, (uint64_t)(off[i + 1] - off[i]));
 
-    ShuffleTables T;
-    build_shuffle_tables(K, s, T);
+    // every buffer of this call lives until the end of the call (see Scratch)
+    Scratch sc(s);
+    const ShuffleTables T = build_shuffle_tables(K, s, sc);
 
-    DevBuf doff((n_sets + 1) * 8, s), dpoff((n_sets + 1) * 8, s);
-    D3_HIP(hipMemcpyAsync(doff.as<void>(), off, (n_sets + 1) * 8, hipMemcpyHostToDevice, s));
-    D3_HIP(hipMemcpyAsync(dpoff.as<void>(), poff ? poff : off, (n_sets + 1) * 8,
-                          hipMemcpyHostToDevice, s));
-    D3_HIP(hipStreamSynchronize(s));
-    DevBuf ta(total * 4, s), tb(total * 4, s);
-    k_chain_init<<<grid_for(total, 256, cus), 256, 0, s>>>(doff.as<int64_t>(), (uint32_t)n_sets,
-                                                           total, T.F.as<uint32_t>(), ta.as<uint32_t>());
+    int64_t* doff = sc.get<int64_t>(n_sets + 1);
+    int64_t* dpoff = sc.get<int64_t>(n_sets + 1);
+    copy_h2d(doff, off, (n_sets + 1) * 8, s);
+    copy_h2d(dpoff, poff ? poff : off, (n_sets + 1) * 8, s);
+    uint32_t* ta = sc.get<uint32_t>(total);
+    uint32_t* tb = sc.get<uint32_t>(total);
+    k_chain_init<<<grid_for(total, 256, cus), 256, 0, s>>>(doff, (uint32_t)n_sets, total, T.F, ta);
     D3_LAUNCH_CHECK();
     int rounds = 1;
     while ((1ull << rounds) < K) ++rounds;
-    uint32_t* cur = ta.as<uint32_t>();
-    uint32_t* nxt = tb.as<uint32_t>();
+    uint32_t* cur = ta;
+    uint32_t* nxt = tb;
     for (int r = 0; r < rounds; ++r) {
         k_jump<<<grid_for(total, 256, cus), 256, 0, s>>>(cur, total, nxt);
         D3_LAUNCH_CHECK();
         std::swap(cur, nxt);
     }
-    DevBuf recA(total * sizeof(Rec), s), recB(total * sizeof(Rec), s), lpos(total * 8, s);
-    Rec* A = recA.as<Rec>();
-    Rec* B = recB.as<Rec>();
-    k_shuffle_place<<<grid_for(total, 256, cus), 256, 0, s>>>(
-        doff.as<int64_t>(), dpoff.as<int64_t>(), (uint32_t)n_sets, total, T.j.as<uint32_t>(),
-        T.F.as<uint32_t>(),
-        T.G.as<uint32_t>(), cur, order, xyz, A);
+    Rec* A = sc.get<Rec>(total);
+    Rec* B = sc.get<Rec>(total);
+    int64_t* lpos = sc.get<int64_t>(total);
+    k_shuffle_place<<<grid_for(total, 256, cus), 256, 0, s>>>(doff, dpoff, (uint32_t)n_sets, total,
+                                                              T.j, T.F, T.G, cur, order, xyz, A);
     D3_LAUNCH_CHECK();
 
     // multiscale segments (src/spatial_sort.cpp:130-137) of every set
@@ -946,14 +931,14 @@ static void sort_sets(const double* xyz, const int64_t* poff, const int64_t* off
         if (big.empty()) break;
         std::vector<SplitReq> px, py, pz;
         for (const Node& nd : big) px.push_back(SplitReq{nd.lo, nd.hi, nd.x, nd.ux});
-        run_splits(A, B, lpos.as<int64_t>(), px, s);
+        run_splits(A, B, lpos, px, s, sc);
         for (const Node& nd : big) {
             const int y = (nd.x + 1) % 3;
             const int64_t m0 = nd.lo, m8 = nd.hi, m4 = mid_of(m0, m8);
             py.push_back(SplitReq{m0, m4, y, nd.uy});
             py.push_back(SplitReq{m4, m8, y, !nd.uy});
         }
-        run_splits(A, B, lpos.as<int64_t>(), py, s);
+        run_splits(A, B, lpos, py, s, sc);
         std::vector<Node> next;
         for (const Node& nd : big) {
             const int x = nd.x, y = (x + 1) % 3, z = (x + 2) % 3;
@@ -975,7 +960,7 @@ static void sort_sets(const double* xyz, const int64_t* poff, const int64_t* off
             next.push_back(Node{m6, m7, y, !uy, uz, !ux});
             next.push_back(Node{m7, m8, z, !uz, !ux, uy});
         }
-        run_splits(A, B, lpos.as<int64_t>(), pz, s);
+        run_splits(A, B, lpos, pz, s, sc);
         level.swap(next);
     }
     std::vector<LeafNode> lv;
@@ -983,15 +968,13 @@ static void sort_sets(const double* xyz, const int64_t* poff, const int64_t* off
     for (const Node& nd : leaves)
         if (nd.hi > nd.lo) lv.push_back(LeafNode{nd.lo, nd.hi, nd.x, nd.ux, nd.uy, nd.uz});
     if (!lv.empty()) {
-        DevBuf dl(lv.size() * sizeof(LeafNode), s);
-        D3_HIP(hipMemcpyAsync(dl.as<void>(), lv.data(), lv.size() * sizeof(LeafNode),
-                              hipMemcpyHostToDevice, s));
-        D3_HIP(hipStreamSynchronize(s));
+        LeafNode* dl = sc.get<LeafNode>(lv.size());
+        copy_h2d(dl, lv.data(), lv.size() * sizeof(LeafNode), s);
         const unsigned g = (unsigned)std::min<size_t>(lv.size(), (size_t)cus * 8);
-        k_wave_subtree<<<g, 64, 0, s>>>(A, dl.as<LeafNode>(), (uint32_t)lv.size(), order);
+        k_wave_subtree<<<g, 64, 0, s>>>(A, dl, (uint32_t)lv.size(), order);
         D3_LAUNCH_CHECK();
     }
-    // keep the host-side tables alive until the stream has used them
+    // the scratch arena is released after this: nothing queued may still use it
     D3_HIP(hipStreamSynchronize(s));
 }
 

@@ -1,0 +1,267 @@
+// hipemu.h - runs this repository's HIP kernels on the CPU, for bounds
+// checking under AddressSanitizer (test infrastructure only).
+//
+// A launch runs its blocks one after another; the threads of a block run as
+// std::threads, __syncthreads() is a barrier over the block, and each
+// wavefront-level operation (__ballot, __shfl, __shfl_up) is a barrier over
+// the 64 threads of its wavefront. Kernels must reach wavefront operations
+// convergently, which the kernels here do. __shared__ variables become
+// function statics: one block runs at a time. Device memory is host memory.
+#pragma once
+
+#include <algorithm>
+#include <atomic>
+#include <barrier>
+#include <condition_variable>
+#include <mutex>
+#include <cmath>
+#include <cstdint>
+#include <cstdlib>
+#include <cstring>
+#include <functional>
+#include <memory>
+#include <thread>
+#include <vector>
+
+#define __global__
+#define __device__
+#define __host__
+#define __forceinline__ inline
+#define __launch_bounds__(...)
+#define __shared__ static
+
+struct dim3 {
+    unsigned x = 1, y = 1, z = 1;
+    dim3(unsigned a = 1, unsigned b = 1, unsigned c = 1) : x(a), y(b), z(c) {}
+};
+
+typedef void* hipStream_t;
+typedef int hipError_t;
+enum { hipSuccess = 0, hipErrorOutOfMemory = 2, hipErrorInvalidValue = 1 };
+enum hipMemcpyKind { hipMemcpyHostToHost = 0, hipMemcpyHostToDevice = 1, hipMemcpyDeviceToHost = 2,
+                     hipMemcpyDeviceToDevice = 3, hipMemcpyDefault = 4 };
+enum { hipStreamNonBlocking = 1, hipHostMallocDefault = 0, hipDeviceAttributeMultiprocessorCount = 63 };
+
+namespace emu {
+struct Wave {
+    std::unique_ptr<std::barrier<>> bar;
+    uint64_t vals[64];
+    int64_t wide[64];
+};
+struct Block {
+    std::unique_ptr<std::barrier<>> bar;
+    std::vector<Wave> waves;
+};
+inline thread_local dim3 tl_tid, tl_bid;
+inline dim3 g_bdim, g_gdim;
+inline thread_local Block* tl_block = nullptr;
+inline int g_cus = 4;
+}  // namespace emu
+
+#define threadIdx (::emu::tl_tid)
+#define blockIdx (::emu::tl_bid)
+#define blockDim (::emu::g_bdim)
+#define gridDim (::emu::g_gdim)
+
+inline void __syncthreads() { emu::tl_block->bar->arrive_and_wait(); }
+inline int __lane_id() { return (int)(threadIdx.x & 63u); }
+inline uint64_t __lanemask_lt() { return (1ull << __lane_id()) - 1ull; }
+inline int __popcll(uint64_t v) { return __builtin_popcountll(v); }
+inline int __ffsll(long long v) { return __builtin_ffsll(v); }
+
+inline emu::Wave& cur_wave() { return emu::tl_block->waves[threadIdx.x >> 6]; }
+
+inline uint64_t __ballot(int pred) {
+    emu::Wave& w = cur_wave();
+    w.vals[__lane_id()] = pred ? 1 : 0;
+    w.bar->arrive_and_wait();
+    uint64_t m = 0;
+    const unsigned base = threadIdx.x & ~63u;
+    for (unsigned l = 0; l < 64 && base + l < emu::g_bdim.x; ++l) m |= (w.vals[l] ? 1ull : 0ull) << l;
+    w.bar->arrive_and_wait();
+    return m;
+}
+
+template <class T>
+inline T emu_exchange(T v, int src) {
+    emu::Wave& w = cur_wave();
+    static_assert(sizeof(T) <= 8, "wide shuffles not emulated");
+    int64_t bits = 0;
+    std::memcpy(&bits, &v, sizeof(T));
+    w.wide[__lane_id()] = bits;
+    w.bar->arrive_and_wait();
+    T r;
+    int64_t got = w.wide[src & 63];
+    std::memcpy(&r, &got, sizeof(T));
+    w.bar->arrive_and_wait();
+    return r;
+}
+template <class T>
+inline T __shfl(T v, int src) { return emu_exchange(v, src); }
+template <class T>
+inline T __shfl_up(T v, unsigned d) {
+    const int lane = __lane_id();
+    return emu_exchange(v, lane >= (int)d ? lane - (int)d : lane);
+}
+
+template <class T>
+inline T atomicAdd(T* p, T v) { return __atomic_fetch_add(p, v, __ATOMIC_SEQ_CST); }
+using std::max;
+using std::min;
+
+namespace emu {
+// EMU_DEFER=1 models the runtime's documented asynchrony: kernel launches and
+// memsets are queued and run at the next stream synchronization, while
+// copies to or from pageable host memory happen at once ("performed
+// synchronously" for unpinned memory) and hipFreeAsync releases at once. A
+// host sequence that relies on a pageable copy or a free being ordered after
+// queued kernels then shows up as a use-after-free under AddressSanitizer.
+inline bool defer_mode() {
+    static const bool d = [] {
+        const char* e = std::getenv("EMU_DEFER");
+        return e && std::atoi(e) != 0;
+    }();
+    return d;
+}
+inline std::vector<std::function<void()>>& queue() {
+    static std::vector<std::function<void()>> q;
+    return q;
+}
+inline void drain() {
+    auto q = std::move(queue());
+    queue().clear();
+    for (auto& f : q) f();
+}
+}  // namespace emu
+
+inline const char* hipGetErrorString(hipError_t) { return "emulated error"; }
+inline hipError_t hipGetLastError() { return hipSuccess; }
+inline hipError_t hipGetDevice(int* d) { *d = 0; return hipSuccess; }
+inline hipError_t hipDeviceGetAttribute(int* v, int, int) { *v = emu::g_cus; return hipSuccess; }
+inline hipError_t hipMallocAsync(void** p, size_t n, hipStream_t) {
+    *p = std::malloc(n ? n : 1);
+    return *p ? hipSuccess : hipErrorOutOfMemory;
+}
+inline hipError_t hipMalloc(void** p, size_t n) { return hipMallocAsync(p, n, nullptr); }
+inline hipError_t hipFreeAsync(void* p, hipStream_t) { std::free(p); return hipSuccess; }
+inline hipError_t hipFree(void* p) { std::free(p); return hipSuccess; }
+inline hipError_t hipMemcpyAsync(void* d, const void* s, size_t n, hipMemcpyKind, hipStream_t) {
+    if (n) std::memcpy(d, s, n);
+    return hipSuccess;
+}
+inline hipError_t hipMemsetAsync(void* d, int v, size_t n, hipStream_t) {
+    if (!n) return hipSuccess;
+    if (emu::defer_mode()) emu::queue().push_back([=] { std::memset(d, v, n); });
+    else std::memset(d, v, n);
+    return hipSuccess;
+}
+inline hipError_t hipStreamSynchronize(hipStream_t) {
+    emu::drain();
+    return hipSuccess;
+}
+inline hipError_t hipStreamCreateWithFlags(hipStream_t* s, unsigned) { *s = nullptr; return hipSuccess; }
+inline hipError_t hipHostMalloc(void** p, size_t n, unsigned) { return hipMallocAsync(p, n, nullptr); }
+
+namespace emu {
+template <class K, class... A>
+void run_launch(dim3 grid, dim3 block, K kernel, A... args);
+
+template <class K, class... A>
+void launch(dim3 grid, dim3 block, K kernel, A... args) {
+    if (defer_mode()) {
+        queue().push_back([=] { run_launch(grid, block, kernel, args...); });
+        return;
+    }
+    run_launch(grid, block, kernel, args...);
+}
+
+// Persistent workers, one per thread of the largest block seen; each block
+// is run by the first block.x workers.
+struct Pool {
+    std::vector<std::thread> workers;
+    std::mutex mu;
+    std::condition_variable cv, done_cv;
+    uint64_t gen = 0;
+    unsigned active = 0, remaining = 0;
+    std::function<void(unsigned)> job;
+    bool quit = false;
+    void ensure(unsigned n) {
+        while (workers.size() < n) {
+            const unsigned id = (unsigned)workers.size();
+            workers.emplace_back([this, id] {
+                uint64_t seen = 0;
+                for (;;) {
+                    std::function<void(unsigned)> j;
+                    {
+                        std::unique_lock<std::mutex> lk(mu);
+                        cv.wait(lk, [&] { return quit || (gen != seen && id < active); });
+                        if (quit) return;
+                        seen = gen;
+                        j = job;
+                    }
+                    j(id);
+                    std::lock_guard<std::mutex> lk(mu);
+                    if (--remaining == 0) done_cv.notify_all();
+                }
+            });
+        }
+    }
+    void run(unsigned n, std::function<void(unsigned)> f) {
+        ensure(n);
+        std::unique_lock<std::mutex> lk(mu);
+        job = std::move(f);
+        active = n;
+        remaining = n;
+        ++gen;
+        cv.notify_all();
+        done_cv.wait(lk, [&] { return remaining == 0; });
+    }
+    ~Pool() {
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            quit = true;
+        }
+        cv.notify_all();
+        for (auto& w : workers) w.join();
+    }
+};
+inline Pool& pool() {
+    static Pool p;
+    return p;
+}
+
+template <class K, class... A>
+void run_launch(dim3 grid, dim3 block, K kernel, A... args) {
+    g_gdim = grid;
+    g_bdim = block;
+    const unsigned nt = block.x;
+    // block order: 0 = launch order, 1 = reversed, 2 = a fixed shuffle; set
+    // through EMU_BLOCK_ORDER to expose dependencies between blocks
+    static const int order_mode = [] {
+        const char* e = std::getenv("EMU_BLOCK_ORDER");
+        return e ? std::atoi(e) : 0;
+    }();
+    std::vector<unsigned> bo(grid.x);
+    for (unsigned b = 0; b < grid.x; ++b) bo[b] = b;
+    if (order_mode == 1) std::reverse(bo.begin(), bo.end());
+    if (order_mode == 2)
+        for (unsigned b = grid.x; b > 1; --b) std::swap(bo[b - 1], bo[(b * 2654435761u) % b]);
+    for (unsigned bi = 0; bi < grid.x; ++bi) {
+        const unsigned b = bo[bi];
+        Block blk;
+        blk.bar = std::make_unique<std::barrier<>>(nt);
+        const unsigned nw = (nt + 63) / 64;
+        blk.waves.resize(nw);
+        for (unsigned w = 0; w < nw; ++w)
+            blk.waves[w].bar = std::make_unique<std::barrier<>>(std::min(64u, nt - 64 * w));
+        pool().run(nt, [&, b](unsigned t) {
+            tl_tid = dim3(t);
+            tl_bid = dim3(b);
+            tl_block = &blk;
+            kernel(args...);
+            blk.waves[t >> 6].bar->arrive_and_drop();
+            blk.bar->arrive_and_drop();
+        });
+    }
+}
+}  // namespace emu

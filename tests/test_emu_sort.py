@@ -1,0 +1,34 @@
+"""The GPU spatial sort's kernels and host driver, run on the CPU by the
+emulator in tests/emu under AddressSanitizer, against the oracle.
+
+EMU_DEFER=1 models the runtime's asynchrony at its least favourable:
+kernels run only when the host synchronizes, while copies to or from
+pageable host memory and stream-ordered frees take effect at once. A host
+sequence that needs a wait it does not have (reading results before the
+kernels producing them ran, or refilling a buffer a queued kernel still
+reads) then fails here instead of on a GPU. EMU_BLOCK_ORDER reverses or
+shuffles the order of blocks, which exposes dependencies between blocks of
+one launch."""
+import os
+import subprocess
+from pathlib import Path
+
+import pytest
+
+HERE = Path(__file__).resolve().parent
+EXE = Path("/tmp") / f"del3d_emu_sort_{os.getuid()}"
+
+
+@pytest.fixture(scope="module")
+def emu_exe():
+    subprocess.run(["python", str(HERE / "emu" / "build_emu.py"), str(EXE),
+                    str(HERE.parent / "del3d_amd" / "csrc" / "spatial_sort.hip")], check=True)
+    return EXE
+
+
+@pytest.mark.parametrize("defer,order", [("1", "2"), ("0", "1")])
+def test_emulated_sort_matches_oracle(emu_exe, defer, order):
+    env = dict(os.environ, EMU_DEFER=defer, EMU_BLOCK_ORDER=order, ASAN_OPTIONS="detect_leaks=0")
+    r = subprocess.run([str(emu_exe), "2", "65", "1100", "2100"],
+                       capture_output=True, text=True, timeout=900, env=env)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-4000:]
