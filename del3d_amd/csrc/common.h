@@ -45,74 +45,44 @@ struct HipError {
 
 #define D3_LAUNCH_CHECK() D3_HIP(hipGetLastError())
 
-// Stream-ordered scratch buffer, released on the same stream.
-class DevBuf {
-public:
-    DevBuf() = default;
-    DevBuf(size_t bytes, hipStream_t s) : s_(s) {
-        if (bytes) D3_HIP(hipMallocAsync(&p_, bytes, s));
-    }
-    ~DevBuf() {
-        if (p_) (void)hipFreeAsync(p_, s_);
-    }
-    DevBuf(const DevBuf&) = delete;
-    DevBuf& operator=(const DevBuf&) = delete;
-    DevBuf(DevBuf&& o) noexcept : p_(o.p_), s_(o.s_) { o.p_ = nullptr; }
-    DevBuf& operator=(DevBuf&& o) noexcept {
-        if (this != &o) {
-            if (p_) (void)hipFreeAsync(p_, s_);
-            p_ = o.p_;
-            s_ = o.s_;
-            o.p_ = nullptr;
-        }
-        return *this;
-    }
-    template <class T>
-    T* as() const {
-        return static_cast<T*>(p_);
-    }
+// Device scratch for one library call, taken from a per-stream cache of
+// hipMalloc'd blocks and given back to that cache when the call ends. A block
+// given back may still be read by work queued on its stream, so it is only
+// handed out again to a later call on the same stream, which the stream
+// orders after that work. Nothing is freed to the runtime while in use.
+void* scratch_take(size_t bytes, hipStream_t s, size_t* got);
+void scratch_give(void* p, size_t bytes, hipStream_t s);
 
-private:
-    void* p_ = nullptr;
-    hipStream_t s_ = nullptr;
-};
-
-// Copies between pageable host memory and device memory. The runtime performs
-// such copies synchronously, without a promise that queued work on `s` has
-// finished (or has not started), so the stream is drained first: a queued
-// kernel can neither read what the copy is about to overwrite nor produce,
-// after the copy, data the host was meant to read.
-inline void copy_h2d(void* dst, const void* src, size_t bytes, hipStream_t s) {
-    if (!bytes) return;
-    D3_HIP(hipStreamSynchronize(s));
-    D3_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s));
-    D3_HIP(hipStreamSynchronize(s));
-}
-
-inline void copy_d2h(void* dst, const void* src, size_t bytes, hipStream_t s) {
-    if (!bytes) return;
-    D3_HIP(hipStreamSynchronize(s));
-    D3_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, s));
-    D3_HIP(hipStreamSynchronize(s));
-}
-
-// Scratch for one library call: every allocation lives until the arena is
-// destroyed, so no buffer is freed and handed out again while work queued
-// earlier in the call may still use it. Destroy it only after the stream has
-// been synchronized (or accept that the frees are stream-ordered).
 class Scratch {
 public:
     explicit Scratch(hipStream_t s) : s_(s) {}
+    ~Scratch() {
+        for (auto& b : blocks_) scratch_give(b.first, b.second, s_);
+    }
+    Scratch(const Scratch&) = delete;
+    Scratch& operator=(const Scratch&) = delete;
     template <class T>
     T* get(size_t count) {
-        bufs_.emplace_back(count * sizeof(T) + 16, s_);
-        return bufs_.back().as<T>();
+        size_t got = 0;
+        void* p = scratch_take(count * sizeof(T) + 16, s_, &got);
+        blocks_.emplace_back(p, got);
+        return static_cast<T*>(p);
     }
 
 private:
     hipStream_t s_;
-    std::vector<DevBuf> bufs_;
+    std::vector<std::pair<void*, size_t>> blocks_;
 };
+
+// Host<->device copies go through a pinned staging buffer of the stream, so
+// they are ordered in the stream like any kernel; they return when the copy
+// has completed.
+void copy_h2d(void* dst, const void* src, size_t bytes, hipStream_t s);
+void copy_d2h(void* dst, const void* src, size_t bytes, hipStream_t s);
+
+// p[0..n) = v, as a kernel on s (no hipMemsetAsync: a kernel is also what a
+// captured graph replays reliably).
+void fill_u32(uint32_t* p, uint32_t v, size_t n, hipStream_t s);
 
 // Number of compute units of the current device (cached per device).
 int device_cu_count();

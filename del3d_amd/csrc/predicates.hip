@@ -288,30 +288,32 @@ void launch_predicates(const double* xyz, const uint8_t* op, const int32_t* idx,
                        int8_t* out, hipStream_t s, unsigned long long* stats_dev) {
     if (n_q <= 0) return;
     const uint32_t chunk = (uint32_t)std::min<int64_t>(n_q, kChunk);
-    DevBuf qbuf((size_t)chunk * 4, s), wbuf((size_t)chunk * 4, s), cbuf(sizeof(PredCounters), s);
+    Scratch sc(s);
+    uint32_t* queue = sc.get<uint32_t>(chunk);
+    uint32_t* wide = sc.get<uint32_t>(chunk);
+    PredCounters* ctr = sc.get<PredCounters>(1);
     const int cus = device_cu_count();
     for (int64_t off = 0; off < n_q; off += chunk) {
         const uint32_t n = (uint32_t)std::min<int64_t>(chunk, n_q - off);
-        PredCounters* ctr = cbuf.as<PredCounters>();
-        D3_HIP(hipMemsetAsync(ctr, 0, sizeof(PredCounters), s));
+        fill_u32(reinterpret_cast<uint32_t*>(ctr), 0u, sizeof(PredCounters) / 4, s);
         const uint32_t tiles = (n + kTile - 1) / kTile;
         const uint32_t fast_grid = std::min<uint32_t>(tiles, (uint32_t)cus * 16);
         const uint8_t* opc = op + off;
         const int32_t* idc = idx + off * 5;
         int8_t* outc = out + off;
         if (stats_dev) {
-            pred_fast<true><<<fast_grid, kTile, 0, s>>>(xyz, opc, idc, n, outc, qbuf.as<uint32_t>(),
+            pred_fast<true><<<fast_grid, kTile, 0, s>>>(xyz, opc, idc, n, outc, queue,
                                                         ctr, stats_dev);
             D3_LAUNCH_CHECK();
         } else {
             pred_fast<false><<<fast_grid, kTile, 0, s>>>(xyz, opc, idc, n, outc,
-                                                         qbuf.as<uint32_t>(), ctr, nullptr);
+                                                         queue, ctr, nullptr);
             D3_LAUNCH_CHECK();
         }
-        pred_exact<<<cus * 4, 256, 0, s>>>(xyz, opc, idc, outc, qbuf.as<uint32_t>(),
-                                           wbuf.as<uint32_t>(), ctr, stats_dev);
+        pred_exact<<<cus * 4, 256, 0, s>>>(xyz, opc, idc, outc, queue,
+                                           wide, ctr, stats_dev);
         D3_LAUNCH_CHECK();
-        pred_wide<<<cus * 4, kWideBlock, 0, s>>>(xyz, opc, idc, outc, wbuf.as<uint32_t>(), ctr,
+        pred_wide<<<cus * 4, kWideBlock, 0, s>>>(xyz, opc, idc, outc, wide, ctr,
                                                  stats_dev);
         D3_LAUNCH_CHECK();
     }

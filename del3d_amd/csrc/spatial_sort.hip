@@ -759,9 +759,9 @@ static ShuffleTables build_shuffle_tables(uint64_t K, hipStream_t s, Scratch& sc
         uint32_t* gstart = sc.get<uint32_t>(K);
         uint32_t* gfill = sc.get<uint32_t>(K);
         uint32_t* grp = sc.get<uint32_t>(K);
-        D3_HIP(hipMemsetAsync(gcount, 0, K * 4, s));
-        D3_HIP(hipMemsetAsync(gfill, 0, K * 4, s));
-        D3_HIP(hipMemsetAsync(T.G, 0xFF, K * 4, s));
+        fill_u32(gcount, 0u, K, s);
+        fill_u32(gfill, 0u, K, s);
+        fill_u32(T.G, kNone, K, s);
         k_steps<<<grid_for(D, 256, cus), 256, 0, s>>>(u, D, rej, metap + 1, last_step, T.j, gcount);
         D3_LAUNCH_CHECK();
         const uint64_t kb = (K + kScanTile - 1) / kScanTile;

@@ -6,6 +6,7 @@ Usage: python tests/emu/build_emu.py OUT SOURCE.hip [SOURCE.hip ...]
   OUT ending in .so builds a shared library; otherwise an executable that
   links emu_main.cpp and the oracle sources.
 """
+import os
 import re
 import subprocess
 import sys
@@ -57,18 +58,22 @@ def main():
             t = Path(td) / (p.stem + ".cpp")
             t.write_text(f'#include "{CSRC}/common.h"\n' + rewrite(p.read_text()).replace(
                 '#include "common.h"', "").replace('#include "pred_core.h"', f'#include "{CSRC}/pred_core.h"'))
+            if p.name == "predicates.hip":
+                # two sources define ScalarQuery; keep the translation units apart
+                t.write_text(t.read_text())
             srcs.append(str(t))
         srcs.append(str(HERE / "emu_driver.cpp"))
+        san = os.environ.get("EMU_SANITIZER", "address")
         cmd = ["g++", "-std=c++20", "-g", "-O1", "-fPIC", "-pthread", "-ffp-contract=off",
-               "-fsanitize=address", "-fno-omit-frame-pointer", f"-I{HERE}", "-o", out] + srcs
+               f"-fsanitize={san}", "-fno-omit-frame-pointer", f"-I{HERE}", "-o", out] + srcs
         if out.endswith(".so"):
             cmd.insert(1, "-shared")
         else:
             oracle = HERE.parents[1] / "oracle"
             objs = []
-            for c in ("spatial_sort.c", "pointsets.c"):
+            for c in ("spatial_sort.c", "pointsets.c", "predicates.c", "exact_int.c"):
                 o = Path(td) / (c + ".o")
-                subprocess.run(["gcc", "-O2", "-std=c11", "-ffp-contract=off", "-c", str(oracle / c),
+                subprocess.run(["gcc", "-O2", "-std=gnu11", "-ffp-contract=off", "-c", str(oracle / c),
                                 "-o", str(o)], check=True)
                 objs.append(str(o))
             cmd += [str(HERE / "emu_main.cpp")] + objs + ["-lm"]

@@ -1,37 +1,28 @@
-// Entry points of the emulated build (test infrastructure): the few host
-// helpers the kernels' host code needs, and C wrappers for ctypes.
+// Entry points of the emulated build (test infrastructure): C wrappers over
+// the library's C ABI, run on the CPU by hipemu.h.
 #include <cstdint>
-#include <string>
+#include <cstdlib>
 
-#include "hipemu.h"
-
-namespace d3 {
-std::string& last_error_slot() {
-    static thread_local std::string s;
-    return s;
-}
-int device_cu_count() { return emu::g_cus; }
-void spatial_sort_device(const double* xyz, int64_t n_pts, int32_t* order, int64_t n_order,
-                         hipStream_t s);
-void spatial_sort_sets_device(const double* xyz, const int64_t* off, int64_t n_sets,
-                              int32_t* order, hipStream_t s);
-}  // namespace d3
+#include "../../include/del3d_amd.h"
 
 extern "C" int emu_spatial_sort(const double* xyz, int64_t n_pts, int32_t* order, int64_t n_order) {
-    try {
-        d3::spatial_sort_device(xyz, n_pts, order, n_order, nullptr);
-        return 0;
-    } catch (...) {
-        return 1;
-    }
+    setenv("DEL3D_ENABLE_GPU_SORT", "1", 1);
+    return del3d_spatial_sort_host(xyz, n_pts, order, n_order);
 }
 
 extern "C" int emu_spatial_sort_sets(const double* xyz, const int64_t* off, int64_t n_sets,
                                      int32_t* order) {
-    try {
-        d3::spatial_sort_sets_device(xyz, off, n_sets, order, nullptr);
-        return 0;
-    } catch (...) {
-        return 1;
-    }
+    setenv("DEL3D_ENABLE_GPU_SORT", "1", 1);
+    // device buffers are host memory in the emulation
+    return del3d_spatial_sort_sets(xyz, off, n_sets, order, nullptr);
+}
+
+extern "C" int emu_predicates(const double* xyz, int64_t n_pts, const uint8_t* op,
+                              const int32_t* idx, int64_t n_q, int8_t* out) {
+    return del3d_predicates_host(xyz, n_pts, op, idx, n_q, out);
+}
+
+extern "C" int emu_predicates_stats(const double* xyz, int64_t n_pts, const uint8_t* op,
+                                    const int32_t* idx, int64_t n_q, int8_t* out, int64_t* st) {
+    return del3d_predicates_stats(xyz, n_pts, op, idx, n_q, out, nullptr, st);
 }

@@ -2,6 +2,7 @@
 // against the oracle (test infrastructure). Usage: emu_sort N [N ...]
 // Environment: EMU_DEFER=1 (queued kernels, immediate pageable copies and
 // frees), EMU_BLOCK_ORDER=0|1|2.
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <vector>
@@ -9,11 +10,64 @@
 #include "../../oracle/oracle.h"
 
 extern "C" int emu_spatial_sort(const double* xyz, int64_t n_pts, int32_t* order, int64_t n_order);
+extern "C" int emu_predicates_stats(const double* xyz, int64_t n_pts, const uint8_t* op,
+                                    const int32_t* idx, int64_t n_q, int8_t* out, int64_t* st);
+
+// xorshift64*, for the predicate workloads
+static uint64_t g_rng = 88172645463325252ull;
+static uint64_t rnd() {
+    g_rng ^= g_rng >> 12;
+    g_rng ^= g_rng << 25;
+    g_rng ^= g_rng >> 27;
+    return g_rng * 2685821657736338717ull;
+}
+
+// Mixed queries over a point set, with aliased slots; returns the failures.
+static int check_predicates(const char* name, const std::vector<double>& xyz, int64_t nq) {
+    const int64_t n = (int64_t)xyz.size() / 3;
+    std::vector<uint8_t> op(nq);
+    std::vector<int32_t> idx(5 * nq);
+    for (int64_t q = 0; q < nq; ++q) {
+        op[q] = (uint8_t)(rnd() % 3);
+        for (int k = 0; k < 5; ++k) idx[5 * q + k] = (int32_t)(rnd() % n);
+        if (q % 7 == 0) idx[5 * q + 4] = idx[5 * q];
+    }
+    std::vector<int8_t> got(nq), want(nq);
+    int64_t st[5] = {0, 0, 0, 0, 0};
+    if (emu_predicates_stats(xyz.data(), n, op.data(), idx.data(), nq, got.data(), st) != 0) {
+        std::printf("%s error\n", name);
+        return 1;
+    }
+    orc_int_pred_batch(xyz.data(), n, op.data(), idx.data(), nq, want.data());
+    const bool ok = got == want && st[0] + st[1] + st[2] + st[3] == nq;
+    std::printf("pred %s %s (filter %lld cert %lld regs %lld wide %lld perturbed %lld)\n", name,
+                ok ? "ok" : "MISMATCH", (long long)st[0], (long long)st[1], (long long)st[2],
+                (long long)st[3], (long long)st[4]);
+    return ok ? 0 : 1;
+}
 extern "C" int emu_spatial_sort_sets(const double* xyz, const int64_t* off, int64_t n_sets,
                                      int32_t* order);
 
 int main(int argc, char** argv) {
     int fails = 0;
+    {
+        std::vector<double> lat;
+        for (int i = 0; i < 5; ++i)
+            for (int j = 0; j < 5; ++j)
+                for (int k = 0; k < 5; ++k) lat.insert(lat.end(), {(double)i, (double)j, (double)k});
+        fails += check_predicates("lattice", lat, 3000);
+        std::vector<double> wide;
+        for (int i = 0; i < 60; ++i)
+            for (int k = 0; k < 3; ++k) {
+                const double m = (double)(int64_t)(rnd() % 2000001) - 1000000.0;
+                wide.push_back(std::ldexp(m + 0.5, (int)(rnd() % 400) - 200));
+            }
+        fails += check_predicates("wide", wide, 1500);
+        std::vector<double> q48;
+        for (int i = 0; i < 200; ++i)
+            for (int k = 0; k < 3; ++k) q48.push_back((double)(rnd() >> 16) * 3.552713678800501e-15);
+        fails += check_predicates("grid48", q48, 2000);
+    }
     for (int a = 1; a < argc; ++a) {
         const int64_t n = std::atoll(argv[a]);
         std::vector<double> xyz(3 * (size_t)n + 3);

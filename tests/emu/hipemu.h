@@ -109,6 +109,11 @@ inline T atomicAdd(T* p, T v) { return __atomic_fetch_add(p, v, __ATOMIC_SEQ_CST
 using std::max;
 using std::min;
 
+
+inline const char* hipGetErrorString(hipError_t) { return "emulated error"; }
+inline hipError_t hipGetLastError() { return hipSuccess; }
+inline hipError_t hipGetDevice(int* d) { *d = 0; return hipSuccess; }
+inline hipError_t hipDeviceGetAttribute(int* v, int, int) { *v = emu::g_cus; return hipSuccess; }
 namespace emu {
 // EMU_DEFER=1 models the runtime's documented asynchrony: kernel launches and
 // memsets are queued and run at the next stream synchronization, while
@@ -134,19 +139,36 @@ inline void drain() {
 }
 }  // namespace emu
 
-inline const char* hipGetErrorString(hipError_t) { return "emulated error"; }
-inline hipError_t hipGetLastError() { return hipSuccess; }
-inline hipError_t hipGetDevice(int* d) { *d = 0; return hipSuccess; }
-inline hipError_t hipDeviceGetAttribute(int* v, int, int) { *v = emu::g_cus; return hipSuccess; }
+namespace emu {
+inline std::vector<std::pair<const char*, size_t>>& pinned() {
+    static std::vector<std::pair<const char*, size_t>> v;
+    return v;
+}
+inline bool is_pinned(const void* p) {
+    for (auto& r : pinned())
+        if ((const char*)p >= r.first && (const char*)p < r.first + r.second) return true;
+    return false;
+}
+}  // namespace emu
 inline hipError_t hipMallocAsync(void** p, size_t n, hipStream_t) {
     *p = std::malloc(n ? n : 1);
     return *p ? hipSuccess : hipErrorOutOfMemory;
 }
 inline hipError_t hipMalloc(void** p, size_t n) { return hipMallocAsync(p, n, nullptr); }
 inline hipError_t hipFreeAsync(void* p, hipStream_t) { std::free(p); return hipSuccess; }
-inline hipError_t hipFree(void* p) { std::free(p); return hipSuccess; }
+inline hipError_t hipFree(void* p) {
+    emu::drain();  // hipFree waits for the device
+    std::free(p);
+    return hipSuccess;
+}
+// Copies touching pinned memory are stream-ordered (queued in the deferred
+// model); copies of pageable memory happen at once.
 inline hipError_t hipMemcpyAsync(void* d, const void* s, size_t n, hipMemcpyKind, hipStream_t) {
-    if (n) std::memcpy(d, s, n);
+    if (!n) return hipSuccess;
+    if (emu::defer_mode() && (emu::is_pinned(d) || emu::is_pinned(s)))
+        emu::queue().push_back([=] { std::memcpy(d, s, n); });
+    else
+        std::memcpy(d, s, n);
     return hipSuccess;
 }
 inline hipError_t hipMemsetAsync(void* d, int v, size_t n, hipStream_t) {
@@ -160,7 +182,19 @@ inline hipError_t hipStreamSynchronize(hipStream_t) {
     return hipSuccess;
 }
 inline hipError_t hipStreamCreateWithFlags(hipStream_t* s, unsigned) { *s = nullptr; return hipSuccess; }
-inline hipError_t hipHostMalloc(void** p, size_t n, unsigned) { return hipMallocAsync(p, n, nullptr); }
+inline hipError_t hipHostMalloc(void** p, size_t n, unsigned) {
+    hipError_t e = hipMallocAsync(p, n, nullptr);
+    if (e == hipSuccess) emu::pinned().emplace_back((const char*)*p, n);
+    return e;
+}
+inline hipError_t hipHostFree(void* p) {
+    emu::drain();
+    auto& v = emu::pinned();
+    for (size_t i = 0; i < v.size(); ++i)
+        if (v[i].first == p) { v.erase(v.begin() + i); break; }
+    std::free(p);
+    return hipSuccess;
+}
 
 namespace emu {
 template <class K, class... A>

@@ -1,5 +1,6 @@
-"""The GPU spatial sort's kernels and host driver, run on the CPU by the
-emulator in tests/emu under AddressSanitizer, against the oracle.
+"""The library's kernels and host code (C ABI, predicate pipeline, spatial
+sort), run on the CPU by the emulator in tests/emu under AddressSanitizer,
+against the oracle.
 
 EMU_DEFER=1 models the runtime's asynchrony at its least favourable:
 kernels run only when the host synchronizes, while copies to or from
@@ -21,8 +22,10 @@ EXE = Path("/tmp") / f"del3d_emu_sort_{os.getuid()}"
 
 @pytest.fixture(scope="module")
 def emu_exe():
+    csrc = HERE.parent / "del3d_amd" / "csrc"
     subprocess.run(["python", str(HERE / "emu" / "build_emu.py"), str(EXE),
-                    str(HERE.parent / "del3d_amd" / "csrc" / "spatial_sort.hip")], check=True)
+                    str(csrc / "capi.hip"), str(csrc / "predicates.hip"),
+                    str(csrc / "spatial_sort.hip")], check=True)
     return EXE
 
 
