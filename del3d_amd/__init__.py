@@ -47,6 +47,7 @@ def lib() -> C.CDLL:
         L.del3d_predicates.argtypes = [vp, i64, vp, vp, i64, vp, vp]
         L.del3d_predicates_stats.argtypes = [vp, i64, vp, vp, i64, vp, vp, C.POINTER(C.c_int64)]
         L.del3d_predicates_host.argtypes = [vp, i64, vp, vp, i64, vp]
+        L.del3d_predicates_timed.argtypes = [vp, i64, vp, vp, i64, vp, vp, C.POINTER(C.c_float)]
         L.del3d_spatial_sort.argtypes = [vp, i64, vp, i64, vp]
         L.del3d_spatial_sort_sets.argtypes = [vp, C.POINTER(C.c_int64), i64, vp, vp]
         L.del3d_spatial_sort_host.argtypes = [vp, i64, vp, i64]
@@ -89,6 +90,16 @@ def predicates_stats(xyz: DeviceArray, op: DeviceArray, idx: DeviceArray, out: D
            "del3d_predicates_stats")
     return {"filter": st[0], "certified": st[1], "exact_regs": st[2], "exact_wide": st[3],
             "perturbed": st[4]}
+
+
+def predicates_timed(xyz: DeviceArray, op: DeviceArray, idx: DeviceArray, out: DeviceArray,
+                     stream: Stream | None = None) -> tuple[float, float, float]:
+    """del3d_predicates plus HIP-event durations (ms) of its three kernels."""
+    ms = (C.c_float * 3)()
+    _check(lib().del3d_predicates_timed(xyz.ptr, int(xyz.shape[0]), op.ptr, idx.ptr,
+                                        int(op.shape[0]), out.ptr, _stream_handle(stream), ms),
+           "del3d_predicates_timed")
+    return float(ms[0]), float(ms[1]), float(ms[2])
 
 
 def spatial_sort_device(xyz: DeviceArray, order: DeviceArray, stream: Stream | None = None) -> None:
@@ -162,6 +173,6 @@ def in_sphere_unperturbed(a, b, c, d, e) -> int:
 __all__ = [
     "NEGATIVE", "ZERO", "POSITIVE", "ORIENT3D", "IN_SPHERE", "IN_SPHERE_UNPERTURBED",
     "DeviceArray", "Stream", "Event", "device_count", "synchronize", "lib", "Del3dError",
-    "predicates", "predicates_stats", "spatial_sort_device", "spatial_sort_sets",
+    "predicates", "predicates_stats", "predicates_timed", "spatial_sort_device", "spatial_sort_sets",
     "predicates_np", "spatial_sort", "orient3d", "in_sphere", "in_sphere_unperturbed",
 ]

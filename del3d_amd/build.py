@@ -58,6 +58,9 @@ def build(verbose: bool = False) -> Path:
     order = sorted(SOURCES, key=lambda s: s != "predicates.hip")
     with ThreadPoolExecutor(max_workers=len(order)) as ex:
         objs = list(ex.map(lambda s: _compile(s, outdir, verbose), order))
+    calib = _compile("calib.hip", outdir, verbose)
+    subprocess.run([hipcc(), "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o",
+                    str(HERE / "libdel3d_calib.so"), calib], check=True)
     tmp = LIB.with_suffix(".so.tmp")
     link = [hipcc(), "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", str(tmp)] + objs
     if verbose:

@@ -12,7 +12,8 @@
 namespace d3 {
 
 void launch_predicates(const double* xyz, const uint8_t* op, const int32_t* idx, int64_t n_q,
-                       int8_t* out, hipStream_t s, unsigned long long* stats_dev);
+                       int8_t* out, hipStream_t s, unsigned long long* stats_dev,
+                       hipEvent_t* ev = nullptr);
 struct ScalarQuery {
     double xyz[15];
     int32_t id[5];
@@ -271,6 +272,35 @@ int del3d_predicates_stats(const double* xyz, int64_t n_pts, const uint8_t* op, 
         unsigned long long h[5];
         copy_d2h(h, st, sizeof h, s);
         for (int k = 0; k < 5; ++k) stats_host[k] = (int64_t)h[k];
+        return DEL3D_OK;
+    });
+}
+
+int del3d_predicates_timed(const double* xyz, int64_t n_pts, const uint8_t* op, const int32_t* idx,
+                           int64_t n_q, int8_t* sign_out, void* stream, float* ms_out) {
+    return guarded([&] {
+        const int rc = check_pred_args(xyz, n_pts, op, idx, n_q, sign_out);
+        if (rc) return rc;
+        if (!ms_out) return set_error(DEL3D_ERR_INVALID, "ms_out is null");
+        if (n_q > (int64_t)(1u << 26))
+            return set_error(DEL3D_ERR_INVALID, "timed form takes at most 2^26 queries per call");
+        hipStream_t s = (hipStream_t)stream;
+        hipEvent_t ev[6] = {};
+        for (auto& e : ev) D3_HIP(hipEventCreate(&e));
+        struct Destroy {
+            hipEvent_t* ev;
+            ~Destroy() {
+                for (int k = 0; k < 6; ++k)
+                    if (ev[k]) (void)hipEventDestroy(ev[k]);
+            }
+        } destroy{ev};
+        if (n_q == 0) {
+            ms_out[0] = ms_out[1] = ms_out[2] = 0.0f;
+            return DEL3D_OK;
+        }
+        launch_predicates(xyz, op, idx, n_q, sign_out, s, nullptr, ev);
+        D3_HIP(hipStreamSynchronize(s));
+        for (int k = 0; k < 3; ++k) D3_HIP(hipEventElapsedTime(&ms_out[k], ev[2 * k], ev[2 * k + 1]));
         return DEL3D_OK;
     });
 }

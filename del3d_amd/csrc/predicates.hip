@@ -284,8 +284,11 @@ __global__ __launch_bounds__(kWideBlock) void pred_wide(const double* __restrict
 // ---------------------------------------------------------------------------
 constexpr uint32_t kChunk = 1u << 26;  // queries per launch sequence (bounds the queues)
 
+// When `ev` is given (6 events), the kernels of the (single) chunk are bracketed
+// by ev[0..1] (fast), ev[2..3] (exact) and ev[4..5] (wide).
 void launch_predicates(const double* xyz, const uint8_t* op, const int32_t* idx, int64_t n_q,
-                       int8_t* out, hipStream_t s, unsigned long long* stats_dev) {
+                       int8_t* out, hipStream_t s, unsigned long long* stats_dev,
+                       hipEvent_t* ev) {
     if (n_q <= 0) return;
     const uint32_t chunk = (uint32_t)std::min<int64_t>(n_q, kChunk);
     Scratch sc(s);
@@ -301,6 +304,7 @@ void launch_predicates(const double* xyz, const uint8_t* op, const int32_t* idx,
         const uint8_t* opc = op + off;
         const int32_t* idc = idx + off * 5;
         int8_t* outc = out + off;
+        if (ev) D3_HIP(hipEventRecord(ev[0], s));
         if (stats_dev) {
             pred_fast<true><<<fast_grid, kTile, 0, s>>>(xyz, opc, idc, n, outc, queue,
                                                         ctr, stats_dev);
@@ -310,12 +314,21 @@ void launch_predicates(const double* xyz, const uint8_t* op, const int32_t* idx,
                                                          queue, ctr, nullptr);
             D3_LAUNCH_CHECK();
         }
+        if (ev) {
+            D3_HIP(hipEventRecord(ev[1], s));
+            D3_HIP(hipEventRecord(ev[2], s));
+        }
         pred_exact<<<cus * 4, 256, 0, s>>>(xyz, opc, idc, outc, queue,
                                            wide, ctr, stats_dev);
         D3_LAUNCH_CHECK();
+        if (ev) {
+            D3_HIP(hipEventRecord(ev[3], s));
+            D3_HIP(hipEventRecord(ev[4], s));
+        }
         pred_wide<<<cus * 4, kWideBlock, 0, s>>>(xyz, opc, idc, outc, wide, ctr,
                                                  stats_dev);
         D3_LAUNCH_CHECK();
+        if (ev) D3_HIP(hipEventRecord(ev[5], s));
     }
 }
 

@@ -82,6 +82,17 @@ int del3d_predicates_stats(const double* xyz, int64_t n_pts, const uint8_t* op,
                            int64_t* stats_host);
 
 /*
+ * Measurement helper (no counterpart in the reference): as del3d_predicates,
+ * and additionally records HIP events around each kernel on `stream`,
+ * synchronizes it, and writes the kernels' durations in milliseconds:
+ *   ms_out[0] fast path (filter, certified grid, in-register perturbation)
+ *   ms_out[1] exact integer tier (registers)   ms_out[2] wide exact tier (LDS)
+ */
+int del3d_predicates_timed(const double* xyz, int64_t n_pts, const uint8_t* op,
+                           const int32_t* idx, int64_t n_q, int8_t* sign_out, void* stream,
+                           float* ms_out);
+
+/*
  * Spatial sort of device-resident data. Replaces
  *   void spatial_sort(const std::vector<Point3>& points, std::vector<int>& order)
  *   include/del3d/spatial_sort.h:37 (src/spatial_sort.cpp:141-151)

@@ -36,6 +36,7 @@ struct dim3 {
 };
 
 typedef void* hipStream_t;
+typedef void* hipEvent_t;
 typedef int hipError_t;
 enum { hipSuccess = 0, hipErrorOutOfMemory = 2, hipErrorInvalidValue = 1 };
 enum hipMemcpyKind { hipMemcpyHostToHost = 0, hipMemcpyHostToDevice = 1, hipMemcpyDeviceToHost = 2,
@@ -187,6 +188,10 @@ inline hipError_t hipHostMalloc(void** p, size_t n, unsigned) {
     if (e == hipSuccess) emu::pinned().emplace_back((const char*)*p, n);
     return e;
 }
+inline hipError_t hipEventCreate(hipEvent_t* e) { *e = (void*)1; return hipSuccess; }
+inline hipError_t hipEventDestroy(hipEvent_t) { return hipSuccess; }
+inline hipError_t hipEventRecord(hipEvent_t, hipStream_t) { return hipSuccess; }
+inline hipError_t hipEventElapsedTime(float* ms, hipEvent_t, hipEvent_t) { *ms = 0.0f; return hipSuccess; }
 inline hipError_t hipHostFree(void* p) {
     emu::drain();
     auto& v = emu::pinned();
