@@ -48,6 +48,16 @@ def lib() -> C.CDLL:
         L.del3d_predicates_stats.argtypes = [vp, i64, vp, vp, i64, vp, vp, C.POINTER(C.c_int64)]
         L.del3d_predicates_host.argtypes = [vp, i64, vp, vp, i64, vp]
         L.del3d_predicates_timed.argtypes = [vp, i64, vp, vp, i64, vp, vp, C.POINTER(C.c_float)]
+        i64p = C.POINTER(C.c_int64)
+        L.del3d_predicates_sets.argtypes = [vp, i64p, i64, vp, vp, i64p, vp, vp]
+        L.del3d_pred_plan_create.argtypes = [i64p, i64p, i64, C.POINTER(C.c_void_p)]
+        L.del3d_pred_plan_run.argtypes = [vp, vp, vp, vp, vp, vp]
+        L.del3d_pred_plan_run_stats.argtypes = [vp, vp, vp, vp, vp, vp, i64p]
+        L.del3d_pred_plan_run_graph.argtypes = [vp, vp, vp, vp, vp, vp]
+        L.del3d_pred_plan_autotune.argtypes = [vp, vp, vp, vp, vp, vp, C.POINTER(C.c_float)]
+        L.del3d_pred_plan_set_knobs.argtypes = [vp, C.POINTER(C.c_int32), C.c_int32]
+        L.del3d_pred_plan_get_knobs.argtypes = [vp, C.POINTER(C.c_int32)]
+        L.del3d_pred_plan_destroy.argtypes = [vp]
         L.del3d_spatial_sort.argtypes = [vp, i64, vp, i64, vp]
         L.del3d_spatial_sort_sets.argtypes = [vp, C.POINTER(C.c_int64), i64, vp, vp]
         L.del3d_spatial_sort_host.argtypes = [vp, i64, vp, i64]
@@ -82,14 +92,84 @@ def predicates(xyz: DeviceArray, op: DeviceArray, idx: DeviceArray, out: DeviceA
                                   _stream_handle(stream)), "del3d_predicates")
 
 
+STAT_NAMES = ("filter", "certified", "exact_regs", "exact_wide", "perturbed", "bad_index")
+
+
 def predicates_stats(xyz: DeviceArray, op: DeviceArray, idx: DeviceArray, out: DeviceArray,
                      stream: Stream | None = None) -> dict:
-    st = (C.c_int64 * 5)()
+    st = (C.c_int64 * 6)()
     _check(lib().del3d_predicates_stats(xyz.ptr, int(xyz.shape[0]), op.ptr, idx.ptr,
                                         int(op.shape[0]), out.ptr, _stream_handle(stream), st),
            "del3d_predicates_stats")
-    return {"filter": st[0], "certified": st[1], "exact_regs": st[2], "exact_wide": st[3],
-            "perturbed": st[4]}
+    return dict(zip(STAT_NAMES, st))
+
+
+def _i64p(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_int64))
+
+
+def predicates_sets(xyz: DeviceArray, pt_off, op: DeviceArray, idx: DeviceArray, q_off,
+                    out: DeviceArray, stream: Stream | None = None) -> None:
+    """Many point sets in one call: set s owns points pt_off[s]:pt_off[s+1] and
+    queries q_off[s]:q_off[s+1], with set-local indices. Synchronizes."""
+    po = np.ascontiguousarray(pt_off, dtype=np.int64)
+    qo = np.ascontiguousarray(q_off, dtype=np.int64)
+    _check(lib().del3d_predicates_sets(xyz.ptr, _i64p(po), len(po) - 1, op.ptr, idx.ptr, _i64p(qo),
+                                       out.ptr, _stream_handle(stream)), "del3d_predicates_sets")
+
+
+class PredPlan:
+    """A plan over a fixed set layout (include/del3d_amd.h, del3d_pred_plan_*)."""
+
+    KNOBS = 5
+
+    def __init__(self, pt_off, q_off):
+        self._po = np.ascontiguousarray(pt_off, dtype=np.int64)
+        self._qo = np.ascontiguousarray(q_off, dtype=np.int64)
+        h = C.c_void_p()
+        _check(lib().del3d_pred_plan_create(_i64p(self._po), _i64p(self._qo), len(self._po) - 1,
+                                            C.byref(h)), "del3d_pred_plan_create")
+        self.handle = h
+
+    def run(self, xyz, op, idx, out, stream=None, graph=False):
+        f = lib().del3d_pred_plan_run_graph if graph else lib().del3d_pred_plan_run
+        _check(f(self.handle, xyz.ptr, op.ptr, idx.ptr, out.ptr, _stream_handle(stream)),
+               "del3d_pred_plan_run")
+
+    def run_stats(self, xyz, op, idx, out, stream=None) -> dict:
+        st = (C.c_int64 * 6)()
+        _check(lib().del3d_pred_plan_run_stats(self.handle, xyz.ptr, op.ptr, idx.ptr, out.ptr,
+                                               _stream_handle(stream), st), "del3d_pred_plan_run_stats")
+        return dict(zip(STAT_NAMES, st))
+
+    def autotune(self, xyz, op, idx, out, stream=None) -> float:
+        ms = C.c_float()
+        _check(lib().del3d_pred_plan_autotune(self.handle, xyz.ptr, op.ptr, idx.ptr, out.ptr,
+                                              _stream_handle(stream), C.byref(ms)),
+               "del3d_pred_plan_autotune")
+        return float(ms.value)
+
+    @property
+    def knobs(self) -> list:
+        k = (C.c_int32 * self.KNOBS)()
+        _check(lib().del3d_pred_plan_get_knobs(self.handle, k), "del3d_pred_plan_get_knobs")
+        return list(k)
+
+    @knobs.setter
+    def knobs(self, values) -> None:
+        k = (C.c_int32 * len(values))(*values)
+        _check(lib().del3d_pred_plan_set_knobs(self.handle, k, len(values)), "del3d_pred_plan_set_knobs")
+
+    def close(self) -> None:
+        if self.handle:
+            lib().del3d_pred_plan_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def predicates_timed(xyz: DeviceArray, op: DeviceArray, idx: DeviceArray, out: DeviceArray,
@@ -173,6 +253,6 @@ def in_sphere_unperturbed(a, b, c, d, e) -> int:
 __all__ = [
     "NEGATIVE", "ZERO", "POSITIVE", "ORIENT3D", "IN_SPHERE", "IN_SPHERE_UNPERTURBED",
     "DeviceArray", "Stream", "Event", "device_count", "synchronize", "lib", "Del3dError",
-    "predicates", "predicates_stats", "predicates_timed", "spatial_sort_device", "spatial_sort_sets",
+    "predicates", "predicates_stats", "predicates_timed", "predicates_sets", "PredPlan", "spatial_sort_device", "spatial_sort_sets",
     "predicates_np", "spatial_sort", "orient3d", "in_sphere", "in_sphere_unperturbed",
 ]

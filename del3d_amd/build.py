@@ -14,7 +14,7 @@ from pathlib import Path
 HERE = Path(__file__).resolve().parent
 CSRC = HERE / "csrc"
 LIB = HERE / "libdel3d_amd.so"
-SOURCES = ["capi.hip", "predicates.hip", "spatial_sort.hip", "dropin.cpp"]
+SOURCES = ["capi.hip", "predicates.hip", "batch.hip", "spatial_sort.hip", "dropin.cpp"]
 ARCH = os.environ.get("DEL3D_OFFLOAD_ARCH", "gfx950")
 
 COMMON = [

@@ -8,19 +8,10 @@
 #include <vector>
 
 #include "common.h"
+#include "predicates.h"
 
 namespace d3 {
 
-void launch_predicates(const double* xyz, const uint8_t* op, const int32_t* idx, int64_t n_q,
-                       int8_t* out, hipStream_t s, unsigned long long* stats_dev,
-                       hipEvent_t* ev = nullptr);
-struct ScalarQuery {
-    double xyz[15];
-    int32_t id[5];
-    uint8_t op;
-    int8_t sign;
-};
-void launch_single(ScalarQuery* qd, hipStream_t s);
 void spatial_sort_device(const double* xyz, int64_t n_pts, int32_t* order, int64_t n_order,
                          hipStream_t s);
 void spatial_sort_sets_device(const double* xyz, const int64_t* off, int64_t n_sets,
@@ -157,27 +148,13 @@ static int sort_disabled() {
 }
 
 // A non-blocking stream per thread and device for the synchronous forms.
-static hipStream_t thread_stream() {
+hipStream_t thread_stream() {
     thread_local std::vector<hipStream_t> streams;
     int dev = 0;
     D3_HIP(hipGetDevice(&dev));
     if ((int)streams.size() <= dev) streams.resize(dev + 1, nullptr);
     if (!streams[dev]) D3_HIP(hipStreamCreateWithFlags(&streams[dev], hipStreamNonBlocking));
     return streams[dev];
-}
-
-template <class F>
-static int guarded(F&& f) {
-    try {
-        last_error_slot().clear();
-        return f();
-    } catch (const HipError& e) {
-        return e.code;
-    } catch (const std::bad_alloc&) {
-        return set_error(DEL3D_ERR_NOMEM, "host allocation failed");
-    } catch (...) {
-        return set_error(DEL3D_ERR_DEVICE, "unexpected exception");
-    }
 }
 
 static int check_pred_args(const double* xyz, int64_t n_pts, const uint8_t* op, const int32_t* idx,
@@ -253,7 +230,7 @@ int del3d_predicates(const double* xyz, int64_t n_pts, const uint8_t* op, const 
     return guarded([&] {
         const int rc = check_pred_args(xyz, n_pts, op, idx, n_q, sign_out);
         if (rc) return rc;
-        launch_predicates(xyz, op, idx, n_q, sign_out, (hipStream_t)stream, nullptr);
+        launch_predicates(xyz, n_pts, op, idx, n_q, sign_out, (hipStream_t)stream, nullptr);
         return DEL3D_OK;
     });
 }
@@ -266,12 +243,12 @@ int del3d_predicates_stats(const double* xyz, int64_t n_pts, const uint8_t* op, 
         if (!stats_host) return set_error(DEL3D_ERR_INVALID, "stats_host is null");
         hipStream_t s = (hipStream_t)stream;
         Scratch sc(s);
-        unsigned long long* st = sc.get<unsigned long long>(5);
-        fill_u32(reinterpret_cast<uint32_t*>(st), 0u, 10, s);
-        launch_predicates(xyz, op, idx, n_q, sign_out, s, st);
-        unsigned long long h[5];
+        unsigned long long* st = sc.get<unsigned long long>(kStats);
+        fill_u32(reinterpret_cast<uint32_t*>(st), 0u, 2 * kStats, s);
+        launch_predicates(xyz, n_pts, op, idx, n_q, sign_out, s, st);
+        unsigned long long h[kStats];
         copy_d2h(h, st, sizeof h, s);
-        for (int k = 0; k < 5; ++k) stats_host[k] = (int64_t)h[k];
+        for (int k = 0; k < kStats; ++k) stats_host[k] = (int64_t)h[k];
         return DEL3D_OK;
     });
 }
@@ -298,7 +275,7 @@ int del3d_predicates_timed(const double* xyz, int64_t n_pts, const uint8_t* op, 
             ms_out[0] = ms_out[1] = ms_out[2] = 0.0f;
             return DEL3D_OK;
         }
-        launch_predicates(xyz, op, idx, n_q, sign_out, s, nullptr, ev);
+        launch_predicates(xyz, n_pts, op, idx, n_q, sign_out, s, nullptr, ev);
         D3_HIP(hipStreamSynchronize(s));
         for (int k = 0; k < 3; ++k) D3_HIP(hipEventElapsedTime(&ms_out[k], ev[2 * k], ev[2 * k + 1]));
         return DEL3D_OK;
@@ -320,7 +297,7 @@ int del3d_predicates_host(const double* xyz, int64_t n_pts, const uint8_t* op, c
         copy_h2d(dx, xyz, (size_t)n_pts * 24, s);
         copy_h2d(dop, op, (size_t)n_q, s);
         copy_h2d(did, idx, (size_t)n_q * 20, s);
-        launch_predicates(dx, dop, did, n_q, dout, s, nullptr);
+        launch_predicates(dx, n_pts, dop, did, n_q, dout, s, nullptr);
         copy_d2h(sign_out, dout, (size_t)n_q, s);
         return DEL3D_OK;
     });

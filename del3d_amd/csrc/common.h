@@ -6,6 +6,7 @@
 #include <cstdarg>
 #include <cstdint>
 #include <cstdio>
+#include <new>
 #include <string>
 #include <vector>
 
@@ -86,5 +87,24 @@ void fill_u32(uint32_t* p, uint32_t v, size_t n, hipStream_t s);
 
 // Number of compute units of the current device (cached per device).
 int device_cu_count();
+
+// A non-blocking stream per thread and device, for the synchronous forms.
+hipStream_t thread_stream();
+
+// Runs an entry point's body: clears the thread's error message, turns
+// exceptions into status codes.
+template <class F>
+int guarded(F&& f) {
+    try {
+        last_error_slot().clear();
+        return f();
+    } catch (const HipError& e) {
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        return set_error(DEL3D_ERR_NOMEM, "host allocation failed");
+    } catch (...) {
+        return set_error(DEL3D_ERR_DEVICE, "unexpected exception");
+    }
+}
 
 }  // namespace d3

@@ -136,6 +136,19 @@ D3_HD Filt insphere_filter(const P3& a, const P3& b, const P3& c, const P3& d, c
     return f;
 }
 
+// The filters' maxd on its own (same differences, same maximum), for callers
+// that evaluate the filter first and need maxd only when it is undecided.
+D3_HD double max_diff(const P3* p, int np) {
+    const P3& r = p[np - 1];
+    double m = 0.0;
+    for (int i = 0; i < np - 1; ++i) {
+        m = fmax(m, fabs(p[i].x - r.x));
+        m = fmax(m, fabs(p[i].y - r.y));
+        m = fmax(m, fabs(p[i].z - r.z));
+    }
+    return m;
+}
+
 // The reference's accept test: |det| > bound * permanent (src/predicates.cpp:143-144).
 D3_HD bool filter_decides(const Filt& f, double bound_const) {
     const double eb = bound_const * f.perm;
@@ -208,7 +221,10 @@ D3_HD int perturbation_walk(const P3 p[5], const int32_t id[5], F&& orient_subst
     top_two_slots(p, s4, s3);
 #pragma unroll
     for (int it = 0; it < 2; ++it) {
-        const int32_t v = id[it == 0 ? s4 : s3];
+        // a select chain rather than id[slot]: a runtime index would put the
+        // array in private (scratch) memory on the GPU
+        const int sl = it == 0 ? s4 : s3;
+        const int32_t v = sl == 0 ? id[0] : sl == 1 ? id[1] : sl == 2 ? id[2] : sl == 3 ? id[3] : id[4];
         if (v == id[4]) return -1;
         int o;
         if (v == id[3] && (o = orient_subst(3)) != 0) return o;

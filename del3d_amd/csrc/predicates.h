@@ -1,0 +1,75 @@
+// predicates.h - shared types of the batched predicate kernels and their
+// host launchers (internal to the library).
+#pragma once
+
+#include <cstdint>
+
+#include "common.h"
+
+namespace d3 {
+
+constexpr int kLdsThreads = 1024;   // workgroup of the LDS kernel (one per CU)
+constexpr int kGlobalThreads = 256; // workgroup of the global-gather kernel
+constexpr int kLdsPts = 5632;       // largest point set held in LDS (132 KB)
+constexpr uint32_t kChunk = 1u << 26;  // queries per launch sequence (bounds the queues)
+constexpr int kStats = 6;           // decision counters (see del3d_predicates_stats)
+
+// Queries [q0, q1) of point set `set`; their indices are local to the set.
+struct PredTask {
+    int64_t q0, q1;
+    int32_t set, pad;
+};
+
+struct PredCounters {
+    uint32_t queued;        // entries in the exact queue
+    uint32_t wide;          // entries in the wide queue
+    uint32_t next_task[2];  // task cursors of the global [0] and LDS [1] kernels
+};
+
+// Kernel arguments of one launch sequence (device pointers).
+struct BatchArgs {
+    const double* xyz;       // all points
+    int64_t n_pts;           // points of the single set (pt_off == nullptr)
+    const int64_t* pt_off;   // n_sets+1 point offsets, or nullptr (one set)
+    const int64_t* q_off;    // n_sets+1 query offsets, or nullptr (one set)
+    uint32_t n_sets;
+    const uint8_t* op;
+    const int32_t* idx;
+    int8_t* out;
+    int64_t n_q;             // queries of an implicit (single-set) task list
+    const PredTask* tasks;   // explicit task table, or nullptr: chunks of `chunk`
+    uint32_t n_tasks;
+    uint32_t chunk;
+    uint32_t* queue;         // exact-tier queue (capacity: the queries of the launch)
+    uint32_t* wide;          // wide-tier queue (same capacity)
+    PredCounters* ctr;
+    unsigned long long* stats;  // 6 counters or nullptr
+};
+
+// How a batch is split between the two fast kernels.
+struct BatchShape {
+    const PredTask* lds_tasks = nullptr;     // nullptr: implicit chunks
+    uint32_t n_lds_tasks = 0, lds_chunk = 0, lds_blocks_per_cu = 1;
+    const PredTask* global_tasks = nullptr;
+    uint32_t n_global_tasks = 0, global_chunk = 0, global_blocks_per_cu = 8;
+};
+
+struct ScalarQuery {
+    double xyz[15];
+    int32_t id[5];
+    uint8_t op;
+    int8_t sign;
+};
+
+// Enqueues the whole pipeline of one batch on s (no host synchronization;
+// capturable into a graph). ev (6 events) brackets fast / exact / wide.
+void enqueue_batch(const BatchArgs& a, const BatchShape& sh, hipStream_t s, hipEvent_t* ev);
+
+// One point set; force_path 0 = choose, 1 = LDS kernel, 2 = global kernel.
+void launch_predicates(const double* xyz, int64_t n_pts, const uint8_t* op, const int32_t* idx,
+                       int64_t n_q, int8_t* out, hipStream_t s, unsigned long long* stats_dev,
+                       hipEvent_t* ev = nullptr, int force_path = 0);
+
+void launch_single(ScalarQuery* qd, hipStream_t s);
+
+}  // namespace d3

@@ -1,38 +1,47 @@
 // predicates.hip - batched exact orient3d / in_sphere for gfx950.
 //
-// Three kernels per batch, stream-ordered, no host round trip in between:
+// A batch is a list of tasks, each a contiguous range of queries of one point
+// set (a plain call is one set). Kernels, stream-ordered, no host round trip:
 //
-//  pred_fast   one lane per query. Index records are staged through LDS with
-//              fully coalesced dword loads (a 256-query tile is 5 KB of
-//              indices), the tile is regrouped so that in_sphere and orient3d
-//              queries occupy separate wavefronts, then each lane gathers its
-//              4-5 points and runs the FP64 filter in registers. Undecided
-//              queries try the certified small-grid tier (T1) and, for
-//              exactly cospherical in_sphere queries, the whole symbolic
-//              perturbation, still in registers. Whatever is left is appended
-//              (one atomic per wavefront) to a compacted queue.
-//  pred_exact  persistent grid over the queue: exact integer arithmetic with
-//              1-2 limbs per difference (T2), perturbation included. Queries
-//              whose exponent span needs more limbs go to a second queue.
-//  pred_wide   one query per workgroup, exact integer arithmetic of any width
-//              in LDS (T3).
+//  pred_batch_lds     sets of at most kLdsPts points: one 1024-thread
+//                     workgroup per CU holds the whole set in LDS (up to
+//                     132 KB) and walks its queries in 1024-query tiles.
+//  pred_batch_global  larger sets: 256-thread workgroups gather points from
+//                     global memory (L2 / Infinity Cache / HBM).
+//     Both: the tile's index records are staged through LDS with coalesced
+//     dword loads, the tile is regrouped so in_sphere and orient3d queries
+//     occupy separate wavefronts, and each lane runs, in registers, the FP64
+//     filter in the reference's operation order, then for undecided queries
+//     the certified small-grid tier (T1) and, for exactly cospherical
+//     in_sphere queries, the whole symbolic perturbation. Whatever is left is
+//     appended (one atomic per wavefront) to a compacted queue.
+//  pred_exact         persistent grid over the queue: exact integer arithmetic
+//                     with 1-2 limbs per difference (T2), perturbation
+//                     included; wider exponent spans go to a second queue.
+//  pred_wide          one query per workgroup, exact integers of any width in
+//                     LDS (T3).
+//
+// Why LDS for small sets: a query gathers 4-5 random points of 24 bytes, two
+// L2 requests each (a 16- and an 8-byte load), so with points in L2 the filter
+// is bound by the L2 request rate (16 requests per clock per XCD), not by
+// FP64. From LDS the same gathers cost a few bank cycles.
 //
 // All device code is compiled with -ffp-contract=off: the filter must round
 // exactly as the reference's (src/predicates.cpp:125-201) does.
+#include <algorithm>
+
 #include "common.h"
 #include "pred_core.h"
+#include "predicates.h"
 
 namespace d3 {
 
-constexpr int kTile = 256;           // queries per fast-kernel tile (one workgroup)
-constexpr int kWideLimbsMax = 67;    // > limbs_for() of any finite input (66)
+constexpr int kWideLimbsMax = 67;  // > limbs_for() of any finite input (66)
 constexpr int kWideBlock = 64;
 
-struct PredCounters {
-    uint32_t queued;  // entries in the exact queue
-    uint32_t wide;    // entries in the wide queue
-};
-
+// Out-of-range point indices are undefined behaviour in the reference (a bad
+// pointer); here such a query gets sign 0 and is counted in stats[5], so a
+// bad index can never make the device read outside the point array.
 __device__ inline uint64_t lanemask_lt() { return __lanemask_lt(); }
 
 // Appends `q` for every lane with `want`; one atomic per wavefront.
@@ -52,120 +61,250 @@ __device__ inline void wave_count(bool flag, unsigned long long* slot) {
     if (m && __lane_id() == __ffsll((long long)m) - 1) atomicAdd(slot, (unsigned long long)__popcll(m));
 }
 
-// ---------------------------------------------------------------------------
-// Fast path
-// ---------------------------------------------------------------------------
-template <bool kStats>
-__global__ __launch_bounds__(kTile) void pred_fast(const double* __restrict__ xyz,
-                                                   const uint8_t* __restrict__ op,
-                                                   const int32_t* __restrict__ idx, uint32_t n_q,
-                                                   int8_t* __restrict__ out,
-                                                   uint32_t* __restrict__ queue,
-                                                   PredCounters* __restrict__ ctr,
-                                                   unsigned long long* __restrict__ stats) {
-    __shared__ int32_t s_idx[kTile * 5];
-    __shared__ uint8_t s_op[kTile];
-    __shared__ uint16_t s_perm[kTile];
-    __shared__ uint32_t s_wave_ins[kTile / 64];
+// A value every lane holds equally, moved to scalar registers.
+__device__ inline int64_t uniform64(int64_t v) {
+    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)(uint64_t)v);
+    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)((uint64_t)v >> 32));
+    return (int64_t)(((uint64_t)hi << 32) | lo);
+}
+
+// Point fetch from global memory (set base already applied) or from LDS.
+struct GlobalPts {
+    const double* p;
+    __device__ P3 operator()(int32_t i) const { return load_p3(p, i); }
+};
+struct LdsPts {
+    const double* p;  // points to the __shared__ array
+    __device__ P3 operator()(int32_t i) const { return P3{p[3 * i], p[3 * i + 1], p[3 * i + 2]}; }
+};
+
+// The fast tiers of one query: T0 filter, T1 certificate, in-register
+// perturbation. Returns the public sign, or 2 when the exact tiers must decide.
+// kReload: fetch the points again for T1 instead of keeping them live across
+// the filter (cheap from LDS; it lowers register pressure on the common path).
+template <bool kReload, class Fetch>
+__device__ __forceinline__ int fast_decide(const Fetch& F, uint8_t o, const int32_t id[5], int& tier,
+                                           bool& perturbed) {
+    const bool ins = o != DEL3D_OP_ORIENT3D;
+    P3 p[5];
+    p[0] = F(id[0]);
+    p[1] = F(id[1]);
+    p[2] = F(id[2]);
+    p[3] = F(id[3]);
+    double det;
+    if (!ins) {
+        const Filt f = orient_filter(p[0], p[1], p[2], p[3]);
+        if (filter_decides(f, kOrientBound)) {
+            tier = kTierFilter;
+            return -sgn(f.det);
+        }
+        det = f.det;
+    } else {
+        p[4] = F(id[4]);
+        const Filt f = insphere_filter(p[0], p[1], p[2], p[3], p[4]);
+        if (filter_decides(f, kInSphereBound)) {
+            tier = kTierFilter;
+            return -sgn(f.det);
+        }
+        det = f.det;
+    }
+    if (kReload) {
+        asm volatile("" ::: "memory");  // the points are fetched again, not kept
+#pragma unroll
+        for (int k = 0; k < 4; ++k) p[k] = F(id[k]);
+        if (ins) p[4] = F(id[4]);
+    }
+    if (!ins) {
+        if (!t1_certify(p, 4, max_diff(p, 4), 15, -358)) return 2;
+        tier = kTierCert;
+        return -sgn(det);  // certified: the filter's determinant is exact
+    }
+    if (!t1_certify(p, 5, max_diff(p, 5), 8, -214)) return 2;
+    tier = kTierCert;
+    int sign = -sgn(det);
+    if (sign == 0 && o == DEL3D_OP_IN_SPHERE) {
+        perturbed = true;
+        // every four of these points certify at degree 3 too, so the FP64
+        // orient3d determinant of a substituted tetrahedron is exact here
+        sign = perturbation_walk(p, id, [&](int k) {
+            P3 t[4];
+            subst_tet(p, k, t);
+            return -sgn(orient_filter(t[0], t[1], t[2], t[3]).det);
+        });
+    }
+    return sign;
+}
+
+// Task `t` of a batch: explicit table, or implicit equal chunks of one set.
+__device__ inline PredTask task_at(const BatchArgs& a, uint32_t t) {
+    if (a.tasks) return a.tasks[t];
+    PredTask r;
+    r.q0 = (int64_t)t * a.chunk;
+    r.q1 = min(r.q0 + (int64_t)a.chunk, a.n_q);
+    r.set = 0;
+    r.pad = 0;
+    return r;
+}
+
+// Body of both fast kernels. kLds: the workgroup loads each task's point
+// set into LDS first. Workgroups take tasks dynamically (one atomic per task).
+template <bool kStats, bool kLds>
+struct BatchLds {
+    static constexpr int kT = kLds ? kLdsThreads : kGlobalThreads;  // queries per tile
+    int32_t idx[kT * 5];
+    uint8_t op[kT];
+    uint16_t perm[kT];
+    uint32_t wave_ins[kT / 64];
+    int64_t task[4];
+    double pts[kLds ? 3 * kLdsPts : 1];
+};
+
+template <bool kStats, bool kLds>
+__device__ __forceinline__ void pred_batch_body(const BatchArgs& a, BatchLds<kStats, kLds>& L) {
+    constexpr int kT = BatchLds<kStats, kLds>::kT;
+    constexpr int kW = kT / 64;
+    int32_t* s_idx = L.idx;
+    uint8_t* s_op = L.op;
+    uint16_t* s_perm = L.perm;
+    uint32_t* s_wave_ins = L.wave_ins;
+    int64_t* s_task = L.task;
+    double* s_pts = L.pts;
 
     const int tid = threadIdx.x;
     const int wave = tid >> 6;
     const int lane = __lane_id();
+    int32_t loaded_set = -1;
+    unsigned long long* stats = a.stats;
+    uint32_t* next = &a.ctr->next_task[kLds ? 1 : 0];
 
-    for (uint32_t base = blockIdx.x * kTile; base < n_q; base += gridDim.x * kTile) {
-        const uint32_t cnt = min((uint32_t)kTile, n_q - base);
-        __syncthreads();  // the previous tile is done with the LDS arrays
-        // Stage the tile's index records: 5*cnt consecutive int32, one dword
-        // per lane per step, consecutive lanes on consecutive addresses.
-        const int32_t* src = idx + (size_t)base * 5;
-        for (int k = tid; k < (int)cnt * 5; k += kTile) s_idx[k] = src[k];
-        const bool live = (uint32_t)tid < cnt;
-        const uint8_t my_op = live ? op[base + tid] : (uint8_t)DEL3D_OP_ORIENT3D;
-
-        // Regroup: in_sphere-type queries first, orient3d after, each in
-        // query order, so wavefronts mostly run one code path.
-        const bool is_ins = live && my_op != DEL3D_OP_ORIENT3D;
-        const uint64_t ins_mask = __ballot(is_ins);
-        if (lane == 0) s_wave_ins[wave] = (uint32_t)__popcll(ins_mask);
-        __syncthreads();
-        uint32_t ins_before = 0, ins_total = 0;
-#pragma unroll
-        for (int w = 0; w < kTile / 64; ++w) {
-            const uint32_t c = s_wave_ins[w];
-            ins_total += c;
-            ins_before += (w < wave) ? c : 0u;
-        }
-        const uint32_t lt = (uint32_t)__popcll(ins_mask & lanemask_lt());
-        // live orient3d lanes before this one = live lanes before - in_sphere lanes before
-        const uint32_t pos = is_ins ? ins_before + lt
-                                    : ins_total + ((uint32_t)(wave * 64 + lane) - (ins_before + lt));
-        if (live) s_perm[pos] = (uint16_t)tid;
-        s_op[tid] = my_op;
-        __syncthreads();
-
-        const bool act = (uint32_t)tid < cnt;
-        const int lq = act ? s_perm[tid] : 0;
-        const uint32_t q = base + lq;
-        const uint8_t o = s_op[lq];
-        int32_t id[5];
-#pragma unroll
-        for (int k = 0; k < 5; ++k) id[k] = s_idx[lq * 5 + k];
-
-        int sign = 0;
-        bool done = false;
-        int tier = kTierFilter;
-        bool perturbed = false;
-        if (act) {
-            P3 p[5];
-            p[0] = load_p3(xyz, id[0]);
-            p[1] = load_p3(xyz, id[1]);
-            p[2] = load_p3(xyz, id[2]);
-            p[3] = load_p3(xyz, id[3]);
-            if (o == DEL3D_OP_ORIENT3D) {
-                const Filt f = orient_filter(p[0], p[1], p[2], p[3]);
-                if (filter_decides(f, kOrientBound)) {
-                    sign = -sgn(f.det);
-                    done = true;
-                } else if (t1_certify(p, 4, f.maxd, 15, -358)) {
-                    sign = -sgn(f.det);
-                    done = true;
-                    tier = kTierCert;
-                }
+    for (;;) {
+        __syncthreads();  // the previous task is done with every LDS array
+        if (tid == 0) {
+            const uint32_t t = atomicAdd(next, 1u);
+            if (t < a.n_tasks) {
+                const PredTask tk = task_at(a, t);
+                const int64_t pb = a.pt_off ? a.pt_off[tk.set] : 0;
+                const int64_t np = a.pt_off ? a.pt_off[tk.set + 1] - pb : a.n_pts;
+                s_task[0] = tk.q0;
+                s_task[1] = tk.q1;
+                s_task[2] = tk.set;
+                s_task[3] = np;
             } else {
-                p[4] = load_p3(xyz, id[4]);
-                const Filt f = insphere_filter(p[0], p[1], p[2], p[3], p[4]);
-                if (filter_decides(f, kInSphereBound)) {
-                    sign = -sgn(f.det);
-                    done = true;
-                } else if (t1_certify(p, 5, f.maxd, 8, -214)) {
-                    tier = kTierCert;
-                    sign = -sgn(f.det);
-                    done = true;
-                    if (sign == 0 && o == DEL3D_OP_IN_SPHERE) {
-                        perturbed = true;
-                        // every four of these points certify at degree 3 too,
-                        // so the FP64 orient3d determinant is exact here
-                        sign = perturbation_walk(p, id, [&](int k) {
-                            P3 t[4];
-                            subst_tet(p, k, t);
-                            return -sgn(orient_filter(t[0], t[1], t[2], t[3]).det);
-                        });
-                    }
-                }
+                s_task[0] = -1;
             }
-            if (done) out[q] = (int8_t)sign;
         }
-        wave_push(act && !done, q, &ctr->queued, queue);
-        if (kStats) {
-            wave_count(act && done && tier == kTierFilter, &stats[0]);
-            wave_count(act && done && tier == kTierCert, &stats[1]);
-            wave_count(act && perturbed, &stats[4]);
+        __syncthreads();
+        // the task is the same for every lane: keep it in scalar registers
+        const int64_t q0 = uniform64(s_task[0]);
+        if (q0 < 0) break;
+        const int64_t q1 = uniform64(s_task[1]);
+        const int32_t set = (int32_t)uniform64(s_task[2]);
+        // points addressable by this task's indices (the LDS copy holds at most kLdsPts)
+        const int64_t np = kLds ? min(uniform64(s_task[3]), (int64_t)kLdsPts) : uniform64(s_task[3]);
+        const int64_t pbase = a.pt_off ? a.pt_off[set] : 0;
+        const double* setpts = a.xyz + 3 * pbase;
+        if (kLds && set != loaded_set) {
+            // np <= kLdsPts is guaranteed by the host, which routes larger sets
+            // to the global kernel
+            const int n3 = 3 * (int)np;
+            for (int i = tid; i < n3; i += kT) s_pts[i] = setpts[i];
+            loaded_set = set;
+        }
+
+        // Each thread's share of a tile's records (dwords tid + k*kT of the
+        // 5*kT, and op byte tid) is loaded one tile ahead into registers, so
+        // the load latency overlaps the current tile's work.
+        int32_t pf_idx[5];
+        uint8_t pf_op = (uint8_t)DEL3D_OP_ORIENT3D;
+        auto prefetch = [&](int64_t b) {
+            const int c = (int)min((int64_t)kT, q1 - b);
+            const int32_t* src = a.idx + b * 5;
+#pragma unroll
+            for (int k = 0; k < 5; ++k) {
+                const int j = tid + k * kT;
+                pf_idx[k] = j < c * 5 ? src[j] : 0;
+            }
+            pf_op = tid < c ? a.op[b + tid] : (uint8_t)DEL3D_OP_ORIENT3D;
+        };
+        prefetch(q0);
+        for (int64_t base = q0; base < q1; base += kT) {
+            const uint32_t cnt = (uint32_t)min((int64_t)kT, q1 - base);
+            __syncthreads();  // the previous tile is done with s_idx / s_op / s_perm
+#pragma unroll
+            for (int k = 0; k < 5; ++k) s_idx[tid + k * kT] = pf_idx[k];
+            const bool live = (uint32_t)tid < cnt;
+            const uint8_t my_op = live ? pf_op : (uint8_t)DEL3D_OP_ORIENT3D;
+            if (base + kT < q1) prefetch(base + kT);
+
+            // regroup: in_sphere-type queries first, orient3d after, each in
+            // query order, so wavefronts mostly run one code path
+            const bool is_ins = live && my_op != DEL3D_OP_ORIENT3D;
+            const uint64_t ins_mask = __ballot(is_ins);
+            if (lane == 0) s_wave_ins[wave] = (uint32_t)__popcll(ins_mask);
+            __syncthreads();
+            uint32_t ins_before = 0, ins_total = 0;
+#pragma unroll
+            for (int w = 0; w < kW; ++w) {
+                const uint32_t c = s_wave_ins[w];
+                ins_total += c;
+                ins_before += (w < wave) ? c : 0u;
+            }
+            const uint32_t lt = (uint32_t)__popcll(ins_mask & lanemask_lt());
+            const uint32_t pos = is_ins ? ins_before + lt
+                                        : ins_total + ((uint32_t)tid - (ins_before + lt));
+            if (live) s_perm[pos] = (uint16_t)tid;
+            s_op[tid] = my_op;
+            __syncthreads();
+
+            const bool act = (uint32_t)tid < cnt;
+            const int lq = act ? s_perm[tid] : 0;
+            const int64_t q = base + lq;
+            const uint8_t o = s_op[lq];
+            int32_t id[5];
+#pragma unroll
+            for (int k = 0; k < 5; ++k) id[k] = s_idx[lq * 5 + k];
+            const int nchk = (o == DEL3D_OP_ORIENT3D) ? 4 : 5;
+            bool valid = true;
+#pragma unroll
+            for (int k = 0; k < 5; ++k) valid = valid && (k >= nchk || (uint32_t)id[k] < (uint64_t)np);
+
+            int sign = 0, tier = kTierFilter;
+            bool perturbed = false;
+            if (act && valid) {
+                if (kLds) sign = fast_decide<true>(LdsPts{s_pts}, o, id, tier, perturbed);
+                else sign = fast_decide<false>(GlobalPts{setpts}, o, id, tier, perturbed);
+            }
+            const bool queued = act && valid && sign == 2;
+            if (act && !queued) a.out[q] = (int8_t)sign;
+            wave_push(queued, (uint32_t)q, &a.ctr->queued, a.queue);
+            if (kStats) {
+                wave_count(act && valid && !queued && tier == kTierFilter, &stats[0]);
+                wave_count(act && valid && !queued && tier == kTierCert, &stats[1]);
+                wave_count(act && perturbed, &stats[4]);
+                wave_count(act && !valid, &stats[5]);
+            }
         }
     }
 }
 
+// The two fast kernels. The global one is held to 128 VGPRs (four
+// wavefronts per SIMD) to hide gather latency; the LDS one gets that bound
+// from its 1024-thread workgroup.
+template <bool kStats>
+__global__ __launch_bounds__(kLdsThreads) void pred_batch_lds(BatchArgs a) {
+    __shared__ BatchLds<kStats, true> L;
+    pred_batch_body<kStats, true>(a, L);
+}
+
+template <bool kStats>
+__global__ __launch_bounds__(kGlobalThreads) __attribute__((amdgpu_waves_per_eu(4)))
+void pred_batch_global(BatchArgs a) {
+    __shared__ BatchLds<kStats, false> L;
+    pred_batch_body<kStats, false>(a, L);
+}
+
 // ---------------------------------------------------------------------------
-// Exact path, registers (1..3 limbs)
+// Exact path, registers (1..2 limbs)
 // ---------------------------------------------------------------------------
 template <int L>
 __device__ int exact_query(uint8_t o, const P3 p[5], const int32_t id[5], int q, bool& perturbed) {
@@ -187,15 +326,21 @@ __device__ int exact_query(uint8_t o, const P3 p[5], const int32_t id[5], int q,
     });
 }
 
-__global__ __launch_bounds__(256) void pred_exact(const double* __restrict__ xyz,
-                                                  const uint8_t* __restrict__ op,
-                                                  const int32_t* __restrict__ idx,
-                                                  int8_t* __restrict__ out,
-                                                  const uint32_t* __restrict__ queue,
-                                                  uint32_t* __restrict__ wide,
-                                                  PredCounters* __restrict__ ctr,
-                                                  unsigned long long* __restrict__ stats) {
-    const uint32_t n = ctr->queued;
+// Point base (in points) of the set holding query q: binary search over the
+// query offsets (one set: 0).
+__device__ inline int64_t point_base_of(const BatchArgs& a, int64_t q) {
+    if (!a.q_off) return 0;
+    uint32_t lo = 0, hi = a.n_sets;  // last s with q_off[s] <= q
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (a.q_off[mid] <= q) lo = mid;
+        else hi = mid;
+    }
+    return a.pt_off[lo];
+}
+
+__global__ __launch_bounds__(256) void pred_exact(BatchArgs a) {
+    const uint32_t n = a.ctr->queued;
     const uint32_t stride = gridDim.x * blockDim.x;
     // i0 is the wavefront's base, so the trip count is uniform per wavefront
     // and the ballots in wave_count see every lane
@@ -204,14 +349,15 @@ __global__ __launch_bounds__(256) void pred_exact(const double* __restrict__ xyz
         const bool act = i < n;
         bool perturbed = false, done = false;
         if (act) {
-            const uint32_t q = queue[i];
-            const uint8_t o = op[q];
+            const uint32_t q = a.queue[i];
+            const uint8_t o = a.op[q];
+            const double* pts = a.xyz + 3 * point_base_of(a, q);
             int32_t id[5];
 #pragma unroll
-            for (int k = 0; k < 5; ++k) id[k] = idx[(size_t)q * 5 + k];
+            for (int k = 0; k < 5; ++k) id[k] = a.idx[(size_t)q * 5 + k];
             P3 p[5];
             const int np = (o == DEL3D_OP_ORIENT3D) ? 4 : 5;
-            for (int k = 0; k < np; ++k) p[k] = load_p3(xyz, id[k]);
+            for (int k = 0; k < np; ++k) p[k] = load_p3(pts, id[k]);
             if (np == 4) p[4] = p[3];
             const Grid g = grid_of(p, np);
             const int L = limbs_for(g);
@@ -223,13 +369,13 @@ __global__ __launch_bounds__(256) void pred_exact(const double* __restrict__ xyz
                 s = exact_query<2>(o, p, id, g.q, perturbed);
             } else {
                 done = false;
-                wide[atomicAdd(&ctr->wide, 1u)] = q;  // rare: exponent span above 2^55
+                a.wide[atomicAdd(&a.ctr->wide, 1u)] = q;  // rare: exponent span above 2^55
             }
-            if (done) out[q] = (int8_t)s;
+            if (done) a.out[q] = (int8_t)s;
         }
-        if (stats) {
-            wave_count(act && done, &stats[2]);
-            wave_count(act && perturbed, &stats[4]);
+        if (a.stats) {
+            wave_count(act && done, &a.stats[2]);
+            wave_count(act && perturbed, &a.stats[4]);
         }
     }
 }
@@ -237,24 +383,19 @@ __global__ __launch_bounds__(256) void pred_exact(const double* __restrict__ xyz
 // ---------------------------------------------------------------------------
 // Exact path, any width, LDS (one query per workgroup, lane 0 computes)
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kWideBlock) void pred_wide(const double* __restrict__ xyz,
-                                                        const uint8_t* __restrict__ op,
-                                                        const int32_t* __restrict__ idx,
-                                                        int8_t* __restrict__ out,
-                                                        const uint32_t* __restrict__ wide,
-                                                        const PredCounters* __restrict__ ctr,
-                                                        unsigned long long* __restrict__ stats) {
+__global__ __launch_bounds__(kWideBlock) void pred_wide(BatchArgs a) {
     __shared__ uint32_t ws[64 * kWideLimbsMax];
-    const uint32_t n = ctr->wide;
+    const uint32_t n = a.ctr->wide;
     if (threadIdx.x != 0) return;
     for (uint32_t i = blockIdx.x; i < n; i += gridDim.x) {
-        const uint32_t q = wide[i];
-        const uint8_t o = op[q];
+        const uint32_t q = a.wide[i];
+        const uint8_t o = a.op[q];
+        const double* pts = a.xyz + 3 * point_base_of(a, q);
         int32_t id[5];
-        for (int k = 0; k < 5; ++k) id[k] = idx[(size_t)q * 5 + k];
+        for (int k = 0; k < 5; ++k) id[k] = a.idx[(size_t)q * 5 + k];
         P3 p[5];
         const int np = (o == DEL3D_OP_ORIENT3D) ? 4 : 5;
-        for (int k = 0; k < np; ++k) p[k] = load_p3(xyz, id[k]);
+        for (int k = 0; k < np; ++k) p[k] = load_p3(pts, id[k]);
         if (np == 4) p[4] = p[3];
         const Grid g = grid_of(p, np);
         const int L = limbs_for(g);
@@ -264,7 +405,7 @@ __global__ __launch_bounds__(kWideBlock) void pred_wide(const double* __restrict
         } else {
             s = -rt_insphere_raw(p[0], p[1], p[2], p[3], p[4], g.q, L, ws);
             if (s == 0 && o == DEL3D_OP_IN_SPHERE) {
-                if (stats) atomicAdd(&stats[4], 1ull);
+                if (a.stats) atomicAdd(&a.stats[4], 1ull);
                 s = perturbation_walk(p, id, [&](int k) {
                     P3 t[4];
                     subst_tet(p, k, t);
@@ -274,21 +415,68 @@ __global__ __launch_bounds__(kWideBlock) void pred_wide(const double* __restrict
                 });
             }
         }
-        out[q] = (int8_t)s;
-        if (stats) atomicAdd(&stats[3], 1ull);
+        a.out[q] = (int8_t)s;
+        if (a.stats) atomicAdd(&a.stats[3], 1ull);
     }
 }
 
 // ---------------------------------------------------------------------------
-// Host launcher
+// Host launchers
 // ---------------------------------------------------------------------------
-constexpr uint32_t kChunk = 1u << 26;  // queries per launch sequence (bounds the queues)
 
-// When `ev` is given (6 events), the kernels of the (single) chunk are bracketed
-// by ev[0..1] (fast), ev[2..3] (exact) and ev[4..5] (wide).
-void launch_predicates(const double* xyz, const uint8_t* op, const int32_t* idx, int64_t n_q,
-                       int8_t* out, hipStream_t s, unsigned long long* stats_dev,
-                       hipEvent_t* ev) {
+// Zeroes the per-chunk counters (a kernel, so the whole sequence can be
+// captured into a graph).
+__global__ void k_clear_counters(PredCounters* c) {
+    if (threadIdx.x < sizeof(PredCounters) / 4) reinterpret_cast<uint32_t*>(c)[threadIdx.x] = 0u;
+}
+
+void enqueue_batch(const BatchArgs& base, const BatchShape& sh, hipStream_t s, hipEvent_t* ev) {
+    const int cus = device_cu_count();
+    k_clear_counters<<<1, 64, 0, s>>>(base.ctr);
+    D3_LAUNCH_CHECK();
+    if (ev) D3_HIP(hipEventRecord(ev[0], s));
+    if (sh.n_lds_tasks) {
+        BatchArgs a = base;
+        a.tasks = sh.lds_tasks;
+        a.n_tasks = sh.n_lds_tasks;
+        a.chunk = sh.lds_chunk;
+        const unsigned g = (unsigned)std::min<uint64_t>(sh.n_lds_tasks, (uint64_t)cus * sh.lds_blocks_per_cu);
+        if (a.stats) pred_batch_lds<true><<<g, kLdsThreads, 0, s>>>(a);
+        else pred_batch_lds<false><<<g, kLdsThreads, 0, s>>>(a);
+        D3_LAUNCH_CHECK();
+    }
+    if (sh.n_global_tasks) {
+        BatchArgs a = base;
+        a.tasks = sh.global_tasks;
+        a.n_tasks = sh.n_global_tasks;
+        a.chunk = sh.global_chunk;
+        const unsigned g =
+            (unsigned)std::min<uint64_t>(sh.n_global_tasks, (uint64_t)cus * sh.global_blocks_per_cu);
+        if (a.stats) pred_batch_global<true><<<g, kGlobalThreads, 0, s>>>(a);
+        else pred_batch_global<false><<<g, kGlobalThreads, 0, s>>>(a);
+        D3_LAUNCH_CHECK();
+    }
+    if (ev) {
+        D3_HIP(hipEventRecord(ev[1], s));
+        D3_HIP(hipEventRecord(ev[2], s));
+    }
+    pred_exact<<<cus * 4, 256, 0, s>>>(base);
+    D3_LAUNCH_CHECK();
+    if (ev) {
+        D3_HIP(hipEventRecord(ev[3], s));
+        D3_HIP(hipEventRecord(ev[4], s));
+    }
+    pred_wide<<<cus * 4, kWideBlock, 0, s>>>(base);
+    D3_LAUNCH_CHECK();
+    if (ev) D3_HIP(hipEventRecord(ev[5], s));
+}
+
+// One point set: the LDS kernel when the set fits, else the global one; the
+// tasks are implicit equal chunks. Queries are processed in launch chunks of
+// at most kChunk (which bounds the queues).
+void launch_predicates(const double* xyz, int64_t n_pts, const uint8_t* op, const int32_t* idx,
+                       int64_t n_q, int8_t* out, hipStream_t s, unsigned long long* stats_dev,
+                       hipEvent_t* ev, int force_path) {
     if (n_q <= 0) return;
     const uint32_t chunk = (uint32_t)std::min<int64_t>(n_q, kChunk);
     Scratch sc(s);
@@ -296,56 +484,39 @@ void launch_predicates(const double* xyz, const uint8_t* op, const int32_t* idx,
     uint32_t* wide = sc.get<uint32_t>(chunk);
     PredCounters* ctr = sc.get<PredCounters>(1);
     const int cus = device_cu_count();
+    const bool lds = force_path == 1 || (force_path == 0 && n_pts <= kLdsPts);
     for (int64_t off = 0; off < n_q; off += chunk) {
-        const uint32_t n = (uint32_t)std::min<int64_t>(chunk, n_q - off);
-        fill_u32(reinterpret_cast<uint32_t*>(ctr), 0u, sizeof(PredCounters) / 4, s);
-        const uint32_t tiles = (n + kTile - 1) / kTile;
-        const uint32_t fast_grid = std::min<uint32_t>(tiles, (uint32_t)cus * 16);
-        const uint8_t* opc = op + off;
-        const int32_t* idc = idx + off * 5;
-        int8_t* outc = out + off;
-        if (ev) D3_HIP(hipEventRecord(ev[0], s));
-        if (stats_dev) {
-            pred_fast<true><<<fast_grid, kTile, 0, s>>>(xyz, opc, idc, n, outc, queue,
-                                                        ctr, stats_dev);
-            D3_LAUNCH_CHECK();
+        const int64_t n = std::min<int64_t>(chunk, n_q - off);
+        BatchArgs a{};
+        a.xyz = xyz;
+        a.n_pts = n_pts;
+        a.op = op + off;
+        a.idx = idx + off * 5;
+        a.out = out + off;
+        a.n_q = n;
+        a.queue = queue;
+        a.wide = wide;
+        a.ctr = ctr;
+        a.stats = stats_dev;
+        BatchShape sh{};
+        if (lds) {
+            // about two tasks per CU, at least 8 tiles each
+            const int64_t per = std::max<int64_t>(8 * kLdsThreads, (n + 2 * cus - 1) / (2 * cus));
+            sh.lds_chunk = (uint32_t)std::min<int64_t>(per, n);
+            sh.n_lds_tasks = (uint32_t)((n + sh.lds_chunk - 1) / sh.lds_chunk);
+            sh.lds_blocks_per_cu = 1;
         } else {
-            pred_fast<false><<<fast_grid, kTile, 0, s>>>(xyz, opc, idc, n, outc,
-                                                         queue, ctr, nullptr);
-            D3_LAUNCH_CHECK();
+            sh.global_chunk = 16 * kGlobalThreads;
+            sh.n_global_tasks = (uint32_t)((n + sh.global_chunk - 1) / sh.global_chunk);
+            sh.global_blocks_per_cu = 8;
         }
-        if (ev) {
-            D3_HIP(hipEventRecord(ev[1], s));
-            D3_HIP(hipEventRecord(ev[2], s));
-        }
-        pred_exact<<<cus * 4, 256, 0, s>>>(xyz, opc, idc, outc, queue,
-                                           wide, ctr, stats_dev);
-        D3_LAUNCH_CHECK();
-        if (ev) {
-            D3_HIP(hipEventRecord(ev[3], s));
-            D3_HIP(hipEventRecord(ev[4], s));
-        }
-        pred_wide<<<cus * 4, kWideBlock, 0, s>>>(xyz, opc, idc, outc, wide, ctr,
-                                                 stats_dev);
-        D3_LAUNCH_CHECK();
-        if (ev) D3_HIP(hipEventRecord(ev[5], s));
+        enqueue_batch(a, sh, s, ev);
     }
 }
-
-}  // namespace d3
-
-namespace d3 {
 
 // ---------------------------------------------------------------------------
 // One query, every tier in one thread: the scalar entry points (C++ drop-in).
 // ---------------------------------------------------------------------------
-struct ScalarQuery {
-    double xyz[15];
-    int32_t id[5];
-    uint8_t op;
-    int8_t sign;
-};
-
 __global__ __launch_bounds__(64) void pred_single(ScalarQuery* qd) {
     __shared__ uint32_t ws[64 * kWideLimbsMax];
     if (threadIdx.x != 0) return;

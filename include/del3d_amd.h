@@ -10,11 +10,11 @@
  * Conventions
  *   - Plain pointers and sizes; no framework types. "Device pointer" means
  *     memory the HIP runtime can address from the current device (hipMalloc,
- *     hipMallocAsync, or a framework's device tensor).
+ *     or a framework's device tensor).
  *   - `stream` is a hipStream_t passed as void*; NULL is the null stream.
  *     Calls are stream-ordered and return once the work is enqueued, except
- *     where stated. Scratch memory is taken with hipMallocAsync on `stream`
- *     and released on it, so concurrent calls on different streams are safe.
+ *     where stated. Scratch memory comes from a per-stream cache of device
+ *     blocks, so concurrent calls on different streams are safe.
  *   - Return value: DEL3D_OK (0) or a DEL3D_ERR_* code; del3d_last_error()
  *     gives the message of the calling thread's last failure.
  *   - Points are the reference's Point3 (include/del3d/spatial_sort.h:31):
@@ -70,12 +70,14 @@ int del3d_predicates(const double* xyz, int64_t n_pts, const uint8_t* op, const 
 
 /*
  * As del3d_predicates, and additionally returns, after synchronizing the
- * stream, how the queries were decided (host array of 5 int64):
+ * stream, how the queries were decided (host array of 6 int64):
  *   [0] by the FP64 semi-static filter
  *   [1] by the certified small-integer re-evaluation in registers
  *   [2] by exact integer arithmetic in registers (1-2 limbs per difference)
  *   [3] by exact integer arithmetic in LDS (wide exponent spans)
  *   [4] in_sphere queries that needed the symbolic perturbation
+ *   [5] queries with a point index outside the point set (sign 0 written;
+ *       the reference's behaviour for such a pointer is undefined)
  */
 int del3d_predicates_stats(const double* xyz, int64_t n_pts, const uint8_t* op,
                            const int32_t* idx, int64_t n_q, int8_t* sign_out, void* stream,
@@ -91,6 +93,66 @@ int del3d_predicates_stats(const double* xyz, int64_t n_pts, const uint8_t* op,
 int del3d_predicates_timed(const double* xyz, int64_t n_pts, const uint8_t* op,
                            const int32_t* idx, int64_t n_q, int8_t* sign_out, void* stream,
                            float* ms_out);
+
+/*
+ * Many point sets in one call (the batched configuration). Set s owns the
+ * points xyz[3*pt_off[s] .. 3*pt_off[s+1]) and the queries
+ * [q_off[s], q_off[s+1]) of op/idx/sign_out, whose indices are local to the
+ * set (in [0, pt_off[s+1]-pt_off[s])). Each query is answered exactly as
+ * del3d_predicates would answer it on its set alone.
+ *   pt_off, q_off: host arrays of n_sets+1 int64, non-decreasing, from 0;
+ *   at most 2^32-1 queries in total.
+ * Builds a plan (below), runs it, synchronizes `stream`, frees the plan.
+ */
+int del3d_predicates_sets(const double* xyz, const int64_t* pt_off, int64_t n_sets,
+                          const uint8_t* op, const int32_t* idx, const int64_t* q_off,
+                          int8_t* sign_out, void* stream);
+
+/*
+ * Plans (no counterpart in the reference, whose predicates are scalar
+ * calls): the task tables of a fixed set layout, built and uploaded once, so
+ * that a run only enqueues kernels - no allocation, copy or host
+ * synchronization - and can be captured into a HIP graph.
+ *
+ *   del3d_pred_plan_create    layout as in del3d_predicates_sets; allocates
+ *                             the plan's device tables and queues on the
+ *                             current device.
+ *   del3d_pred_plan_run       enqueues the batch on `stream`.
+ *   del3d_pred_plan_run_stats as run, then synchronizes and returns the
+ *                             6 counters of del3d_predicates_stats.
+ *   del3d_pred_plan_run_graph captures a run into a graph on first use (and
+ *                             whenever the buffers or the stream change),
+ *                             then replays it with one graph launch.
+ *   del3d_pred_plan_autotune  times candidate knob settings on these buffers
+ *                             (synchronizing), keeps the fastest; *best_ms
+ *                             receives its time. Results never depend on the
+ *                             knobs, only speed does.
+ *   del3d_pred_plan_set_knobs / get_knobs
+ *                             knobs[0] largest set (points) routed to the
+ *                             LDS kernel (at most 5632; 0 = none),
+ *                             [1] queries per LDS task, [2] queries per
+ *                             global task, [3] global workgroups per CU,
+ *                             [4] reserved.
+ *   del3d_pred_plan_destroy   frees the plan; call it once its runs finished.
+ * A plan holds one set of queues: runs of one plan must not overlap (use one
+ * stream per plan, or one plan per stream).
+ */
+#define DEL3D_PLAN_KNOBS 5
+typedef struct del3d_pred_plan_s* del3d_pred_plan;
+int del3d_pred_plan_create(const int64_t* pt_off, const int64_t* q_off, int64_t n_sets,
+                           del3d_pred_plan* plan_out);
+int del3d_pred_plan_run(del3d_pred_plan plan, const double* xyz, const uint8_t* op,
+                        const int32_t* idx, int8_t* sign_out, void* stream);
+int del3d_pred_plan_run_stats(del3d_pred_plan plan, const double* xyz, const uint8_t* op,
+                              const int32_t* idx, int8_t* sign_out, void* stream,
+                              int64_t* stats_host);
+int del3d_pred_plan_run_graph(del3d_pred_plan plan, const double* xyz, const uint8_t* op,
+                              const int32_t* idx, int8_t* sign_out, void* stream);
+int del3d_pred_plan_autotune(del3d_pred_plan plan, const double* xyz, const uint8_t* op,
+                             const int32_t* idx, int8_t* sign_out, void* stream, float* best_ms);
+int del3d_pred_plan_set_knobs(del3d_pred_plan plan, const int32_t* knobs, int32_t n);
+int del3d_pred_plan_get_knobs(del3d_pred_plan plan, int32_t* knobs);
+int del3d_pred_plan_destroy(del3d_pred_plan plan);
 
 /*
  * Spatial sort of device-resident data. Replaces

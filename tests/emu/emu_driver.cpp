@@ -26,3 +26,29 @@ extern "C" int emu_predicates_stats(const double* xyz, int64_t n_pts, const uint
                                     const int32_t* idx, int64_t n_q, int8_t* out, int64_t* st) {
     return del3d_predicates_stats(xyz, n_pts, op, idx, n_q, out, nullptr, st);
 }
+
+extern "C" int emu_predicates_forced(const double* xyz, int64_t n_pts, const uint8_t* op,
+                                     const int32_t* idx, int64_t n_q, int8_t* out, int path) {
+    // path 1 = LDS kernel, 2 = global kernel: a one-set plan whose routing
+    // threshold sends the set to the chosen kernel
+    const int64_t po[2] = {0, n_pts}, qo[2] = {0, n_q};
+    del3d_pred_plan p = nullptr;
+    int rc = del3d_pred_plan_create(po, qo, 1, &p);
+    if (rc) return rc;
+    int32_t knobs[DEL3D_PLAN_KNOBS];
+    del3d_pred_plan_get_knobs(p, knobs);
+    knobs[0] = path == 1 ? 5632 : 0;
+    knobs[1] = 2048;   // small tasks: several per set, so task switching is exercised
+    knobs[2] = 512;
+    rc = del3d_pred_plan_set_knobs(p, knobs, DEL3D_PLAN_KNOBS);
+    int64_t st[6];
+    if (!rc) rc = del3d_pred_plan_run_stats(p, xyz, op, idx, out, nullptr, st);
+    del3d_pred_plan_destroy(p);
+    return rc;
+}
+
+extern "C" int emu_predicates_sets(const double* xyz, const int64_t* pt_off, int64_t n_sets,
+                                   const uint8_t* op, const int32_t* idx, const int64_t* q_off,
+                                   int8_t* out) {
+    return del3d_predicates_sets(xyz, pt_off, n_sets, op, idx, q_off, out, nullptr);
+}

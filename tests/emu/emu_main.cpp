@@ -33,7 +33,7 @@ static int check_predicates(const char* name, const std::vector<double>& xyz, in
         if (q % 7 == 0) idx[5 * q + 4] = idx[5 * q];
     }
     std::vector<int8_t> got(nq), want(nq);
-    int64_t st[5] = {0, 0, 0, 0, 0};
+    int64_t st[6] = {0, 0, 0, 0, 0, 0};
     if (emu_predicates_stats(xyz.data(), n, op.data(), idx.data(), nq, got.data(), st) != 0) {
         std::printf("%s error\n", name);
         return 1;
@@ -43,6 +43,71 @@ static int check_predicates(const char* name, const std::vector<double>& xyz, in
     std::printf("pred %s %s (filter %lld cert %lld regs %lld wide %lld perturbed %lld)\n", name,
                 ok ? "ok" : "MISMATCH", (long long)st[0], (long long)st[1], (long long)st[2],
                 (long long)st[3], (long long)st[4]);
+    return ok ? 0 : 1;
+}
+extern "C" int emu_predicates_forced(const double* xyz, int64_t n_pts, const uint8_t* op,
+                                     const int32_t* idx, int64_t n_q, int8_t* out, int path);
+extern "C" int emu_predicates_sets(const double* xyz, const int64_t* pt_off, int64_t n_sets,
+                                   const uint8_t* op, const int32_t* idx, const int64_t* q_off,
+                                   int8_t* out);
+
+// The same queries through the LDS kernel (path 1) and the global kernel
+// (path 2), each against the integer oracle.
+static int check_paths(const char* name, const std::vector<double>& xyz, int64_t nq) {
+    const int64_t n = (int64_t)xyz.size() / 3;
+    std::vector<uint8_t> op(nq);
+    std::vector<int32_t> idx(5 * nq);
+    for (int64_t q = 0; q < nq; ++q) {
+        op[q] = (uint8_t)(rnd() % 3);
+        for (int k = 0; k < 5; ++k) idx[5 * q + k] = (int32_t)(rnd() % n);
+        if (q % 5 == 0) idx[5 * q + 4] = idx[5 * q + 1];
+    }
+    idx[5 * 3 + 2] = (int32_t)n;  // one out-of-range index: sign 0, no read outside the set
+    std::vector<int8_t> want(nq);
+    orc_int_pred_batch(xyz.data(), n, op.data(), idx.data(), nq, want.data());
+    want[3] = 0;
+    int fails = 0;
+    for (int path = 1; path <= 2; ++path) {
+        std::vector<int8_t> got(nq, 9);
+        const int rc = emu_predicates_forced(xyz.data(), n, op.data(), idx.data(), nq, got.data(), path);
+        const bool ok = rc == 0 && got == want;
+        std::printf("pred %s path %d %s\n", name, path, ok ? "ok" : "MISMATCH");
+        fails += ok ? 0 : 1;
+    }
+    return fails;
+}
+
+// Several sets of different sizes (LDS-sized and larger) in one call.
+static int check_sets() {
+    const int64_t sizes[5] = {7, 300, 5632, 5633, 9000};
+    const int64_t nqs[5] = {40, 1500, 3000, 2500, 2100};
+    std::vector<double> xyz;
+    std::vector<int64_t> po = {0}, qo = {0};
+    std::vector<uint8_t> op;
+    std::vector<int32_t> idx;
+    std::vector<int8_t> want;
+    for (int s = 0; s < 5; ++s) {
+        std::vector<double> p(3 * (size_t)sizes[s]);
+        for (auto& c : p) c = (double)(int)(rnd() % 9) * 0.25;  // coarse grid: many degeneracies
+        std::vector<uint8_t> o(nqs[s]);
+        std::vector<int32_t> ix(5 * nqs[s]);
+        for (int64_t q = 0; q < nqs[s]; ++q) {
+            o[q] = (uint8_t)(rnd() % 3);
+            for (int k = 0; k < 5; ++k) ix[5 * q + k] = (int32_t)(rnd() % sizes[s]);
+        }
+        std::vector<int8_t> w(nqs[s]);
+        orc_int_pred_batch(p.data(), sizes[s], o.data(), ix.data(), nqs[s], w.data());
+        xyz.insert(xyz.end(), p.begin(), p.end());
+        op.insert(op.end(), o.begin(), o.end());
+        idx.insert(idx.end(), ix.begin(), ix.end());
+        want.insert(want.end(), w.begin(), w.end());
+        po.push_back(po.back() + sizes[s]);
+        qo.push_back(qo.back() + nqs[s]);
+    }
+    std::vector<int8_t> got(want.size(), 9);
+    const int rc = emu_predicates_sets(xyz.data(), po.data(), 5, op.data(), idx.data(), qo.data(), got.data());
+    const bool ok = rc == 0 && got == want;
+    std::printf("pred sets %s\n", ok ? "ok" : "MISMATCH");
     return ok ? 0 : 1;
 }
 extern "C" int emu_spatial_sort_sets(const double* xyz, const int64_t* off, int64_t n_sets,
@@ -67,6 +132,10 @@ int main(int argc, char** argv) {
         for (int i = 0; i < 200; ++i)
             for (int k = 0; k < 3; ++k) q48.push_back((double)(rnd() >> 16) * 3.552713678800501e-15);
         fails += check_predicates("grid48", q48, 2000);
+        fails += check_paths("lattice", lat, 5000);
+        fails += check_paths("wide", wide, 2500);
+        fails += check_sets();
+        if (std::getenv("EMU_PRED_ONLY")) return fails ? 1 : 0;
     }
     for (int a = 1; a < argc; ++a) {
         const int64_t n = std::atoll(argv[a]);

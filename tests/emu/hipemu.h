@@ -182,6 +182,10 @@ inline hipError_t hipStreamSynchronize(hipStream_t) {
     emu::drain();
     return hipSuccess;
 }
+inline hipError_t hipEventSynchronize(hipEvent_t) {
+    emu::drain();
+    return hipSuccess;
+}
 inline hipError_t hipStreamCreateWithFlags(hipStream_t* s, unsigned) { *s = nullptr; return hipSuccess; }
 inline hipError_t hipHostMalloc(void** p, size_t n, unsigned) {
     hipError_t e = hipMallocAsync(p, n, nullptr);
@@ -192,6 +196,20 @@ inline hipError_t hipEventCreate(hipEvent_t* e) { *e = (void*)1; return hipSucce
 inline hipError_t hipEventDestroy(hipEvent_t) { return hipSuccess; }
 inline hipError_t hipEventRecord(hipEvent_t, hipStream_t) { return hipSuccess; }
 inline hipError_t hipEventElapsedTime(float* ms, hipEvent_t, hipEvent_t) { *ms = 0.0f; return hipSuccess; }
+inline hipError_t hipEventSynchronize(hipEvent_t);
+// Graph capture: the emulation records nothing and replays by running the
+// captured sequence again, which is what a graph launch does.
+typedef void* hipGraph_t;
+typedef void* hipGraphExec_t;
+enum { hipStreamCaptureModeThreadLocal = 1 };
+inline hipError_t hipStreamBeginCapture(hipStream_t, int) { return hipErrorInvalidValue; }
+inline hipError_t hipStreamEndCapture(hipStream_t, hipGraph_t* g) { *g = nullptr; return hipErrorInvalidValue; }
+inline hipError_t hipGraphInstantiate(hipGraphExec_t*, hipGraph_t, void*, void*, size_t) { return hipErrorInvalidValue; }
+inline hipError_t hipGraphLaunch(hipGraphExec_t, hipStream_t) { return hipErrorInvalidValue; }
+inline hipError_t hipGraphExecDestroy(hipGraphExec_t) { return hipSuccess; }
+inline hipError_t hipGraphDestroy(hipGraph_t) { return hipSuccess; }
+inline uint32_t __builtin_amdgcn_readfirstlane(uint32_t v) { return v; }
+#define amdgpu_waves_per_eu(...) unused
 inline hipError_t hipHostFree(void* p) {
     emu::drain();
     auto& v = emu::pinned();

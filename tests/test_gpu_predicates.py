@@ -184,3 +184,131 @@ def test_c1_trace_replay():
     got, st = predicates_dev(pts, op, idx, stats=True)
     assert np.array_equal(got, sign), int((got != sign).sum())
     say("trace stats", st)
+
+
+def _plan_signs(pts_list, op_list, idx_list, knobs=None, graph=False, replays=1):
+    """Runs a plan over several sets (set-local indices); returns the signs."""
+    sizes = [len(p) for p in pts_list]
+    nqs = [len(o) for o in op_list]
+    po = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
+    qo = np.concatenate([[0], np.cumsum(nqs)]).astype(np.int64)
+    xyz = D.DeviceArray.from_host(np.vstack(pts_list).astype(np.float64))
+    dop = D.DeviceArray.from_host(np.concatenate(op_list).astype(np.uint8))
+    didx = D.DeviceArray.from_host(np.vstack(idx_list).astype(np.int32))
+    out = D.DeviceArray((int(qo[-1]),), np.int8)
+    plan = D.PredPlan(po, qo)
+    if knobs is not None:
+        k = plan.knobs
+        for i, v in knobs.items():
+            k[i] = v
+        plan.knobs = k
+    s = D.Stream()
+    results = []
+    for _ in range(replays):
+        out.zero()
+        plan.run(xyz, dop, didx, out, s, graph=graph)
+        s.synchronize()
+        results.append(out.to_host())
+    plan.close()
+    return results
+
+
+@pytest.mark.parametrize("name", ["lattice_5^3", "very_wide", "quantised_2^-10_sphere", "denormals"])
+def test_lds_and_global_paths_agree_with_oracle(name):
+    """The same queries through the LDS kernel (set held in LDS) and the
+    global-gather kernel (routing threshold 0), both against the oracle."""
+    pts = O.lattice(5, 5, 5) if name == "lattice_5^3" else hard_sets()[name]
+    rng = np.random.default_rng(3)
+    n = 30000
+    idx = rng.integers(0, len(pts), size=(n, 5)).astype(np.int32)
+    idx[:4000, 4] = idx[:4000, 2]
+    op = rng.integers(0, 3, size=n).astype(np.uint8)
+    want = O.pred_batch(pts, op, idx, "int")
+    say("paths", name)
+    lds = _plan_signs([pts], [op], [idx], knobs={0: 5632, 1: 4096})[0]
+    glob = _plan_signs([pts], [op], [idx], knobs={0: 0, 2: 1024})[0]
+    assert np.array_equal(lds, want)
+    assert np.array_equal(glob, want)
+
+
+def test_out_of_range_index_gives_zero():
+    pts = O.lattice(3, 3, 3)
+    op = np.array([0, 1, 2, 1], np.uint8)
+    idx = np.array([[0, 1, 3, 27, 0], [0, 1, 3, 9, -1], [0, 1, 3, 9, 13], [0, 1, 3, 9, 1 << 30]], np.int32)
+    xyz = D.DeviceArray.from_host(pts)
+    dop, didx = D.DeviceArray.from_host(op), D.DeviceArray.from_host(idx)
+    out = D.DeviceArray((4,), np.int8)
+    st = D.predicates_stats(xyz, dop, didx, out)
+    got = out.to_host()
+    assert st["bad_index"] == 3
+    assert got[0] == 0 and got[1] == 0 and got[3] == 0
+    assert got[2] == O.pred_batch(pts, op[2:3], idx[2:3], "oracle")[0]
+
+
+def _mixed_sets(n_sets, seed, max_pts=12000):
+    """Sets of log-uniform size (uniform cube, quantised sphere, jittered
+    lattice), 4 queries per point, set-local indices."""
+    rng = np.random.default_rng(seed)
+    pts_l, op_l, idx_l = [], [], []
+    for s in range(n_sets):
+        n = int(10 * (max_pts / 10) ** rng.random())
+        kind = s % 3
+        if kind == 0:
+            p = rng.random((n, 3))
+        elif kind == 1:
+            v = rng.normal(size=(n, 3))
+            p = np.round(v / np.linalg.norm(v, axis=1, keepdims=True) * 1024) / 1024
+        else:
+            m = max(2, int(round(n ** (1 / 3))))
+            g = np.stack(np.meshgrid(*[np.arange(m)] * 3, indexing="ij"), -1).reshape(-1, 3)[:n]
+            p = g + np.round(rng.random(g.shape) * 64) / 1024
+            n = len(p)
+        nq = 4 * n
+        pts_l.append(p)
+        op_l.append(rng.integers(0, 3, size=nq).astype(np.uint8))
+        idx_l.append(rng.integers(0, n, size=(nq, 5)).astype(np.int32))
+    return pts_l, op_l, idx_l
+
+
+def test_sets_match_single_set_calls():
+    pts_l, op_l, idx_l = _mixed_sets(60, seed=9)
+    say("sets", sum(len(p) for p in pts_l), "points", sum(len(o) for o in op_l), "queries")
+    got = _plan_signs(pts_l, op_l, idx_l)[0]
+    off = 0
+    for p, o, i in zip(pts_l, op_l, idx_l):
+        alone, _ = predicates_dev(p, o, i)
+        assert np.array_equal(got[off:off + len(o)], alone)
+        off += len(o)
+    want = np.concatenate([O.pred_batch(p, o, i, "oracle") for p, o, i in zip(pts_l[:12], op_l[:12], idx_l[:12])])
+    assert np.array_equal(got[:len(want)], want)
+
+
+def test_sets_call_and_graph_replays():
+    """del3d_predicates_sets, a plan run, and five replays of the captured
+    graph (with other GPU work between them) all give the same signs."""
+    pts_l, op_l, idx_l = _mixed_sets(40, seed=10)
+    eager = _plan_signs(pts_l, op_l, idx_l)[0]
+    sizes = [len(p) for p in pts_l]
+    po = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
+    qo = np.concatenate([[0], np.cumsum([len(o) for o in op_l])]).astype(np.int64)
+    xyz = D.DeviceArray.from_host(np.vstack(pts_l))
+    dop = D.DeviceArray.from_host(np.concatenate(op_l))
+    didx = D.DeviceArray.from_host(np.vstack(idx_l))
+    out = D.DeviceArray((int(qo[-1]),), np.int8)
+    D.predicates_sets(xyz, po, dop, didx, qo, out)
+    assert np.array_equal(out.to_host(), eager)
+    plan = D.PredPlan(po, qo)
+    s = D.Stream()
+    for r in range(5):
+        out.zero()
+        plan.run(xyz, dop, didx, out, s, graph=True)
+        s.synchronize()
+        assert np.array_equal(out.to_host(), eager), r
+        predicates_dev(pts_l[0], op_l[0], idx_l[0])  # eager work between replays
+    ms = plan.autotune(xyz, dop, didx, out, s)
+    say("autotuned", plan.knobs, ms)
+    out.zero()
+    plan.run(xyz, dop, didx, out, s, graph=True)
+    s.synchronize()
+    assert np.array_equal(out.to_host(), eager)
+    plan.close()
