@@ -72,6 +72,249 @@ def load_traffic():
         return None
 
 
+def _workloads_lib():
+    import ctypes as C
+    lib = C.CDLL(str(ROOT / "del3d_amd" / "libdel3d_workloads.so"))
+    lib.del3d_gen_queries.argtypes = [C.c_uint64, C.c_void_p, C.c_void_p, C.c_int64, C.c_int64,
+                                      C.c_void_p, C.c_void_p, C.c_void_p]
+    return lib
+
+
+def _oracle_sample(pts_host, op, idx):
+    sys.path.insert(0, str(ROOT / "tests"))
+    import oracle_lib as O
+    return O.pred_batch(pts_host, op, idx, "ref" if O.have_ref() else "oracle")
+
+
+def _events_ms(D, stream, fn, steps):
+    e0, e1 = D.Event(), D.Event()
+    D.synchronize()
+    e0.record(stream)
+    for _ in range(steps):
+        fn()
+    e1.record(stream)
+    e1.synchronize()
+    return e0.elapsed_ms(e1)
+
+
+def run_large(args) -> dict:
+    """10^7 points (half N(0,1)^3, half uniform [-1,1)^3) and 2^28 random
+    queries over them (half orient3d, half in_sphere), generated on the
+    device; one step = one batched call over all queries."""
+    import del3d_amd as D
+    from del3d_amd import workloads as W
+    n_pts, n_q, seed = 10_000_000, 1 << 28, 3
+    pts = W.c3_points(n_pts, seed=seed)
+    stream = D.Stream()
+    xyz = D.DeviceArray.from_host(pts)
+    po = np.array([0, n_pts], np.int64)
+    qo = np.array([0, n_q], np.int64)
+    dpo, dqo = D.DeviceArray.from_host(po), D.DeviceArray.from_host(qo)
+    op = D.DeviceArray((n_q,), np.uint8)
+    idx = D.DeviceArray((n_q, 5), np.int32)
+    out = D.DeviceArray((n_q,), np.int8)
+    G = _workloads_lib()
+    assert G.del3d_gen_queries(seed, dpo.ptr, dqo.ptr, 1, n_q, op.ptr, idx.ptr, stream.handle) == 0
+    st = D.predicates_stats(xyz, op, idx, out, stream)
+    for _ in range(args.warmup):
+        D.predicates(xyz, op, idx, out, stream)
+    ms = _events_ms(D, stream, lambda: D.predicates(xyz, op, idx, out, stream), args.steps)
+    sample = np.random.default_rng(0).choice(n_q, size=1 << 18, replace=False)
+    sample.sort()
+    sop, sidx = W.hashed_queries(seed, n_pts, sample)
+    got = out.to_host()[sample]
+    want = _oracle_sample(pts, sop, sidx)
+    return {"config": "large", "points": n_pts, "queries": n_q,
+            "gpred_per_s": round(n_q * args.steps / (ms / 1e3) / 1e9, 3),
+            "ms_per_step": round(ms / args.steps, 3), "decided_by": st,
+            "bit_exact_sample": bool(np.array_equal(got, want)), "sample": len(sample),
+            "note": "the spatial-sort half of this configuration is not run (GPU sort disabled)"}
+
+
+def run_batched(args) -> dict:
+    """4096 point sets of log-uniform size 100..200,000 (uniform cube,
+    quantised sphere, jittered lattice), 10 queries per point, in one plan:
+    autotuned, captured into a graph and replayed, against the same jobs
+    launched one call per set."""
+    import del3d_amd as D
+    from del3d_amd import workloads as W
+    seed = 4
+    sizes = W.c4_sizes(4096, seed=seed)
+    t0 = time.perf_counter()
+    pts = W.c4_points(sizes, seed=seed)
+    t_gen = time.perf_counter() - t0
+    po = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
+    qo = np.concatenate([[0], np.cumsum(10 * sizes)]).astype(np.int64)
+    n_q = int(qo[-1])
+    stream = D.Stream()
+    xyz = D.DeviceArray.from_host(pts)
+    dpo, dqo = D.DeviceArray.from_host(po), D.DeviceArray.from_host(qo)
+    op = D.DeviceArray((n_q,), np.uint8)
+    idx = D.DeviceArray((n_q, 5), np.int32)
+    out = D.DeviceArray((n_q,), np.int8)
+    out1 = D.DeviceArray((n_q,), np.int8)
+    G = _workloads_lib()
+    assert G.del3d_gen_queries(seed, dpo.ptr, dqo.ptr, len(sizes), n_q, op.ptr, idx.ptr,
+                               stream.handle) == 0
+    plan = D.PredPlan(po, qo)
+    st = plan.run_stats(xyz, op, idx, out, stream)
+    t0 = time.perf_counter()
+    tuned_ms = plan.autotune(xyz, op, idx, out, stream)
+    t_tune = time.perf_counter() - t0
+    plan.run(xyz, op, idx, out, stream, graph=True)
+    for _ in range(args.warmup):
+        plan.run(xyz, op, idx, out, stream, graph=True)
+    ms_graph = _events_ms(D, stream, lambda: plan.run(xyz, op, idx, out, stream, graph=True), args.steps)
+
+    lib = D.lib()
+
+    def one_by_one():
+        for s in range(len(sizes)):
+            a, b = int(qo[s]), int(qo[s + 1])
+            rc = lib.del3d_predicates(xyz.addr + 24 * int(po[s]), int(sizes[s]), op.addr + a,
+                                      idx.addr + 20 * a, b - a, out1.addr + a, stream.handle)
+            assert rc == 0
+    one_by_one()
+    steps1 = max(1, args.steps // 5)
+    w0 = time.perf_counter()
+    ms_single = _events_ms(D, stream, one_by_one, steps1)
+    wall_single = time.perf_counter() - w0
+    signs, signs1 = out.to_host(), out1.to_host()
+    same = bool(np.array_equal(signs, signs1))
+    # oracle on sample sets: the largest of each kind and a few others
+    rng = np.random.default_rng(1)
+    pick = sorted(set([int(np.argmax(np.where(np.arange(len(sizes)) % 3 == k, sizes, 0))) for k in range(3)]
+                      + list(rng.choice(len(sizes), 6, replace=False))))
+    exact = True
+    checked = 0
+    for s in pick:
+        a, b = int(qo[s]), int(qo[s + 1])
+        q = np.arange(a, min(b, a + 200_000))
+        sop, sidx = W.hashed_queries(seed, int(sizes[s]), q)
+        want = _oracle_sample(pts[po[s]:po[s + 1]], sop, sidx)
+        exact &= bool(np.array_equal(signs[q], want))
+        checked += len(q)
+    plan.close()
+    return {"config": "batched", "sets": len(sizes), "points": int(po[-1]), "queries": n_q,
+            "gpred_per_s_graph": round(n_q * args.steps / (ms_graph / 1e3) / 1e9, 3),
+            "ms_per_step_graph": round(ms_graph / args.steps, 3),
+            "gpred_per_s_one_call_per_set": round(n_q * steps1 / (ms_single / 1e3) / 1e9, 3),
+            "ms_per_step_one_call_per_set": round(ms_single / steps1, 3),
+            "speedup_vs_one_call_per_set": round((ms_single / steps1) / (ms_graph / args.steps), 2),
+            "one_call_per_set_wall_s": round(wall_single / steps1, 3),
+            "autotuned_knobs": None, "autotune_best_ms": round(tuned_ms, 3),
+            "autotune_s": round(t_tune, 2), "points_gen_s": round(t_gen, 1), "decided_by": st,
+            "same_signs_batched_vs_per_set": same, "oracle_checked_queries": checked,
+            "bit_exact_oracle_sample": exact,
+            "note": "the spatial-sort half of this configuration is not run (GPU sort disabled)"}
+
+
+def run_served(args) -> dict:
+    """Poisson stream of predicate jobs on resident point sets of 1,000 to
+    1,000,000 points (log-uniform; 30% lattice or quantised sets), each job
+    10 queries per point of its set (at most 2^22), follow-up queries on the
+    same resident sets, under a 200 GB HBM budget. Phase 1 submits every job
+    at once (sustained throughput); phase 2 offers half that rate as Poisson
+    arrivals and reports p50 / p99 submit-to-result latency. Baseline: the
+    same jobs as plain per-request calls (upload the job's queries, one
+    batched call on the resident set, download the signs), one at a time.
+    Every job's signs are compared with the baseline's (tolerance 0)."""
+    import threading
+
+    import del3d_amd as D
+    from del3d_amd import workloads as W
+    rng = np.random.default_rng(5)
+    n_sets, n_jobs = 48, 240
+    sizes = np.floor(1000 * 1000 ** rng.random(n_sets)).astype(np.int64)
+    sets = []
+    for i, n in enumerate(sizes):
+        if rng.random() < 0.3:
+            sets.append(W.set_points(1 + (i % 2), int(n), rng))  # quantised sphere / lattice
+        else:
+            sets.append(W.set_points(0, int(n), rng))
+    pools = []
+    for p in sets:
+        nq = min(10 * len(p), 1 << 22)
+        pools.append((rng.integers(0, 2, size=nq).astype(np.uint8),
+                      rng.integers(0, len(p), size=(nq, 5)).astype(np.int32)))
+    job_set = rng.integers(0, n_sets, size=n_jobs)
+    total_q = int(sum(len(pools[s][0]) for s in job_set))
+
+    # baseline: one synchronous request at a time on device-resident points
+    H = D._hip._lib()
+    stream = D.Stream()
+    dev_sets = [D.DeviceArray.from_host(p) for p in sets]
+    qmax = max(len(p[0]) for p in pools)
+    dop, didx = D.DeviceArray((qmax,), np.uint8), D.DeviceArray((qmax, 5), np.int32)
+    dout = D.DeviceArray((qmax,), np.int8)
+    lib = D.lib()
+    base_out = []
+    t0 = time.perf_counter()
+    for s in job_set:
+        op, idx = pools[s]
+        out = np.empty(len(op), np.int8)
+        D._hip.check(H.hipMemcpyAsync(dop.ptr, op.ctypes.data, op.nbytes, 1, stream.handle), "H2D")
+        D._hip.check(H.hipMemcpyAsync(didx.ptr, idx.ctypes.data, idx.nbytes, 1, stream.handle), "H2D")
+        assert lib.del3d_predicates(dev_sets[s].ptr, len(sets[s]), dop.ptr, didx.ptr, len(op),
+                                    dout.ptr, stream.handle) == 0
+        D._hip.check(H.hipMemcpyAsync(out.ctypes.data, dout.ptr, out.nbytes, 2, stream.handle), "D2H")
+        stream.synchronize()
+        base_out.append(out)
+    t_base = time.perf_counter() - t0
+    del dev_sets, dop, didx, dout
+
+    srv = D.Server(200 << 30, slots=3, max_batch_queries=1 << 24)
+    ids = [srv.put_points(p) for p in sets]
+    # warm-up: one job per slot size class
+    for s in range(3):
+        srv.wait(srv.submit(ids[s], *pools[s]))
+
+    def phase(rate):
+        res, lat = [None] * n_jobs, [0.0] * n_jobs
+        sub = [None] * n_jobs
+        done = threading.Event()
+
+        def waiter():
+            for k in range(n_jobs):
+                while sub[k] is None:
+                    time.sleep(0.0001)
+                res[k], lat[k] = srv.wait(sub[k])
+            done.set()
+
+        th = threading.Thread(target=waiter)
+        th.start()
+        t_start = time.perf_counter()
+        r = np.random.default_rng(11)
+        for k, s in enumerate(job_set):
+            sub[k] = srv.submit(ids[s], *pools[s])
+            if rate:
+                time.sleep(r.exponential(1.0 / rate))
+        th.join()
+        return res, lat, time.perf_counter() - t_start
+
+    res1, lat1, t1 = phase(0)
+    rate = 0.5 * n_jobs / t1
+    res2, lat2, t2 = phase(rate)
+    info = srv.info()
+    srv.close()
+    same = all(np.array_equal(a, b) for a, b in zip(res1, base_out)) and \
+        all(np.array_equal(a, b) for a, b in zip(res2, base_out))
+    return {"config": "served", "sets": n_sets, "jobs": n_jobs, "queries": total_q,
+            "points_resident": int(sizes.sum()),
+            "saturated_gpred_per_s": round(total_q / t1 / 1e9, 3),
+            "per_request_gpred_per_s": round(total_q / t_base / 1e9, 3),
+            "speedup_vs_per_request": round(t_base / t1, 2),
+            "poisson_rate_jobs_per_s": round(rate, 1),
+            "poisson_gpred_per_s": round(total_q / t2 / 1e9, 3),
+            "p50_ms": round(float(np.percentile(lat2, 50)), 3),
+            "p99_ms": round(float(np.percentile(lat2, 99)), 3),
+            "saturated_p99_ms": round(float(np.percentile(lat1, 99)), 3),
+            "per_request_mean_ms": round(1e3 * t_base / n_jobs, 3),
+            "same_signs_as_per_request": bool(same), "server": info,
+            "note": "predicate jobs only: sort jobs are not served (GPU sort disabled); "
+                    "latencies include the host-side copies of each job's queries and signs"}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -80,6 +323,8 @@ def main() -> int:
     ap.add_argument("--queries", type=int, default=1 << 24)
     ap.add_argument("--cpu-sample", type=int, default=1 << 23)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--config", default="lattice", choices=["lattice", "large", "batched", "served"],
+                    help="lattice: the headline line; large / batched: the other configurations")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="only run the calibration copy kernel (for counter scaling)")
     args = ap.parse_args()
@@ -103,6 +348,11 @@ def main() -> int:
             lib.del3d_calib_copy(a.ptr, b.ptr, nbytes, None)
         D.synchronize()
         print(json.dumps({"calib_copy_bytes": nbytes, "launches": 5}))
+        return 0
+
+    if args.config != "lattice":
+        res = {"large": run_large, "batched": run_batched, "served": run_served}[args.config](args)
+        print(json.dumps(res), flush=True)
         return 0
 
     t_gen = time.perf_counter()

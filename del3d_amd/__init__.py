@@ -58,6 +58,13 @@ def lib() -> C.CDLL:
         L.del3d_pred_plan_set_knobs.argtypes = [vp, C.POINTER(C.c_int32), C.c_int32]
         L.del3d_pred_plan_get_knobs.argtypes = [vp, C.POINTER(C.c_int32)]
         L.del3d_pred_plan_destroy.argtypes = [vp]
+        L.del3d_server_create.argtypes = [i64, C.c_int32, i64, C.POINTER(C.c_void_p)]
+        L.del3d_server_put_points.argtypes = [vp, vp, i64, i64p]
+        L.del3d_server_drop_points.argtypes = [vp, i64]
+        L.del3d_server_submit.argtypes = [vp, i64, vp, vp, i64, vp, i64p]
+        L.del3d_server_wait.argtypes = [vp, i64, C.POINTER(C.c_double)]
+        L.del3d_server_info.argtypes = [vp, i64p]
+        L.del3d_server_destroy.argtypes = [vp]
         L.del3d_spatial_sort.argtypes = [vp, i64, vp, i64, vp]
         L.del3d_spatial_sort_sets.argtypes = [vp, C.POINTER(C.c_int64), i64, vp, vp]
         L.del3d_spatial_sort_host.argtypes = [vp, i64, vp, i64]
@@ -195,6 +202,71 @@ def spatial_sort_sets(xyz: DeviceArray, offsets: np.ndarray, order: DeviceArray,
            "del3d_spatial_sort_sets")
 
 
+class Server:
+    """The serving loop (include/del3d_amd.h, del3d_server_*): resident point
+    sets and continuously batched predicate jobs. Host arrays handed to it
+    are kept referenced until they may be released."""
+
+    INFO = ("used_bytes", "resident_bytes", "resident_sets", "evictions", "uploads", "batches",
+            "jobs_done", "budget")
+
+    def __init__(self, budget_bytes: int, slots: int = 3, max_batch_queries: int = 1 << 24):
+        h = C.c_void_p()
+        _check(lib().del3d_server_create(int(budget_bytes), int(slots), int(max_batch_queries),
+                                         C.byref(h)), "del3d_server_create")
+        self.handle = h
+        self._sets = {}
+        self._jobs = {}
+
+    def put_points(self, points) -> int:
+        xyz = np.ascontiguousarray(np.asarray(points, dtype=np.float64).reshape(-1, 3))
+        sid = C.c_int64()
+        _check(lib().del3d_server_put_points(self.handle, xyz.ctypes.data, len(xyz), C.byref(sid)),
+               "del3d_server_put_points")
+        self._sets[sid.value] = xyz
+        return sid.value
+
+    def drop_points(self, set_id: int) -> None:
+        _check(lib().del3d_server_drop_points(self.handle, int(set_id)), "del3d_server_drop_points")
+        self._sets.pop(set_id, None)
+
+    def submit(self, set_id: int, op, idx) -> int:
+        op = np.ascontiguousarray(op, dtype=np.uint8)
+        idx = np.ascontiguousarray(np.asarray(idx, dtype=np.int32).reshape(-1, 5))
+        out = np.empty(len(op), dtype=np.int8)
+        jid = C.c_int64()
+        _check(lib().del3d_server_submit(self.handle, int(set_id), op.ctypes.data, idx.ctypes.data,
+                                         len(op), out.ctypes.data, C.byref(jid)), "del3d_server_submit")
+        self._jobs[jid.value] = (op, idx, out)
+        return jid.value
+
+    def wait(self, job_id: int):
+        """(signs, latency in ms) of a finished job."""
+        ms = C.c_double()
+        rc = lib().del3d_server_wait(self.handle, int(job_id), C.byref(ms))
+        op, idx, out = self._jobs.pop(job_id)
+        _check(rc, "del3d_server_wait")
+        return out, ms.value
+
+    def info(self) -> dict:
+        v = (C.c_int64 * 8)()
+        _check(lib().del3d_server_info(self.handle, v), "del3d_server_info")
+        return dict(zip(self.INFO, v))
+
+    def close(self) -> None:
+        if self.handle:
+            lib().del3d_server_destroy(self.handle)
+            self.handle = None
+            self._jobs.clear()
+            self._sets.clear()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 # ---- host-memory conveniences ------------------------------------------------
 
 def predicates_np(points, op, idx) -> np.ndarray:
@@ -253,6 +325,6 @@ def in_sphere_unperturbed(a, b, c, d, e) -> int:
 __all__ = [
     "NEGATIVE", "ZERO", "POSITIVE", "ORIENT3D", "IN_SPHERE", "IN_SPHERE_UNPERTURBED",
     "DeviceArray", "Stream", "Event", "device_count", "synchronize", "lib", "Del3dError",
-    "predicates", "predicates_stats", "predicates_timed", "predicates_sets", "PredPlan", "spatial_sort_device", "spatial_sort_sets",
+    "predicates", "predicates_stats", "predicates_timed", "predicates_sets", "PredPlan", "Server", "spatial_sort_device", "spatial_sort_sets",
     "predicates_np", "spatial_sort", "orient3d", "in_sphere", "in_sphere_unperturbed",
 ]

@@ -14,7 +14,7 @@ from pathlib import Path
 HERE = Path(__file__).resolve().parent
 CSRC = HERE / "csrc"
 LIB = HERE / "libdel3d_amd.so"
-SOURCES = ["capi.hip", "predicates.hip", "batch.hip", "spatial_sort.hip", "dropin.cpp"]
+SOURCES = ["capi.hip", "predicates.hip", "batch.hip", "server.hip", "spatial_sort.hip", "dropin.cpp"]
 ARCH = os.environ.get("DEL3D_OFFLOAD_ARCH", "gfx950")
 
 COMMON = [
@@ -58,9 +58,12 @@ def build(verbose: bool = False) -> Path:
     order = sorted(SOURCES, key=lambda s: s != "predicates.hip")
     with ThreadPoolExecutor(max_workers=len(order)) as ex:
         objs = list(ex.map(lambda s: _compile(s, outdir, verbose), order))
-    calib = _compile("calib.hip", outdir, verbose)
-    subprocess.run([hipcc(), "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o",
-                    str(HERE / "libdel3d_calib.so"), calib], check=True)
+    # measurement helpers, separate from the product library: a copy kernel of
+    # known size (counter calibration) and the benchmarks' query generator
+    for src, lib in (("calib.hip", "libdel3d_calib.so"), ("workload_gen.hip", "libdel3d_workloads.so")):
+        obj = _compile(src, outdir, verbose)
+        subprocess.run([hipcc(), "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o",
+                        str(HERE / lib), obj], check=True)
     tmp = LIB.with_suffix(".so.tmp")
     link = [hipcc(), "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", str(tmp)] + objs
     if verbose:

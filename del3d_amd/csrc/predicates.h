@@ -33,6 +33,11 @@ struct BatchArgs {
     const int64_t* pt_off;   // n_sets+1 point offsets, or nullptr (one set)
     const int64_t* q_off;    // n_sets+1 query offsets, or nullptr (one set)
     uint32_t n_sets;
+    // alternatively, one device pointer and point count per set (sets in
+    // separate allocations, as resident point sets of a server are); used
+    // when set_ptr != nullptr, in place of xyz + pt_off
+    const double* const* set_ptr;
+    const int64_t* set_np;
     const uint8_t* op;
     const int32_t* idx;
     int8_t* out;

@@ -135,6 +135,12 @@ __device__ __forceinline__ int fast_decide(const Fetch& F, uint8_t o, const int3
     return sign;
 }
 
+// The points of set s.
+__device__ inline const double* set_points(const BatchArgs& a, int32_t s) {
+    if (a.set_ptr) return a.set_ptr[s];
+    return a.xyz + 3 * (a.pt_off ? a.pt_off[s] : 0);
+}
+
 // Task `t` of a batch: explicit table, or implicit equal chunks of one set.
 __device__ inline PredTask task_at(const BatchArgs& a, uint32_t t) {
     if (a.tasks) return a.tasks[t];
@@ -183,8 +189,9 @@ __device__ __forceinline__ void pred_batch_body(const BatchArgs& a, BatchLds<kSt
             const uint32_t t = atomicAdd(next, 1u);
             if (t < a.n_tasks) {
                 const PredTask tk = task_at(a, t);
-                const int64_t pb = a.pt_off ? a.pt_off[tk.set] : 0;
-                const int64_t np = a.pt_off ? a.pt_off[tk.set + 1] - pb : a.n_pts;
+                int64_t np = a.n_pts;
+                if (a.set_ptr) np = a.set_np[tk.set];
+                else if (a.pt_off) np = a.pt_off[tk.set + 1] - a.pt_off[tk.set];
                 s_task[0] = tk.q0;
                 s_task[1] = tk.q1;
                 s_task[2] = tk.set;
@@ -201,8 +208,7 @@ __device__ __forceinline__ void pred_batch_body(const BatchArgs& a, BatchLds<kSt
         const int32_t set = (int32_t)uniform64(s_task[2]);
         // points addressable by this task's indices (the LDS copy holds at most kLdsPts)
         const int64_t np = kLds ? min(uniform64(s_task[3]), (int64_t)kLdsPts) : uniform64(s_task[3]);
-        const int64_t pbase = a.pt_off ? a.pt_off[set] : 0;
-        const double* setpts = a.xyz + 3 * pbase;
+        const double* setpts = set_points(a, set);
         if (kLds && set != loaded_set) {
             // np <= kLdsPts is guaranteed by the host, which routes larger sets
             // to the global kernel
@@ -326,17 +332,17 @@ __device__ int exact_query(uint8_t o, const P3 p[5], const int32_t id[5], int q,
     });
 }
 
-// Point base (in points) of the set holding query q: binary search over the
-// query offsets (one set: 0).
-__device__ inline int64_t point_base_of(const BatchArgs& a, int64_t q) {
-    if (!a.q_off) return 0;
+// The points of the set holding query q: binary search over the query
+// offsets (one set: set 0).
+__device__ inline const double* points_of_query(const BatchArgs& a, int64_t q) {
+    if (!a.q_off) return set_points(a, 0);
     uint32_t lo = 0, hi = a.n_sets;  // last s with q_off[s] <= q
     while (hi - lo > 1) {
         const uint32_t mid = (lo + hi) >> 1;
         if (a.q_off[mid] <= q) lo = mid;
         else hi = mid;
     }
-    return a.pt_off[lo];
+    return set_points(a, (int32_t)lo);
 }
 
 __global__ __launch_bounds__(256) void pred_exact(BatchArgs a) {
@@ -351,7 +357,7 @@ __global__ __launch_bounds__(256) void pred_exact(BatchArgs a) {
         if (act) {
             const uint32_t q = a.queue[i];
             const uint8_t o = a.op[q];
-            const double* pts = a.xyz + 3 * point_base_of(a, q);
+            const double* pts = points_of_query(a, q);
             int32_t id[5];
 #pragma unroll
             for (int k = 0; k < 5; ++k) id[k] = a.idx[(size_t)q * 5 + k];
@@ -390,7 +396,7 @@ __global__ __launch_bounds__(kWideBlock) void pred_wide(BatchArgs a) {
     for (uint32_t i = blockIdx.x; i < n; i += gridDim.x) {
         const uint32_t q = a.wide[i];
         const uint8_t o = a.op[q];
-        const double* pts = a.xyz + 3 * point_base_of(a, q);
+        const double* pts = points_of_query(a, q);
         int32_t id[5];
         for (int k = 0; k < 5; ++k) id[k] = a.idx[(size_t)q * 5 + k];
         P3 p[5];

@@ -169,3 +169,59 @@ def random_queries(n_pts: int, n_q: int, seed: int):
     op = rng.integers(0, 2, size=n_q).astype(np.uint8)
     idx = rng.integers(0, n_pts, size=(n_q, 5), dtype=np.int64).astype(np.int32)
     return op, idx
+
+
+# ---- batched and served configurations ---------------------------------------
+
+def c4_sizes(n_sets: int = 4096, seed: int = 4, lo: int = 100, hi: int = 200_000) -> np.ndarray:
+    """Log-uniform set sizes n_s = floor(lo * (hi/lo)^U)."""
+    rng = np.random.default_rng(seed)
+    return np.floor(lo * (hi / lo) ** rng.random(n_sets)).astype(np.int64)
+
+
+def set_points(kind: int, n: int, rng) -> np.ndarray:
+    """kind 0: uniform in the unit cube; 1: unit-sphere surface snapped to the
+    2^-10 grid (many ties and near-cospherical points); 2: an integer lattice
+    plus a jitter snapped to 2^-10."""
+    if kind == 0:
+        return rng.random((n, 3))
+    if kind == 1:
+        v = rng.standard_normal((n, 3))
+        v /= np.linalg.norm(v, axis=1, keepdims=True)
+        return np.round(v * 1024.0) / 1024.0
+    m = max(2, int(np.ceil(n ** (1.0 / 3.0))))
+    g = np.stack(np.unravel_index(np.arange(n), (m, m, m)), axis=1).astype(np.float64)
+    return g + np.round(rng.random((n, 3)) * 64.0) / 1024.0
+
+
+def c4_points(sizes: np.ndarray, seed: int = 4) -> np.ndarray:
+    """All sets' points, concatenated (set s has kind s mod 3)."""
+    rng = np.random.default_rng(seed + 1)
+    out = np.empty((int(sizes.sum()), 3))
+    off = 0
+    for s, n in enumerate(sizes):
+        out[off:off + n] = set_points(s % 3, int(n), rng)
+        off += n
+    return out
+
+
+def _splitmix64(x: np.ndarray) -> np.ndarray:
+    with np.errstate(over="ignore"):
+        z = x + np.uint64(0x9E3779B97F4A7C15)
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        return z ^ (z >> np.uint64(31))
+
+
+def hashed_queries(seed: int, n_set: int, q: np.ndarray):
+    """Host regeneration of the device generator's queries (op, idx) with
+    global indices q of a set of n_set points (libdel3d_workloads.so)."""
+    q = np.asarray(q, dtype=np.uint64)
+    base = np.uint64(seed) * np.uint64(1 << 40) + np.uint64(8) * q
+    idx = np.empty((len(q), 5), dtype=np.int32)
+    with np.errstate(over="ignore"):
+        for k in range(5):
+            h = _splitmix64(base + np.uint64(k)) >> np.uint64(32)
+            idx[:, k] = ((h * np.uint64(n_set)) >> np.uint64(32)).astype(np.int32)
+        op = (_splitmix64(base + np.uint64(5)) & np.uint64(1)).astype(np.uint8)
+    return op, idx

@@ -155,6 +155,50 @@ int del3d_pred_plan_get_knobs(del3d_pred_plan plan, int32_t* knobs);
 int del3d_pred_plan_destroy(del3d_pred_plan plan);
 
 /*
+ * Serving loop (no counterpart in the reference): continuous batching of
+ * predicate jobs on resident point sets, under an HBM budget.
+ *
+ *   del3d_server_create      budget in bytes for resident sets plus batch
+ *                            buffers; n_slots batches in flight at once (1..8,
+ *                            each with its own stream); at most
+ *                            max_batch_queries (<= 2^26) queries per batch and
+ *                            per job. Starts the scheduler thread.
+ *   del3d_server_put_points  uploads a point set (host array of 3*n_pts
+ *                            doubles) and keeps it resident. The host array
+ *                            must stay valid until the set is dropped: a set
+ *                            no running batch uses may be evicted (least
+ *                            recently used first) to admit other work and is
+ *                            uploaded again from it when a job needs it.
+ *   del3d_server_drop_points forgets a set (it must have no pending job).
+ *   del3d_server_submit      queues a job: n_q queries (op, idx as in
+ *                            del3d_predicates, host arrays, indices local to
+ *                            the set); signs are written to the host array
+ *                            sign_out. The arrays must stay valid until
+ *                            del3d_server_wait returns for the job. Returns at
+ *                            once; jobs that arrive while batches run are
+ *                            admitted into the next free slot.
+ *   del3d_server_wait        blocks until the job finished; latency_ms (may be
+ *                            NULL) receives submit-to-completion time. Returns
+ *                            the job's status (DEL3D_ERR_NOMEM if its set or
+ *                            batch could not fit the budget).
+ *   del3d_server_info        info[8]: device bytes used, resident set bytes,
+ *                            resident sets, evictions, uploads, batches, jobs
+ *                            finished, budget.
+ *   del3d_server_destroy     lets queued jobs finish, then frees everything.
+ * Each job's signs are exactly those of del3d_predicates on its set alone.
+ */
+typedef struct del3d_server_s* del3d_server;
+int del3d_server_create(int64_t hbm_budget_bytes, int32_t n_slots, int64_t max_batch_queries,
+                        del3d_server* server_out);
+int del3d_server_put_points(del3d_server server, const double* xyz, int64_t n_pts, int64_t* set_id);
+int del3d_server_drop_points(del3d_server server, int64_t set_id);
+int del3d_server_submit(del3d_server server, int64_t set_id, const uint8_t* op, const int32_t* idx,
+                        int64_t n_q, int8_t* sign_out, int64_t* job_id);
+int del3d_server_wait(del3d_server server, int64_t job_id, double* latency_ms);
+int del3d_server_info(del3d_server server, int64_t* info);
+int del3d_server_destroy(del3d_server server);
+
+/*
  * Spatial sort of device-resident data. Replaces
  *   void spatial_sort(const std::vector<Point3>& points, std::vector<int>& order)
  *   include/del3d/spatial_sort.h:37 (src/spatial_sort.cpp:141-151)

@@ -110,6 +110,85 @@ static int check_sets() {
     std::printf("pred sets %s\n", ok ? "ok" : "MISMATCH");
     return ok ? 0 : 1;
 }
+// The serving loop: jobs from two client threads over four sets whose sizes
+// exceed the HBM budget together (so idle sets are evicted and uploaded
+// again), each job's signs against the integer oracle; a set larger than the
+// budget is refused.
+#include <thread>
+#include "../../include/del3d_amd.h"
+static int check_server() {
+    const int64_t sizes[4] = {50, 300, 6000, 3000};
+    std::vector<std::vector<double>> pts(4);
+    for (int s = 0; s < 4; ++s) {
+        pts[s].resize(3 * (size_t)sizes[s]);
+        for (auto& c : pts[s]) c = (double)(int)(rnd() % 7) * 0.5;
+    }
+    del3d_server sv = nullptr;
+    if (del3d_server_create(430000, 2, 4096, &sv) != 0) { std::printf("server create failed\n"); return 1; }
+    int64_t ids[4];
+    for (int s = 0; s < 4; ++s)
+        if (del3d_server_put_points(sv, pts[s].data(), sizes[s], &ids[s]) != 0) {
+            std::printf("put %d failed: %s\n", s, del3d_last_error());
+            return 1;
+        }
+    std::vector<double> huge(3 * 20000, 1.0);
+    int64_t hid = 0;
+    int fails = 0;
+    if (del3d_server_put_points(sv, huge.data(), 20000, &hid) != DEL3D_ERR_NOMEM) {
+        std::printf("oversized set admitted\n");
+        ++fails;
+    }
+    const int kJobs = 24;
+    struct J { int set; std::vector<uint8_t> op; std::vector<int32_t> idx; std::vector<int8_t> out, want; int64_t id; };
+    std::vector<J> jobs(kJobs);
+    for (int k = 0; k < kJobs; ++k) {
+        J& j = jobs[k];
+        j.set = (int)(rnd() % 4);
+        const int64_t nq = 1 + (int64_t)(rnd() % 3000);
+        j.op.resize(nq);
+        j.idx.resize(5 * nq);
+        for (int64_t q = 0; q < nq; ++q) {
+            j.op[q] = (uint8_t)(rnd() % 3);
+            for (int c = 0; c < 5; ++c) j.idx[5 * q + c] = (int32_t)(rnd() % sizes[j.set]);
+        }
+        j.out.assign(nq, 9);
+        j.want.resize(nq);
+        orc_int_pred_batch(pts[j.set].data(), sizes[j.set], j.op.data(), j.idx.data(), nq, j.want.data());
+    }
+    auto client = [&](int first) {
+        for (int k = first; k < kJobs; k += 2) {
+            J& j = jobs[k];
+            if (del3d_server_submit(sv, ids[j.set], j.op.data(), j.idx.data(), (int64_t)j.op.size(),
+                                    j.out.data(), &j.id) != 0) j.id = -1;
+        }
+        for (int k = first; k < kJobs; k += 2) {
+            double ms = 0;
+            if (jobs[k].id < 0 || del3d_server_wait(sv, jobs[k].id, &ms) != 0) {
+                std::printf("server job %d failed: %s\n", k, del3d_last_error());
+                jobs[k].out.assign(1, 9);
+            }
+        }
+    };
+    std::thread t0(client, 0), t1(client, 1);
+    t0.join();
+    t1.join();
+    for (int k = 0; k < kJobs; ++k)
+        if (jobs[k].out != jobs[k].want) {
+            std::printf("server job %d (set %d) MISMATCH\n", k, jobs[k].set);
+            ++fails;
+        }
+    int64_t info[8];
+    del3d_server_info(sv, info);
+    std::printf("server jobs %d batches %lld evictions %lld uploads %lld used %lld %s\n", kJobs,
+                (long long)info[5], (long long)info[3], (long long)info[4], (long long)info[0],
+                fails ? "MISMATCH" : "ok");
+    if (info[3] == 0) {
+        std::printf("server: no eviction happened\n");
+        ++fails;
+    }
+    del3d_server_destroy(sv);
+    return fails;
+}
 extern "C" int emu_spatial_sort_sets(const double* xyz, const int64_t* off, int64_t n_sets,
                                      int32_t* order);
 
@@ -135,6 +214,7 @@ int main(int argc, char** argv) {
         fails += check_paths("lattice", lat, 5000);
         fails += check_paths("wide", wide, 2500);
         fails += check_sets();
+        if (!std::getenv("EMU_DEFER") || std::atoi(std::getenv("EMU_DEFER")) == 0) fails += check_server();
         if (std::getenv("EMU_PRED_ONLY")) return fails ? 1 : 0;
     }
     for (int a = 1; a < argc; ++a) {

@@ -24,7 +24,7 @@ EXE = Path("/tmp") / f"del3d_emu_sort_{os.getuid()}"
 def emu_exe():
     csrc = HERE.parent / "del3d_amd" / "csrc"
     subprocess.run(["python", str(HERE / "emu" / "build_emu.py"), str(EXE),
-                    str(csrc / "capi.hip"), str(csrc / "predicates.hip"), str(csrc / "batch.hip"),
+                    str(csrc / "capi.hip"), str(csrc / "predicates.hip"), str(csrc / "batch.hip"), str(csrc / "server.hip"),
                     str(csrc / "spatial_sort.hip")], check=True)
     return EXE
 
