@@ -31,6 +31,8 @@ def _lib() -> C.CDLL:
         h.hipGetErrorString.restype = C.c_char_p
         h.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
         h.hipFree.argtypes = [C.c_void_p]
+        h.hipHostMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t, C.c_uint]
+        h.hipHostFree.argtypes = [C.c_void_p]
         h.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
         h.hipMemcpyAsync.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int, C.c_void_p]
         h.hipMemset.argtypes = [C.c_void_p, C.c_int, C.c_size_t]
@@ -130,6 +132,46 @@ class DeviceArray:
             self.free()
         except Exception:
             pass
+
+
+class _PinnedBlock:
+    """Owns one hipHostMalloc block; freed when the last array using it goes."""
+
+    def __init__(self, nbytes: int):
+        self.ptr = C.c_void_p(0)
+        check(_lib().hipHostMalloc(C.byref(self.ptr), max(1, nbytes), 0), "hipHostMalloc")
+
+    def __del__(self):
+        try:
+            if self.ptr.value:
+                _lib().hipHostFree(self.ptr)
+        except Exception:
+            pass
+
+
+class PinnedArray(np.ndarray):
+    """numpy array over pinned host memory; views keep their parent (and so
+    the pinned block) alive through numpy's base chain."""
+
+
+def pinned_empty(shape, dtype) -> np.ndarray:
+    """A numpy array in pinned (page-locked) host memory: copies between it
+    and the device run as asynchronous DMA without staging."""
+    dtype = np.dtype(dtype)
+    shape = tuple(int(s) for s in (shape if isinstance(shape, (tuple, list)) else (shape,)))
+    nbytes = int(np.prod(shape, dtype=np.int64)) * dtype.itemsize
+    blk = _PinnedBlock(nbytes)
+    buf = (C.c_char * max(1, nbytes)).from_address(blk.ptr.value)
+    arr = np.frombuffer(buf, dtype=np.uint8, count=nbytes).view(PinnedArray)
+    arr._pinned_block = blk
+    return arr.view(dtype).reshape(shape) if nbytes else arr[:0].view(dtype).reshape(shape)
+
+
+def pinned_copy(a) -> np.ndarray:
+    a = np.ascontiguousarray(a)
+    out = pinned_empty(a.shape, a.dtype)
+    out[...] = a
+    return out
 
 
 class Stream:

@@ -2,10 +2,10 @@
 //
 // Clients register point sets (uploaded once, kept resident in HBM) and
 // submit jobs: a batch of queries on one resident set, with host buffers.
-// A scheduler thread admits queued jobs into batches and launches each batch
-// into a free slot; each slot has its own stream, pinned staging and device
-// arenas, so the copies of one batch overlap the kernels of another, and jobs
-// that arrive while batches run are admitted as soon as a slot frees up. A
+// A scheduler thread admits queued jobs into batches and hands each batch to
+// a free slot; each slot has its own thread, stream and device arenas, so the
+// copies of one batch overlap the kernels and copies of others, and jobs that
+// arrive while batches run are admitted as soon as a slot frees up. A
 // batch is one launch sequence of the batched predicate kernels over its jobs,
 // each job a set of the batch (per-set point pointers), so every job's signs
 // are exactly those of del3d_predicates on its set alone.
@@ -72,8 +72,6 @@ struct Slot {
     int64_t* d_qoff = nullptr;
     PredTask* d_tasks = nullptr;
     // pinned host staging
-    uint8_t* h_op = nullptr;
-    int32_t* h_idx = nullptr;
     int8_t* h_out = nullptr;
     const double** h_ptr = nullptr;
     int64_t* h_np = nullptr;
@@ -82,8 +80,12 @@ struct Slot {
     int64_t cap_q = 0, cap_jobs = 0, cap_tasks = 0;
     int64_t bytes = 0;  // device bytes held
     bool busy = false;
+    int64_t q_run = 0, n_tasks = 0, n_lds = 0;  // the batch in flight
     std::vector<std::shared_ptr<Job>> jobs;
-    std::thread waiter;
+    // the slot's worker thread runs each batch handed to it
+    std::thread worker;
+    std::condition_variable cv;
+    bool has_work = false, quit = false;
 };
 
 }  // namespace
@@ -114,11 +116,9 @@ namespace {
 // Frees every buffer of a slot (device and pinned) and returns its bytes.
 void slot_release(del3d_server_s* sv, Slot& sl) {
     for (void* p : {(void*)sl.d_op, (void*)sl.d_idx, (void*)sl.d_out, (void*)sl.d_queue,
-                    (void*)sl.d_wide, (void*)sl.d_ctr, (void*)sl.d_ptr, (void*)sl.d_np,
-                    (void*)sl.d_qoff, (void*)sl.d_tasks})
+                    (void*)sl.d_wide, (void*)sl.d_ctr, (void*)sl.d_ptr})
         if (p) (void)hipFree(p);
-    for (void* p : {(void*)sl.h_op, (void*)sl.h_idx, (void*)sl.h_out, (void*)sl.h_ptr,
-                    (void*)sl.h_np, (void*)sl.h_qoff, (void*)sl.h_tasks})
+    for (void* p : {(void*)sl.h_out, (void*)sl.h_ptr})
         if (p) (void)hipHostFree(p);
     sl.d_op = nullptr;
     sl.d_idx = nullptr;
@@ -128,8 +128,6 @@ void slot_release(del3d_server_s* sv, Slot& sl) {
     sl.d_ptr = nullptr;
     sl.d_np = sl.d_qoff = nullptr;
     sl.d_tasks = nullptr;
-    sl.h_op = nullptr;
-    sl.h_idx = nullptr;
     sl.h_out = nullptr;
     sl.h_ptr = nullptr;
     sl.h_np = sl.h_qoff = nullptr;
@@ -139,16 +137,18 @@ void slot_release(del3d_server_s* sv, Slot& sl) {
     sl.cap_q = sl.cap_jobs = sl.cap_tasks = 0;
 }
 
+size_t table_bytes(int64_t j, int64_t t) { return (size_t)(24 * j + 8 + t * (int64_t)sizeof(PredTask)); }
+
 // Device bytes of a slot sized for q queries, j jobs and t tasks.
 int64_t slot_bytes(int64_t q, int64_t j, int64_t t) {
-    return q * (1 + 20 + 1 + 4 + 4) + (int64_t)sizeof(PredCounters) + j * (8 + 8 + 8) + 8 +
-           t * (int64_t)sizeof(PredTask);
+    return q * (1 + 20 + 1 + 4 + 4) + (int64_t)sizeof(PredCounters) + (int64_t)table_bytes(j, t);
 }
 
 // Evicts resident sets no running job uses, least recently used first, then
-// frees the buffers of idle slots other than `keep`, until `need` more bytes
-// fit the budget. Returns whether they do. Caller holds sv->mu.
-bool make_room(del3d_server_s* sv, int64_t need, const Slot* keep) {
+// frees the buffers of idle slots (a slot being filled reallocates its own),
+// until `need` more bytes fit the budget. Returns whether they do. Caller
+// holds sv->mu.
+bool make_room(del3d_server_s* sv, int64_t need) {
     while (sv->used + need > sv->budget) {
         int64_t victim = -1;
         uint64_t oldest = ~0ull;
@@ -163,7 +163,7 @@ bool make_room(del3d_server_s* sv, int64_t need, const Slot* keep) {
             // then the buffers of idle slots
             Slot* idle = nullptr;
             for (auto& sl : sv->slots)
-                if (sl.get() != keep && !sl->busy && sl->bytes > 0) idle = sl.get();
+                if (!sl->busy && sl->bytes > 0) idle = sl.get();
             if (!idle) return false;
             slot_release(sv, *idle);
             continue;
@@ -179,10 +179,10 @@ bool make_room(del3d_server_s* sv, int64_t need, const Slot* keep) {
 
 // Makes a registered set resident (uploading it on the slot's stream).
 // Caller holds sv->mu. Throws HipError on failure.
-void ensure_resident(del3d_server_s* sv, ResidentSet& r, hipStream_t s, const Slot* keep) {
+void ensure_resident(del3d_server_s* sv, ResidentSet& r, hipStream_t s) {
     if (r.dev) return;
     const int64_t bytes = r.n * 24;
-    if (!make_room(sv, bytes, keep))
+    if (!make_room(sv, bytes))
         throw HipError{set_error(DEL3D_ERR_NOMEM, "point set of %lld bytes does not fit the HBM budget",
                                  (long long)bytes)};
     D3_HIP(hipMalloc((void**)&r.dev, (size_t)std::max<int64_t>(bytes, 24)));
@@ -199,7 +199,7 @@ void slot_reserve(del3d_server_s* sv, Slot& sl, int64_t q, int64_t j, int64_t t)
     t = std::max(t, sl.cap_tasks);
     const int64_t want = slot_bytes(q, j, t);
     slot_release(sv, sl);
-    if (!make_room(sv, want, &sl))
+    if (!make_room(sv, want))
         throw HipError{set_error(DEL3D_ERR_NOMEM, "batch buffers of %lld bytes do not fit the HBM budget",
                                  (long long)want)};
     sv->used += want;
@@ -210,17 +210,22 @@ void slot_reserve(del3d_server_s* sv, Slot& sl, int64_t q, int64_t j, int64_t t)
     D3_HIP(hipMalloc((void**)&sl.d_queue, (size_t)q * 4));
     D3_HIP(hipMalloc((void**)&sl.d_wide, (size_t)q * 4));
     D3_HIP(hipMalloc((void**)&sl.d_ctr, sizeof(PredCounters)));
-    D3_HIP(hipMalloc((void**)&sl.d_ptr, (size_t)j * 8));
-    D3_HIP(hipMalloc((void**)&sl.d_np, (size_t)j * 8));
-    D3_HIP(hipMalloc((void**)&sl.d_qoff, (size_t)(j + 1) * 8));
-    D3_HIP(hipMalloc((void**)&sl.d_tasks, (size_t)t * sizeof(PredTask)));
-    D3_HIP(hipHostMalloc((void**)&sl.h_op, (size_t)q, hipHostMallocDefault));
-    D3_HIP(hipHostMalloc((void**)&sl.h_idx, (size_t)q * 20, hipHostMallocDefault));
+    // the batch tables live in one block (one copy per batch):
+    // set pointers [j], point counts [j], query offsets [j+1], tasks [t]
+    const size_t tab = table_bytes(j, t);
+    char* dt = nullptr;
+    D3_HIP(hipMalloc((void**)&dt, tab));
+    sl.d_ptr = reinterpret_cast<const double**>(dt);
+    sl.d_np = reinterpret_cast<int64_t*>(dt + 8 * j);
+    sl.d_qoff = reinterpret_cast<int64_t*>(dt + 16 * j);
+    sl.d_tasks = reinterpret_cast<PredTask*>(dt + 16 * j + 8 * (j + 1));
     D3_HIP(hipHostMalloc((void**)&sl.h_out, (size_t)q, hipHostMallocDefault));
-    D3_HIP(hipHostMalloc((void**)&sl.h_ptr, (size_t)j * 8, hipHostMallocDefault));
-    D3_HIP(hipHostMalloc((void**)&sl.h_np, (size_t)j * 8, hipHostMallocDefault));
-    D3_HIP(hipHostMalloc((void**)&sl.h_qoff, (size_t)(j + 1) * 8, hipHostMallocDefault));
-    D3_HIP(hipHostMalloc((void**)&sl.h_tasks, (size_t)t * sizeof(PredTask), hipHostMallocDefault));
+    char* ht = nullptr;
+    D3_HIP(hipHostMalloc((void**)&ht, tab, hipHostMallocDefault));
+    sl.h_ptr = reinterpret_cast<const double**>(ht);
+    sl.h_np = reinterpret_cast<int64_t*>(ht + 8 * j);
+    sl.h_qoff = reinterpret_cast<int64_t*>(ht + 16 * j);
+    sl.h_tasks = reinterpret_cast<PredTask*>(ht + 16 * j + 8 * (j + 1));
     sl.cap_q = q;
     sl.cap_jobs = j;
     sl.cap_tasks = t;
@@ -249,17 +254,57 @@ bool other_slot_busy(del3d_server_s* sv, const Slot* me) {
     return false;
 }
 
-// Waits for a slot's batch, hands the signs to the jobs, frees the slot.
-void slot_wait(del3d_server_s* sv, Slot* sl) {
-    hipError_t e = hipEventSynchronize(sl->done);
+// Runs a slot's batch: copies the jobs' inputs from the clients' host arrays
+// (the runtime stages pageable memory itself), enqueues the kernels and the
+// copy of the signs into pinned memory, waits, hands each job its signs.
+void slot_run(del3d_server_s* sv, Slot* sl) {
+    const int64_t q_run = sl->q_run, t = sl->n_tasks, n_lds = sl->n_lds;
+    const size_t nj = sl->jobs.size();
+    int rc = guarded([&] {
+        hipStream_t s = sl->s;
+        int64_t off = 0;
+        for (auto& j : sl->jobs) {
+            D3_HIP(hipMemcpyAsync(sl->d_op + off, j->op, (size_t)j->n_q, hipMemcpyHostToDevice, s));
+            D3_HIP(hipMemcpyAsync(sl->d_idx + 5 * off, j->idx, (size_t)j->n_q * 20,
+                                  hipMemcpyHostToDevice, s));
+            off += j->n_q;
+        }
+        // the tables were written for cap_jobs slots: copy the block up to the tasks used
+        const size_t tab_used = reinterpret_cast<char*>(sl->h_tasks + t) - reinterpret_cast<char*>(sl->h_ptr);
+        D3_HIP(hipMemcpyAsync((void*)sl->d_ptr, sl->h_ptr, tab_used, hipMemcpyHostToDevice, s));
+        BatchArgs a{};
+        a.set_ptr = sl->d_ptr;
+        a.set_np = sl->d_np;
+        a.q_off = sl->d_qoff;
+        a.n_sets = (uint32_t)nj;
+        a.op = sl->d_op;
+        a.idx = sl->d_idx;
+        a.out = sl->d_out;
+        a.n_q = q_run;
+        a.queue = sl->d_queue;
+        a.wide = sl->d_wide;
+        a.ctr = sl->d_ctr;
+        BatchShape sh;
+        sh.lds_tasks = sl->d_tasks;
+        sh.n_lds_tasks = (uint32_t)n_lds;
+        sh.global_tasks = sl->d_tasks + n_lds;
+        sh.n_global_tasks = (uint32_t)(t - n_lds);
+        enqueue_batch(a, sh, s, nullptr);
+        D3_HIP(hipMemcpyAsync(sl->h_out, sl->d_out, (size_t)q_run, hipMemcpyDeviceToHost, s));
+        D3_HIP(hipEventRecord(sl->done, s));
+        D3_HIP(hipEventSynchronize(sl->done));
+        return DEL3D_OK;
+    });
+    std::string err = rc ? last_error_slot() : std::string();
+    if (rc) (void)hipStreamSynchronize(sl->s);  // let whatever was enqueued drain
     std::lock_guard<std::mutex> lock(sv->mu);
     int64_t off = 0;
     for (auto& j : sl->jobs) {
-        if (e == hipSuccess) {
+        if (rc == DEL3D_OK) {
             std::memcpy(j->out, sl->h_out + off, (size_t)j->n_q);
             finish_job(sv, j, DEL3D_OK, nullptr);
         } else {
-            finish_job(sv, j, DEL3D_ERR_DEVICE, hipGetErrorString(e));
+            finish_job(sv, j, rc, err.c_str());
         }
         off += j->n_q;
     }
@@ -305,7 +350,7 @@ bool launch_batch(del3d_server_s* sv, Slot& sl, std::unique_lock<std::mutex>& lo
         auto& j = take[k];
         ResidentSet& r = sv->sets[j->set];
         const int rc = guarded([&] {
-            ensure_resident(sv, r, sl.s, &sl);
+            ensure_resident(sv, r, sl.s);
             return DEL3D_OK;
         });
         if (rc == DEL3D_ERR_NOMEM && other_slot_busy(sv, &sl)) {
@@ -327,13 +372,24 @@ bool launch_batch(del3d_server_s* sv, Slot& sl, std::unique_lock<std::mutex>& lo
         sv->cv_done.notify_all();
         return true;
     }
-    const int rc = guarded([&] {
+    int rc = guarded([&] {
         slot_reserve(sv, sl, q_run, (int64_t)run.size(), t_run);
         return DEL3D_OK;
     });
     if (rc == DEL3D_ERR_NOMEM && other_slot_busy(sv, &sl)) {
         defer(run, 0);
         return true;
+    }
+    if (rc == DEL3D_ERR_NOMEM && run.size() > 1) {
+        // nothing else runs: try the first job alone, the others wait
+        defer(run, 1);
+        sv->blocked = false;
+        q_run = run[0]->n_q;
+        t_run = tasks_for(q_run);
+        rc = guarded([&] {
+            slot_reserve(sv, sl, q_run, 1, t_run);
+            return DEL3D_OK;
+        });
     }
     if (rc != DEL3D_OK) {
         for (auto& j : run) finish_job(sv, j, rc, last_error_slot().c_str());
@@ -365,59 +421,28 @@ bool launch_batch(del3d_server_s* sv, Slot& sl, std::unique_lock<std::mutex>& lo
     for (int64_t i = 0; i < t; ++i) sl.h_tasks[i].pad = 0;
     sl.jobs = run;
     sl.busy = true;
+    sl.q_run = q_run;
+    sl.n_tasks = t;
+    sl.n_lds = n_lds;
     ++sv->stat_batches;
-    lock.unlock();  // copying inputs and enqueueing need no lock
-    off = 0;
-    for (auto& j : run) {
-        std::memcpy(sl.h_op + off, j->op, (size_t)j->n_q);
-        std::memcpy(sl.h_idx + 5 * off, j->idx, (size_t)j->n_q * 20);
-        off += j->n_q;
-    }
-    const int rc2 = guarded([&] {
-        hipStream_t s = sl.s;
-        D3_HIP(hipMemcpyAsync(sl.d_op, sl.h_op, (size_t)q_run, hipMemcpyHostToDevice, s));
-        D3_HIP(hipMemcpyAsync(sl.d_idx, sl.h_idx, (size_t)q_run * 20, hipMemcpyHostToDevice, s));
-        D3_HIP(hipMemcpyAsync(sl.d_ptr, sl.h_ptr, run.size() * 8, hipMemcpyHostToDevice, s));
-        D3_HIP(hipMemcpyAsync(sl.d_np, sl.h_np, run.size() * 8, hipMemcpyHostToDevice, s));
-        D3_HIP(hipMemcpyAsync(sl.d_qoff, sl.h_qoff, (run.size() + 1) * 8, hipMemcpyHostToDevice, s));
-        D3_HIP(hipMemcpyAsync(sl.d_tasks, sl.h_tasks, (size_t)t * sizeof(PredTask),
-                              hipMemcpyHostToDevice, s));
-        BatchArgs a{};
-        a.set_ptr = sl.d_ptr;
-        a.set_np = sl.d_np;
-        a.q_off = sl.d_qoff;
-        a.n_sets = (uint32_t)run.size();
-        a.op = sl.d_op;
-        a.idx = sl.d_idx;
-        a.out = sl.d_out;
-        a.n_q = q_run;
-        a.queue = sl.d_queue;
-        a.wide = sl.d_wide;
-        a.ctr = sl.d_ctr;
-        BatchShape sh;
-        sh.lds_tasks = sl.d_tasks;
-        sh.n_lds_tasks = (uint32_t)n_lds;
-        sh.global_tasks = sl.d_tasks + n_lds;
-        sh.n_global_tasks = (uint32_t)(t - n_lds);
-        enqueue_batch(a, sh, s, nullptr);
-        D3_HIP(hipMemcpyAsync(sl.h_out, sl.d_out, (size_t)q_run, hipMemcpyDeviceToHost, s));
-        D3_HIP(hipEventRecord(sl.done, s));
-        return DEL3D_OK;
-    });
-    lock.lock();
-    if (rc2 != DEL3D_OK) {
-        // the stream may hold part of the batch: wait for it, then fail the jobs
-        (void)hipStreamSynchronize(sl.s);
-        for (auto& j : run) finish_job(sv, j, rc2, last_error_slot().c_str());
-        sl.jobs.clear();
-        sl.busy = false;
-        sv->cv_done.notify_all();
-        return true;
-    }
-    if (sl.waiter.joinable()) sl.waiter.join();
-    Slot* slp = &sl;
-    sl.waiter = std::thread([sv, slp] { slot_wait(sv, slp); });
+    // the slot's own thread copies the inputs, runs and completes the batch,
+    // so the batches of different slots move their data concurrently
+    sl.has_work = true;
+    sl.cv.notify_one();
     return true;
+}
+
+void slot_worker(del3d_server_s* sv, Slot* sl) {
+    (void)hipSetDevice(sv->dev);
+    for (;;) {
+        {
+            std::unique_lock<std::mutex> lock(sv->mu);
+            sl->cv.wait(lock, [&] { return sl->has_work || sl->quit; });
+            if (!sl->has_work) return;
+            sl->has_work = false;
+        }
+        slot_run(sv, sl);
+    }
 }
 
 void scheduler(del3d_server_s* sv) {
@@ -464,6 +489,7 @@ int del3d_server_create(int64_t hbm_budget_bytes, int32_t n_slots, int64_t max_b
             sv->slots.push_back(std::move(sl));
         }
         del3d_server_s* raw = sv.release();
+        for (auto& sl : raw->slots) sl->worker = std::thread(slot_worker, raw, sl.get());
         raw->sched = std::thread(scheduler, raw);
         *out = raw;
         return DEL3D_OK;
@@ -481,7 +507,7 @@ int del3d_server_put_points(del3d_server sv, const double* xyz, int64_t n_pts, i
         r.n = n_pts;
         r.last_use = ++sv->tick;
         const int rc = guarded([&] {
-            ensure_resident(sv, r, thread_stream(), nullptr);
+            ensure_resident(sv, r, thread_stream());
             return DEL3D_OK;
         });
         if (rc != DEL3D_OK) {
@@ -589,8 +615,15 @@ int del3d_server_destroy(del3d_server sv) {
         sv->cv_sched.notify_all();
     }
     sv->sched.join();
+    {
+        std::lock_guard<std::mutex> lock(sv->mu);
+        for (auto& s : sv->slots) {
+            s->quit = true;
+            s->cv.notify_one();
+        }
+    }
     for (auto& s : sv->slots) {
-        if (s->waiter.joinable()) s->waiter.join();
+        if (s->worker.joinable()) s->worker.join();
         slot_release(sv, *s);
         (void)hipEventDestroy(s->done);
         (void)hipStreamDestroy(s->s);

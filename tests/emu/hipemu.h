@@ -133,10 +133,22 @@ inline std::vector<std::function<void()>>& queue() {
     static std::vector<std::function<void()>> q;
     return q;
 }
+inline std::mutex& queue_mutex() {
+    static std::mutex m;
+    return m;
+}
 inline void drain() {
-    auto q = std::move(queue());
-    queue().clear();
+    std::vector<std::function<void()>> q;
+    {
+        std::lock_guard<std::mutex> lk(queue_mutex());
+        q = std::move(queue());
+        queue().clear();
+    }
     for (auto& f : q) f();
+}
+inline void enqueue(std::function<void()> f) {
+    std::lock_guard<std::mutex> lk(queue_mutex());
+    queue().push_back(std::move(f));
 }
 }  // namespace emu
 
@@ -167,14 +179,14 @@ inline hipError_t hipFree(void* p) {
 inline hipError_t hipMemcpyAsync(void* d, const void* s, size_t n, hipMemcpyKind, hipStream_t) {
     if (!n) return hipSuccess;
     if (emu::defer_mode() && (emu::is_pinned(d) || emu::is_pinned(s)))
-        emu::queue().push_back([=] { std::memcpy(d, s, n); });
+        emu::enqueue([=] { std::memcpy(d, s, n); });
     else
         std::memcpy(d, s, n);
     return hipSuccess;
 }
 inline hipError_t hipMemsetAsync(void* d, int v, size_t n, hipStream_t) {
     if (!n) return hipSuccess;
-    if (emu::defer_mode()) emu::queue().push_back([=] { std::memset(d, v, n); });
+    if (emu::defer_mode()) emu::enqueue([=] { std::memset(d, v, n); });
     else std::memset(d, v, n);
     return hipSuccess;
 }
@@ -230,7 +242,7 @@ void run_launch(dim3 grid, dim3 block, K kernel, A... args);
 template <class K, class... A>
 void launch(dim3 grid, dim3 block, K kernel, A... args) {
     if (defer_mode()) {
-        queue().push_back([=] { run_launch(grid, block, kernel, args...); });
+        enqueue([=] { run_launch(grid, block, kernel, args...); });
         return;
     }
     run_launch(grid, block, kernel, args...);
@@ -291,8 +303,16 @@ inline Pool& pool() {
     return p;
 }
 
+inline std::mutex& launch_mutex() {
+    static std::mutex m;
+    return m;
+}
+
+// Kernels run one at a time, whatever host thread launches them (__shared__
+// variables are statics of the kernel function).
 template <class K, class... A>
 void run_launch(dim3 grid, dim3 block, K kernel, A... args) {
+    std::lock_guard<std::mutex> launch_lock(launch_mutex());
     g_gdim = grid;
     g_bdim = block;
     const unsigned nt = block.x;

@@ -49,8 +49,9 @@ def test_served_jobs_equal_alone():
     sets = _sets(rng)
     jobs = _jobs(rng, sets, 120)
     alone = [predicates_dev(sets[s], op, idx)[0] for s, op, idx in jobs]
-    # 48 MB: the sets (about 9 MB) and three slots of 2^20-query buffers do not fit at once
-    srv = D.Server(48 << 20, slots=3, max_batch_queries=1 << 20)
+    # 16 MB: the sets (about 9 MB) and the buffers of a few in-flight batches
+    # (up to 9 MB each) do not fit at once
+    srv = D.Server(16 << 20, slots=3, max_batch_queries=1 << 19)
     ids = [srv.put_points(p) for p in sets]
     results = [None] * len(jobs)
     lat = [0.0] * len(jobs)
