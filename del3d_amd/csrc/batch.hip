@@ -32,6 +32,7 @@ struct del3d_pred_plan_s {
     uint32_t* queue = nullptr;
     uint32_t* wide = nullptr;
     d3::PredCounters* ctr = nullptr;
+    int32_t* grid = nullptr;  // per set SetGrid summary, recomputed by every run
     uint32_t queue_cap = 0;
     struct Chunk {
         uint32_t lds0, n_lds, glob0, n_glob;  // task ranges in d_tasks
@@ -128,6 +129,12 @@ void build_tasks(del3d_pred_plan_s* p, hipStream_t s) {
 
 void run_plan(del3d_pred_plan_s* p, const double* xyz, const uint8_t* op, const int32_t* idx,
               int8_t* out, hipStream_t s, unsigned long long* stats, hipEvent_t* ev) {
+    {
+        BatchArgs g{};
+        g.xyz = xyz;
+        g.pt_off = p->d_pt_off;
+        enqueue_set_grids(g, (uint32_t)p->n_sets, p->n_pts, p->grid, s);
+    }
     for (const auto& c : p->chunks) {
         BatchArgs a{};
         a.xyz = xyz;
@@ -143,6 +150,7 @@ void run_plan(del3d_pred_plan_s* p, const double* xyz, const uint8_t* op, const 
         a.wide = p->wide;
         a.ctr = p->ctr;
         a.stats = stats;
+        a.grid = p->grid;
         BatchShape sh;
         sh.lds_tasks = p->d_tasks + c.lds0;
         sh.n_lds_tasks = c.n_lds;
@@ -221,6 +229,7 @@ int del3d_pred_plan_create(const int64_t* pt_off, const int64_t* q_off, int64_t 
         D3_HIP(hipMalloc((void**)&p->queue, (size_t)p->queue_cap * 4));
         D3_HIP(hipMalloc((void**)&p->wide, (size_t)p->queue_cap * 4));
         D3_HIP(hipMalloc((void**)&p->ctr, sizeof(PredCounters)));
+        D3_HIP(hipMalloc((void**)&p->grid, (size_t)std::max<int64_t>(n_sets, 1) * 8));
         build_tasks(p, s);
         guard.p = nullptr;
         *plan_out = p;
@@ -364,6 +373,7 @@ int del3d_pred_plan_destroy(del3d_pred_plan p) {
     (void)hipFree(p->queue);
     (void)hipFree(p->wide);
     (void)hipFree(p->ctr);
+    (void)hipFree(p->grid);
     delete p;
     return DEL3D_OK;
 }

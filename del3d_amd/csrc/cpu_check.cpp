@@ -86,3 +86,14 @@ extern "C" void predcore_host_batch(const double* xyz, int64_t n_pts, const uint
         if (tier) tier[q] = (int8_t)t;
     }
 }
+
+// The two T1 certificates of a query's points (np = 4 or 5 points, 3 doubles
+// each) against a set summary: out[0] = per-coordinate test, out[1] = set test.
+extern "C" void predcore_host_t1(const double* pts, int np, int32_t gmin, int32_t mexp, int* out) {
+    P3 p[5];
+    for (int k = 0; k < np; ++k) p[k] = P3{pts[3 * k], pts[3 * k + 1], pts[3 * k + 2]};
+    const int k = np == 5 ? 8 : 15, qmin = np == 5 ? -214 : -358;
+    const double md = max_diff(p, np);
+    out[0] = t1_certify(p, np, md, k, qmin) ? 1 : 0;
+    out[1] = t1_certify_set(md, k, qmin, SetGrid{gmin, mexp}) ? 1 : 0;
+}

@@ -165,31 +165,61 @@ D3_HD bool filter_decides(const Filt& f, double bound_const) {
 // in units of 2^(deg*q), hence exactly representable, so the FP64 determinant
 // is exact. q is bounded so that 2^(deg*q) stays at or above 2^-1074
 // (representable) and the values stay far below overflow.
+// The tests below combine conditions with & and | rather than && and ||:
+// every term is cheap, and branch-free code keeps a wavefront's lanes together.
 D3_HD bool on_grid(double c, double inv_unit) {
     const double t = c * inv_unit;  // exact scaling by a power of two unless it underflows
     // an underflowed or fractional t fails the integer test; t == 0 with c != 0
     // means c underflowed entirely, i.e. it is not on the grid
-    return t == trunc(t) && fabs(t) < 4503599627370496.0 && (t != 0.0 || c == 0.0);
+    return (t == trunc(t)) & (fabs(t) < 4503599627370496.0) & ((t != 0.0) | (c == 0.0));
+}
+
+// ilogb for the magnitudes T1 sees: the unbiased exponent of a normal
+// positive double; zero and subnormals give -2000 (below any accepted q).
+D3_HD int exponent_of(double v) {
+    const int f = (int)((__builtin_bit_cast(uint64_t, v) >> 52) & 0x7FF);
+    return f == 0 ? -2000 : f - 1023;
+}
+
+// 2^e for -1022 <= e <= 1023, exactly.
+D3_HD double pow2i(int e) { return __builtin_bit_cast(double, (uint64_t)(e + 1023) << 52); }
+
+// The same certificate from a summary of the whole point set: if every
+// nonzero coordinate of the set is a multiple of 2^gmin and below 2^(mexp+1)
+// in magnitude, then for q <= gmin every coordinate is a multiple of 2^q, and
+// mexp <= q + 51 gives |c| < 2^(52+q). One comparison chain per query instead
+// of a test per coordinate; when it fails the per-coordinate test still may
+// pass (gmin is the set's coarsest common grid, a query's may be coarser).
+struct SetGrid {
+    int32_t gmin;  // min over nonzero coordinates of the lowest set bit's exponent
+    int32_t mexp;  // max over nonzero coordinates of floor(log2 |c|)
+};
+
+D3_HD bool t1_certify_set(double maxd, int k, int qmin_allowed, SetGrid g) {
+    const int e = exponent_of(maxd);
+    const int q = e - k;
+    return (maxd == 0.0) | ((q >= qmin_allowed) & (e <= 180) & (q <= g.gmin) & (g.mexp <= q + 51));
 }
 
 // deg 5 (insphere): k = 8, q >= -214; deg 3 (orient3d): k = 15, q >= -358
 D3_HD bool t1_certify(const P3* p, int np, double maxd, int k, int qmin_allowed) {
-    if (maxd == 0.0) return true;  // all points identical: every difference is exactly 0
-    const int e = ilogb(maxd);
+    const int e = exponent_of(maxd);
     const int q = e - k;
-    if (q < qmin_allowed || e > 180) return false;
-    const double inv = ldexp(1.0, -q);
-    bool ok = true;
-    for (int i = 0; i < np; ++i) ok = ok && on_grid(p[i].x, inv) && on_grid(p[i].y, inv) && on_grid(p[i].z, inv);
-    return ok;
+    const bool in_range = (q >= qmin_allowed) & (e <= 180);
+    const double inv = pow2i(in_range ? -q : 0);  // -q in [k-180, -qmin]: representable
+    bool ok = in_range;
+    for (int i = 0; i < np; ++i)
+        ok = ok & on_grid(p[i].x, inv) & on_grid(p[i].y, inv) & on_grid(p[i].z, inv);
+    // maxd == 0: all points identical, every difference is exactly 0
+    return (maxd == 0.0) | ok;
 }
 
 // ---- the perturbation walk (src/predicates.cpp:211-231) ---------------------
 
+// (x, y, z) lexicographic <, with IEEE comparisons (so -0.0 == +0.0), as
+// src/predicates.cpp:117-121 (for finite coordinates).
 D3_HD bool lex_less(const P3& p, const P3& q) {
-    if (p.x != q.x) return p.x < q.x;
-    if (p.y != q.y) return p.y < q.y;
-    return p.z < q.z;
+    return (p.x < q.x) | ((p.x == q.x) & ((p.y < q.y) | ((p.y == q.y) & (p.z < q.z))));
 }
 
 // Slots 0..4 are (a, b, c, d, e). The walk needs the slots at sorted ranks 4
@@ -226,18 +256,28 @@ D3_HD int perturbation_walk(const P3 p[5], const int32_t id[5], F&& orient_subst
         const int sl = it == 0 ? s4 : s3;
         const int32_t v = sl == 0 ? id[0] : sl == 1 ? id[1] : sl == 2 ? id[2] : sl == 3 ? id[3] : id[4];
         if (v == id[4]) return -1;
-        int o;
-        if (v == id[3] && (o = orient_subst(3)) != 0) return o;
-        if (v == id[2] && (o = orient_subst(2)) != 0) return o;
-        if (v == id[1] && (o = orient_subst(1)) != 0) return o;
-        if (v == id[0] && (o = orient_subst(0)) != 0) return o;
+        // slots d, c, b, a in that order; with distinct indices exactly one
+        // matches, so the common case evaluates one substitution, in one
+        // place for every lane; aliased slots continue in order below
+        const int k = v == id[3] ? 3 : v == id[2] ? 2 : v == id[1] ? 1 : v == id[0] ? 0 : -1;
+        if (k >= 0) {
+            int o = orient_subst(k);
+            if (o != 0) return o;
+            for (int k2 = k - 1; k2 >= 0; --k2) {
+                const int32_t w = k2 == 2 ? id[2] : k2 == 1 ? id[1] : id[0];
+                if (v == w && (o = orient_subst(k2)) != 0) return o;
+            }
+        }
     }
     return -1;
 }
 
 D3_HD void subst_tet(const P3 p[5], int k, P3 t[4]) {
 #pragma unroll
-    for (int i = 0; i < 4; ++i) t[i] = (i == k) ? p[4] : p[i];
+    for (int i = 0; i < 4; ++i) {
+        const bool sub = i == k;
+        t[i] = P3{sub ? p[4].x : p[i].x, sub ? p[4].y : p[i].y, sub ? p[4].z : p[i].z};
+    }
 }
 
 // ---- exact integer arithmetic: two's complement, 32-bit limbs --------------

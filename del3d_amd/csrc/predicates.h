@@ -38,6 +38,7 @@ struct BatchArgs {
     // when set_ptr != nullptr, in place of xyz + pt_off
     const double* const* set_ptr;
     const int64_t* set_np;
+    const int64_t* pt_first;  // with set_ptr: n_sets+1 prefix of set_np (for the grid prepass)
     const uint8_t* op;
     const int32_t* idx;
     int8_t* out;
@@ -49,6 +50,7 @@ struct BatchArgs {
     uint32_t* wide;          // wide-tier queue (same capacity)
     PredCounters* ctr;
     unsigned long long* stats;  // 6 counters or nullptr
+    const int32_t* grid;        // per set {gmin, mexp} (SetGrid), or nullptr
 };
 
 // How a batch is split between the two fast kernels.
@@ -65,6 +67,11 @@ struct ScalarQuery {
     uint8_t op;
     int8_t sign;
 };
+
+// Writes the SetGrid summary of every set of `a` into grid (2 int32 per set,
+// n_sets sets; one set when a.pt_off and a.set_ptr are null), on s.
+void enqueue_set_grids(const BatchArgs& a, uint32_t n_sets, int64_t total_pts, int32_t* grid,
+                       hipStream_t s);
 
 // Enqueues the whole pipeline of one batch on s (no host synchronization;
 // capturable into a graph). ev (6 events) brackets fast / exact / wide.

@@ -83,8 +83,8 @@ struct LdsPts {
 // kReload: fetch the points again for T1 instead of keeping them live across
 // the filter (cheap from LDS; it lowers register pressure on the common path).
 template <bool kReload, class Fetch>
-__device__ __forceinline__ int fast_decide(const Fetch& F, uint8_t o, const int32_t id[5], int& tier,
-                                           bool& perturbed) {
+__device__ __forceinline__ int fast_decide(const Fetch& F, uint8_t o, const int32_t id[5], SetGrid sg,
+                                           int& tier, bool& perturbed) {
     const bool ins = o != DEL3D_OP_ORIENT3D;
     P3 p[5];
     p[0] = F(id[0]);
@@ -114,12 +114,15 @@ __device__ __forceinline__ int fast_decide(const Fetch& F, uint8_t o, const int3
         for (int k = 0; k < 4; ++k) p[k] = F(id[k]);
         if (ins) p[4] = F(id[4]);
     }
+    // T1: the set-level grid summary first, the per-coordinate test if it fails
     if (!ins) {
-        if (!t1_certify(p, 4, max_diff(p, 4), 15, -358)) return 2;
+        const double md = max_diff(p, 4);
+        if (!t1_certify_set(md, 15, -358, sg) && !t1_certify(p, 4, md, 15, -358)) return 2;
         tier = kTierCert;
         return -sgn(det);  // certified: the filter's determinant is exact
     }
-    if (!t1_certify(p, 5, max_diff(p, 5), 8, -214)) return 2;
+    const double md = max_diff(p, 5);
+    if (!t1_certify_set(md, 8, -214, sg) && !t1_certify(p, 5, md, 8, -214)) return 2;
     tier = kTierCert;
     int sign = -sgn(det);
     if (sign == 0 && o == DEL3D_OP_IN_SPHERE) {
@@ -209,6 +212,12 @@ __device__ __forceinline__ void pred_batch_body(const BatchArgs& a, BatchLds<kSt
         // points addressable by this task's indices (the LDS copy holds at most kLdsPts)
         const int64_t np = kLds ? min(uniform64(s_task[3]), (int64_t)kLdsPts) : uniform64(s_task[3]);
         const double* setpts = set_points(a, set);
+        // no summary: an impossible grid, so only the per-coordinate test runs
+        SetGrid sg{-100000, 100000};
+        if (a.grid) {
+            sg.gmin = __builtin_amdgcn_readfirstlane(a.grid[2 * set]);
+            sg.mexp = __builtin_amdgcn_readfirstlane(a.grid[2 * set + 1]);
+        }
         if (kLds && set != loaded_set) {
             // np <= kLdsPts is guaranteed by the host, which routes larger sets
             // to the global kernel
@@ -277,8 +286,8 @@ __device__ __forceinline__ void pred_batch_body(const BatchArgs& a, BatchLds<kSt
             int sign = 0, tier = kTierFilter;
             bool perturbed = false;
             if (act && valid) {
-                if (kLds) sign = fast_decide<true>(LdsPts{s_pts}, o, id, tier, perturbed);
-                else sign = fast_decide<false>(GlobalPts{setpts}, o, id, tier, perturbed);
+                if (kLds) sign = fast_decide<true>(LdsPts{s_pts}, o, id, sg, tier, perturbed);
+                else sign = fast_decide<false>(GlobalPts{setpts}, o, id, sg, tier, perturbed);
             }
             const bool queued = act && valid && sign == 2;
             if (act && !queued) a.out[q] = (int8_t)sign;
@@ -430,6 +439,76 @@ __global__ __launch_bounds__(kWideBlock) void pred_wide(BatchArgs a) {
 // Host launchers
 // ---------------------------------------------------------------------------
 
+// ---------------------------------------------------------------------------
+// Set-level grid summaries (SetGrid) for the T1 certificate
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_grid_init(int32_t* grid, uint32_t n_sets) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n_sets; i += gridDim.x * blockDim.x) {
+        grid[2 * i] = 0x7FFFFFFF;      // min lowest-bit exponent
+        grid[2 * i + 1] = -0x7FFFFFFF;  // max magnitude exponent
+    }
+}
+
+// One thread per point. Points of one set are contiguous in the global
+// numbering (set s owns [first point of s, first point of s+1)), so a
+// wavefront usually covers one set and reduces before its atomics.
+__global__ __launch_bounds__(256) void k_grid_points(BatchArgs a, uint32_t n_sets, int64_t total,
+                                                     const int64_t* __restrict__ first,
+                                                     int32_t* __restrict__ grid) {
+    for (int64_t g0 = (int64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63); g0 < total;
+         g0 += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t g = g0 + __lane_id();
+        int32_t lo = 0x7FFFFFFF, hi = -0x7FFFFFFF;
+        uint32_t set = 0;
+        if (g < total) {
+            if (first) {
+                uint32_t l = 0, h = n_sets;  // last s with first[s] <= g
+                while (h - l > 1) {
+                    const uint32_t mid = (l + h) >> 1;
+                    if (first[mid] <= g) l = mid;
+                    else h = mid;
+                }
+                set = l;
+            }
+            const double* pts = set_points(a, (int32_t)set);
+            const int64_t local = first ? g - first[set] : g;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const Bits d = decompose(pts[3 * local + c]);
+                if (d.m) {
+                    lo = min(lo, d.e + __builtin_ctzll(d.m));
+                    hi = max(hi, d.e + 63 - __builtin_clzll(d.m));
+                }
+            }
+        }
+        const uint32_t set0 = __shfl(set, 0);
+        if (__ballot(g < total && set != set0) == 0) {
+            for (int off = 32; off > 0; off >>= 1) {
+                lo = min(lo, __shfl_xor(lo, off));
+                hi = max(hi, __shfl_xor(hi, off));
+            }
+            if (__lane_id() == 0 && g0 < total) {
+                atomicMin(&grid[2 * set0], lo);
+                atomicMax(&grid[2 * set0 + 1], hi);
+            }
+        } else if (g < total) {
+            atomicMin(&grid[2 * set], lo);
+            atomicMax(&grid[2 * set + 1], hi);
+        }
+    }
+}
+
+void enqueue_set_grids(const BatchArgs& a, uint32_t n_sets, int64_t total_pts, int32_t* grid,
+                       hipStream_t s) {
+    k_grid_init<<<(n_sets + 255) / 256, 256, 0, s>>>(grid, n_sets);
+    D3_LAUNCH_CHECK();
+    if (total_pts <= 0) return;
+    const int cus = device_cu_count();
+    const unsigned g = (unsigned)std::min<int64_t>((total_pts + 255) / 256, (int64_t)cus * 16);
+    k_grid_points<<<g, 256, 0, s>>>(a, n_sets, total_pts, a.set_ptr ? a.pt_first : a.pt_off, grid);
+    D3_LAUNCH_CHECK();
+}
+
 // Zeroes the per-chunk counters (a kernel, so the whole sequence can be
 // captured into a graph).
 __global__ void k_clear_counters(PredCounters* c) {
@@ -489,8 +568,15 @@ void launch_predicates(const double* xyz, int64_t n_pts, const uint8_t* op, cons
     uint32_t* queue = sc.get<uint32_t>(chunk);
     uint32_t* wide = sc.get<uint32_t>(chunk);
     PredCounters* ctr = sc.get<PredCounters>(1);
+    int32_t* grid = sc.get<int32_t>(2);
     const int cus = device_cu_count();
     const bool lds = force_path == 1 || (force_path == 0 && n_pts <= kLdsPts);
+    {
+        BatchArgs g{};
+        g.xyz = xyz;
+        g.n_pts = n_pts;
+        enqueue_set_grids(g, 1, n_pts, grid, s);
+    }
     for (int64_t off = 0; off < n_q; off += chunk) {
         const int64_t n = std::min<int64_t>(chunk, n_q - off);
         BatchArgs a{};
@@ -504,6 +590,7 @@ void launch_predicates(const double* xyz, int64_t n_pts, const uint8_t* op, cons
         a.wide = wide;
         a.ctr = ctr;
         a.stats = stats_dev;
+        a.grid = grid;
         BatchShape sh{};
         if (lds) {
             // about two tasks per CU, at least 8 tiles each

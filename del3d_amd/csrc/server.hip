@@ -70,12 +70,15 @@ struct Slot {
     const double** d_ptr = nullptr;
     int64_t* d_np = nullptr;
     int64_t* d_qoff = nullptr;
+    int64_t* d_first = nullptr;
+    int32_t* d_grid = nullptr;
     PredTask* d_tasks = nullptr;
     // pinned host staging
     int8_t* h_out = nullptr;
     const double** h_ptr = nullptr;
     int64_t* h_np = nullptr;
     int64_t* h_qoff = nullptr;
+    int64_t* h_first = nullptr;
     PredTask* h_tasks = nullptr;
     int64_t cap_q = 0, cap_jobs = 0, cap_tasks = 0;
     int64_t bytes = 0;  // device bytes held
@@ -137,7 +140,11 @@ void slot_release(del3d_server_s* sv, Slot& sl) {
     sl.cap_q = sl.cap_jobs = sl.cap_tasks = 0;
 }
 
-size_t table_bytes(int64_t j, int64_t t) { return (size_t)(24 * j + 8 + t * (int64_t)sizeof(PredTask)); }
+// set pointers [j], point counts [j], query offsets [j+1], first points [j+1],
+// grid summaries [2j int32], tasks [t]
+size_t table_bytes(int64_t j, int64_t t) {
+    return (size_t)(8 * j + 8 * j + 8 * (j + 1) + 8 * (j + 1) + 8 * j + t * (int64_t)sizeof(PredTask));
+}
 
 // Device bytes of a slot sized for q queries, j jobs and t tasks.
 int64_t slot_bytes(int64_t q, int64_t j, int64_t t) {
@@ -218,14 +225,17 @@ void slot_reserve(del3d_server_s* sv, Slot& sl, int64_t q, int64_t j, int64_t t)
     sl.d_ptr = reinterpret_cast<const double**>(dt);
     sl.d_np = reinterpret_cast<int64_t*>(dt + 8 * j);
     sl.d_qoff = reinterpret_cast<int64_t*>(dt + 16 * j);
-    sl.d_tasks = reinterpret_cast<PredTask*>(dt + 16 * j + 8 * (j + 1));
+    sl.d_first = reinterpret_cast<int64_t*>(dt + 16 * j + 8 * (j + 1));
+    sl.d_grid = reinterpret_cast<int32_t*>(dt + 16 * j + 16 * (j + 1));
+    sl.d_tasks = reinterpret_cast<PredTask*>(dt + 24 * j + 16 * (j + 1));
     D3_HIP(hipHostMalloc((void**)&sl.h_out, (size_t)q, hipHostMallocDefault));
     char* ht = nullptr;
     D3_HIP(hipHostMalloc((void**)&ht, tab, hipHostMallocDefault));
     sl.h_ptr = reinterpret_cast<const double**>(ht);
     sl.h_np = reinterpret_cast<int64_t*>(ht + 8 * j);
     sl.h_qoff = reinterpret_cast<int64_t*>(ht + 16 * j);
-    sl.h_tasks = reinterpret_cast<PredTask*>(ht + 16 * j + 8 * (j + 1));
+    sl.h_first = reinterpret_cast<int64_t*>(ht + 16 * j + 8 * (j + 1));
+    sl.h_tasks = reinterpret_cast<PredTask*>(ht + 24 * j + 16 * (j + 1));
     sl.cap_q = q;
     sl.cap_jobs = j;
     sl.cap_tasks = t;
@@ -275,7 +285,10 @@ void slot_run(del3d_server_s* sv, Slot* sl) {
         BatchArgs a{};
         a.set_ptr = sl->d_ptr;
         a.set_np = sl->d_np;
+        a.pt_first = sl->d_first;
         a.q_off = sl->d_qoff;
+        a.grid = sl->d_grid;
+        enqueue_set_grids(a, (uint32_t)nj, sl->h_first[nj], sl->d_grid, s);
         a.n_sets = (uint32_t)nj;
         a.op = sl->d_op;
         a.idx = sl->d_idx;
@@ -399,12 +412,14 @@ bool launch_batch(del3d_server_s* sv, Slot& sl, std::unique_lock<std::mutex>& lo
     // host tables and inputs (the set pointers are stable while pinned)
     int64_t off = 0, t = 0;
     sl.h_qoff[0] = 0;
+    sl.h_first[0] = 0;
     for (size_t k = 0; k < run.size(); ++k) {
         const Job& j = *run[k];
         const ResidentSet& r = sv->sets[j.set];
         sl.h_ptr[k] = r.dev;
         sl.h_np[k] = r.n;
         sl.h_qoff[k + 1] = off + j.n_q;
+        sl.h_first[k + 1] = sl.h_first[k] + r.n;
         for (int64_t a = 0; a < j.n_q; a += kTaskQ) {
             PredTask tk;
             tk.q0 = off + a;

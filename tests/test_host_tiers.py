@@ -65,3 +65,44 @@ def test_tiers_match_int_oracle(name, mode):
         assert set(np.unique(tier)) <= {2, 3}
     if mode == 2:
         assert set(np.unique(tier)) == {3}
+
+
+def _set_grid(pts):
+    """(gmin, mexp) of a point set from the doubles' bits: the lowest set
+    bit's exponent and floor(log2 |c|) over nonzero coordinates."""
+    c = np.asarray(pts, dtype=np.float64).ravel()
+    c = c[c != 0]
+    m, e = np.frexp(np.abs(c))  # |c| = m * 2^e, m in [0.5, 1)
+    mexp = int((e - 1).max())
+    lsb = []
+    for v in c:
+        n, ex = float(abs(v)).as_integer_ratio()  # v = n / 2^k exactly
+        t = (ex & -ex).bit_length() - 1  # ex is a power of two
+        lsb.append((n & -n).bit_length() - 1 - t)
+    return int(min(lsb)), mexp
+
+
+@pytest.mark.parametrize("scale", [0, -10, -48, 20, -300])
+def test_set_level_certificate_implies_per_coordinate(scale):
+    """Whenever the set summary certifies a query, the per-coordinate test
+    does too (so both tiers accept exactly the exact cases); on coarse grids
+    the set test accepts most degenerate queries."""
+    lib = C.CDLL(str(LIB))
+    rng = np.random.default_rng(1000 + scale)
+    for trial in range(40):
+        span = int(rng.integers(1, 40))
+        pts = rng.integers(-2 ** span, 2 ** span, size=(60, 3)).astype(np.float64) * 2.0 ** scale
+        if trial % 4 == 1:
+            pts[0, 0] += 2.0 ** (scale - 3)  # one finer coordinate lowers gmin
+        gmin, mexp = _set_grid(pts)
+        both = np.zeros(2, np.intc)
+        accepted = 0
+        for _ in range(60):
+            np_ = int(rng.choice([4, 5]))
+            q = np.ascontiguousarray(pts[rng.integers(0, len(pts), np_)])
+            lib.predcore_host_t1(q.ctypes.data_as(C.c_void_p), C.c_int(np_), C.c_int32(gmin),
+                                 C.c_int32(mexp), both.ctypes.data_as(C.c_void_p))
+            assert not (both[1] and not both[0]), (scale, span, q)
+            accepted += int(both[1])
+        if span <= 5 and trial % 4 != 1:
+            assert accepted > 0
