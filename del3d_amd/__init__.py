@@ -48,6 +48,7 @@ def lib() -> C.CDLL:
         L.del3d_predicates_stats.argtypes = [vp, i64, vp, vp, i64, vp, vp, C.POINTER(C.c_int64)]
         L.del3d_predicates_host.argtypes = [vp, i64, vp, vp, i64, vp]
         L.del3d_predicates_timed.argtypes = [vp, i64, vp, vp, i64, vp, vp, C.POINTER(C.c_float)]
+        L.del3d_predicates_ev.argtypes = [vp, i64, vp, vp, i64, vp, vp, vp, vp]
         i64p = C.POINTER(C.c_int64)
         L.del3d_predicates_sets.argtypes = [vp, i64p, i64, vp, vp, i64p, vp, vp]
         L.del3d_pred_plan_create.argtypes = [i64p, i64p, i64, C.POINTER(C.c_void_p)]
@@ -187,6 +188,15 @@ def predicates_timed(xyz: DeviceArray, op: DeviceArray, idx: DeviceArray, out: D
                                         int(op.shape[0]), out.ptr, _stream_handle(stream), ms),
            "del3d_predicates_timed")
     return float(ms[0]), float(ms[1]), float(ms[2])
+
+
+def predicates_ev(xyz: DeviceArray, op: DeviceArray, idx: DeviceArray, out: DeviceArray,
+                  stream: Stream, ev_begin: Event, ev_end: Event) -> None:
+    """del3d_predicates, with the two events recorded around the fast kernel
+    (no synchronization)."""
+    _check(lib().del3d_predicates_ev(xyz.ptr, int(xyz.shape[0]), op.ptr, idx.ptr, int(op.shape[0]),
+                                     out.ptr, _stream_handle(stream), ev_begin.handle, ev_end.handle),
+           "del3d_predicates_ev")
 
 
 def spatial_sort_device(xyz: DeviceArray, order: DeviceArray, stream: Stream | None = None) -> None:

@@ -75,12 +75,15 @@ void enqueue_set_grids(const BatchArgs& a, uint32_t n_sets, int64_t total_pts, i
 
 // Enqueues the whole pipeline of one batch on s (no host synchronization;
 // capturable into a graph). ev (6 events) brackets fast / exact / wide.
-void enqueue_batch(const BatchArgs& a, const BatchShape& sh, hipStream_t s, hipEvent_t* ev);
+void enqueue_batch(const BatchArgs& a, const BatchShape& sh, hipStream_t s, hipEvent_t* ev,
+                   hipEvent_t* ev2 = nullptr);
 
 // One point set; force_path 0 = choose, 1 = LDS kernel, 2 = global kernel.
+// ev: 6 events (fast, exact, wide brackets) or nullptr; ev2: 2 events around
+// the fast kernels only, or nullptr.
 void launch_predicates(const double* xyz, int64_t n_pts, const uint8_t* op, const int32_t* idx,
                        int64_t n_q, int8_t* out, hipStream_t s, unsigned long long* stats_dev,
-                       hipEvent_t* ev = nullptr, int force_path = 0);
+                       hipEvent_t* ev = nullptr, int force_path = 0, hipEvent_t* ev2 = nullptr);
 
 void launch_single(ScalarQuery* qd, hipStream_t s);
 

@@ -449,52 +449,79 @@ __global__ __launch_bounds__(256) void k_grid_init(int32_t* grid, uint32_t n_set
     }
 }
 
-// One thread per point. Points of one set are contiguous in the global
-// numbering (set s owns [first point of s, first point of s+1)), so a
-// wavefront usually covers one set and reduces before its atomics.
+// Each workgroup takes one contiguous range of the global point numbering
+// (set s owns [first[s], first[s+1])), so a thread's points change set only
+// at set boundaries: it accumulates its current set's minimum and maximum in
+// registers, flushes them with atomics when the set changes, and at the end
+// thread 0 merges the workgroup's runs of equal sets (one atomic pair per run).
+__device__ inline void grid_flush(int32_t* grid, int32_t set, int32_t lo, int32_t hi) {
+    if (lo <= hi) {
+        atomicMin(&grid[2 * set], lo);
+        atomicMax(&grid[2 * set + 1], hi);
+    }
+}
+
 __global__ __launch_bounds__(256) void k_grid_points(BatchArgs a, uint32_t n_sets, int64_t total,
                                                      const int64_t* __restrict__ first,
                                                      int32_t* __restrict__ grid) {
-    for (int64_t g0 = (int64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63); g0 < total;
-         g0 += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t g = g0 + __lane_id();
-        int32_t lo = 0x7FFFFFFF, hi = -0x7FFFFFFF;
-        uint32_t set = 0;
-        if (g < total) {
+    __shared__ int32_t s_lo[256], s_hi[256], s_set[256];
+    const int64_t per = (total + gridDim.x - 1) / gridDim.x;
+    const int64_t b0 = (int64_t)blockIdx.x * per, b1 = min(total, b0 + per);
+    int32_t cur = -1, lo = 0x7FFFFFFF, hi = -0x7FFFFFFF;
+    int64_t cur_end = 0, cur_first = 0;
+    const double* pts = nullptr;
+    for (int64_t g = b0 + threadIdx.x; g < b1; g += blockDim.x) {
+        if (cur < 0 || g >= cur_end) {
+            grid_flush(grid, cur, lo, hi);
+            lo = 0x7FFFFFFF;
+            hi = -0x7FFFFFFF;
             if (first) {
-                uint32_t l = 0, h = n_sets;  // last s with first[s] <= g
+                uint32_t l = cur < 0 ? 0 : (uint32_t)cur, h = n_sets;  // last s with first[s] <= g
                 while (h - l > 1) {
                     const uint32_t mid = (l + h) >> 1;
                     if (first[mid] <= g) l = mid;
                     else h = mid;
                 }
-                set = l;
+                cur = (int32_t)l;
+                cur_first = first[l];
+                cur_end = first[l + 1];
+            } else {
+                cur = 0;
+                cur_first = 0;
+                cur_end = total;
             }
-            const double* pts = set_points(a, (int32_t)set);
-            const int64_t local = first ? g - first[set] : g;
+            pts = set_points(a, cur);
+        }
+        const int64_t local = g - cur_first;
 #pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                const Bits d = decompose(pts[3 * local + c]);
-                if (d.m) {
-                    lo = min(lo, d.e + __builtin_ctzll(d.m));
-                    hi = max(hi, d.e + 63 - __builtin_clzll(d.m));
-                }
+        for (int c = 0; c < 3; ++c) {
+            const Bits d = decompose(pts[3 * local + c]);
+            if (d.m) {
+                lo = min(lo, d.e + __builtin_ctzll(d.m));
+                hi = max(hi, d.e + 63 - __builtin_clzll(d.m));
             }
         }
-        const uint32_t set0 = __shfl(set, 0);
-        if (__ballot(g < total && set != set0) == 0) {
-            for (int off = 32; off > 0; off >>= 1) {
-                lo = min(lo, __shfl_xor(lo, off));
-                hi = max(hi, __shfl_xor(hi, off));
+    }
+    s_lo[threadIdx.x] = lo;
+    s_hi[threadIdx.x] = hi;
+    s_set[threadIdx.x] = cur;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int32_t c = -1, l = 0x7FFFFFFF, h = -0x7FFFFFFF;
+        for (int t = 0; t < (int)blockDim.x; ++t) {
+            const int32_t st = s_set[t];
+            if (st < 0) continue;
+            if (st != c) {
+                if (c >= 0) grid_flush(grid, c, l, h);
+                c = st;
+                l = s_lo[t];
+                h = s_hi[t];
+            } else {
+                l = min(l, s_lo[t]);
+                h = max(h, s_hi[t]);
             }
-            if (__lane_id() == 0 && g0 < total) {
-                atomicMin(&grid[2 * set0], lo);
-                atomicMax(&grid[2 * set0 + 1], hi);
-            }
-        } else if (g < total) {
-            atomicMin(&grid[2 * set], lo);
-            atomicMax(&grid[2 * set + 1], hi);
         }
+        if (c >= 0) grid_flush(grid, c, l, h);
     }
 }
 
@@ -504,7 +531,7 @@ void enqueue_set_grids(const BatchArgs& a, uint32_t n_sets, int64_t total_pts, i
     D3_LAUNCH_CHECK();
     if (total_pts <= 0) return;
     const int cus = device_cu_count();
-    const unsigned g = (unsigned)std::min<int64_t>((total_pts + 255) / 256, (int64_t)cus * 16);
+    const unsigned g = (unsigned)std::min<int64_t>((total_pts + 4095) / 4096, (int64_t)cus * 4);
     k_grid_points<<<g, 256, 0, s>>>(a, n_sets, total_pts, a.set_ptr ? a.pt_first : a.pt_off, grid);
     D3_LAUNCH_CHECK();
 }
@@ -515,11 +542,13 @@ __global__ void k_clear_counters(PredCounters* c) {
     if (threadIdx.x < sizeof(PredCounters) / 4) reinterpret_cast<uint32_t*>(c)[threadIdx.x] = 0u;
 }
 
-void enqueue_batch(const BatchArgs& base, const BatchShape& sh, hipStream_t s, hipEvent_t* ev) {
+void enqueue_batch(const BatchArgs& base, const BatchShape& sh, hipStream_t s, hipEvent_t* ev,
+                   hipEvent_t* ev2) {
     const int cus = device_cu_count();
     k_clear_counters<<<1, 64, 0, s>>>(base.ctr);
     D3_LAUNCH_CHECK();
     if (ev) D3_HIP(hipEventRecord(ev[0], s));
+    if (ev2) D3_HIP(hipEventRecord(ev2[0], s));
     if (sh.n_lds_tasks) {
         BatchArgs a = base;
         a.tasks = sh.lds_tasks;
@@ -541,6 +570,7 @@ void enqueue_batch(const BatchArgs& base, const BatchShape& sh, hipStream_t s, h
         else pred_batch_global<false><<<g, kGlobalThreads, 0, s>>>(a);
         D3_LAUNCH_CHECK();
     }
+    if (ev2) D3_HIP(hipEventRecord(ev2[1], s));
     if (ev) {
         D3_HIP(hipEventRecord(ev[1], s));
         D3_HIP(hipEventRecord(ev[2], s));
@@ -561,7 +591,7 @@ void enqueue_batch(const BatchArgs& base, const BatchShape& sh, hipStream_t s, h
 // at most kChunk (which bounds the queues).
 void launch_predicates(const double* xyz, int64_t n_pts, const uint8_t* op, const int32_t* idx,
                        int64_t n_q, int8_t* out, hipStream_t s, unsigned long long* stats_dev,
-                       hipEvent_t* ev, int force_path) {
+                       hipEvent_t* ev, int force_path, hipEvent_t* ev2) {
     if (n_q <= 0) return;
     const uint32_t chunk = (uint32_t)std::min<int64_t>(n_q, kChunk);
     Scratch sc(s);
@@ -603,7 +633,7 @@ void launch_predicates(const double* xyz, int64_t n_pts, const uint8_t* op, cons
             sh.n_global_tasks = (uint32_t)((n + sh.global_chunk - 1) / sh.global_chunk);
             sh.global_blocks_per_cu = 8;
         }
-        enqueue_batch(a, sh, s, ev);
+        enqueue_batch(a, sh, s, ev, ev2);
     }
 }
 

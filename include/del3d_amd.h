@@ -95,6 +95,15 @@ int del3d_predicates_timed(const double* xyz, int64_t n_pts, const uint8_t* op,
                            float* ms_out);
 
 /*
+ * Measurement helper (no counterpart in the reference): as del3d_predicates
+ * (at most 2^26 queries), and records the caller's events (hipEvent_t passed
+ * as void*) on `stream` immediately before and after the fast-path kernel,
+ * without synchronizing, so a timed loop can bracket every launch.
+ */
+int del3d_predicates_ev(const double* xyz, int64_t n_pts, const uint8_t* op, const int32_t* idx,
+                        int64_t n_q, int8_t* sign_out, void* stream, void* ev_begin, void* ev_end);
+
+/*
  * Many point sets in one call (the batched configuration). Set s owns the
  * points xyz[3*pt_off[s] .. 3*pt_off[s+1]) and the queries
  * [q_off[s], q_off[s+1]) of op/idx/sign_out, whose indices are local to the
