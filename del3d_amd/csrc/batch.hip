@@ -133,8 +133,10 @@ void run_plan(del3d_pred_plan_s* p, const double* xyz, const uint8_t* op, const 
         BatchArgs g{};
         g.xyz = xyz;
         g.pt_off = p->d_pt_off;
+        g.ctr = p->ctr;
         enqueue_set_grids(g, (uint32_t)p->n_sets, p->n_pts, p->grid, s);
     }
+    bool first_chunk = true;
     for (const auto& c : p->chunks) {
         BatchArgs a{};
         a.xyz = xyz;
@@ -160,6 +162,8 @@ void run_plan(del3d_pred_plan_s* p, const double* xyz, const uint8_t* op, const 
         sh.n_global_tasks = c.n_glob;
         sh.global_chunk = 0;
         sh.global_blocks_per_cu = (uint32_t)std::max(1, p->knobs[3]);
+        sh.counters_cleared = first_chunk;  // the grid prepass zeroed them
+        first_chunk = false;
         enqueue_batch(a, sh, s, p->chunks.size() == 1 ? ev : nullptr);
     }
 }

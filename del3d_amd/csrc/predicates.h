@@ -59,6 +59,7 @@ struct BatchShape {
     uint32_t n_lds_tasks = 0, lds_chunk = 0, lds_blocks_per_cu = 1;
     const PredTask* global_tasks = nullptr;
     uint32_t n_global_tasks = 0, global_chunk = 0, global_blocks_per_cu = 8;
+    bool counters_cleared = false;  // the grid prepass already zeroed the counters
 };
 
 struct ScalarQuery {
@@ -69,7 +70,8 @@ struct ScalarQuery {
 };
 
 // Writes the SetGrid summary of every set of `a` into grid (2 int32 per set,
-// n_sets sets; one set when a.pt_off and a.set_ptr are null), on s.
+// n_sets sets; one set when a.pt_off and a.set_ptr are null), on s; also
+// zeroes a.ctr when it is set.
 void enqueue_set_grids(const BatchArgs& a, uint32_t n_sets, int64_t total_pts, int32_t* grid,
                        hipStream_t s);
 

@@ -442,11 +442,14 @@ __global__ __launch_bounds__(kWideBlock) void pred_wide(BatchArgs a) {
 // ---------------------------------------------------------------------------
 // Set-level grid summaries (SetGrid) for the T1 certificate
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_grid_init(int32_t* grid, uint32_t n_sets) {
+// Also zeroes the batch's counters when given (one launch for both).
+__global__ __launch_bounds__(256) void k_grid_init(int32_t* grid, uint32_t n_sets, PredCounters* ctr) {
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n_sets; i += gridDim.x * blockDim.x) {
         grid[2 * i] = 0x7FFFFFFF;      // min lowest-bit exponent
         grid[2 * i + 1] = -0x7FFFFFFF;  // max magnitude exponent
     }
+    if (ctr && blockIdx.x == 0 && threadIdx.x < sizeof(PredCounters) / 4)
+        reinterpret_cast<uint32_t*>(ctr)[threadIdx.x] = 0u;
 }
 
 // Each workgroup takes one contiguous range of the global point numbering
@@ -527,11 +530,11 @@ __global__ __launch_bounds__(256) void k_grid_points(BatchArgs a, uint32_t n_set
 
 void enqueue_set_grids(const BatchArgs& a, uint32_t n_sets, int64_t total_pts, int32_t* grid,
                        hipStream_t s) {
-    k_grid_init<<<(n_sets + 255) / 256, 256, 0, s>>>(grid, n_sets);
+    k_grid_init<<<(n_sets + 255) / 256, 256, 0, s>>>(grid, n_sets, a.ctr);
     D3_LAUNCH_CHECK();
     if (total_pts <= 0) return;
     const int cus = device_cu_count();
-    const unsigned g = (unsigned)std::min<int64_t>((total_pts + 4095) / 4096, (int64_t)cus * 4);
+    const unsigned g = (unsigned)std::min<int64_t>((total_pts + 1023) / 1024, (int64_t)cus * 4);
     k_grid_points<<<g, 256, 0, s>>>(a, n_sets, total_pts, a.set_ptr ? a.pt_first : a.pt_off, grid);
     D3_LAUNCH_CHECK();
 }
@@ -545,8 +548,10 @@ __global__ void k_clear_counters(PredCounters* c) {
 void enqueue_batch(const BatchArgs& base, const BatchShape& sh, hipStream_t s, hipEvent_t* ev,
                    hipEvent_t* ev2) {
     const int cus = device_cu_count();
-    k_clear_counters<<<1, 64, 0, s>>>(base.ctr);
-    D3_LAUNCH_CHECK();
+    if (!sh.counters_cleared) {
+        k_clear_counters<<<1, 64, 0, s>>>(base.ctr);
+        D3_LAUNCH_CHECK();
+    }
     if (ev) D3_HIP(hipEventRecord(ev[0], s));
     if (ev2) D3_HIP(hipEventRecord(ev2[0], s));
     if (sh.n_lds_tasks) {
@@ -601,12 +606,7 @@ void launch_predicates(const double* xyz, int64_t n_pts, const uint8_t* op, cons
     int32_t* grid = sc.get<int32_t>(2);
     const int cus = device_cu_count();
     const bool lds = force_path == 1 || (force_path == 0 && n_pts <= kLdsPts);
-    {
-        BatchArgs g{};
-        g.xyz = xyz;
-        g.n_pts = n_pts;
-        enqueue_set_grids(g, 1, n_pts, grid, s);
-    }
+    bool grids_done = false;
     for (int64_t off = 0; off < n_q; off += chunk) {
         const int64_t n = std::min<int64_t>(chunk, n_q - off);
         BatchArgs a{};
@@ -622,6 +622,12 @@ void launch_predicates(const double* xyz, int64_t n_pts, const uint8_t* op, cons
         a.stats = stats_dev;
         a.grid = grid;
         BatchShape sh{};
+        if (!grids_done) {
+            // the set's grid summary, and this chunk's counters, in one prepass
+            enqueue_set_grids(a, 1, n_pts, grid, s);
+            grids_done = true;
+            sh.counters_cleared = true;
+        }
         if (lds) {
             // about two tasks per CU, at least 8 tiles each
             const int64_t per = std::max<int64_t>(8 * kLdsThreads, (n + 2 * cus - 1) / (2 * cus));

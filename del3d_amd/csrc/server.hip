@@ -288,7 +288,6 @@ void slot_run(del3d_server_s* sv, Slot* sl) {
         a.pt_first = sl->d_first;
         a.q_off = sl->d_qoff;
         a.grid = sl->d_grid;
-        enqueue_set_grids(a, (uint32_t)nj, sl->h_first[nj], sl->d_grid, s);
         a.n_sets = (uint32_t)nj;
         a.op = sl->d_op;
         a.idx = sl->d_idx;
@@ -297,11 +296,13 @@ void slot_run(del3d_server_s* sv, Slot* sl) {
         a.queue = sl->d_queue;
         a.wide = sl->d_wide;
         a.ctr = sl->d_ctr;
+        enqueue_set_grids(a, (uint32_t)nj, sl->h_first[nj], sl->d_grid, s);  // also zeroes a.ctr
         BatchShape sh;
         sh.lds_tasks = sl->d_tasks;
         sh.n_lds_tasks = (uint32_t)n_lds;
         sh.global_tasks = sl->d_tasks + n_lds;
         sh.n_global_tasks = (uint32_t)(t - n_lds);
+        sh.counters_cleared = true;
         enqueue_batch(a, sh, s, nullptr);
         D3_HIP(hipMemcpyAsync(sl->h_out, sl->d_out, (size_t)q_run, hipMemcpyDeviceToHost, s));
         D3_HIP(hipEventRecord(sl->done, s));
