@@ -7,6 +7,8 @@ reference's interface:
   in_sphere(a, b, c, d, e)             include/del3d/predicates.h:49-50
   in_sphere_unperturbed(a, b, c, d, e) include/del3d/predicates.h:54-55
   spatial_sort(points, order)          include/del3d/spatial_sort.h:37
+  orient2d, coplanar_orientation, collinear, equal, collinear_position,
+  coplanar_in_circle(_unperturbed)     include/del3d/predicates2.h:26-69
 
 with the same sign convention (NEGATIVE=-1, ZERO=0, POSITIVE=1) and the same
 results, plus batched forms over device buffers (``predicates``,
@@ -26,6 +28,10 @@ from ._hip import DeviceArray, Event, Stream, device_count, synchronize
 
 NEGATIVE, ZERO, POSITIVE = -1, 0, 1
 ORIENT3D, IN_SPHERE, IN_SPHERE_UNPERTURBED = 0, 1, 2
+# coplanar batch op codes (DEL3D_OP2_*)
+(COPLANAR_ORIENTATION, COPLANAR_ORIENTATION4, COLLINEAR, EQUAL, COLLINEAR_POSITION,
+ IN_CIRCLE, IN_CIRCLE_UNPERTURBED, ORIENT2D_XY) = range(8)
+CP_BEFORE, CP_SOURCE, CP_MIDDLE, CP_TARGET, CP_AFTER = range(5)
 
 LIB_PATH = Path(__file__).resolve().parent / "libdel3d_amd.so"
 
@@ -70,6 +76,13 @@ def lib() -> C.CDLL:
         L.del3d_spatial_sort_sets.argtypes = [vp, C.POINTER(C.c_int64), i64, vp, vp]
         L.del3d_spatial_sort_host.argtypes = [vp, i64, vp, i64]
         L.del3d_orient3d.argtypes = [dp] * 4
+        L.del3d_predicates2.argtypes = [vp, i64, vp, vp, i64, vp, vp]
+        L.del3d_predicates2_host.argtypes = [vp, i64, vp, vp, i64, vp]
+        L.del3d_orient2d.argtypes = [C.c_double] * 6
+        for name, n in (("del3d_coplanar_orientation", 3), ("del3d_coplanar_orientation4", 4),
+                        ("del3d_collinear", 3), ("del3d_equal", 2), ("del3d_collinear_position", 3),
+                        ("del3d_coplanar_in_circle", 4), ("del3d_coplanar_in_circle_unperturbed", 4)):
+            getattr(L, name).argtypes = [dp] * n
         L.del3d_in_sphere.argtypes = [dp] * 5
         L.del3d_in_sphere_unperturbed.argtypes = [dp] * 5
         _lib = L
@@ -98,6 +111,16 @@ def predicates(xyz: DeviceArray, op: DeviceArray, idx: DeviceArray, out: DeviceA
     n_q = int(op.shape[0])
     _check(lib().del3d_predicates(xyz.ptr, int(xyz.shape[0]), op.ptr, idx.ptr, n_q, out.ptr,
                                   _stream_handle(stream)), "del3d_predicates")
+
+
+def predicates2(xyz: DeviceArray, op: DeviceArray, idx: DeviceArray, out: DeviceArray,
+                stream: Stream | None = None) -> None:
+    """Results of a coplanar batch: op[q] one of COPLANAR_ORIENTATION ..
+    ORIENT2D_XY, idx[q] = the op's point indices in argument order (5 int32,
+    the unused ones ignored). Enqueued on `stream`."""
+    n_q = int(op.shape[0])
+    _check(lib().del3d_predicates2(xyz.ptr, int(xyz.shape[0]), op.ptr, idx.ptr, n_q, out.ptr,
+                                   _stream_handle(stream)), "del3d_predicates2")
 
 
 STAT_NAMES = ("filter", "certified", "exact_regs", "exact_wide", "perturbed", "bad_index")
@@ -289,6 +312,18 @@ def predicates_np(points, op, idx) -> np.ndarray:
     return out
 
 
+def predicates2_np(points, op, idx) -> np.ndarray:
+    """The coplanar batch on host arrays (copied in and out)."""
+    xyz = np.ascontiguousarray(np.asarray(points, dtype=np.float64).reshape(-1, 3))
+    op = np.ascontiguousarray(op, dtype=np.uint8)
+    idx = np.ascontiguousarray(np.asarray(idx, dtype=np.int32).reshape(-1, 5))
+    out = np.zeros(len(op), dtype=np.int8)
+    _check(lib().del3d_predicates2_host(xyz.ctypes.data, len(xyz), op.ctypes.data,
+                                        idx.ctypes.data, len(op), out.ctypes.data),
+           "del3d_predicates2_host")
+    return out
+
+
 def spatial_sort(points, order=None) -> np.ndarray:
     """The reference's spatial_sort(points, order) on host arrays; returns the
     permuted order (a new array)."""
@@ -332,9 +367,56 @@ def in_sphere_unperturbed(a, b, c, d, e) -> int:
     return int(lib().del3d_in_sphere_unperturbed(*p))
 
 
+def orient2d(px, py, qx, qy, rx, ry) -> int:
+    """Sign of |q-p, r-p| in the plane (one GPU query)."""
+    return int(lib().del3d_orient2d(px, py, qx, qy, rx, ry))
+
+
+def coplanar_orientation(p, q, r, s=None) -> int:
+    """Orientation of (p,q,r) in the first non-degenerate of the xy, yz, xz
+    projections; with s, the side of s against line (p,q), signed by (p,q,r)."""
+    if s is None:
+        keep, ptr = _pts(p, q, r)
+        return int(lib().del3d_coplanar_orientation(*ptr))
+    keep, ptr = _pts(p, q, r, s)
+    return int(lib().del3d_coplanar_orientation4(*ptr))
+
+
+def collinear(p, q, r) -> bool:
+    keep, ptr = _pts(p, q, r)
+    return bool(lib().del3d_collinear(*ptr))
+
+
+def equal(p, q) -> bool:
+    keep, ptr = _pts(p, q)
+    return bool(lib().del3d_equal(*ptr))
+
+
+def collinear_position(p, q, r) -> int:
+    """CP_BEFORE, CP_SOURCE, CP_MIDDLE, CP_TARGET or CP_AFTER."""
+    keep, ptr = _pts(p, q, r)
+    return int(lib().del3d_collinear_position(*ptr))
+
+
+def coplanar_in_circle(p0, p1, p2, p) -> int:
+    """POSITIVE iff p is inside the circle through the coplanar p0, p1, p2;
+    never ZERO (symbolic perturbation)."""
+    keep, ptr = _pts(p0, p1, p2, p)
+    return int(lib().del3d_coplanar_in_circle(*ptr))
+
+
+def coplanar_in_circle_unperturbed(p0, p1, p2, p) -> int:
+    keep, ptr = _pts(p0, p1, p2, p)
+    return int(lib().del3d_coplanar_in_circle_unperturbed(*ptr))
+
+
 __all__ = [
     "NEGATIVE", "ZERO", "POSITIVE", "ORIENT3D", "IN_SPHERE", "IN_SPHERE_UNPERTURBED",
     "DeviceArray", "Stream", "Event", "device_count", "synchronize", "lib", "Del3dError",
     "predicates", "predicates_stats", "predicates_timed", "predicates_sets", "PredPlan", "Server", "spatial_sort_device", "spatial_sort_sets",
     "predicates_np", "spatial_sort", "orient3d", "in_sphere", "in_sphere_unperturbed",
+    "predicates2", "predicates2_np", "orient2d", "coplanar_orientation", "collinear", "equal",
+    "collinear_position", "coplanar_in_circle", "coplanar_in_circle_unperturbed",
+    "COPLANAR_ORIENTATION", "COPLANAR_ORIENTATION4", "COLLINEAR", "EQUAL", "COLLINEAR_POSITION",
+    "IN_CIRCLE", "IN_CIRCLE_UNPERTURBED", "ORIENT2D_XY",
 ]

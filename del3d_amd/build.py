@@ -14,12 +14,13 @@ from pathlib import Path
 HERE = Path(__file__).resolve().parent
 CSRC = HERE / "csrc"
 LIB = HERE / "libdel3d_amd.so"
-SOURCES = ["capi.hip", "predicates.hip", "batch.hip", "server.hip", "spatial_sort.hip", "dropin.cpp"]
+SOURCES = ["capi.hip", "predicates.hip", "predicates2.hip", "batch.hip", "server.hip", "spatial_sort.hip",
+           "dropin.cpp"]
 ARCH = os.environ.get("DEL3D_OFFLOAD_ARCH", "gfx950")
 
 COMMON = [
     "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-fno-fast-math",
-    "-Wall", "-Wno-unused-result",
+    "-Wall", "-Wno-unused-result", "-Wno-bitwise-instead-of-logical",
 ]
 HIP_ONLY = ["-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 

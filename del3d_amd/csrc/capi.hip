@@ -171,7 +171,7 @@ struct ScalarSlot {
     ScalarQuery* dev = nullptr;
 };
 
-static int scalar_query(uint8_t op, const double* const* pts, int np) {
+static int scalar_query(uint8_t op, const double* const* pts, int np, bool coplanar) {
     thread_local std::vector<ScalarSlot> slots;
     int dev = 0;
     D3_HIP(hipGetDevice(&dev));
@@ -196,16 +196,19 @@ static int scalar_query(uint8_t op, const double* const* pts, int np) {
     }
     hipStream_t s = thread_stream();
     D3_HIP(hipMemcpyAsync(sl.dev, sl.host, sizeof(ScalarQuery), hipMemcpyHostToDevice, s));
-    launch_single(sl.dev, s);
+    if (coplanar)
+        launch_single2(sl.dev, s);
+    else
+        launch_single(sl.dev, s);
     D3_HIP(hipMemcpyAsync(sl.host, sl.dev, sizeof(ScalarQuery), hipMemcpyDeviceToHost, s));
     D3_HIP(hipStreamSynchronize(s));
     return q.sign;
 }
 
-static int scalar_or_abort(uint8_t op, const double* const* pts, int np) {
+static int scalar_or_abort(uint8_t op, const double* const* pts, int np, bool coplanar = false) {
     int sign = 0;
     const int rc = guarded([&] {
-        sign = scalar_query(op, pts, np);
+        sign = scalar_query(op, pts, np, coplanar);
         return DEL3D_OK;
     });
     if (rc != DEL3D_OK) {
@@ -323,6 +326,37 @@ int del3d_predicates_host(const double* xyz, int64_t n_pts, const uint8_t* op, c
     });
 }
 
+int del3d_predicates2(const double* xyz, int64_t n_pts, const uint8_t* op, const int32_t* idx,
+                      int64_t n_q, int8_t* out, void* stream) {
+    return guarded([&] {
+        const int rc = check_pred_args(xyz, n_pts, op, idx, n_q, out);
+        if (rc) return rc;
+        launch_predicates2(xyz, n_pts, op, idx, n_q, out, (hipStream_t)stream);
+        return DEL3D_OK;
+    });
+}
+
+int del3d_predicates2_host(const double* xyz, int64_t n_pts, const uint8_t* op, const int32_t* idx,
+                           int64_t n_q, int8_t* out) {
+    return guarded([&] {
+        const int rc = check_pred_args(xyz, n_pts, op, idx, n_q, out);
+        if (rc) return rc;
+        if (n_q == 0) return DEL3D_OK;
+        hipStream_t s = thread_stream();
+        Scratch sc(s);
+        double* dx = sc.get<double>((size_t)n_pts * 3);
+        uint8_t* dop = sc.get<uint8_t>((size_t)n_q);
+        int32_t* did = sc.get<int32_t>((size_t)n_q * 5);
+        int8_t* dout = sc.get<int8_t>((size_t)n_q);
+        copy_h2d(dx, xyz, (size_t)n_pts * 24, s);
+        copy_h2d(dop, op, (size_t)n_q, s);
+        copy_h2d(did, idx, (size_t)n_q * 20, s);
+        launch_predicates2(dx, n_pts, dop, did, n_q, dout, s);
+        copy_d2h(out, dout, (size_t)n_q, s);
+        return DEL3D_OK;
+    });
+}
+
 int del3d_spatial_sort(const double* xyz, int64_t n_pts, int32_t* order, int64_t n_order,
                        void* stream) {
     return guarded([&] {
@@ -387,6 +421,48 @@ int del3d_in_sphere_unperturbed(const double* a, const double* b, const double* 
                                 const double* e) {
     const double* p[5] = {a, b, c, d, e};
     return scalar_or_abort(DEL3D_OP_IN_SPHERE_UNPERTURBED, p, 5);
+}
+
+int del3d_orient2d(double px, double py, double qx, double qy, double rx, double ry) {
+    const double p[3] = {px, py, 0.0}, q[3] = {qx, qy, 0.0}, r[3] = {rx, ry, 0.0};
+    const double* v[3] = {p, q, r};
+    return scalar_or_abort(DEL3D_OP2_ORIENT2D_XY, v, 3, true);
+}
+
+int del3d_coplanar_orientation(const double* p, const double* q, const double* r) {
+    const double* v[3] = {p, q, r};
+    return scalar_or_abort(DEL3D_OP2_COPLANAR_ORIENTATION, v, 3, true);
+}
+
+int del3d_coplanar_orientation4(const double* p, const double* q, const double* r, const double* s) {
+    const double* v[4] = {p, q, r, s};
+    return scalar_or_abort(DEL3D_OP2_COPLANAR_ORIENTATION4, v, 4, true);
+}
+
+int del3d_collinear(const double* p, const double* q, const double* r) {
+    const double* v[3] = {p, q, r};
+    return scalar_or_abort(DEL3D_OP2_COLLINEAR, v, 3, true);
+}
+
+int del3d_equal(const double* p, const double* q) {
+    const double* v[2] = {p, q};
+    return scalar_or_abort(DEL3D_OP2_EQUAL, v, 2, true);
+}
+
+int del3d_collinear_position(const double* p, const double* q, const double* r) {
+    const double* v[3] = {p, q, r};
+    return scalar_or_abort(DEL3D_OP2_COLLINEAR_POSITION, v, 3, true);
+}
+
+int del3d_coplanar_in_circle(const double* p0, const double* p1, const double* p2, const double* p) {
+    const double* v[4] = {p0, p1, p2, p};
+    return scalar_or_abort(DEL3D_OP2_IN_CIRCLE, v, 4, true);
+}
+
+int del3d_coplanar_in_circle_unperturbed(const double* p0, const double* p1, const double* p2,
+                                         const double* p) {
+    const double* v[4] = {p0, p1, p2, p};
+    return scalar_or_abort(DEL3D_OP2_IN_CIRCLE_UNPERTURBED, v, 4, true);
 }
 
 }  // extern "C"

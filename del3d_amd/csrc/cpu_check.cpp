@@ -8,6 +8,7 @@
 #include <cstdint>
 #include <vector>
 
+#include "pred2_core.h"
 #include "pred_core.h"
 
 using namespace d3;
@@ -96,4 +97,38 @@ extern "C" void predcore_host_t1(const double* pts, int np, int32_t gmin, int32_
     const double md = max_diff(p, np);
     out[0] = t1_certify(p, np, md, k, qmin) ? 1 : 0;
     out[1] = t1_certify_set(md, k, qmin, SetGrid{gmin, mexp}) ? 1 : 0;
+}
+
+// Host build of the coplanar predicates (pred2_core.h). mode 0: the full
+// evaluation (filters, exact tiers, perturbation); 1: the fast kernel's
+// filter-only result (2 where it leaves the query to the exact kernel);
+// 2: the exact tiers forced (the filters skipped) for orient2d and the raw
+// in-circle determinant.
+extern "C" void predcore2_host_batch(const double* xyz, int64_t n_pts, const uint8_t* op,
+                                     const int32_t* idx, int64_t n_q, int8_t* out, int mode) {
+    (void)n_pts;
+    std::vector<uint32_t> ws((size_t)in_circle_workspace_limbs(kMaxLimbs2));
+    for (int64_t q = 0; q < n_q; ++q) {
+        const uint8_t o = op[q];
+        const int np = o == kOp2Equal ? 2
+                     : (o == kOp2CoplanarOrientation4 || o == kOp2InCircle ||
+                        o == kOp2InCircleUnperturbed) ? 4 : 3;
+        P3 p[4];
+        int32_t id[4];
+        for (int k = 0; k < 4; ++k) {
+            id[k] = idx[5 * q + k];
+            p[k] = k < np ? load_p3(xyz, id[k]) : P3{0.0, 0.0, 0.0};
+        }
+        int s;
+        if (mode == 1) {
+            s = pred2_fast(o, p);
+        } else if (mode == 2 && o == kOp2Orient2dXY) {
+            s = orient2d_exact(p[0].x, p[0].y, p[1].x, p[1].y, p[2].x, p[2].y, ws.data());
+        } else if (mode == 2 && o == kOp2InCircleUnperturbed) {
+            s = in_circle_exact(p[0], p[1], p[2], p[3], ws.data());
+        } else {
+            s = pred2_eval(o, p, id, ws.data());
+        }
+        out[q] = (int8_t)s;
+    }
 }

@@ -1,6 +1,7 @@
 // dropin.cpp - the reference's C++ symbols over the C ABI.
 //
-// Scalar predicate calls run one query on the GPU (del3d_orient3d & co.);
+// Scalar predicate calls, 3D and coplanar, run one query on the GPU
+// (del3d_orient3d & co.);
 // spatial_sort runs the GPU sort on a host copy of the data. The reference's
 // signatures have no error channel, so a device failure aborts with the
 // library's message rather than returning a sign that was never computed.
@@ -23,6 +24,37 @@ Sign in_sphere(const double* a, const double* b, const double* c, const double* 
 Sign in_sphere_unperturbed(const double* a, const double* b, const double* c, const double* d,
                            const double* e) {
     return static_cast<Sign>(del3d_in_sphere_unperturbed(a, b, c, d, e));
+}
+
+Sign orient2d(double px, double py, double qx, double qy, double rx, double ry) {
+    return static_cast<Sign>(del3d_orient2d(px, py, qx, qy, rx, ry));
+}
+
+Sign coplanar_orientation(const double* p, const double* q, const double* r) {
+    return static_cast<Sign>(del3d_coplanar_orientation(p, q, r));
+}
+
+Sign coplanar_orientation(const double* p, const double* q, const double* r, const double* s) {
+    return static_cast<Sign>(del3d_coplanar_orientation4(p, q, r, s));
+}
+
+bool collinear(const double* p, const double* q, const double* r) {
+    return del3d_collinear(p, q, r) != 0;
+}
+
+bool equal(const double* p, const double* q) { return del3d_equal(p, q) != 0; }
+
+CollinearPosition collinear_position(const double* p, const double* q, const double* r) {
+    return static_cast<CollinearPosition>(del3d_collinear_position(p, q, r));
+}
+
+Sign coplanar_in_circle(const double* p0, const double* p1, const double* p2, const double* p) {
+    return static_cast<Sign>(del3d_coplanar_in_circle(p0, p1, p2, p));
+}
+
+Sign coplanar_in_circle_unperturbed(const double* p0, const double* p1, const double* p2,
+                                    const double* p) {
+    return static_cast<Sign>(del3d_coplanar_in_circle_unperturbed(p0, p1, p2, p));
 }
 
 void spatial_sort(const std::vector<Point3>& points, std::vector<int>& order) {

@@ -89,4 +89,12 @@ void launch_predicates(const double* xyz, int64_t n_pts, const uint8_t* op, cons
 
 void launch_single(ScalarQuery* qd, hipStream_t s);
 
+// Coplanar batch (predicates2.hip): op codes DEL3D_OP2_*, idx 5 per query
+// (the op's points in argument order), out = the predicate's result. Queries
+// with an unknown op or an index outside [0, n_pts) get 0.
+void launch_predicates2(const double* xyz, int64_t n_pts, const uint8_t* op, const int32_t* idx,
+                        int64_t n_q, int8_t* out, hipStream_t s);
+// One coplanar query of a ScalarQuery record (points by value, op DEL3D_OP2_*).
+void launch_single2(ScalarQuery* qd, hipStream_t s);
+
 }  // namespace d3

@@ -1,0 +1,138 @@
+// predicates2.hip - batched coplanar predicates for gfx950.
+//
+// Replaces, per query, one call of the reference's coplanar predicates
+// (src/predicates2.cpp:111-259; include/del3d/predicates2.h:26-69). Two
+// kernels per batch on the caller's stream:
+//
+//  pred2_fast   one lane per query: the filters in the reference's operation
+//               order (and the predicates that are plain comparisons); what
+//               they leave undecided is appended, one atomic per wavefront,
+//               to a queue;
+//  pred2_exact  one query per workgroup: the whole predicate with exact
+//               integer fallbacks and the cocircular perturbation, on a
+//               workspace in LDS sized for any finite input.
+//
+// The coplanar predicates are called rarely by the reference's driver (during
+// the dimension bootstrap and on exactly coplanar hull facets), so the exact
+// tier favours a bounded, input-independent workspace over speed.
+#include "common.h"
+#include "pred2_core.h"
+#include "predicates.h"
+
+namespace d3 {
+
+constexpr int kWs2 = 207 * kMaxLimbs2 + 16;  // in_circle_workspace_limbs(kMaxLimbs2)
+
+__device__ inline int points_of_op2(uint8_t o) {
+    switch (o) {
+        case kOp2Equal: return 2;
+        case kOp2CoplanarOrientation4:
+        case kOp2InCircle:
+        case kOp2InCircleUnperturbed: return 4;
+        default: return 3;
+    }
+}
+
+__global__ __launch_bounds__(256) void pred2_fast_k(const double* __restrict__ xyz, int64_t n_pts,
+                                                    const uint8_t* __restrict__ op,
+                                                    const int32_t* __restrict__ idx, uint32_t n_q,
+                                                    int8_t* __restrict__ out, uint32_t* __restrict__ queue,
+                                                    uint32_t* __restrict__ counter) {
+    for (uint32_t q0 = blockIdx.x * blockDim.x + (threadIdx.x & ~63u); q0 < n_q;
+         q0 += gridDim.x * blockDim.x) {
+        const uint32_t q = q0 + __lane_id();
+        bool pending = false;
+        if (q < n_q) {
+            const uint8_t o = op[q];
+            const int np = points_of_op2(o);
+            P3 pt[4];
+            bool valid = o <= kOp2Orient2dXY;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const int32_t id = idx[(size_t)q * 5 + k];
+                const bool used = k < np;
+                valid = valid && (!used || (uint32_t)id < (uint64_t)n_pts);
+                pt[k] = (used && valid) ? load_p3(xyz, id) : P3{0.0, 0.0, 0.0};
+            }
+            int s = 0;
+            if (valid) {
+                s = pred2_fast(o, pt);
+                pending = s == 2;
+            }
+            if (!pending) out[q] = (int8_t)s;
+        }
+        const uint64_t m = __ballot(pending);
+        if (m) {
+            const int leader = __ffsll((long long)m) - 1;
+            uint32_t base = 0;
+            if (__lane_id() == leader) base = atomicAdd(counter, (uint32_t)__popcll(m));
+            base = __shfl(base, leader);
+            if (pending) queue[base + (uint32_t)__popcll(m & __lanemask_lt())] = q;
+        }
+    }
+}
+
+__global__ __launch_bounds__(64) void pred2_exact_k(const double* __restrict__ xyz,
+                                                    const uint8_t* __restrict__ op,
+                                                    const int32_t* __restrict__ idx,
+                                                    int8_t* __restrict__ out,
+                                                    const uint32_t* __restrict__ queue,
+                                                    const uint32_t* __restrict__ counter) {
+    __shared__ uint32_t ws[kWs2];
+    if (threadIdx.x != 0) return;
+    const uint32_t n = *counter;
+    for (uint32_t i = blockIdx.x; i < n; i += gridDim.x) {
+        const uint32_t q = queue[i];
+        const uint8_t o = op[q];
+        const int np = points_of_op2(o);
+        P3 pt[4];
+        int32_t id[4];
+        for (int k = 0; k < 4; ++k) {
+            id[k] = idx[(size_t)q * 5 + k];
+            pt[k] = k < np ? load_p3(xyz, id[k]) : P3{0.0, 0.0, 0.0};
+        }
+        out[q] = (int8_t)pred2_eval(o, pt, id, ws);
+    }
+}
+
+__global__ void k_clear_u32(uint32_t* c) { *c = 0u; }
+
+void launch_predicates2(const double* xyz, int64_t n_pts, const uint8_t* op, const int32_t* idx,
+                        int64_t n_q, int8_t* out, hipStream_t s) {
+    if (n_q <= 0) return;
+    const uint32_t chunk = (uint32_t)std::min<int64_t>(n_q, kChunk);
+    Scratch sc(s);
+    uint32_t* queue = sc.get<uint32_t>(chunk);
+    uint32_t* counter = sc.get<uint32_t>(1);
+    const int cus = device_cu_count();
+    for (int64_t off = 0; off < n_q; off += chunk) {
+        const uint32_t n = (uint32_t)std::min<int64_t>(chunk, n_q - off);
+        k_clear_u32<<<1, 1, 0, s>>>(counter);
+        D3_LAUNCH_CHECK();
+        const unsigned g = (unsigned)std::min<uint32_t>((n + 255) / 256, (uint32_t)cus * 16);
+        pred2_fast_k<<<g, 256, 0, s>>>(xyz, n_pts, op + off, idx + off * 5, n, out + off, queue, counter);
+        D3_LAUNCH_CHECK();
+        pred2_exact_k<<<cus * 2, 64, 0, s>>>(xyz, op + off, idx + off * 5, out + off, queue, counter);
+        D3_LAUNCH_CHECK();
+    }
+}
+
+// One query given by its points' coordinates (scalar drop-in calls).
+__global__ __launch_bounds__(64) void pred2_single_k(ScalarQuery* qd) {
+    __shared__ uint32_t ws[kWs2];
+    if (threadIdx.x != 0) return;
+    P3 pt[4];
+    int32_t id[4];
+    for (int k = 0; k < 4; ++k) {
+        pt[k] = P3{qd->xyz[3 * k], qd->xyz[3 * k + 1], qd->xyz[3 * k + 2]};
+        id[k] = qd->id[k];
+    }
+    qd->sign = (int8_t)pred2_eval(qd->op, pt, id, ws);
+}
+
+void launch_single2(ScalarQuery* qd, hipStream_t s) {
+    pred2_single_k<<<1, 64, 0, s>>>(qd);
+    D3_LAUNCH_CHECK();
+}
+
+}  // namespace d3

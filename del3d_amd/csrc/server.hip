@@ -333,12 +333,11 @@ void slot_run(del3d_server_s* sv, Slot* sl) {
 // holds the lock; returns false if there was nothing to launch.
 bool launch_batch(del3d_server_s* sv, Slot& sl, std::unique_lock<std::mutex>& lock) {
     std::vector<std::shared_ptr<Job>> take;
-    int64_t q = 0, ntasks = 0;
+    int64_t q = 0;
     while (!sv->queue.empty()) {
         auto& j = sv->queue.front();
         if (!take.empty() && q + j->n_q > sv->max_batch_q) break;
         q += j->n_q;
-        ntasks += tasks_for(j->n_q);
         take.push_back(j);
         sv->queue.pop_front();
     }

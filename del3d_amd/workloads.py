@@ -225,3 +225,67 @@ def hashed_queries(seed: int, n_set: int, q: np.ndarray):
             idx[:, k] = ((h * np.uint64(n_set)) >> np.uint64(32)).astype(np.int32)
         op = (_splitmix64(base + np.uint64(5)) & np.uint64(1)).astype(np.uint8)
     return op, idx
+
+
+# ---- coplanar predicates -----------------------------------------------------
+
+_RING25 = [(5, 0), (4, 3), (3, 4), (0, 5), (-3, 4), (-4, 3), (-5, 0), (-4, -3), (-3, -4),
+           (0, -5), (3, -4), (4, -3)]
+# M M^T = 9 I: maps the z = 0 plane isometrically (up to the factor 3) onto a
+# tilted plane with integer coordinates, so circles stay circles.
+_TILT = np.array([[1, 2, 2], [2, 1, -2], [2, -2, 1]], dtype=np.float64)
+
+
+def coplanar_points(kind: str, rng, n: int = 400) -> np.ndarray:
+    """Point sets for the coplanar predicates.
+
+    grid_z0      integer grid [-3,3]^2 in the plane z = 0
+    tilted       the same grid on the plane z = x + y
+    ring         the 12 lattice points of x^2 + y^2 = 25, its centre and (9,9)
+    ring_tilted  the ring mapped by an integer similarity onto a tilted plane
+    dyadic_plane random 2^-20-grid points on z = 3x/4 - 5y/8 + 1/2 (exact)
+    line         points p + t d on one line, t small integers (collinear input)
+    scaled       2^-20-grid points on z = 3x/4 - 5y/8, half scaled by 2^-60, half
+                 by 2^+60 (two clusters on one plane)
+    random       uniform points in the unit cube (no precondition holds)
+    """
+    if kind in ("grid_z0", "tilted"):
+        g = np.array([(i, j, 0.0) for i in range(-3, 4) for j in range(-3, 4)])
+        if kind == "tilted":
+            g[:, 2] = g[:, 0] + g[:, 1]
+        return g
+    if kind in ("ring", "ring_tilted"):
+        r = np.array([(x, y, 0.0) for x, y in _RING25] + [(0, 0, 0.0), (9, 9, 0.0)])
+        return r @ _TILT.T if kind == "ring_tilted" else r
+    if kind in ("dyadic_plane", "scaled"):
+        xy = rng.integers(-(1 << 20), 1 << 20, size=(n, 2)) * 2.0 ** -20
+        z = 0.75 * xy[:, 0] - 0.625 * xy[:, 1] + (0.5 if kind == "dyadic_plane" else 0.0)
+        p = np.column_stack([xy, z])
+        if kind == "scaled":  # the plane through the origin, so both clusters lie on it
+            p = np.vstack([p[: n // 2] * 2.0 ** -60, p[n // 2:] * 2.0 ** 60])
+        return p
+    if kind == "line":
+        t = rng.integers(-40, 40, size=n).astype(np.float64)
+        p0 = np.array([0.5, -1.25, 3.0])
+        d = np.array([0.125, 0.75, -0.5])
+        return p0 + t[:, None] * d
+    if kind == "random":
+        return rng.random((n, 3))
+    raise ValueError(kind)
+
+
+COPLANAR_KINDS = ("grid_z0", "tilted", "ring", "ring_tilted", "dyadic_plane", "line", "scaled",
+                  "random")
+
+
+def coplanar_queries(n_pts: int, n_q: int, rng, ops=tuple(range(8))):
+    """Random coplanar-batch queries: op codes from ops, indices drawn
+    independently, with about one query in eight repeating an earlier slot
+    (equal index = the same point)."""
+    op = rng.choice(np.asarray(ops, dtype=np.uint8), size=n_q).astype(np.uint8)
+    idx = rng.integers(0, n_pts, size=(n_q, 5)).astype(np.int32)
+    m = n_q // 8
+    src = rng.integers(0, 3, size=m)
+    idx[np.arange(m), 3] = idx[np.arange(m), src]
+    idx[:, 4] = 0
+    return op, idx

@@ -1,7 +1,8 @@
 /*
  * del3d_amd.h - C ABI of the MI355X (gfx950) implementation of del3d's hot path:
  * the exact filtered predicates orient3d / in_sphere (with symbolic
- * perturbation) and the rand48-shuffled multiscale Hilbert spatial sort.
+ * perturbation), their coplanar counterparts, and the rand48-shuffled
+ * multiscale Hilbert spatial sort.
  *
  * Every result is bit-exact with the reference on the same input: predicate
  * signs are the signs the reference returns, sort permutations are the
@@ -40,6 +41,17 @@ extern "C" {
 #define DEL3D_OP_IN_SPHERE 1             /* in_sphere(r0, r1, r2, r3, r4) */
 #define DEL3D_OP_IN_SPHERE_UNPERTURBED 2 /* in_sphere_unperturbed(r0..r4) */
 
+/* Coplanar predicate op codes (del3d_predicates2), one byte per query. The
+ * query's points are idx[5q+0 ..] in the reference's argument order. */
+#define DEL3D_OP2_COPLANAR_ORIENTATION 0   /* coplanar_orientation(p,q,r): -1/0/+1 */
+#define DEL3D_OP2_COPLANAR_ORIENTATION4 1  /* coplanar_orientation(p,q,r,s): -1/0/+1 */
+#define DEL3D_OP2_COLLINEAR 2              /* collinear(p,q,r): 0/1 */
+#define DEL3D_OP2_EQUAL 3                  /* equal(p,q): 0/1 */
+#define DEL3D_OP2_COLLINEAR_POSITION 4     /* collinear_position(p,q,r): 0..4 */
+#define DEL3D_OP2_IN_CIRCLE 5              /* coplanar_in_circle(p0,p1,p2,p): -1/+1 */
+#define DEL3D_OP2_IN_CIRCLE_UNPERTURBED 6  /* coplanar_in_circle_unperturbed: -1/0/+1 */
+#define DEL3D_OP2_ORIENT2D_XY 7            /* orient2d(p.x,p.y, q.x,q.y, r.x,r.y): -1/0/+1 */
+
 /* Message of the calling thread's last failed call ("" if none). */
 const char* del3d_last_error(void);
 
@@ -62,8 +74,10 @@ const char* del3d_version(void);
  *   op       device, n_q bytes (DEL3D_OP_*)
  *   idx      device, 5*n_q int32, each in [0, n_pts)
  *   sign_out device, n_q int8, written with -1 / 0 / +1
- * Indices are not range-checked on the device; out-of-range indices are
- * undefined behaviour, as out-of-range pointers are in the reference.
+ * Out-of-range pointers are undefined behaviour in the reference; here a
+ * query with an index outside [0, n_pts) or an unknown op code gets sign 0
+ * (counted in stats[5] of the stats form), and the device never reads
+ * outside xyz.
  */
 int del3d_predicates(const double* xyz, int64_t n_pts, const uint8_t* op, const int32_t* idx,
                      int64_t n_q, int8_t* sign_out, void* stream);
@@ -230,10 +244,39 @@ int del3d_spatial_sort(const double* xyz, int64_t n_pts, int32_t* order, int64_t
 int del3d_spatial_sort_sets(const double* xyz, const int64_t* off, int64_t n_sets,
                             int32_t* order, void* stream);
 
+/*
+ * Batched coplanar predicates over device-resident data.
+ *
+ * Replaces, per query q, one call of the reference's coplanar predicates
+ * (include/del3d/predicates2.h:26-69, src/predicates2.cpp:111-259):
+ *   orient2d                          predicates2.h:26  (src/predicates2.cpp:111)
+ *   coplanar_orientation(p,q,r)       predicates2.h:33  (src/predicates2.cpp:122)
+ *   coplanar_orientation(p,q,r,s)     predicates2.h:39-40 (src/predicates2.cpp:133)
+ *   collinear                         predicates2.h:44  (src/predicates2.cpp:148)
+ *   equal                             predicates2.h:47  (src/predicates2.cpp:158)
+ *   collinear_position                predicates2.h:51-52 (src/predicates2.cpp:162)
+ *   coplanar_in_circle                predicates2.h:64-65 (src/predicates2.cpp:232)
+ *   coplanar_in_circle_unperturbed    predicates2.h:68-69 (src/predicates2.cpp:182)
+ * with the op's points at xyz + 3*idx[5q+k], k = 0 .. (points of the op)-1;
+ * the remaining idx entries are ignored. Equal indices are equal pointers
+ * (the in-circle perturbation compares pointer identity,
+ * src/predicates2.cpp:249-255). orient2d reads the x and y coordinates of
+ * its three points. out[q] is the predicate's result as listed with the op
+ * codes above (collinear_position: 0 before, 1 source, 2 middle, 3 target,
+ * 4 after, the reference's CollinearPosition order). A query with an index
+ * outside [0, n_pts) or an unknown op code gets 0.
+ *   xyz device, 3*n_pts doubles; op device, n_q bytes (DEL3D_OP2_*);
+ *   idx device, 5*n_q int32; out device, n_q int8.
+ */
+int del3d_predicates2(const double* xyz, int64_t n_pts, const uint8_t* op, const int32_t* idx,
+                      int64_t n_q, int8_t* out, void* stream);
+
 /* Host-memory forms: copy in, compute on the current device, copy out,
  * synchronously. For callers without device buffers. */
 int del3d_predicates_host(const double* xyz, int64_t n_pts, const uint8_t* op,
                           const int32_t* idx, int64_t n_q, int8_t* sign_out);
+int del3d_predicates2_host(const double* xyz, int64_t n_pts, const uint8_t* op,
+                           const int32_t* idx, int64_t n_q, int8_t* out);
 int del3d_spatial_sort_host(const double* xyz, int64_t n_pts, int32_t* order, int64_t n_order);
 
 /*
@@ -247,6 +290,20 @@ int del3d_in_sphere(const double* a, const double* b, const double* c, const dou
                     const double* e);
 int del3d_in_sphere_unperturbed(const double* a, const double* b, const double* c,
                                 const double* d, const double* e);
+
+/* The coplanar predicates, scalar (include/del3d/predicates2.h:26-69); same
+ * conventions. orient2d takes the six coordinates as the reference does. */
+int del3d_orient2d(double px, double py, double qx, double qy, double rx, double ry);
+int del3d_coplanar_orientation(const double* p, const double* q, const double* r);
+int del3d_coplanar_orientation4(const double* p, const double* q, const double* r,
+                                const double* s);
+int del3d_collinear(const double* p, const double* q, const double* r);
+int del3d_equal(const double* p, const double* q);
+int del3d_collinear_position(const double* p, const double* q, const double* r);
+int del3d_coplanar_in_circle(const double* p0, const double* p1, const double* p2,
+                             const double* p);
+int del3d_coplanar_in_circle_unperturbed(const double* p0, const double* p1, const double* p2,
+                                         const double* p);
 
 #ifdef __cplusplus
 }

@@ -6,8 +6,12 @@
 //   Sign in_sphere(...)              include/del3d/predicates.h:49-50
 //   Sign in_sphere_unperturbed(...)  include/del3d/predicates.h:54-55
 //   void spatial_sort(points, order) include/del3d/spatial_sort.h:37
+//   the coplanar predicates          include/del3d/predicates2.h:26-69
+//     (orient2d, coplanar_orientation x2, collinear, equal,
+//      collinear_position, coplanar_in_circle(_unperturbed))
 // The enum and struct layouts are the reference's (predicates.h:21,
-// spatial_sort.h:31), which the mangled symbols and the data layout depend on.
+// predicates2.h:50, spatial_sort.h:31), which the mangled symbols and the
+// data layout depend on.
 #ifndef DEL3D_AMD_DROPIN_HPP
 #define DEL3D_AMD_DROPIN_HPP
 
@@ -26,6 +30,18 @@ Sign in_sphere(const double* a, const double* b, const double* c, const double* 
 Sign in_sphere_unperturbed(const double* a, const double* b, const double* c, const double* d,
                            const double* e);
 void spatial_sort(const std::vector<Point3>& points, std::vector<int>& order);
+
+enum CollinearPosition { CP_BEFORE, CP_SOURCE, CP_MIDDLE, CP_TARGET, CP_AFTER };
+
+Sign orient2d(double px, double py, double qx, double qy, double rx, double ry);
+Sign coplanar_orientation(const double* p, const double* q, const double* r);
+Sign coplanar_orientation(const double* p, const double* q, const double* r, const double* s);
+bool collinear(const double* p, const double* q, const double* r);
+bool equal(const double* p, const double* q);
+CollinearPosition collinear_position(const double* p, const double* q, const double* r);
+Sign coplanar_in_circle(const double* p0, const double* p1, const double* p2, const double* p);
+Sign coplanar_in_circle_unperturbed(const double* p0, const double* p1, const double* p2,
+                                    const double* p);
 
 }  // namespace del3d
 

@@ -23,9 +23,10 @@
 #include <stdlib.h>
 #include <string.h>
 
-/* 32-bit limbs, sign-magnitude, fixed capacity. 5 factors of up to
- * 2099+53 bits each need < 10800 bits; 352 limbs hold 11264. */
-#define BI_CAP 352
+/* 32-bit limbs, sign-magnitude, fixed capacity. The widest product is the
+ * coplanar in-circle's degree-7 term: 7 factors of up to 2099+53 bits each
+ * need < 15100 bits; 512 limbs hold 16384. */
+#define BI_CAP 512
 
 typedef struct {
     int sg;                 /* -1, 0, +1 */
@@ -322,6 +323,166 @@ void orc_int_pred_batch(const double* xyz, int64_t n_pts, const uint8_t* op,
             const double* e = xyz + 3 * (int64_t)r[4];
             s = op[q] == ORC_OP_INSPHERE ? orc_int_in_sphere(a, b, c, d, e)
                                          : orc_int_in_sphere_raw(a, b, c, d, e);
+        }
+        sign_out[q] = (int8_t)s;
+    }
+}
+
+/* ---- coplanar predicates, method of tests/exact.h:294-378 --------------- */
+
+int orc_int_orient2d(double px, double py, double qx, double qy, double rx, double ry) {
+    Frame f = {0, 0};
+    frame_add(&f, px); frame_add(&f, py); frame_add(&f, qx);
+    frame_add(&f, qy); frame_add(&f, rx); frame_add(&f, ry);
+    Big* v = (Big*)malloc(sizeof(Big) * 8);
+    if (!v) abort();
+    frame_conv(&f, px, v + 0); frame_conv(&f, py, v + 1);
+    frame_conv(&f, qx, v + 2); frame_conv(&f, qy, v + 3);
+    frame_conv(&f, rx, v + 4); frame_conv(&f, ry, v + 5);
+    big_sub(v + 2, v + 2, v + 0); big_sub(v + 3, v + 3, v + 1);  /* q - p */
+    big_sub(v + 4, v + 4, v + 0); big_sub(v + 5, v + 5, v + 1);  /* r - p */
+    minor2(v + 6, v + 2, v + 5, v + 3, v + 4);
+    const int s = v[6].sg;
+    free(v);
+    return s;
+}
+
+static int int_cop_orient3(const double* p, const double* q, const double* r) {
+    int o = orc_int_orient2d(p[0], p[1], q[0], q[1], r[0], r[1]);
+    if (o) return o;
+    o = orc_int_orient2d(p[1], p[2], q[1], q[2], r[1], r[2]);
+    if (o) return o;
+    return orc_int_orient2d(p[0], p[2], q[0], q[2], r[0], r[2]);
+}
+
+static int int_cop_orient4(const double* p, const double* q, const double* r, const double* s) {
+    int o = orc_int_orient2d(p[0], p[1], q[0], q[1], r[0], r[1]);
+    if (o) return o * orc_int_orient2d(p[0], p[1], q[0], q[1], s[0], s[1]);
+    o = orc_int_orient2d(p[1], p[2], q[1], q[2], r[1], r[2]);
+    if (o) return o * orc_int_orient2d(p[1], p[2], q[1], q[2], s[1], s[2]);
+    o = orc_int_orient2d(p[0], p[2], q[0], q[2], r[0], r[2]);
+    return o * orc_int_orient2d(p[0], p[2], q[0], q[2], s[0], s[2]);
+}
+
+/* all three projections degenerate, in the order xy, yz, xz of tests/exact.h */
+static int int_collinear(const double* p, const double* q, const double* r) {
+    return orc_int_orient2d(p[0], p[1], q[0], q[1], r[0], r[1]) == 0 &&
+           orc_int_orient2d(p[1], p[2], q[1], q[2], r[1], r[2]) == 0 &&
+           orc_int_orient2d(p[0], p[2], q[0], q[2], r[0], r[2]) == 0;
+}
+
+/* In-sphere of p against (p0, p1, p2, apex), apex = p0 + (p1-p0) x (p2-p0),
+ * all relative to p: -sign det4_lifted(A, B, C, D). */
+static int int_in_circle_raw(const double* p0, const double* p1, const double* p2,
+                             const double* p) {
+    Frame f = {0, 0};
+    for (int k = 0; k < 3; ++k) {
+        frame_add(&f, p0[k]); frame_add(&f, p1[k]); frame_add(&f, p2[k]); frame_add(&f, p[k]);
+    }
+    V3* V = (V3*)malloc(sizeof(V3) * 9);
+    Big* t = (Big*)malloc(sizeof(Big) * 2);
+    if (!V || !t) abort();
+    V3 *P = V, *A = V + 1, *B = V + 2, *C = V + 3, *D = V + 4, *T = V + 5, *u = V + 6, *w = V + 7;
+    to_v3(&f, p, P);
+    to_v3(&f, p0, T); v3_sub(A, T, P);
+    to_v3(&f, p1, T); v3_sub(B, T, P);
+    to_v3(&f, p2, T); v3_sub(C, T, P);
+    v3_sub(u, B, A);
+    v3_sub(w, C, A);
+    /* n = u x w; D = A + n */
+    minor2(&t[0], &u->y, &w->z, &u->z, &w->y); big_add(&D->x, &A->x, &t[0]);
+    minor2(&t[0], &u->z, &w->x, &u->x, &w->z); big_add(&D->y, &A->y, &t[0]);
+    minor2(&t[0], &u->x, &w->y, &u->y, &w->x); big_add(&D->z, &A->z, &t[0]);
+    det4_lifted(&t[1], A, B, C, D);
+    const int s = -t[1].sg;
+    free(t);
+    free(V);
+    return s;
+}
+
+static int int_in_circle(const double* p0, const double* p1, const double* p2,
+                         const double* p) {
+    const int bs = int_in_circle_raw(p0, p1, p2, p);
+    if (bs != 0) return bs;
+    /* rank the four slots by (coordinates, slot): a stable order */
+    const double* s[4] = {p0, p1, p2, p};
+    const double* sorted[4];
+    for (int i = 0; i < 4; ++i) {
+        int r = 0;
+        for (int j = 0; j < 4; ++j) {
+            if (j == i) continue;
+            const double* x = s[j];
+            const double* y = s[i];
+            int c = 0;
+            for (int k = 0; k < 3 && c == 0; ++k) c = x[k] < y[k] ? -1 : (x[k] > y[k] ? 1 : 0);
+            if (c < 0 || (c == 0 && j < i)) ++r;
+        }
+        sorted[r] = s[i];
+    }
+    const int local = int_cop_orient3(p0, p1, p2);
+    for (int i = 3; i > 0; --i) {
+        if (sorted[i] == p) return -1;
+        int o;
+        if (sorted[i] == p2 && (o = int_cop_orient3(p0, p1, p)) != 0) return o * local;
+        if (sorted[i] == p1 && (o = int_cop_orient3(p0, p, p2)) != 0) return o * local;
+        if (sorted[i] == p0 && (o = int_cop_orient3(p, p1, p2)) != 0) return o * local;
+    }
+    return -local;
+}
+
+static int int_equal(const double* p, const double* q) {
+    return p[0] == q[0] && p[1] == q[1] && p[2] == q[2];
+}
+
+/* Position of q on the line (p, r) from the parameter t of q = p + t (r - p),
+ * decided in exact integers along the coordinate of largest extent: a second
+ * route to the answer of src/predicates2.cpp:162-180 on collinear input. */
+static int int_position(const double* p, const double* q, const double* r) {
+    if (int_equal(q, p)) return 1;
+    if (int_equal(q, r)) return 3;
+    int k = 0;
+    for (int j = 1; j < 3; ++j)
+        if (fabs(r[j] - p[j]) > fabs(r[k] - p[k])) k = j;
+    /* compare (q-p)_k with 0 and with (r-p)_k, in the direction of r-p */
+    Frame f = {0, 0};
+    frame_add(&f, p[k]); frame_add(&f, q[k]); frame_add(&f, r[k]);
+    Big* v = (Big*)malloc(sizeof(Big) * 5);
+    if (!v) abort();
+    frame_conv(&f, p[k], v); frame_conv(&f, q[k], v + 1); frame_conv(&f, r[k], v + 2);
+    big_sub(v + 3, v + 1, v);      /* q - p */
+    big_sub(v + 4, v + 2, v);      /* r - p */
+    const int dir = v[4].sg;
+    big_sub(v + 2, v + 3, v + 4);  /* q - r */
+    const int s0 = v[3].sg * dir, s1 = v[2].sg * dir;
+    free(v);
+    if (s0 < 0) return 0;
+    if (s1 > 0) return 4;
+    return 2;
+}
+
+void orc_int_pred2_batch(const double* xyz, int64_t n_pts, const uint8_t* op,
+                         const int32_t* idx5, int64_t n_q, int8_t* sign_out) {
+    (void)n_pts;
+    for (int64_t q = 0; q < n_q; ++q) {
+        const int32_t* r = idx5 + 5 * q;
+        const int np = op[q] == ORC_OP2_EQUAL ? 2
+                     : (op[q] == ORC_OP2_COPLANAR_ORIENTATION4 || op[q] == ORC_OP2_IN_CIRCLE ||
+                        op[q] == ORC_OP2_IN_CIRCLE_UNPERTURBED) ? 4 : 3;
+        const double* p[4];
+        for (int k = 0; k < 4; ++k) p[k] = k < np ? xyz + 3 * (int64_t)r[k] : xyz;
+        int s;
+        switch (op[q]) {
+            case ORC_OP2_COPLANAR_ORIENTATION: s = int_cop_orient3(p[0], p[1], p[2]); break;
+            case ORC_OP2_COPLANAR_ORIENTATION4: s = int_cop_orient4(p[0], p[1], p[2], p[3]); break;
+            case ORC_OP2_COLLINEAR: s = int_collinear(p[0], p[1], p[2]); break;
+            case ORC_OP2_EQUAL: s = int_equal(p[0], p[1]); break;
+            case ORC_OP2_COLLINEAR_POSITION: s = int_position(p[0], p[1], p[2]); break;
+            case ORC_OP2_IN_CIRCLE: s = int_in_circle(p[0], p[1], p[2], p[3]); break;
+            case ORC_OP2_IN_CIRCLE_UNPERTURBED: s = int_in_circle_raw(p[0], p[1], p[2], p[3]); break;
+            case ORC_OP2_ORIENT2D_XY:
+                s = orc_int_orient2d(p[0][0], p[0][1], p[1][0], p[1][1], p[2][0], p[2][1]);
+                break;
+            default: s = 0; break;
         }
         sign_out[q] = (int8_t)s;
     }

@@ -9,6 +9,8 @@
  * What it restates (reference paths relative to the del3d source tree):
  *   - predicates   src/predicates.cpp:22-232 (filter, expansion exact stage,
  *                  symbolic perturbation) with src/expansion.h:33-178
+ *   - coplanar     src/predicates2.cpp:22-259 (orient2d, coplanar orientation,
+ *                  collinear, equal, collinear_position, in-circle)
  *   - an independent big-integer predicate oracle following the method of
  *                  tests/exact.h:161-291 (power-of-two frame, integer dets)
  *   - spatial sort src/spatial_sort.cpp:17-151 with src/rand48.h:26-153
@@ -19,6 +21,9 @@
  *     the known answers of tests/test_predicates.cpp:64-76;
  *   - perturbation: the rule text of include/del3d/predicates.h:28-48 and a
  *     second restatement over the big-integer oracle;
+ *   - coplanar: the known answers of tests/test_predicates2.cpp:64-90 and the
+ *     big-integer oracle's apex method (tests/exact.h:294-378) on exactly
+ *     coplanar queries; the compiled reference where it builds;
  *   - rand48 draws: glibc srand48/lrand48 (same LCG constants and output);
  *   - spatial sort: the 17 pinned FNV-1a hashes of
  *     tests/test_spatial_sort.cpp:115-147 and, where the reference builds,
@@ -53,6 +58,25 @@ void orc_pred_batch(const double* xyz, int64_t n_pts, const uint8_t* op,
  * [0]=filter-decided, [1]=exact stage entered, [2]=perturbation entered. */
 void orc_last_stage_counts(int64_t out[3]);
 
+/* ---- coplanar predicates (restating src/predicates2.cpp) --------------- */
+/* Op codes of the coplanar batch (idx[5*q ..]: the op's points in argument
+ * order). Results: orientations -1/0/+1, collinear/equal 0/1,
+ * collinear_position 0..4 (before, source, middle, target, after). */
+enum {
+    ORC_OP2_COPLANAR_ORIENTATION = 0,  /* (p,q,r) */
+    ORC_OP2_COPLANAR_ORIENTATION4 = 1, /* (p,q,r,s) */
+    ORC_OP2_COLLINEAR = 2,             /* (p,q,r) */
+    ORC_OP2_EQUAL = 3,                 /* (p,q) */
+    ORC_OP2_COLLINEAR_POSITION = 4,    /* (p,q,r) */
+    ORC_OP2_IN_CIRCLE = 5,             /* (p0,p1,p2,p), perturbed */
+    ORC_OP2_IN_CIRCLE_UNPERTURBED = 6, /* (p0,p1,p2,p) */
+    ORC_OP2_ORIENT2D_XY = 7            /* orient2d on the xy coordinates of (p,q,r) */
+};
+int orc_pred2_one(uint8_t op, const double* a, const double* b, const double* c,
+                  const double* d);
+void orc_pred2_batch(const double* xyz, int64_t n_pts, const uint8_t* op,
+                     const int32_t* idx5, int64_t n_q, int8_t* sign_out);
+
 /* ---- big-integer oracle (method of tests/exact.h) ---------------------- */
 int orc_int_orient3d(const double* a, const double* b, const double* c, const double* d);
 int orc_int_in_sphere_raw(const double* a, const double* b, const double* c,
@@ -61,6 +85,13 @@ int orc_int_in_sphere(const double* a, const double* b, const double* c,
                       const double* d, const double* e);
 void orc_int_pred_batch(const double* xyz, int64_t n_pts, const uint8_t* op,
                         const int32_t* idx5, int64_t n_q, int8_t* sign_out);
+/* Coplanar predicates by the method of tests/exact.h:294-378: orient2d on an
+ * integer frame, and the in-circle test as an in-sphere test against the apex
+ * p0 + (p1-p0) x (p2-p0). The in-circle answers equal the library's only for
+ * exactly coplanar queries (its precondition). Same op codes as above. */
+int orc_int_orient2d(double px, double py, double qx, double qy, double rx, double ry);
+void orc_int_pred2_batch(const double* xyz, int64_t n_pts, const uint8_t* op,
+                         const int32_t* idx5, int64_t n_q, int8_t* sign_out);
 
 /* ---- spatial sort (restating src/spatial_sort.cpp + src/rand48.h) ------ */
 /* order[0..n) is permuted in place; xyz is AoS (Point3). */

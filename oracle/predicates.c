@@ -112,26 +112,35 @@ static Exp exp_diff2(double a, double b) {
     return r;
 }
 
-/* Add one double into an expansion (the Grow step of src/expansion.h:137-150). */
-static Exp exp_grow(Exp x, double b) {
-    if (b == 0.0) return x;
-    Exp r;
-    r.c = arena_alloc((size_t)x.n + 1);
-    r.n = 0;
+/* Add one double into an expansion (the Grow step of src/expansion.h:137-150),
+ * writing the result to out (x.n + 1 doubles, not overlapping x). */
+static int grow_into(const double* x, int n, double b, double* out) {
+    int m = 0;
     double q = b;
-    for (int i = 0; i < x.n; ++i) {
+    for (int i = 0; i < n; ++i) {
         double s, e;
-        eft_sum(q, x.c[i], &s, &e);
-        if (e != 0.0) r.c[r.n++] = e;
+        eft_sum(q, x[i], &s, &e);
+        if (e != 0.0) out[m++] = e;
         q = s;
     }
-    if (q != 0.0) r.c[r.n++] = q;
-    return r;
+    if (q != 0.0) out[m++] = q;
+    return m;
 }
 
+/* x + y: y's components grown into x one at a time, ping-ponging between two
+ * buffers (the intermediate expansions are dropped as soon as used). */
 static Exp exp_add(Exp x, Exp y) {
+    if (y.n == 0) return x;
+    const size_t cap = (size_t)x.n + (size_t)y.n;
+    double* buf[2] = {arena_alloc(cap), arena_alloc(cap)};
     Exp r = x;
-    for (int i = 0; i < y.n; ++i) r = exp_grow(r, y.c[i]);
+    int w = 0;
+    for (int i = 0; i < y.n; ++i) {
+        if (y.c[i] == 0.0) continue;
+        r.n = grow_into(r.c, r.n, y.c[i], buf[w]);
+        r.c = buf[w];
+        w ^= 1;
+    }
     return r;
 }
 
@@ -145,38 +154,59 @@ static Exp exp_neg(Exp x) {
 
 static Exp exp_sub(Exp x, Exp y) { return exp_add(x, exp_neg(y)); }
 
-/* Scale an expansion by one double (src/expansion.h:155-175). */
-static Exp exp_scale(Exp x, double b) {
-    Exp r;
-    r.n = 0;
-    r.c = NULL;
-    if (b == 0.0 || x.n == 0) return r;
-    r.c = arena_alloc(2 * (size_t)x.n);
+/* Scale an expansion by one double (src/expansion.h:155-175) into out
+ * (2 * x.n doubles); returns the length. */
+static int scale_into(Exp x, double b, double* out) {
+    int m = 0;
+    if (b == 0.0 || x.n == 0) return 0;
     double q, lo;
     eft_product(x.c[0], b, &q, &lo);
-    if (lo != 0.0) r.c[r.n++] = lo;
+    if (lo != 0.0) out[m++] = lo;
     for (int i = 1; i < x.n; ++i) {
         double ph, pl;
         eft_product(x.c[i], b, &ph, &pl);
         double s1, e1;
         eft_sum(q, pl, &s1, &e1);
-        if (e1 != 0.0) r.c[r.n++] = e1;
+        if (e1 != 0.0) out[m++] = e1;
         double s2, e2;
         eft_sum(ph, s1, &s2, &e2);
-        if (e2 != 0.0) r.c[r.n++] = e2;
+        if (e2 != 0.0) out[m++] = e2;
         q = s2;
     }
-    if (q != 0.0) r.c[r.n++] = q;
-    return r;
+    if (q != 0.0) out[m++] = q;
+    return m;
 }
 
-/* Product: accumulate x scaled by each component of y (src/expansion.h:127-131). */
+/* Product: accumulate x scaled by each component of y (src/expansion.h:127-131).
+ * The accumulator ping-pongs between two heap buffers of the largest possible
+ * length, so memory stays O(x.n * y.n) however long the operands are; the
+ * result is copied into the arena. */
 static Exp exp_mul(Exp x, Exp y) {
-    Exp acc;
-    acc.n = 0;
-    acc.c = NULL;
-    for (int i = 0; i < y.n; ++i) acc = exp_add(acc, exp_scale(x, y.c[i]));
-    return acc;
+    Exp r;
+    r.n = 0;
+    r.c = NULL;
+    if (x.n == 0 || y.n == 0) return r;
+    const size_t cap = 2 * (size_t)x.n * (size_t)y.n + 1;
+    double* buf[3];
+    buf[0] = (double*)malloc(cap * sizeof(double));
+    buf[1] = (double*)malloc(cap * sizeof(double));
+    buf[2] = (double*)malloc(2 * (size_t)x.n * sizeof(double));
+    if (!buf[0] || !buf[1] || !buf[2]) abort();
+    int n = 0, cur = 0;
+    for (int i = 0; i < y.n; ++i) {
+        const int m = scale_into(x, y.c[i], buf[2]);
+        for (int k = 0; k < m; ++k) {
+            n = grow_into(buf[cur], n, buf[2][k], buf[cur ^ 1]);
+            cur ^= 1;
+        }
+    }
+    r.n = n;
+    r.c = arena_alloc((size_t)n);
+    memcpy(r.c, buf[cur], sizeof(double) * (size_t)n);
+    free(buf[0]);
+    free(buf[1]);
+    free(buf[2]);
+    return r;
 }
 
 static int exp_sign(Exp x) {
@@ -385,4 +415,200 @@ void orc_last_stage_counts(int64_t out[3]) {
     out[0] = g_stage[0];
     out[1] = g_stage[1];
     out[2] = g_stage[2];
+}
+
+/* ====================================================================== */
+/* The coplanar predicates (restating src/predicates2.cpp).                */
+/*   src/predicates2.cpp:22-26   error-bound constants                     */
+/*   src/predicates2.cpp:34-40   orient2d_exact                            */
+/*   src/predicates2.cpp:59-101  in_circle_exact (rows pt, rt, qt, v)      */
+/*   src/predicates2.cpp:111-180 orient2d .. collinear_position            */
+/*   src/predicates2.cpp:182-259 coplanar_in_circle(_unperturbed)          */
+/* ====================================================================== */
+#define ORIENT2D_BOUND ((3.0 + 16.0 * kEps) * kEps)
+#define INCIRCLE_BOUND ((16.0 + 224.0 * kEps) * kEps)
+
+static int orient2d_impl(double px, double py, double qx, double qy, double rx, double ry) {
+    const double qpx = qx - px, qpy = qy - py;
+    const double rpx = rx - px, rpy = ry - py;
+    const double a = qpx * rpy, b = qpy * rpx;
+    const double det = a - b;
+    const double bound = ORIENT2D_BOUND * (fabs(a) + fabs(b));
+    if (det > bound || -det > bound) { g_stage[0]++; return sgn(det); }
+    g_stage[1]++;
+    Exp ex = exp_sub(exp_mul(exp_diff2(qx, px), exp_diff2(ry, py)),
+                     exp_mul(exp_diff2(qy, py), exp_diff2(rx, px)));
+    return exp_sign(ex);
+}
+
+static int cop_orient3(const double* p, const double* q, const double* r) {
+    int o = orient2d_impl(p[0], p[1], q[0], q[1], r[0], r[1]);
+    if (o) return o;
+    o = orient2d_impl(p[1], p[2], q[1], q[2], r[1], r[2]);
+    if (o) return o;
+    return orient2d_impl(p[0], p[2], q[0], q[2], r[0], r[2]);
+}
+
+static int cop_orient4(const double* p, const double* q, const double* r, const double* s) {
+    int o = orient2d_impl(p[0], p[1], q[0], q[1], r[0], r[1]);
+    if (o) return o * orient2d_impl(p[0], p[1], q[0], q[1], s[0], s[1]);
+    o = orient2d_impl(p[1], p[2], q[1], q[2], r[1], r[2]);
+    if (o) return o * orient2d_impl(p[1], p[2], q[1], q[2], s[1], s[2]);
+    o = orient2d_impl(p[0], p[2], q[0], q[2], r[0], r[2]);
+    return o * orient2d_impl(p[0], p[2], q[0], q[2], s[0], s[2]);
+}
+
+static int cop_collinear(const double* p, const double* q, const double* r) {
+    if (orient2d_impl(r[0], r[1], p[0], p[1], q[0], q[1]) != 0) return 0;
+    if (orient2d_impl(r[0], r[2], p[0], p[2], q[0], q[2]) != 0) return 0;
+    return orient2d_impl(r[1], r[2], p[1], p[2], q[1], q[2]) == 0;
+}
+
+static int cop_equal(const double* p, const double* q) {
+    return p[0] == q[0] && p[1] == q[1] && p[2] == q[2];
+}
+
+/* 0 before p, 1 at p, 2 between, 3 at r, 4 after r */
+static int cop_position(const double* p, const double* q, const double* r) {
+    if (cop_equal(q, p)) return 1;
+    if (cop_equal(q, r)) return 3;
+    const int k = p[0] != r[0] ? 0 : (p[1] != r[1] ? 1 : 2);
+    const double a = p[k], b = q[k], c = r[k];
+    if (a < c) return b < a ? 0 : (b > c ? 4 : 2);
+    return b > a ? 0 : (b < c ? 4 : 2);
+}
+
+/* The 4x4 determinant of rows (pt, rt, qt, v) lifted, v = pq x pr, its 2x2
+ * minors formed as m_ij = a_j0 a_i1 - a_i0 a_j1, in expansion arithmetic. */
+static int in_circle_exact_raw(const double* p0, const double* p1, const double* p2,
+                               const double* t) {
+    Exp a[4][4];
+    for (int k = 0; k < 3; ++k) {
+        a[0][k] = exp_diff2(p0[k], t[k]);
+        a[1][k] = exp_diff2(p2[k], t[k]);
+        a[2][k] = exp_diff2(p1[k], t[k]);
+    }
+    for (int i = 0; i < 3; ++i)
+        a[i][3] = exp_add(exp_add(exp_mul(a[i][0], a[i][0]), exp_mul(a[i][1], a[i][1])),
+                          exp_mul(a[i][2], a[i][2]));
+    Exp pq[3], pr[3];
+    for (int k = 0; k < 3; ++k) { pq[k] = exp_diff2(p1[k], p0[k]); pr[k] = exp_diff2(p2[k], p0[k]); }
+    a[3][0] = exp_sub(exp_mul(pq[1], pr[2]), exp_mul(pq[2], pr[1]));
+    a[3][1] = exp_sub(exp_mul(pq[2], pr[0]), exp_mul(pq[0], pr[2]));
+    a[3][2] = exp_sub(exp_mul(pq[0], pr[1]), exp_mul(pq[1], pr[0]));
+    a[3][3] = exp_add(exp_add(exp_mul(a[3][0], a[3][0]), exp_mul(a[3][1], a[3][1])),
+                      exp_mul(a[3][2], a[3][2]));
+#define M2(i, j) exp_sub(exp_mul(a[j][0], a[i][1]), exp_mul(a[i][0], a[j][1]))
+    const Exp m01 = M2(0, 1), m02 = M2(0, 2), m03 = M2(0, 3);
+    const Exp m12 = M2(1, 2), m13 = M2(1, 3), m23 = M2(2, 3);
+#undef M2
+#define M3(x, y, z, r0, r1, r2) \
+    exp_add(exp_sub(exp_mul(x, a[r0][2]), exp_mul(y, a[r1][2])), exp_mul(z, a[r2][2]))
+    const Exp m012 = M3(m12, m02, m01, 0, 1, 2);
+    const Exp m013 = M3(m13, m03, m01, 0, 1, 3);
+    const Exp m023 = M3(m23, m03, m02, 0, 2, 3);
+    const Exp m123 = M3(m23, m13, m12, 1, 2, 3);
+#undef M3
+    const Exp det = exp_sub(exp_add(exp_sub(exp_mul(m123, a[0][3]), exp_mul(m023, a[1][3])),
+                                    exp_mul(m013, a[2][3])),
+                            exp_mul(m012, a[3][3]));
+    return exp_sign(det);
+}
+
+static int in_circle_raw_impl(const double* p0, const double* p1, const double* p2,
+                              const double* p) {
+    double a[4][4];
+    for (int k = 0; k < 3; ++k) {
+        a[0][k] = p0[k] - p[k];
+        a[2][k] = p1[k] - p[k];
+        a[1][k] = p2[k] - p[k];
+    }
+    a[0][3] = a[0][0] * a[0][0] + a[0][1] * a[0][1] + a[0][2] * a[0][2];
+    a[2][3] = a[2][0] * a[2][0] + a[2][1] * a[2][1] + a[2][2] * a[2][2];
+    a[1][3] = a[1][0] * a[1][0] + a[1][1] * a[1][1] + a[1][2] * a[1][2];
+    double pq[3], pr[3];
+    for (int k = 0; k < 3; ++k) { pq[k] = p1[k] - p0[k]; pr[k] = p2[k] - p0[k]; }
+    a[3][0] = pq[1] * pr[2] - pq[2] * pr[1];
+    a[3][1] = pq[2] * pr[0] - pq[0] * pr[2];
+    a[3][2] = pq[0] * pr[1] - pq[1] * pr[0];
+    a[3][3] = a[3][0] * a[3][0] + a[3][1] * a[3][1] + a[3][2] * a[3][2];
+#define M2(i, j) (a[j][0] * a[i][1] - a[i][0] * a[j][1])
+    const double m01 = M2(0, 1), m02 = M2(0, 2), m03 = M2(0, 3);
+    const double m12 = M2(1, 2), m13 = M2(1, 3), m23 = M2(2, 3);
+#undef M2
+    const double m012 = m12 * a[0][2] - m02 * a[1][2] + m01 * a[2][2];
+    const double m013 = m13 * a[0][2] - m03 * a[1][2] + m01 * a[3][2];
+    const double m023 = m23 * a[0][2] - m03 * a[2][2] + m02 * a[3][2];
+    const double m123 = m23 * a[1][2] - m13 * a[2][2] + m12 * a[3][2];
+    const double det = m123 * a[0][3] - m023 * a[1][3] + m013 * a[2][3] - m012 * a[3][3];
+    double mag = 0.0;
+    for (int i = 0; i < 4; ++i) {
+        double row = 0.0;
+        for (int j = 0; j < 4; ++j) row += fabs(a[i][j]);
+        mag = mag == 0.0 ? row : mag * row;
+    }
+    const double bound = INCIRCLE_BOUND * mag;
+    if (det > bound || -det > bound) { g_stage[0]++; return sgn(det); }
+    g_stage[1]++;
+    return in_circle_exact_raw(p0, p1, p2, p);
+}
+
+static void sort4_stable(const double* p[4]) {
+    for (int i = 1; i < 4; ++i) {
+        const double* v = p[i];
+        int j = i;
+        while (j > 0 && lex_less(v, p[j - 1])) { p[j] = p[j - 1]; --j; }
+        p[j] = v;
+    }
+}
+
+static int in_circle_impl(const double* p0, const double* p1, const double* p2,
+                          const double* p) {
+    const int s = in_circle_raw_impl(p0, p1, p2, p);
+    if (s != 0) return s;
+    g_stage[2]++;
+    const double* pts[4] = {p0, p1, p2, p};
+    sort4_stable(pts);
+    const int local = cop_orient3(p0, p1, p2);
+    for (int i = 3; i > 0; --i) {
+        if (pts[i] == p) return -1;
+        int o;
+        if (pts[i] == p2 && (o = cop_orient3(p0, p1, p)) != 0) return o * local;
+        if (pts[i] == p1 && (o = cop_orient3(p0, p, p2)) != 0) return o * local;
+        if (pts[i] == p0 && (o = cop_orient3(p, p1, p2)) != 0) return o * local;
+    }
+    return -local;
+}
+
+int orc_pred2_one(uint8_t op, const double* a, const double* b, const double* c,
+                  const double* d) {
+    int s;
+    switch (op) {
+        case ORC_OP2_COPLANAR_ORIENTATION: s = cop_orient3(a, b, c); break;
+        case ORC_OP2_COPLANAR_ORIENTATION4: s = cop_orient4(a, b, c, d); break;
+        case ORC_OP2_COLLINEAR: s = cop_collinear(a, b, c); break;
+        case ORC_OP2_EQUAL: s = cop_equal(a, b); break;
+        case ORC_OP2_COLLINEAR_POSITION: s = cop_position(a, b, c); break;
+        case ORC_OP2_IN_CIRCLE: s = in_circle_impl(a, b, c, d); break;
+        case ORC_OP2_IN_CIRCLE_UNPERTURBED: s = in_circle_raw_impl(a, b, c, d); break;
+        case ORC_OP2_ORIENT2D_XY: s = orient2d_impl(a[0], a[1], b[0], b[1], c[0], c[1]); break;
+        default: s = 0; break;
+    }
+    arena_reset();
+    return s;
+}
+
+void orc_pred2_batch(const double* xyz, int64_t n_pts, const uint8_t* op,
+                     const int32_t* idx5, int64_t n_q, int8_t* sign_out) {
+    (void)n_pts;
+    g_stage[0] = g_stage[1] = g_stage[2] = 0;
+    for (int64_t q = 0; q < n_q; ++q) {
+        const int32_t* r = idx5 + 5 * q;
+        const int np = op[q] == ORC_OP2_EQUAL ? 2
+                     : (op[q] == ORC_OP2_COPLANAR_ORIENTATION4 || op[q] == ORC_OP2_IN_CIRCLE ||
+                        op[q] == ORC_OP2_IN_CIRCLE_UNPERTURBED) ? 4 : 3;
+        const double* p[4];
+        for (int k = 0; k < 4; ++k) p[k] = k < np ? xyz + 3 * (int64_t)r[k] : xyz;
+        sign_out[q] = (int8_t)orc_pred2_one(op[q], p[0], p[1], p[2], p[3]);
+    }
 }

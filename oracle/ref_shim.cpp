@@ -6,10 +6,12 @@
 // that tests can compare against the real reference and bench.py can time it
 // as the CPU baseline. The declarations used are the reference's public ones:
 //   include/del3d/predicates.h:26,49-50,54-55  orient3d / in_sphere(_unperturbed)
+//   include/del3d/predicates2.h:26-69          the coplanar predicates
 //   include/del3d/spatial_sort.h:31,37         Point3 / spatial_sort
 //   include/del3d/delaunay.h:62-92             Delaunay (build, finite_cells)
 #include "del3d/delaunay.h"
 #include "del3d/predicates.h"
+#include "del3d/predicates2.h"
 #include "del3d/spatial_sort.h"
 
 #include <cstdint>
@@ -50,6 +52,34 @@ void ref_pred_batch(const double* xyz, int64_t n_pts, const uint8_t* op, const i
             const double* p4 = xyz + 3 * int64_t(r[4]);
             s = op[q] == 1 ? del3d::in_sphere(p0, p1, p2, p3, p4)
                            : del3d::in_sphere_unperturbed(p0, p1, p2, p3, p4);
+        }
+        out[q] = static_cast<int8_t>(s);
+    }
+}
+
+// Coplanar batch: op 0 coplanar_orientation(p,q,r), 1 coplanar_orientation(p,q,r,s),
+// 2 collinear, 3 equal(p,q), 4 collinear_position, 5 coplanar_in_circle,
+// 6 coplanar_in_circle_unperturbed, 7 orient2d on the xy coordinates of (p,q,r).
+// Points are pointers into xyz, so equal indices are the same pointer.
+void ref_pred2_batch(const double* xyz, int64_t n_pts, const uint8_t* op, const int32_t* idx5,
+                     int64_t n_q, int8_t* out) {
+    (void)n_pts;
+    for (int64_t q = 0; q < n_q; ++q) {
+        const int32_t* r = idx5 + 5 * q;
+        const int np = op[q] == 3 ? 2 : (op[q] == 1 || op[q] == 5 || op[q] == 6) ? 4 : 3;
+        const double* p[4];
+        for (int k = 0; k < 4; ++k) p[k] = k < np ? xyz + 3 * int64_t(r[k]) : xyz;
+        int s = 0;
+        switch (op[q]) {
+            case 0: s = del3d::coplanar_orientation(p[0], p[1], p[2]); break;
+            case 1: s = del3d::coplanar_orientation(p[0], p[1], p[2], p[3]); break;
+            case 2: s = del3d::collinear(p[0], p[1], p[2]) ? 1 : 0; break;
+            case 3: s = del3d::equal(p[0], p[1]) ? 1 : 0; break;
+            case 4: s = static_cast<int>(del3d::collinear_position(p[0], p[1], p[2])); break;
+            case 5: s = del3d::coplanar_in_circle(p[0], p[1], p[2], p[3]); break;
+            case 6: s = del3d::coplanar_in_circle_unperturbed(p[0], p[1], p[2], p[3]); break;
+            case 7: s = del3d::orient2d(p[0][0], p[0][1], p[1][0], p[1][1], p[2][0], p[2][1]); break;
+            default: break;
         }
         out[q] = static_cast<int8_t>(s);
     }

@@ -52,3 +52,8 @@ extern "C" int emu_predicates_sets(const double* xyz, const int64_t* pt_off, int
                                    int8_t* out) {
     return del3d_predicates_sets(xyz, pt_off, n_sets, op, idx, q_off, out, nullptr);
 }
+
+extern "C" int emu_predicates2(const double* xyz, int64_t n_pts, const uint8_t* op,
+                               const int32_t* idx, int64_t n_q, int8_t* out) {
+    return del3d_predicates2_host(xyz, n_pts, op, idx, n_q, out);
+}

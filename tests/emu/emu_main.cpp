@@ -192,6 +192,56 @@ static int check_server() {
 extern "C" int emu_spatial_sort_sets(const double* xyz, const int64_t* off, int64_t n_sets,
                                      int32_t* order);
 
+extern "C" int emu_predicates2(const double* xyz, int64_t n_pts, const uint8_t* op,
+                               const int32_t* idx, int64_t n_q, int8_t* out);
+
+// The coplanar batch (fast kernel, queue, exact kernel) on a cocircular ring
+// mapped onto a tilted plane, an integer grid and random points, all ops, with
+// aliased slots, plus queries with bad indices or op codes (result 0).
+static int check_coplanar() {
+    static const int ring[12][2] = {{5, 0}, {4, 3}, {3, 4}, {0, 5}, {-3, 4}, {-4, 3},
+                                    {-5, 0}, {-4, -3}, {-3, -4}, {0, -5}, {3, -4}, {4, -3}};
+    std::vector<double> xyz;
+    for (auto& r : ring) {  // (x, y, 0) mapped by [[1,2,2],[2,1,-2],[2,-2,1]]
+        const double x = r[0], y = r[1];
+        xyz.insert(xyz.end(), {x + 2 * y, 2 * x + y, 2 * x - 2 * y});
+    }
+    for (int i = -3; i <= 3; ++i)
+        for (int j = -3; j <= 3; ++j) xyz.insert(xyz.end(), {(double)i, (double)j, (double)(i + j)});
+    for (int i = 0; i < 40; ++i)
+        for (int k = 0; k < 3; ++k) xyz.push_back((double)(rnd() >> 11) * 0x1p-53);
+    const int64_t n = (int64_t)xyz.size() / 3, nq = 30000;
+    std::vector<uint8_t> op(nq);
+    std::vector<int32_t> idx(5 * nq, 0);
+    for (int64_t q = 0; q < nq; ++q) {
+        op[q] = (uint8_t)(rnd() % 8);
+        // mostly within one group (ring, grid or random) so the queries meet
+        // the predicates' preconditions; some across groups
+        const int64_t g = rnd() % 4, lo = g == 0 ? 0 : g == 1 ? 12 : g == 2 ? 61 : 0;
+        const int64_t w = g == 0 ? 12 : g == 1 ? 49 : g == 2 ? 40 : n;
+        for (int k = 0; k < 4; ++k) idx[5 * q + k] = (int32_t)(lo + rnd() % w);
+        if (q % 9 == 0) idx[5 * q + 3] = idx[5 * q + (rnd() % 3)];
+    }
+    std::vector<int8_t> got(nq), want(nq);
+    if (emu_predicates2(xyz.data(), n, op.data(), idx.data(), nq, got.data()) != 0) {
+        std::printf("coplanar error\n");
+        return 1;
+    }
+    orc_pred2_batch(xyz.data(), n, op.data(), idx.data(), nq, want.data());
+    // bad inputs: an index out of range, a negative index, unknown op codes
+    const uint8_t bop[4] = {5, 0, 8, 200};
+    const int32_t bidx[20] = {0, 1, 2, (int32_t)n, 0, -1, 1, 2, 0, 0, 0, 1, 2, 3, 0, 0, 1, 2, 3, 0};
+    int8_t bout[4] = {9, 9, 9, 9};
+    const int rc = emu_predicates2(xyz.data(), n, bop, bidx, 4, bout);
+    const bool ok = got == want && rc == 0 && bout[0] == 0 && bout[1] == 0 && bout[2] == 0 &&
+                    bout[3] == 0;
+    int64_t st[3];
+    orc_last_stage_counts(st);
+    std::printf("coplanar %s (oracle: filter %lld exact %lld perturbed %lld)\n", ok ? "ok" : "MISMATCH",
+                (long long)st[0], (long long)st[1], (long long)st[2]);
+    return ok ? 0 : 1;
+}
+
 int main(int argc, char** argv) {
     int fails = 0;
     {
@@ -214,6 +264,7 @@ int main(int argc, char** argv) {
         fails += check_paths("lattice", lat, 5000);
         fails += check_paths("wide", wide, 2500);
         fails += check_sets();
+        fails += check_coplanar();
         if (!std::getenv("EMU_DEFER") || std::atoi(std::getenv("EMU_DEFER")) == 0) fails += check_server();
         if (std::getenv("EMU_PRED_ONLY")) return fails ? 1 : 0;
     }

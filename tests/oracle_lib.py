@@ -48,7 +48,8 @@ def oracle() -> C.CDLL:
                      "orc_int_in_sphere", "orc_int_in_sphere_raw"):
             getattr(lib, name).argtypes = [_dp] * 5
             getattr(lib, name).restype = C.c_int
-        for name in ("orc_pred_batch", "orc_int_pred_batch"):
+        for name in ("orc_pred_batch", "orc_int_pred_batch", "orc_pred2_batch",
+                     "orc_int_pred2_batch"):
             getattr(lib, name).argtypes = [_dp, C.c_int64, _u8p, _i32p, C.c_int64, _i8p]
             getattr(lib, name).restype = None
         lib.orc_last_stage_counts.argtypes = [_i64p]
@@ -90,6 +91,7 @@ def ref():
         lib.ref_in_sphere_unperturbed.argtypes = [_dp] * 5
         lib.ref_in_sphere_unperturbed.restype = C.c_int
         lib.ref_pred_batch.argtypes = [_dp, C.c_int64, _u8p, _i32p, C.c_int64, _i8p]
+        lib.ref_pred2_batch.argtypes = [_dp, C.c_int64, _u8p, _i32p, C.c_int64, _i8p]
         lib.ref_pred_batch_mt.argtypes = [_dp, C.c_int64, _u8p, _i32p, C.c_int64, _i8p, C.c_int]
         lib.ref_spatial_sort.argtypes = [_dp, C.c_int64, _i32p]
         lib.ref_delaunay_cells.argtypes = [_dp, C.c_int64, _i32p, C.c_int64]
@@ -124,6 +126,29 @@ def pred_batch(pts, op, idx5, which: str = "oracle") -> np.ndarray:
             raise RuntimeError("reference library not built")
         r.ref_pred_batch(_ptr(xyz, _dp), len(xyz), _ptr(op, _u8p), _ptr(idx5, _i32p),
                          len(op), _ptr(out, _i8p))
+    else:
+        raise ValueError(which)
+    return out
+
+
+def pred2_batch(pts, op, idx5, which: str = "oracle") -> np.ndarray:
+    """Results of a coplanar batch (op codes ORC_OP2_*) from the restatement
+    ("oracle"), the big-integer apex oracle ("int", exactly coplanar queries
+    only) or the compiled reference ("ref")."""
+    xyz = _xyz(pts)
+    op = np.ascontiguousarray(op, dtype=np.uint8)
+    idx5 = np.ascontiguousarray(np.asarray(idx5, dtype=np.int32).reshape(-1, 5))
+    out = np.zeros(len(op), dtype=np.int8)
+    args = (_ptr(xyz, _dp), len(xyz), _ptr(op, _u8p), _ptr(idx5, _i32p), len(op), _ptr(out, _i8p))
+    if which == "oracle":
+        oracle().orc_pred2_batch(*args)
+    elif which == "int":
+        oracle().orc_int_pred2_batch(*args)
+    elif which == "ref":
+        r = ref()
+        if r is None:
+            raise RuntimeError("reference library not built")
+        r.ref_pred2_batch(*args)
     else:
         raise ValueError(which)
     return out

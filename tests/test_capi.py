@@ -17,6 +17,15 @@ DROPIN_SYMBOLS = [
     "_ZN5del3d9in_sphereEPKdS1_S1_S1_S1_",
     "_ZN5del3d21in_sphere_unperturbedEPKdS1_S1_S1_S1_",
     "_ZN5del3d12spatial_sortERKSt6vectorINS_6Point3ESaIS1_EERS0_IiSaIiEE",
+    # include/del3d/predicates2.h:26-69
+    "_ZN5del3d8orient2dEdddddd",
+    "_ZN5del3d20coplanar_orientationEPKdS1_S1_",
+    "_ZN5del3d20coplanar_orientationEPKdS1_S1_S1_",
+    "_ZN5del3d9collinearEPKdS1_S1_",
+    "_ZN5del3d5equalEPKdS1_",
+    "_ZN5del3d18collinear_positionEPKdS1_S1_",
+    "_ZN5del3d18coplanar_in_circleEPKdS1_S1_S1_",
+    "_ZN5del3d30coplanar_in_circle_unperturbedEPKdS1_S1_S1_",
 ]
 
 
@@ -62,6 +71,11 @@ def test_argument_errors_without_device(lib):
     lib.del3d_spatial_sort_sets.argtypes = [vp, ctypes.POINTER(ctypes.c_int64), ctypes.c_int64, vp, vp]
     assert lib.del3d_spatial_sort_sets(None, off, 2, None, None) == 1
     assert b"decrease" in lib.del3d_last_error()
+    lib.del3d_predicates2.argtypes = [vp, ctypes.c_int64, vp, vp, ctypes.c_int64, vp, vp]
+    assert lib.del3d_predicates2(None, 0, None, None, 0, None, None) == 0
+    assert lib.del3d_predicates2(None, 0, None, None, -3, None, None) == 1
+    assert lib.del3d_predicates2(None, 0, None, None, 4, None, None) == 1
+    assert b"null" in lib.del3d_last_error()
 
 
 def test_version(lib):

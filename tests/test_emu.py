@@ -1,5 +1,5 @@
-"""The library's kernels and host code (C ABI, predicate pipeline, spatial
-sort), run on the CPU by the emulator in tests/emu under AddressSanitizer,
+"""The library's kernels and host code (C ABI, predicate pipelines 3D and
+coplanar, spatial sort), run on the CPU by the emulator in tests/emu under AddressSanitizer,
 against the oracle.
 
 EMU_DEFER=1 models the runtime's asynchrony at its least favourable:
@@ -24,8 +24,9 @@ EXE = Path("/tmp") / f"del3d_emu_sort_{os.getuid()}"
 def emu_exe():
     csrc = HERE.parent / "del3d_amd" / "csrc"
     subprocess.run(["python", str(HERE / "emu" / "build_emu.py"), str(EXE),
-                    str(csrc / "capi.hip"), str(csrc / "predicates.hip"), str(csrc / "batch.hip"), str(csrc / "server.hip"),
-                    str(csrc / "spatial_sort.hip")], check=True)
+                    str(csrc / "capi.hip"), str(csrc / "predicates.hip"), str(csrc / "predicates2.hip"),
+                    str(csrc / "batch.hip"), str(csrc / "server.hip"), str(csrc / "spatial_sort.hip")],
+                   check=True)
     return EXE
 
 
@@ -43,8 +44,9 @@ def test_emulated_server_under_thread_sanitizer():
     exe = Path("/tmp") / f"del3d_emu_tsan_{os.getuid()}"
     csrc = HERE.parent / "del3d_amd" / "csrc"
     subprocess.run(["python", str(HERE / "emu" / "build_emu.py"), str(exe),
-                    str(csrc / "capi.hip"), str(csrc / "predicates.hip"), str(csrc / "batch.hip"),
-                    str(csrc / "server.hip"), str(csrc / "spatial_sort.hip")], check=True,
+                    str(csrc / "capi.hip"), str(csrc / "predicates.hip"), str(csrc / "predicates2.hip"),
+                    str(csrc / "batch.hip"), str(csrc / "server.hip"), str(csrc / "spatial_sort.hip")],
+                   check=True,
                    env=dict(os.environ, EMU_SANITIZER="thread"))
     env = dict(os.environ, EMU_PRED_ONLY="1", TSAN_OPTIONS="halt_on_error=1")
     r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=900, env=env)

@@ -20,6 +20,7 @@ SORT_FAULT_REASON = (
 
 @pytest.mark.parametrize("name", [
     "predicates",
+    "predicates2",
     pytest.param("spatial_sort", marks=pytest.mark.skip(reason=SORT_FAULT_REASON)),
     pytest.param("delaunay", marks=pytest.mark.skip(reason=SORT_FAULT_REASON)),
 ])
