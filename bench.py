@@ -21,8 +21,9 @@ other_configs (skipped with --headline-only): the large configuration (10^7
 points, 2^28 gathered queries), the batched one (4096 point sets, one
 autotuned plan replayed as a graph, against one call per set) and the served
 one (a Poisson job stream through the serving loop, p50/p99 latency, against
-plain per-request calls), each with its own bit-exactness checks; --config
-runs one of them alone.
+plain per-request calls) and the coplanar predicates (2^24 queries on a
+tilted planar grid), each with its own bit-exactness checks; --config runs
+one of them alone.
 """
 from __future__ import annotations
 
@@ -363,6 +364,54 @@ def run_served(args) -> dict:
                     "pinned host buffers and the device"}
 
 
+# FP64 operations of the coplanar filters per query (src/predicates2.cpp:
+# 111-118 and 182-230): orient2d 9; in-circle 109 (differences 15, lifts 15,
+# normal 9 and its square 5, 2x2 minors 18, 3x3 minors 20, determinant 7,
+# magnitude bound 19, error bound 1); coplanar_orientation with s and collinear
+# evaluate two and one orient2d filters first; equal and collinear_position
+# are comparisons only.
+OPS2 = {0: 9, 1: 18, 2: 9, 3: 0, 4: 0, 5: 109, 6: 109, 7: 9}
+
+
+def run_coplanar(args) -> dict:
+    """The coplanar predicates: 2^24 queries from 4x4 windows of a 256x256
+    grid on a tilted plane (many exactly cocircular / collinear), one batched
+    call per step; a 2^18 sample against the oracle."""
+    import ctypes as C
+    import del3d_amd as D
+    from del3d_amd import workloads as W
+    pts, op, idx = W.coplanar_lattice(1 << 24)
+    n_q = len(op)
+    stream = D.Stream()
+    xyz = D.DeviceArray.from_host(pts)
+    dop = D.DeviceArray.from_host(op)
+    didx = D.DeviceArray.from_host(idx)
+    out = D.DeviceArray((n_q,), np.int8)
+    for _ in range(args.warmup + 1):
+        D.predicates2(xyz, dop, didx, out, stream)
+    ms = _events_ms(D, stream, lambda: D.predicates2(xyz, dop, didx, out, stream), args.steps)
+    got = out.to_host()
+    s = np.sort(np.random.default_rng(0).choice(n_q, size=1 << 18, replace=False))
+    sys.path.insert(0, str(ROOT / "tests"))
+    import oracle_lib as O
+    want = O.pred2_batch(pts, op[s], idx[s], "ref" if O.have_ref() else "oracle")
+    # which sampled queries the fast kernel leaves to the exact kernel: the
+    # host build of the same per-query code, filter-only mode
+    host = C.CDLL(str(ROOT / "del3d_amd" / "libpredcore_host.so"))
+    fast = np.zeros(len(s), np.int8)
+    so, si = np.ascontiguousarray(op[s]), np.ascontiguousarray(idx[s])
+    host.predcore2_host_batch(pts.ctypes.data_as(C.c_void_p), C.c_int64(len(pts)),
+                              so.ctypes.data_as(C.c_void_p), si.ctypes.data_as(C.c_void_p),
+                              C.c_int64(len(s)), fast.ctypes.data_as(C.c_void_p), C.c_int(1))
+    flops = int(sum(OPS2[k] * int((op == k).sum()) for k in OPS2))
+    t = ms / args.steps / 1e3
+    return {"config": "coplanar", "points": len(pts), "queries": n_q,
+            "gpred_per_s": round(n_q / t / 1e9, 3), "ms_per_step": round(ms / args.steps, 3),
+            "filter_tflops": round(flops / t / 1e12, 3),
+            "exact_kernel_share_sample": round(float((fast == 9).mean()), 4),
+            "bit_exact_sample": bool(np.array_equal(got[s], want)), "sample": len(s)}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -373,7 +422,8 @@ def main() -> int:
     ap.add_argument("--headline-only", action="store_true",
                     help="only the degenerate-lattice line, without the other configurations")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--config", default="lattice", choices=["lattice", "large", "batched", "served"],
+    ap.add_argument("--config", default="lattice",
+                    choices=["lattice", "large", "batched", "served", "coplanar"],
                     help="lattice: the headline line; large / batched: the other configurations")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="only run the calibration copy kernel (for counter scaling)")
@@ -401,7 +451,8 @@ def main() -> int:
         return 0
 
     if args.config != "lattice":
-        res = {"large": run_large, "batched": run_batched, "served": run_served}[args.config](args)
+        res = {"large": run_large, "batched": run_batched, "served": run_served,
+               "coplanar": run_coplanar}[args.config](args)
         print(json.dumps(res), flush=True)
         return 0
 
@@ -491,7 +542,8 @@ def main() -> int:
         # own bit-exactness checks included); a failure is reported, not fatal
         sub = argparse.Namespace(steps=max(3, args.steps // 5), warmup=1)
         others = {}
-        for name, fn in (("large", run_large), ("batched", run_batched), ("served", run_served)):
+        for name, fn in (("large", run_large), ("batched", run_batched), ("served", run_served),
+                         ("coplanar", run_coplanar)):
             t0 = time.perf_counter()
             try:
                 others[name] = fn(sub)

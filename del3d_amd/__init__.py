@@ -78,6 +78,7 @@ def lib() -> C.CDLL:
         L.del3d_orient3d.argtypes = [dp] * 4
         L.del3d_predicates2.argtypes = [vp, i64, vp, vp, i64, vp, vp]
         L.del3d_predicates2_host.argtypes = [vp, i64, vp, vp, i64, vp]
+        L.del3d_predicates_mixed.argtypes = [vp, i64, vp, vp, i64, vp, vp]
         L.del3d_orient2d.argtypes = [C.c_double] * 6
         for name, n in (("del3d_coplanar_orientation", 3), ("del3d_coplanar_orientation4", 4),
                         ("del3d_collinear", 3), ("del3d_equal", 2), ("del3d_collinear_position", 3),
@@ -121,6 +122,19 @@ def predicates2(xyz: DeviceArray, op: DeviceArray, idx: DeviceArray, out: Device
     n_q = int(op.shape[0])
     _check(lib().del3d_predicates2(xyz.ptr, int(xyz.shape[0]), op.ptr, idx.ptr, n_q, out.ptr,
                                    _stream_handle(stream)), "del3d_predicates2")
+
+
+MIXED_OP2_BASE = 16
+INEXACT_ORIENTATION = 8  # DEL3D_OP2_INEXACT_ORIENTATION
+
+
+def predicates_mixed(xyz: DeviceArray, op: DeviceArray, idx: DeviceArray, out: DeviceArray,
+                     stream: Stream | None = None) -> None:
+    """A mixed batch: op bytes ORIENT3D / IN_SPHERE / IN_SPHERE_UNPERTURBED or
+    MIXED_OP2_BASE + a coplanar op code (the predicate trace format's codes)."""
+    n_q = int(op.shape[0])
+    _check(lib().del3d_predicates_mixed(xyz.ptr, int(xyz.shape[0]), op.ptr, idx.ptr, n_q, out.ptr,
+                                        _stream_handle(stream)), "del3d_predicates_mixed")
 
 
 STAT_NAMES = ("filter", "certified", "exact_regs", "exact_wide", "perturbed", "bad_index")
@@ -418,5 +432,6 @@ __all__ = [
     "predicates2", "predicates2_np", "orient2d", "coplanar_orientation", "collinear", "equal",
     "collinear_position", "coplanar_in_circle", "coplanar_in_circle_unperturbed",
     "COPLANAR_ORIENTATION", "COPLANAR_ORIENTATION4", "COLLINEAR", "EQUAL", "COLLINEAR_POSITION",
-    "IN_CIRCLE", "IN_CIRCLE_UNPERTURBED", "ORIENT2D_XY",
+    "IN_CIRCLE", "IN_CIRCLE_UNPERTURBED", "ORIENT2D_XY", "INEXACT_ORIENTATION", "MIXED_OP2_BASE",
+    "predicates_mixed",
 ]

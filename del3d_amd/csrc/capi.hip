@@ -336,6 +336,21 @@ int del3d_predicates2(const double* xyz, int64_t n_pts, const uint8_t* op, const
     });
 }
 
+int del3d_predicates_mixed(const double* xyz, int64_t n_pts, const uint8_t* op, const int32_t* idx,
+                           int64_t n_q, int8_t* out, void* stream) {
+    return guarded([&] {
+        const int rc = check_pred_args(xyz, n_pts, op, idx, n_q, out);
+        if (rc) return rc;
+        hipStream_t s = (hipStream_t)stream;
+        // the 3D kernels answer ops 0-2 and write 0 for every other byte; the
+        // second pass then answers DEL3D_MIXED_OP2_BASE + k and leaves the
+        // 3D queries untouched (stream order puts its writes last)
+        launch_predicates(xyz, n_pts, op, idx, n_q, out, s, nullptr);
+        launch_predicates2(xyz, n_pts, op, idx, n_q, out, s, DEL3D_MIXED_OP2_BASE);
+        return DEL3D_OK;
+    });
+}
+
 int del3d_predicates2_host(const double* xyz, int64_t n_pts, const uint8_t* op, const int32_t* idx,
                            int64_t n_q, int8_t* out) {
     return guarded([&] {

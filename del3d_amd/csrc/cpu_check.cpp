@@ -101,7 +101,7 @@ extern "C" void predcore_host_t1(const double* pts, int np, int32_t gmin, int32_
 
 // Host build of the coplanar predicates (pred2_core.h). mode 0: the full
 // evaluation (filters, exact tiers, perturbation); 1: the fast kernel's
-// filter-only result (2 where it leaves the query to the exact kernel);
+// result (9 where it leaves the query to the exact kernel);
 // 2: the exact tiers forced (the filters skipped) for orient2d and the raw
 // in-circle determinant.
 extern "C" void predcore2_host_batch(const double* xyz, int64_t n_pts, const uint8_t* op,
@@ -110,9 +110,7 @@ extern "C" void predcore2_host_batch(const double* xyz, int64_t n_pts, const uin
     std::vector<uint32_t> ws((size_t)in_circle_workspace_limbs(kMaxLimbs2));
     for (int64_t q = 0; q < n_q; ++q) {
         const uint8_t o = op[q];
-        const int np = o == kOp2Equal ? 2
-                     : (o == kOp2CoplanarOrientation4 || o == kOp2InCircle ||
-                        o == kOp2InCircleUnperturbed) ? 4 : 3;
+        const int np = points_of_op2(o);
         P3 p[4];
         int32_t id[4];
         for (int k = 0; k < 4; ++k) {
@@ -121,7 +119,7 @@ extern "C" void predcore2_host_batch(const double* xyz, int64_t n_pts, const uin
         }
         int s;
         if (mode == 1) {
-            s = pred2_fast(o, p);
+            s = pred2_fast(o, p, id);
         } else if (mode == 2 && o == kOp2Orient2dXY) {
             s = orient2d_exact(p[0].x, p[0].y, p[1].x, p[1].y, p[2].x, p[2].y, ws.data());
         } else if (mode == 2 && o == kOp2InCircleUnperturbed) {

@@ -3,7 +3,8 @@
 // The reference's predicates on coplanar points (src/predicates2.cpp:111-259):
 // orient2d, coplanar_orientation (three and four points), collinear, equal,
 // collinear_position and coplanar_in_circle(_unperturbed) with its symbolic
-// perturbation. As for the 3D predicates, the filters are the reference's,
+// perturbation; and the Delaunay driver's plain-double pre-walk orientation
+// (src/triangulation.cpp:296-308). As for the 3D predicates, the filters are the reference's,
 // operation for operation (compiled with -ffp-contract=off), and whatever
 // they leave undecided is decided by exact integer arithmetic on the query's
 // own power-of-two grid (the method of tests/exact.h:161-203), whose sign is
@@ -41,24 +42,44 @@ enum : uint8_t {
     kOp2InCircle = 5,
     kOp2InCircleUnperturbed = 6,
     kOp2Orient2dXY = 7,
+    kOp2InexactOrientation = 8,
+    kOp2Last = 8,
 };
+
+// "Not decided without the exact stage" (collinear_position's results use 0-4).
+constexpr int kPending2 = 9;
+
+// Points of each op (the rest of a query's indices are ignored).
+D3_HD int points_of_op2(uint8_t o) {
+    switch (o) {
+        case kOp2Equal: return 2;
+        case kOp2CoplanarOrientation4:
+        case kOp2InCircle:
+        case kOp2InCircleUnperturbed:
+        case kOp2InexactOrientation: return 4;
+        default: return 3;
+    }
+}
+
+// src/triangulation.cpp:296-308: the driver's pre-walk orientation, the sign
+// of det(q-p, r-p, s-p) in plain double precision with the driver's fixed
+// association of the products (no filter, no exact stage: near zero its sign
+// is the rounded determinant's, which the pre-walk depends on).
+D3_HD int inexact_orientation(const P3& p, const P3& q, const P3& r, const P3& s) {
+    const double a00 = q.x - p.x, a01 = q.y - p.y, a02 = q.z - p.z;
+    const double a10 = r.x - p.x, a11 = r.y - p.y, a12 = r.z - p.z;
+    const double a20 = s.x - p.x, a21 = s.y - p.y, a22 = s.z - p.z;
+    const double m01 = a00 * a11 - a10 * a01;
+    const double m02 = a00 * a21 - a20 * a01;
+    const double m12 = a10 * a21 - a20 * a11;
+    const double det = m01 * a22 - m02 * a12 + m12 * a02;
+    return sgn(det);
+}
 
 // ---- filters ---------------------------------------------------------------
 
-// src/predicates2.cpp:111-118: sign if the filter decides, else 2.
-D3_HD int orient2d_filter(double px, double py, double qx, double qy, double rx, double ry) {
-    const double qpx = qx - px, qpy = qy - py;
-    const double rpx = rx - px, rpy = ry - py;
-    const double a = qpx * rpy, b = qpy * rpx;
-    const double det = a - b;
-    const double permanent = fabs(a) + fabs(b);
-    const double errbound = kOrient2dBound * permanent;
-    if (det > errbound || -det > errbound) return sgn(det);
-    return 2;
-}
-
-// src/predicates2.cpp:182-230: sign if the filter decides, else 2.
-D3_HD int in_circle_filter(const P3& p0, const P3& p1, const P3& p2, const P3& p) {
+// src/predicates2.cpp:182-230: sign if the filter decides, else kPending2.
+D3_HD int in_circle_filter(const P3& p0, const P3& p1, const P3& p2, const P3& p, double& det_out) {
     const double ptx = p0.x - p.x, pty = p0.y - p.y, ptz = p0.z - p.z;
     const double qtx = p1.x - p.x, qty = p1.y - p.y, qtz = p1.z - p.z;
     const double rtx = p2.x - p.x, rty = p2.y - p.y, rtz = p2.z - p.z;
@@ -100,8 +121,9 @@ D3_HD int in_circle_filter(const P3& p0, const P3& p1, const P3& p2, const P3& p
         mag = (mag == 0.0) ? row : mag * row;
     }
     const double errbound = kInCircleBound * mag;
+    det_out = det;
     if (det > errbound || -det > errbound) return sgn(det);
-    return 2;
+    return kPending2;
 }
 
 // ---- exact tiers on a caller workspace (runtime limb counts) ----------------
@@ -289,37 +311,111 @@ D3_HD_CALL int in_circle_exact(const P3& p0, const P3& p1, const P3& p2, const P
     return sign_combine(D6, 8 * L, E7, 10 * L, q, E7 + 10 * L);
 }
 
-// ---- the predicates, with exact fallbacks ----------------------------------
-
-D3_HD int orient2d_full(double px, double py, double qx, double qy, double rx, double ry, uint32_t* ws) {
-    const int f = orient2d_filter(px, py, qx, qy, rx, ry);
-    return f != 2 ? f : orient2d_exact(px, py, qx, qy, rx, ry, ws);
+// ---- T1: certified FP64 evaluation on a small power-of-two grid --------------
+//
+// As for the 3D predicates: with M the largest |difference| the filter forms
+// from the query point and q = ilogb(M) - k, if every coordinate is an integer
+// multiple of 2^q below 2^(52+q) in magnitude, every difference is exact and
+// below 2^(k+1) units of 2^q (the in-circle's pq, pr below 2^(k+2)), and every
+// product and partial sum of the filter's expression is an integer below 2^53
+// in units of 2^(deg*q): the FP64 determinant is exact, so its sign (zero
+// included) is the exact stage's sign.
+//   orient2d (degree 2): k = 25, |det| < 2 * 2^52 = 2^53; q >= -537.
+//   in-circle (degree 7 for the |v|^2 term): k = 5, a = 2^6 bounds the
+//     differences, and the terms sum to below 432 a^6 + 1152 a^7 < 2^52.2;
+//     7q >= -1074 needs q >= -153, and 53 + 7q < 1024 needs q <= 138 (e <= 140).
+D3_HD bool t1_certify_values(const double* v, int n, double maxd, int k, int qmin, int emax) {
+    const int e = exponent_of(maxd);
+    const int q = e - k;
+    const bool in_range = (q >= qmin) & (e <= emax);
+    const double inv = pow2i(in_range ? -q : 0);
+    bool ok = in_range;
+    for (int i = 0; i < n; ++i) ok = ok & on_grid(v[i], inv);
+    return (maxd == 0.0) | ok;
 }
 
+// orient2d without the exact stage: the sign when the filter or the
+// certificate decides, else kPending2 (src/predicates2.cpp:111-118).
+D3_HD int orient2d_fast(double px, double py, double qx, double qy, double rx, double ry) {
+    const double qpx = qx - px, qpy = qy - py;
+    const double rpx = rx - px, rpy = ry - py;
+    const double a = qpx * rpy, b = qpy * rpx;
+    const double det = a - b;
+    const double permanent = fabs(a) + fabs(b);
+    const double errbound = kOrient2dBound * permanent;
+    if ((det > errbound) | (-det > errbound)) return sgn(det);
+    const double maxd = fmax(fmax(fabs(qpx), fabs(qpy)), fmax(fabs(rpx), fabs(rpy)));
+    const double v[6] = {px, py, qx, qy, rx, ry};
+    return t1_certify_values(v, 6, maxd, 25, -537, 180) ? sgn(det) : kPending2;
+}
+
+// The in-circle determinant's sign without the exact stage, else kPending2.
+D3_HD int in_circle_fast(const P3& p0, const P3& p1, const P3& p2, const P3& p) {
+    double det;
+    const int f = in_circle_filter(p0, p1, p2, p, det);
+    if (f != kPending2) return f;
+    const P3 pts[4] = {p0, p1, p2, p};
+    const double maxd = max_diff(pts, 4);
+    const double v[12] = {p0.x, p0.y, p0.z, p1.x, p1.y, p1.z, p2.x, p2.y, p2.z, p.x, p.y, p.z};
+    return t1_certify_values(v, 12, maxd, 5, -153, 140) ? sgn(det) : kPending2;
+}
+
+// ---- the predicates -----------------------------------------------------------
+//
+// Each predicate is written once over an orient2d evaluator O and run in two
+// modes: kFast (filters and certificates only; any undecided step makes the
+// whole query kPending2, for the exact kernel) and full (exact tiers behind them).
+
+template <bool kFast>
+struct Orient2 {
+    uint32_t* ws;
+    D3_HD int operator()(double px, double py, double qx, double qy, double rx, double ry) const {
+        const int f = orient2d_fast(px, py, qx, qy, rx, ry);
+        if (kFast || f != kPending2) return f;
+        return orient2d_exact(px, py, qx, qy, rx, ry, ws);
+    }
+};
+
 // src/predicates2.cpp:122-131
-D3_HD int coplanar_orientation3(const P3& p, const P3& q, const P3& r, uint32_t* ws) {
-    int o = orient2d_full(p.x, p.y, q.x, q.y, r.x, r.y, ws);
+template <class O>
+D3_HD int coplanar_orientation3(const P3& p, const P3& q, const P3& r, const O& o2) {
+    int o = o2(p.x, p.y, q.x, q.y, r.x, r.y);
     if (o != 0) return o;
-    o = orient2d_full(p.y, p.z, q.y, q.z, r.y, r.z, ws);
+    o = o2(p.y, p.z, q.y, q.z, r.y, r.z);
     if (o != 0) return o;
-    return orient2d_full(p.x, p.z, q.x, q.z, r.x, r.z, ws);
+    return o2(p.x, p.z, q.x, q.z, r.x, r.z);
 }
 
 // src/predicates2.cpp:133-146
-D3_HD int coplanar_orientation4(const P3& p, const P3& q, const P3& r, const P3& s, uint32_t* ws) {
-    int o = orient2d_full(p.x, p.y, q.x, q.y, r.x, r.y, ws);
-    if (o != 0) return o * orient2d_full(p.x, p.y, q.x, q.y, s.x, s.y, ws);
-    o = orient2d_full(p.y, p.z, q.y, q.z, r.y, r.z, ws);
-    if (o != 0) return o * orient2d_full(p.y, p.z, q.y, q.z, s.y, s.z, ws);
-    o = orient2d_full(p.x, p.z, q.x, q.z, r.x, r.z, ws);
-    return o * orient2d_full(p.x, p.z, q.x, q.z, s.x, s.z, ws);
+template <class O>
+D3_HD int coplanar_orientation4(const P3& p, const P3& q, const P3& r, const P3& s, const O& o2) {
+    int o = o2(p.x, p.y, q.x, q.y, r.x, r.y);
+    if (o == kPending2) return kPending2;
+    if (o != 0) {
+        const int t = o2(p.x, p.y, q.x, q.y, s.x, s.y);
+        return t == kPending2 ? kPending2 : o * t;
+    }
+    o = o2(p.y, p.z, q.y, q.z, r.y, r.z);
+    if (o == kPending2) return kPending2;
+    if (o != 0) {
+        const int t = o2(p.y, p.z, q.y, q.z, s.y, s.z);
+        return t == kPending2 ? kPending2 : o * t;
+    }
+    o = o2(p.x, p.z, q.x, q.z, r.x, r.z);
+    if (o == kPending2) return kPending2;
+    const int t = o2(p.x, p.z, q.x, q.z, s.x, s.z);
+    return t == kPending2 ? kPending2 : o * t;
 }
 
-// src/predicates2.cpp:148-156
-D3_HD bool collinear3(const P3& p, const P3& q, const P3& r, uint32_t* ws) {
-    if (orient2d_full(r.x, r.y, p.x, p.y, q.x, q.y, ws) != 0) return false;
-    if (orient2d_full(r.x, r.z, p.x, p.z, q.x, q.z, ws) != 0) return false;
-    return orient2d_full(r.y, r.z, p.y, p.z, q.y, q.z, ws) == 0;
+// src/predicates2.cpp:148-156 (1 collinear, 0 not, kPending2 undecided)
+template <class O>
+D3_HD int collinear3(const P3& p, const P3& q, const P3& r, const O& o2) {
+    int o = o2(r.x, r.y, p.x, p.y, q.x, q.y);
+    if (o != 0) return o == kPending2 ? kPending2 : 0;
+    o = o2(r.x, r.z, p.x, p.z, q.x, q.z);
+    if (o != 0) return o == kPending2 ? kPending2 : 0;
+    o = o2(r.y, r.z, p.y, p.z, q.y, q.z);
+    return o == kPending2 ? kPending2 : (o == 0 ? 1 : 0);
 }
 
 // src/predicates2.cpp:158-160
@@ -348,7 +444,8 @@ D3_HD int collinear_position3(const P3& p, const P3& q, const P3& r) {
 
 // src/predicates2.cpp:236-259: the cocircular perturbation. Slots 0..3 are
 // (p0, p1, p2, p); id[] gives their identities (equal index = same pointer).
-D3_HD_CALL int in_circle_perturb(const P3 pt[4], const int32_t id[4], uint32_t* ws) {
+template <class O>
+D3_HD int in_circle_perturb(const P3 pt[4], const int32_t id[4], const O& o2) {
     // stable lexicographic order of the four slots (std::sort on four
     // elements is an insertion sort, so equal points keep argument order)
     int ord[4] = {0, 1, 2, 3};
@@ -361,50 +458,57 @@ D3_HD_CALL int in_circle_perturb(const P3 pt[4], const int32_t id[4], uint32_t* 
         }
         ord[j] = v;
     }
-    const int local = coplanar_orientation3(pt[0], pt[1], pt[2], ws);
+    const int local = coplanar_orientation3(pt[0], pt[1], pt[2], o2);
+    if (local == kPending2) return kPending2;
     for (int i = 3; i > 0; --i) {
         const int32_t v = id[ord[i]];
         if (v == id[3]) return -1;
         int o;
-        if (v == id[2] && (o = coplanar_orientation3(pt[0], pt[1], pt[3], ws)) != 0) return o * local;
-        if (v == id[1] && (o = coplanar_orientation3(pt[0], pt[3], pt[2], ws)) != 0) return o * local;
-        if (v == id[0] && (o = coplanar_orientation3(pt[3], pt[1], pt[2], ws)) != 0) return o * local;
+        if (v == id[2] && (o = coplanar_orientation3(pt[0], pt[1], pt[3], o2)) != 0)
+            return o == kPending2 ? kPending2 : o * local;
+        if (v == id[1] && (o = coplanar_orientation3(pt[0], pt[3], pt[2], o2)) != 0)
+            return o == kPending2 ? kPending2 : o * local;
+        if (v == id[0] && (o = coplanar_orientation3(pt[3], pt[1], pt[2], o2)) != 0)
+            return o == kPending2 ? kPending2 : o * local;
     }
     return -local;
 }
 
-// One coplanar query, every tier (ws: in_circle_workspace_limbs(kMaxLimbs2)
-// limbs covers every finite input). pt: the op's points in argument order.
-D3_HD_CALL int pred2_eval(uint8_t op, const P3 pt[4], const int32_t id[4], uint32_t* ws) {
+// One query, every op. pt: the op's points in argument order; id: their
+// identities. kFast: result or kPending2 (undecided by filters and certificates);
+// full: the result (ws: in_circle_workspace_limbs(kMaxLimbs2) limbs cover
+// every finite input).
+template <bool kFast>
+D3_HD int pred2_run(uint8_t op, const P3 pt[4], const int32_t id[4], uint32_t* ws) {
+    const Orient2<kFast> o2{ws};
     switch (op) {
-        case kOp2CoplanarOrientation: return coplanar_orientation3(pt[0], pt[1], pt[2], ws);
-        case kOp2CoplanarOrientation4: return coplanar_orientation4(pt[0], pt[1], pt[2], pt[3], ws);
-        case kOp2Collinear: return collinear3(pt[0], pt[1], pt[2], ws) ? 1 : 0;
+        case kOp2CoplanarOrientation: return coplanar_orientation3(pt[0], pt[1], pt[2], o2);
+        case kOp2CoplanarOrientation4: return coplanar_orientation4(pt[0], pt[1], pt[2], pt[3], o2);
+        case kOp2Collinear: return collinear3(pt[0], pt[1], pt[2], o2);
         case kOp2Equal: return equal3(pt[0], pt[1]) ? 1 : 0;
         case kOp2CollinearPosition: return collinear_position3(pt[0], pt[1], pt[2]);
-        case kOp2Orient2dXY: return orient2d_full(pt[0].x, pt[0].y, pt[1].x, pt[1].y, pt[2].x, pt[2].y, ws);
-        default: break;
+        case kOp2Orient2dXY: return o2(pt[0].x, pt[0].y, pt[1].x, pt[1].y, pt[2].x, pt[2].y);
+        case kOp2InexactOrientation: return inexact_orientation(pt[0], pt[1], pt[2], pt[3]);
+        case kOp2InCircle:
+        case kOp2InCircleUnperturbed: {
+            int s = in_circle_fast(pt[0], pt[1], pt[2], pt[3]);
+            if (s == kPending2) {
+                if (kFast) return kPending2;
+                s = in_circle_exact(pt[0], pt[1], pt[2], pt[3], ws);
+            }
+            if (s != 0 || op == kOp2InCircleUnperturbed) return s;
+            return in_circle_perturb(pt, id, o2);
+        }
+        default: return 0;
     }
-    int s = in_circle_filter(pt[0], pt[1], pt[2], pt[3]);
-    if (s == 2) s = in_circle_exact(pt[0], pt[1], pt[2], pt[3], ws);
-    if (s != 0 || op == kOp2InCircleUnperturbed) return s;
-    return in_circle_perturb(pt, id, ws);
 }
 
-// The filter-only part: the result if no exact evaluation is needed, else 2.
-D3_HD int pred2_fast(uint8_t op, const P3 pt[4]) {
-    switch (op) {
-        case kOp2Equal: return equal3(pt[0], pt[1]) ? 1 : 0;
-        case kOp2CollinearPosition: return collinear_position3(pt[0], pt[1], pt[2]);
-        case kOp2Orient2dXY: return orient2d_filter(pt[0].x, pt[0].y, pt[1].x, pt[1].y, pt[2].x, pt[2].y);
-        case kOp2CoplanarOrientation: {
-            const int o = orient2d_filter(pt[0].x, pt[0].y, pt[1].x, pt[1].y, pt[2].x, pt[2].y);
-            return o == 0 ? 2 : o;  // a filter never returns 0; 2 stays 2
-        }
-        case kOp2InCircle:
-        case kOp2InCircleUnperturbed: return in_circle_filter(pt[0], pt[1], pt[2], pt[3]);
-        default: return 2;
-    }
+D3_HD_CALL int pred2_eval(uint8_t op, const P3 pt[4], const int32_t id[4], uint32_t* ws) {
+    return pred2_run<false>(op, pt, id, ws);
+}
+
+D3_HD int pred2_fast(uint8_t op, const P3 pt[4], const int32_t id[4]) {
+    return pred2_run<true>(op, pt, id, nullptr);
 }
 
 }  // namespace d3

@@ -89,11 +89,13 @@ void launch_predicates(const double* xyz, int64_t n_pts, const uint8_t* op, cons
 
 void launch_single(ScalarQuery* qd, hipStream_t s);
 
-// Coplanar batch (predicates2.hip): op codes DEL3D_OP2_*, idx 5 per query
-// (the op's points in argument order), out = the predicate's result. Queries
-// with an unknown op or an index outside [0, n_pts) get 0.
+// Coplanar batch (predicates2.hip): op codes DEL3D_OP2_* + op_base, idx 5
+// per query (the op's points in argument order), out = the predicate's
+// result. Queries with an op byte below op_base are left untouched (another
+// kernel of a mixed batch owns them); an unknown op or an index outside
+// [0, n_pts) gives 0.
 void launch_predicates2(const double* xyz, int64_t n_pts, const uint8_t* op, const int32_t* idx,
-                        int64_t n_q, int8_t* out, hipStream_t s);
+                        int64_t n_q, int8_t* out, hipStream_t s, uint8_t op_base = 0);
 // One coplanar query of a ScalarQuery record (points by value, op DEL3D_OP2_*).
 void launch_single2(ScalarQuery* qd, hipStream_t s);
 

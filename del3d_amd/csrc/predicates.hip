@@ -40,8 +40,9 @@ constexpr int kWideLimbsMax = 67;  // > limbs_for() of any finite input (66)
 constexpr int kWideBlock = 64;
 
 // Out-of-range point indices are undefined behaviour in the reference (a bad
-// pointer); here such a query gets sign 0 and is counted in stats[5], so a
-// bad index can never make the device read outside the point array.
+// pointer); here such a query, or one with an unknown op code, gets sign 0 and
+// is counted in stats[5], so a bad index can never make the device read
+// outside the point array.
 __device__ inline uint64_t lanemask_lt() { return __lanemask_lt(); }
 
 // Appends `q` for every lane with `want`; one atomic per wavefront.
@@ -279,7 +280,7 @@ __device__ __forceinline__ void pred_batch_body(const BatchArgs& a, BatchLds<kSt
 #pragma unroll
             for (int k = 0; k < 5; ++k) id[k] = s_idx[lq * 5 + k];
             const int nchk = (o == DEL3D_OP_ORIENT3D) ? 4 : 5;
-            bool valid = true;
+            bool valid = o <= DEL3D_OP_IN_SPHERE_UNPERTURBED;
 #pragma unroll
             for (int k = 0; k < 5; ++k) valid = valid && (k >= nchk || (uint32_t)id[k] < (uint64_t)np);
 

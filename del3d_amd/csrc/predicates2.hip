@@ -1,13 +1,15 @@
 // predicates2.hip - batched coplanar predicates for gfx950.
 //
 // Replaces, per query, one call of the reference's coplanar predicates
-// (src/predicates2.cpp:111-259; include/del3d/predicates2.h:26-69). Two
+// (src/predicates2.cpp:111-259; include/del3d/predicates2.h:26-69) or of the
+// Delaunay driver's plain-double orientation (src/triangulation.cpp:296-308). Two
 // kernels per batch on the caller's stream:
 //
-//  pred2_fast   one lane per query: the filters in the reference's operation
-//               order (and the predicates that are plain comparisons); what
-//               they leave undecided is appended, one atomic per wavefront,
-//               to a queue;
+//  pred2_fast   one lane per query: the whole predicate (perturbation
+//               included) with the filters in the reference's operation order
+//               and a certified FP64 evaluation for points on a small
+//               power-of-two grid; a query any step leaves undecided is
+//               appended, one atomic per wavefront, to a queue;
 //  pred2_exact  one query per workgroup: the whole predicate with exact
 //               integer fallbacks and the cocircular perturbation, on a
 //               workspace in LDS sized for any finite input.
@@ -23,41 +25,34 @@ namespace d3 {
 
 constexpr int kWs2 = 207 * kMaxLimbs2 + 16;  // in_circle_workspace_limbs(kMaxLimbs2)
 
-__device__ inline int points_of_op2(uint8_t o) {
-    switch (o) {
-        case kOp2Equal: return 2;
-        case kOp2CoplanarOrientation4:
-        case kOp2InCircle:
-        case kOp2InCircleUnperturbed: return 4;
-        default: return 3;
-    }
-}
-
+// Queries whose op byte is below op_base belong to another kernel of a mixed
+// batch and are left untouched; the others have op code (byte - op_base).
 __global__ __launch_bounds__(256) void pred2_fast_k(const double* __restrict__ xyz, int64_t n_pts,
                                                     const uint8_t* __restrict__ op,
                                                     const int32_t* __restrict__ idx, uint32_t n_q,
                                                     int8_t* __restrict__ out, uint32_t* __restrict__ queue,
-                                                    uint32_t* __restrict__ counter) {
+                                                    uint32_t* __restrict__ counter, uint8_t op_base) {
     for (uint32_t q0 = blockIdx.x * blockDim.x + (threadIdx.x & ~63u); q0 < n_q;
          q0 += gridDim.x * blockDim.x) {
         const uint32_t q = q0 + __lane_id();
         bool pending = false;
-        if (q < n_q) {
-            const uint8_t o = op[q];
+        if (q < n_q && op[q] >= op_base) {
+            const uint8_t o = (uint8_t)(op[q] - op_base);
             const int np = points_of_op2(o);
             P3 pt[4];
-            bool valid = o <= kOp2Orient2dXY;
+            int32_t id[4];
+            bool valid = o <= kOp2Last;
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
-                const int32_t id = idx[(size_t)q * 5 + k];
+                id[k] = idx[(size_t)q * 5 + k];
                 const bool used = k < np;
-                valid = valid && (!used || (uint32_t)id < (uint64_t)n_pts);
-                pt[k] = (used && valid) ? load_p3(xyz, id) : P3{0.0, 0.0, 0.0};
+                valid = valid && (!used || (uint32_t)id[k] < (uint64_t)n_pts);
+                pt[k] = (used && valid) ? load_p3(xyz, id[k]) : P3{0.0, 0.0, 0.0};
             }
             int s = 0;
             if (valid) {
-                s = pred2_fast(o, pt);
-                pending = s == 2;
+                s = pred2_fast(o, pt, id);
+                pending = s == kPending2;
             }
             if (!pending) out[q] = (int8_t)s;
         }
@@ -77,13 +72,14 @@ __global__ __launch_bounds__(64) void pred2_exact_k(const double* __restrict__ x
                                                     const int32_t* __restrict__ idx,
                                                     int8_t* __restrict__ out,
                                                     const uint32_t* __restrict__ queue,
-                                                    const uint32_t* __restrict__ counter) {
+                                                    const uint32_t* __restrict__ counter,
+                                                    uint8_t op_base) {
     __shared__ uint32_t ws[kWs2];
     if (threadIdx.x != 0) return;
     const uint32_t n = *counter;
     for (uint32_t i = blockIdx.x; i < n; i += gridDim.x) {
         const uint32_t q = queue[i];
-        const uint8_t o = op[q];
+        const uint8_t o = (uint8_t)(op[q] - op_base);
         const int np = points_of_op2(o);
         P3 pt[4];
         int32_t id[4];
@@ -98,7 +94,7 @@ __global__ __launch_bounds__(64) void pred2_exact_k(const double* __restrict__ x
 __global__ void k_clear_u32(uint32_t* c) { *c = 0u; }
 
 void launch_predicates2(const double* xyz, int64_t n_pts, const uint8_t* op, const int32_t* idx,
-                        int64_t n_q, int8_t* out, hipStream_t s) {
+                        int64_t n_q, int8_t* out, hipStream_t s, uint8_t op_base) {
     if (n_q <= 0) return;
     const uint32_t chunk = (uint32_t)std::min<int64_t>(n_q, kChunk);
     Scratch sc(s);
@@ -110,9 +106,11 @@ void launch_predicates2(const double* xyz, int64_t n_pts, const uint8_t* op, con
         k_clear_u32<<<1, 1, 0, s>>>(counter);
         D3_LAUNCH_CHECK();
         const unsigned g = (unsigned)std::min<uint32_t>((n + 255) / 256, (uint32_t)cus * 16);
-        pred2_fast_k<<<g, 256, 0, s>>>(xyz, n_pts, op + off, idx + off * 5, n, out + off, queue, counter);
+        pred2_fast_k<<<g, 256, 0, s>>>(xyz, n_pts, op + off, idx + off * 5, n, out + off, queue, counter,
+                                       op_base);
         D3_LAUNCH_CHECK();
-        pred2_exact_k<<<cus * 2, 64, 0, s>>>(xyz, op + off, idx + off * 5, out + off, queue, counter);
+        pred2_exact_k<<<cus * 2, 64, 0, s>>>(xyz, op + off, idx + off * 5, out + off, queue, counter,
+                                             op_base);
         D3_LAUNCH_CHECK();
     }
 }

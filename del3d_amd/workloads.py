@@ -289,3 +289,25 @@ def coplanar_queries(n_pts: int, n_q: int, rng, ops=tuple(range(8))):
     idx[np.arange(m), 3] = idx[np.arange(m), src]
     idx[:, 4] = 0
     return op, idx
+
+
+def coplanar_lattice(n_q: int = 1 << 24, seed: int = 6, side: int = 256):
+    """(points, op, idx) of the coplanar benchmark: a side x side integer grid
+    mapped by the similarity _TILT onto a tilted plane (so its cocircular and
+    collinear subsets stay so in 3D), and n_q queries whose points come from a
+    4x4 window at a random position (squares, rectangles and grid lines: many
+    exactly cocircular or collinear). Op mix: 40% coplanar_in_circle, 10%
+    unperturbed, 20% coplanar_orientation(p,q,r,s), 10% coplanar_orientation,
+    10% collinear, 5% collinear_position, 5% orient2d."""
+    rng = np.random.default_rng(seed)
+    i, j = np.meshgrid(np.arange(side), np.arange(side), indexing="ij")
+    g = np.stack([i.ravel(), j.ravel(), np.zeros(side * side)], axis=1).astype(np.float64)
+    pts = g @ _TILT.T
+    ops = np.array([5] * 8 + [6] * 2 + [1] * 4 + [0] * 2 + [2] * 2 + [4] + [7], np.uint8)
+    op = ops[rng.integers(0, len(ops), size=n_q)]
+    base = rng.integers(0, side - 3, size=(n_q, 2))
+    off = rng.integers(0, 4, size=(n_q, 4, 2))
+    ij = base[:, None, :] + off
+    idx = np.zeros((n_q, 5), np.int32)
+    idx[:, :4] = (ij[..., 0] * side + ij[..., 1]).astype(np.int32)
+    return pts, op, idx

@@ -51,6 +51,14 @@ extern "C" {
 #define DEL3D_OP2_IN_CIRCLE 5              /* coplanar_in_circle(p0,p1,p2,p): -1/+1 */
 #define DEL3D_OP2_IN_CIRCLE_UNPERTURBED 6  /* coplanar_in_circle_unperturbed: -1/0/+1 */
 #define DEL3D_OP2_ORIENT2D_XY 7            /* orient2d(p.x,p.y, q.x,q.y, r.x,r.y): -1/0/+1 */
+#define DEL3D_OP2_INEXACT_ORIENTATION 8    /* the Delaunay driver's plain-double pre-walk
+                                              orientation (p,q,r,s), src/triangulation.cpp:
+                                              296-308: -1/0/+1, not exact by design */
+
+/* Mixed batches (del3d_predicates_mixed, and the predicate trace format):
+ * op bytes 0-2 are the DEL3D_OP_* codes, DEL3D_MIXED_OP2_BASE + k the
+ * DEL3D_OP2_* code k. */
+#define DEL3D_MIXED_OP2_BASE 16
 
 /* Message of the calling thread's last failed call ("" if none). */
 const char* del3d_last_error(void);
@@ -90,8 +98,9 @@ int del3d_predicates(const double* xyz, int64_t n_pts, const uint8_t* op, const 
  *   [2] by exact integer arithmetic in registers (1-2 limbs per difference)
  *   [3] by exact integer arithmetic in LDS (wide exponent spans)
  *   [4] in_sphere queries that needed the symbolic perturbation
- *   [5] queries with a point index outside the point set (sign 0 written;
- *       the reference's behaviour for such a pointer is undefined)
+ *   [5] queries with a point index outside the point set or an unknown op
+ *       code (sign 0 written; the reference's behaviour for such a pointer is
+ *       undefined)
  */
 int del3d_predicates_stats(const double* xyz, int64_t n_pts, const uint8_t* op,
                            const int32_t* idx, int64_t n_q, int8_t* sign_out, void* stream,
@@ -270,6 +279,17 @@ int del3d_spatial_sort_sets(const double* xyz, const int64_t* off, int64_t n_set
  */
 int del3d_predicates2(const double* xyz, int64_t n_pts, const uint8_t* op, const int32_t* idx,
                       int64_t n_q, int8_t* out, void* stream);
+
+/*
+ * A mixed batch: any of the 3D and coplanar predicates and the driver's
+ * inexact orientation in one call, op bytes as above (DEL3D_OP_* or
+ * DEL3D_MIXED_OP2_BASE + DEL3D_OP2_*), each query's indices and result as in
+ * del3d_predicates / del3d_predicates2. This is the replay form of the
+ * predicate trace format (the op, result and index records of a run of the
+ * reference's Delaunay driver). Unknown op bytes give 0.
+ */
+int del3d_predicates_mixed(const double* xyz, int64_t n_pts, const uint8_t* op,
+                           const int32_t* idx, int64_t n_q, int8_t* out, void* stream);
 
 /* Host-memory forms: copy in, compute on the current device, copy out,
  * synchronously. For callers without device buffers. */

@@ -465,9 +465,7 @@ void orc_int_pred2_batch(const double* xyz, int64_t n_pts, const uint8_t* op,
     (void)n_pts;
     for (int64_t q = 0; q < n_q; ++q) {
         const int32_t* r = idx5 + 5 * q;
-        const int np = op[q] == ORC_OP2_EQUAL ? 2
-                     : (op[q] == ORC_OP2_COPLANAR_ORIENTATION4 || op[q] == ORC_OP2_IN_CIRCLE ||
-                        op[q] == ORC_OP2_IN_CIRCLE_UNPERTURBED) ? 4 : 3;
+        const int np = orc_pred2_points(op[q]);
         const double* p[4];
         for (int k = 0; k < 4; ++k) p[k] = k < np ? xyz + 3 * (int64_t)r[k] : xyz;
         int s;
@@ -481,6 +479,9 @@ void orc_int_pred2_batch(const double* xyz, int64_t n_pts, const uint8_t* op,
             case ORC_OP2_IN_CIRCLE_UNPERTURBED: s = int_in_circle_raw(p[0], p[1], p[2], p[3]); break;
             case ORC_OP2_ORIENT2D_XY:
                 s = orc_int_orient2d(p[0][0], p[0][1], p[1][0], p[1][1], p[2][0], p[2][1]);
+                break;
+            case ORC_OP2_INEXACT_ORIENTATION:  /* the exact sign (the op itself is not exact) */
+                s = orc_int_orient3d(p[0], p[1], p[2], p[3]);
                 break;
             default: s = 0; break;
         }

@@ -10,7 +10,8 @@
  *   - predicates   src/predicates.cpp:22-232 (filter, expansion exact stage,
  *                  symbolic perturbation) with src/expansion.h:33-178
  *   - coplanar     src/predicates2.cpp:22-259 (orient2d, coplanar orientation,
- *                  collinear, equal, collinear_position, in-circle)
+ *                  collinear, equal, collinear_position, in-circle) and the
+ *                  driver's plain-double orientation, src/triangulation.cpp:296-308
  *   - an independent big-integer predicate oracle following the method of
  *                  tests/exact.h:161-291 (power-of-two frame, integer dets)
  *   - spatial sort src/spatial_sort.cpp:17-151 with src/rand48.h:26-153
@@ -70,8 +71,11 @@ enum {
     ORC_OP2_COLLINEAR_POSITION = 4,    /* (p,q,r) */
     ORC_OP2_IN_CIRCLE = 5,             /* (p0,p1,p2,p), perturbed */
     ORC_OP2_IN_CIRCLE_UNPERTURBED = 6, /* (p0,p1,p2,p) */
-    ORC_OP2_ORIENT2D_XY = 7            /* orient2d on the xy coordinates of (p,q,r) */
+    ORC_OP2_ORIENT2D_XY = 7,           /* orient2d on the xy coordinates of (p,q,r) */
+    ORC_OP2_INEXACT_ORIENTATION = 8    /* the driver's plain-double orientation (p,q,r,s) */
 };
+/* Points an op reads. */
+int orc_pred2_points(uint8_t op);
 int orc_pred2_one(uint8_t op, const double* a, const double* b, const double* c,
                   const double* d);
 void orc_pred2_batch(const double* xyz, int64_t n_pts, const uint8_t* op,
@@ -88,7 +92,9 @@ void orc_int_pred_batch(const double* xyz, int64_t n_pts, const uint8_t* op,
 /* Coplanar predicates by the method of tests/exact.h:294-378: orient2d on an
  * integer frame, and the in-circle test as an in-sphere test against the apex
  * p0 + (p1-p0) x (p2-p0). The in-circle answers equal the library's only for
- * exactly coplanar queries (its precondition). Same op codes as above. */
+ * exactly coplanar queries (its precondition). Same op codes as above; for
+ * ORC_OP2_INEXACT_ORIENTATION it gives the exact orientation sign, which the
+ * plain-double one must equal wherever its error bound is small. */
 int orc_int_orient2d(double px, double py, double qx, double qy, double rx, double ry);
 void orc_int_pred2_batch(const double* xyz, int64_t n_pts, const uint8_t* op,
                          const int32_t* idx5, int64_t n_q, int8_t* sign_out);

@@ -424,6 +424,7 @@ void orc_last_stage_counts(int64_t out[3]) {
 /*   src/predicates2.cpp:59-101  in_circle_exact (rows pt, rt, qt, v)      */
 /*   src/predicates2.cpp:111-180 orient2d .. collinear_position            */
 /*   src/predicates2.cpp:182-259 coplanar_in_circle(_unperturbed)          */
+/*   src/triangulation.cpp:296-308 the driver's inexact orientation        */
 /* ====================================================================== */
 #define ORIENT2D_BOUND ((3.0 + 16.0 * kEps) * kEps)
 #define INCIRCLE_BOUND ((16.0 + 224.0 * kEps) * kEps)
@@ -553,6 +554,19 @@ static int in_circle_raw_impl(const double* p0, const double* p1, const double* 
     return in_circle_exact_raw(p0, p1, p2, p);
 }
 
+/* src/triangulation.cpp:296-308: plain double, the driver's association. */
+static int inexact_orientation(const double* p, const double* q, const double* r,
+                               const double* s) {
+    const double a00 = q[0] - p[0], a01 = q[1] - p[1], a02 = q[2] - p[2];
+    const double a10 = r[0] - p[0], a11 = r[1] - p[1], a12 = r[2] - p[2];
+    const double a20 = s[0] - p[0], a21 = s[1] - p[1], a22 = s[2] - p[2];
+    const double m01 = a00 * a11 - a10 * a01;
+    const double m02 = a00 * a21 - a20 * a01;
+    const double m12 = a10 * a21 - a20 * a11;
+    const double det = m01 * a22 - m02 * a12 + m12 * a02;
+    return sgn(det);
+}
+
 static void sort4_stable(const double* p[4]) {
     for (int i = 1; i < 4; ++i) {
         const double* v = p[i];
@@ -592,10 +606,22 @@ int orc_pred2_one(uint8_t op, const double* a, const double* b, const double* c,
         case ORC_OP2_IN_CIRCLE: s = in_circle_impl(a, b, c, d); break;
         case ORC_OP2_IN_CIRCLE_UNPERTURBED: s = in_circle_raw_impl(a, b, c, d); break;
         case ORC_OP2_ORIENT2D_XY: s = orient2d_impl(a[0], a[1], b[0], b[1], c[0], c[1]); break;
+        case ORC_OP2_INEXACT_ORIENTATION: s = inexact_orientation(a, b, c, d); break;
         default: s = 0; break;
     }
     arena_reset();
     return s;
+}
+
+int orc_pred2_points(uint8_t op) {
+    switch (op) {
+        case ORC_OP2_EQUAL: return 2;
+        case ORC_OP2_COPLANAR_ORIENTATION4:
+        case ORC_OP2_IN_CIRCLE:
+        case ORC_OP2_IN_CIRCLE_UNPERTURBED:
+        case ORC_OP2_INEXACT_ORIENTATION: return 4;
+        default: return 3;
+    }
 }
 
 void orc_pred2_batch(const double* xyz, int64_t n_pts, const uint8_t* op,
@@ -604,9 +630,7 @@ void orc_pred2_batch(const double* xyz, int64_t n_pts, const uint8_t* op,
     g_stage[0] = g_stage[1] = g_stage[2] = 0;
     for (int64_t q = 0; q < n_q; ++q) {
         const int32_t* r = idx5 + 5 * q;
-        const int np = op[q] == ORC_OP2_EQUAL ? 2
-                     : (op[q] == ORC_OP2_COPLANAR_ORIENTATION4 || op[q] == ORC_OP2_IN_CIRCLE ||
-                        op[q] == ORC_OP2_IN_CIRCLE_UNPERTURBED) ? 4 : 3;
+        const int np = orc_pred2_points(op[q]);
         const double* p[4];
         for (int k = 0; k < 4; ++k) p[k] = k < np ? xyz + 3 * (int64_t)r[k] : xyz;
         sign_out[q] = (int8_t)orc_pred2_one(op[q], p[0], p[1], p[2], p[3]);

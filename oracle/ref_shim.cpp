@@ -60,13 +60,15 @@ void ref_pred_batch(const double* xyz, int64_t n_pts, const uint8_t* op, const i
 // Coplanar batch: op 0 coplanar_orientation(p,q,r), 1 coplanar_orientation(p,q,r,s),
 // 2 collinear, 3 equal(p,q), 4 collinear_position, 5 coplanar_in_circle,
 // 6 coplanar_in_circle_unperturbed, 7 orient2d on the xy coordinates of (p,q,r).
-// Points are pointers into xyz, so equal indices are the same pointer.
+// Points are pointers into xyz, so equal indices are the same pointer. Op 8
+// (the driver's private plain-double orientation) is not reachable from here:
+// it gets 0 (libdel3d_ref_driver.so answers it).
 void ref_pred2_batch(const double* xyz, int64_t n_pts, const uint8_t* op, const int32_t* idx5,
                      int64_t n_q, int8_t* out) {
     (void)n_pts;
     for (int64_t q = 0; q < n_q; ++q) {
         const int32_t* r = idx5 + 5 * q;
-        const int np = op[q] == 3 ? 2 : (op[q] == 1 || op[q] == 5 || op[q] == 6) ? 4 : 3;
+        const int np = op[q] == 3 ? 2 : (op[q] == 1 || op[q] == 5 || op[q] == 6 || op[q] == 8) ? 4 : 3;
         const double* p[4];
         for (int k = 0; k < 4; ++k) p[k] = k < np ? xyz + 3 * int64_t(r[k]) : xyz;
         int s = 0;

@@ -214,7 +214,7 @@ static int check_coplanar() {
     std::vector<uint8_t> op(nq);
     std::vector<int32_t> idx(5 * nq, 0);
     for (int64_t q = 0; q < nq; ++q) {
-        op[q] = (uint8_t)(rnd() % 8);
+        op[q] = (uint8_t)(rnd() % 9);
         // mostly within one group (ring, grid or random) so the queries meet
         // the predicates' preconditions; some across groups
         const int64_t g = rnd() % 4, lo = g == 0 ? 0 : g == 1 ? 12 : g == 2 ? 61 : 0;

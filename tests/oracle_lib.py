@@ -18,6 +18,7 @@ import numpy as np
 ROOT = Path(__file__).resolve().parents[1]
 ORACLE_SO = ROOT / "oracle" / "liboracle.so"
 REF_SO = ROOT / "oracle" / "_ref" / "libdel3d_ref.so"
+REF_DRIVER_SO = ROOT / "oracle" / "_ref" / "libdel3d_ref_driver.so"
 TRACE_RECORDER = ROOT / "oracle" / "_ref" / "trace_recorder"
 
 _dp = C.POINTER(C.c_double)
@@ -29,6 +30,7 @@ _i8p = C.POINTER(C.c_int8)
 
 _oracle = None
 _ref = None
+_ref_driver = None
 
 
 def _ptr(a: np.ndarray, t):
@@ -149,9 +151,29 @@ def pred2_batch(pts, op, idx5, which: str = "oracle") -> np.ndarray:
         if r is None:
             raise RuntimeError("reference library not built")
         r.ref_pred2_batch(*args)
+        inexact = np.nonzero(op == 8)[0]
+        if len(inexact):
+            sub = np.ascontiguousarray(idx5[inexact])
+            res = np.zeros(len(inexact), np.int8)
+            ref_driver().ref_inexact_orientation_batch(_ptr(xyz, _dp), _ptr(sub, _i32p),
+                                                       len(inexact), _ptr(res, _i8p))
+            out[inexact] = res
     else:
         raise ValueError(which)
     return out
+
+
+def ref_driver():
+    """The reference driver's private plain-double orientation (oracle/
+    ref_driver.cpp), or an error where it was not built."""
+    global _ref_driver
+    if _ref_driver is None:
+        if not REF_DRIVER_SO.exists():
+            raise RuntimeError("reference driver library not built")
+        lib = C.CDLL(str(REF_DRIVER_SO))
+        lib.ref_inexact_orientation_batch.argtypes = [_dp, _i32p, C.c_int64, _i8p]
+        _ref_driver = lib
+    return _ref_driver
 
 
 def stage_counts() -> tuple[int, int, int]:

@@ -20,8 +20,9 @@ import oracle_lib as O
 from del3d_amd import workloads as W
 
 HOST = Path(__file__).resolve().parents[1] / "del3d_amd" / "libpredcore_host.so"
+PENDING = 9  # the fast path's "left to the exact kernel" (pred2_core.h kPending2)
 OPS = {"orient3": 0, "orient4": 1, "collinear": 2, "equal": 3, "position": 4, "in_circle": 5,
-       "in_circle_raw": 6, "orient2d_xy": 7}
+       "in_circle_raw": 6, "orient2d_xy": 7, "inexact": 8}
 
 
 def host_core(pts, op, idx, mode):
@@ -154,8 +155,8 @@ def test_gpu_core_on_host_matches_oracle(kind, mode):
     op, idx = W.coplanar_queries(len(pts), 6000, rng)
     want = O.pred2_batch(pts, op, idx, "oracle")
     got = host_core(pts, op, idx, mode)
-    if mode == 1:  # the filter-only result: where it decides, it is final
-        dec = got != 2
+    if mode == 1:  # the fast path's result: where it decides, it is final
+        dec = got != PENDING
         assert dec.mean() > 0.2
         assert np.array_equal(got[dec], want[dec])
     else:
@@ -179,3 +180,76 @@ def test_gpu_core_extreme_magnitudes():
     want = O.pred2_batch(pts, op, idx, "int")
     for mode in (0, 2):
         assert np.array_equal(host_core(pts, op, idx, mode), want), mode
+
+
+def inexact_points(rng, n=300):
+    """Points for the plain-double orientation: random doubles, near-coplanar
+    quadruples (a random plane plus perturbations of a few ulps) and a lattice,
+    so that many determinants round to tiny values of either sign or to 0."""
+    a = rng.random((n, 3))
+    base = rng.random((n, 2))
+    plane = np.column_stack([base, 0.3 * base[:, 0] + 0.7 * base[:, 1] + 0.1])
+    plane[:, 2] += rng.integers(-3, 4, size=n) * np.spacing(plane[:, 2])
+    return np.vstack([a, plane, W.lattice(4, 4, 4)])
+
+
+def test_inexact_orientation_matches_compiled_driver():
+    """The driver's plain-double orientation (src/triangulation.cpp:296-308)
+    against the driver's own code, compiled from its source (bit-exact, also
+    where the rounded determinant's sign is wrong), and against the exact sign
+    where the determinant is far from zero."""
+    rng = np.random.default_rng(8)
+    pts = inexact_points(rng)
+    n = 20000
+    idx = rng.integers(0, len(pts), size=(n, 5)).astype(np.int32)
+    near = rng.integers(300, 600, size=(n // 2, 4))
+    idx[: n // 2, :4] = near  # quadruples from the near-coplanar group
+    op = np.full(n, OPS["inexact"], np.uint8)
+    got = O.pred2_batch(pts, op, idx, "oracle")
+    ex = O.pred2_batch(pts, op, idx, "int")
+    assert (got != ex).sum() > 20          # the near-coplanar group does flip signs
+    far = np.abs(np.linalg.det(pts[idx[:, 1:4]] - pts[idx[:, :1]])) > 1e-6
+    assert np.array_equal(got[far], ex[far]) and far.sum() > n // 3
+    if HOST.exists():
+        assert np.array_equal(host_core(pts, op, idx, 0), got)
+        assert np.array_equal(host_core(pts, op, idx, 1), got)
+    if O.REF_DRIVER_SO.exists():
+        assert np.array_equal(O.pred2_batch(pts, op, idx, "ref"), got)
+
+
+@pytest.mark.parametrize("scale_exp", [-100, 0, 90])
+def test_certificate_decides_small_grids_and_agrees(scale_exp):
+    """The certified small-grid tier of the fast path (filter-only mode): on
+    integer points whose differences stay below 2^6 (in-circle) every query is
+    decided, including exactly zero determinants, and agrees with the oracle;
+    with one coordinate planted half a unit off the grid, or differences of
+    2^7 and more, the certificate must refuse, and whatever the fast path does
+    decide still agrees with the oracle. Points are not coplanar: the bound
+    holds for any points."""
+    if not HOST.exists():
+        pytest.fail(f"{HOST} not built")
+    rng = np.random.default_rng(400 + scale_exp)
+    s = 2.0 ** scale_exp
+    small = rng.integers(0, 31, size=(200, 3)).astype(np.float64)   # differences < 2^5
+    coplanar = np.column_stack([rng.integers(0, 31, size=(200, 2)), np.zeros(200)])
+    coplanar[:, 2] = coplanar[:, 0] + 2 * coplanar[:, 1] - 40       # z = x + 2y - 40
+    wide = rng.integers(0, 1 << 9, size=(200, 3)).astype(np.float64)
+    planted = small.copy()
+    planted[::7, 1] += 0.5
+    n = 8000
+    for name, pts, must_decide in (("small", small, True), ("coplanar", coplanar, False),
+                                   ("wide", wide, False), ("planted", planted, False)):
+        pts = pts * s
+        op = rng.choice(np.array([OPS["in_circle_raw"], OPS["in_circle"], OPS["orient3"],
+                                  OPS["collinear"], OPS["orient4"]], np.uint8), size=n)
+        idx = rng.integers(0, len(pts), size=(n, 5)).astype(np.int32)
+        idx[: n // 10, 3] = idx[: n // 10, 0]
+        want = O.pred2_batch(pts, op, idx, "oracle")
+        fast = host_core(pts, op, idx, 1)
+        dec = fast != PENDING
+        assert np.array_equal(fast[dec], want[dec]), name
+        if name == "small":
+            raw = op == OPS["in_circle_raw"]
+            assert dec[raw].all(), name      # every in-circle determinant certified
+        if name == "coplanar":
+            assert dec.mean() > 0.95, (name, dec.mean())

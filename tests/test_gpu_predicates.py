@@ -2,9 +2,6 @@
 on the reference's own test workloads, the measured configurations and the
 hard cases (wide exponent spans, denormals, aliases, ties)."""
 import json
-import struct
-import subprocess
-import tempfile
 from pathlib import Path
 
 import numpy as np
@@ -153,39 +150,6 @@ def test_host_memory_form():
     assert np.array_equal(D.predicates_np(pts, op, idx), O.pred_batch(pts, op, idx, "oracle"))
 
 
-def _read_trace(path):
-    raw = Path(path).read_bytes()
-    assert raw[:4] == b"D3TR"
-    n, nrec = struct.unpack_from("<qq", raw, 8)
-    off = 24
-    pts = np.frombuffer(raw, dtype=np.float64, count=3 * n, offset=off).reshape(n, 3)
-    off += 24 * n
-    op = np.frombuffer(raw, dtype=np.uint8, count=nrec, offset=off)
-    off += nrec
-    sign = np.frombuffer(raw, dtype=np.int8, count=nrec, offset=off)
-    off += nrec
-    idx = np.frombuffer(raw, dtype=np.int32, count=5 * nrec, offset=off).reshape(nrec, 5)
-    return pts, op, sign, idx
-
-
-@pytest.mark.skipif(not O.TRACE_RECORDER.exists(), reason="trace recorder not built (needs the reference sources at build time)")
-def test_c1_trace_replay():
-    """Every orient3d / in_sphere call of the reference's Delaunay build on
-    10,000 rand48 points, replayed on the GPU: identical signs."""
-    with tempfile.TemporaryDirectory() as td:
-        out = Path(td) / "c1.trace"
-        say("recording C1 trace")
-        subprocess.run([str(O.TRACE_RECORDER), "10000", "1", str(out)], check=True, timeout=300)
-        pts, op, sign, idx = _read_trace(out)
-    assert np.array_equal(pts, W.c1_points(10000, 1))
-    idx = idx.copy()
-    idx[op == 0, 4] = idx[op == 0, 3]
-    say("replaying", len(op))
-    got, st = predicates_dev(pts, op, idx, stats=True)
-    assert np.array_equal(got, sign), int((got != sign).sum())
-    say("trace stats", st)
-
-
 def _plan_signs(pts_list, op_list, idx_list, knobs=None, graph=False, replays=1):
     """Runs a plan over several sets (set-local indices); returns the signs."""
     sizes = [len(p) for p in pts_list]
@@ -233,15 +197,16 @@ def test_lds_and_global_paths_agree_with_oracle(name):
 
 def test_out_of_range_index_gives_zero():
     pts = O.lattice(3, 3, 3)
-    op = np.array([0, 1, 2, 1], np.uint8)
-    idx = np.array([[0, 1, 3, 27, 0], [0, 1, 3, 9, -1], [0, 1, 3, 9, 13], [0, 1, 3, 9, 1 << 30]], np.int32)
+    op = np.array([0, 1, 2, 1, 3, 200], np.uint8)
+    idx = np.array([[0, 1, 3, 27, 0], [0, 1, 3, 9, -1], [0, 1, 3, 9, 13], [0, 1, 3, 9, 1 << 30],
+                    [0, 1, 3, 9, 13], [0, 1, 3, 9, 13]], np.int32)
     xyz = D.DeviceArray.from_host(pts)
     dop, didx = D.DeviceArray.from_host(op), D.DeviceArray.from_host(idx)
-    out = D.DeviceArray((4,), np.int8)
+    out = D.DeviceArray((6,), np.int8)
     st = D.predicates_stats(xyz, dop, didx, out)
     got = out.to_host()
-    assert st["bad_index"] == 3
-    assert got[0] == 0 and got[1] == 0 and got[3] == 0
+    assert st["bad_index"] == 5  # three bad indices, two unknown op codes
+    assert got[0] == 0 and got[1] == 0 and got[3] == 0 and got[4] == 0 and got[5] == 0
     assert got[2] == O.pred_batch(pts, op[2:3], idx[2:3], "oracle")[0]
 
 

@@ -142,3 +142,73 @@ def test_scalar_entry_points_known_answers():
         idx = np.array([quad + [0]], np.int32)
         want = O.pred2_batch(pts, np.array([D.IN_CIRCLE], np.uint8), idx, "oracle")[0]
         assert D.coplanar_in_circle(*args) == want, quad
+
+
+TRACES = {
+    "rand48_10000": ["10000", "1"],        # the 471,224 calls of the C1 build
+    "lattice_16^3": ["lattice", "16", "16", "16"],
+    "lattice_40x30x1": ["lattice", "40", "30", "1"],
+    "lattice_64x1x1": ["lattice", "64", "1", "1"],
+}
+
+
+@pytest.mark.skipif(not O.TRACE_RECORDER.exists(), reason="trace recorder not built")
+@pytest.mark.parametrize("name", list(TRACES))
+def test_driver_trace_replay(name, tmp_path):
+    """Every exact predicate call (3D and coplanar) the reference's Delaunay
+    build makes, recorded by the reference itself, replayed in one mixed
+    batch on the GPU: identical results."""
+    import subprocess
+    from del3d_amd import trace as T
+    out = tmp_path / "t.trace"
+    say("recording", name)
+    subprocess.run([str(O.TRACE_RECORDER)] + TRACES[name] + [str(out)], check=True, timeout=300,
+                   capture_output=True)
+    pts, op, res, idx = T.read(out)
+    say("replaying", name, len(op), "records,", int((op >= 16).sum()), "coplanar")
+    got = T.replay(pts, op, idx)
+    bad = np.nonzero(got != res)[0]
+    assert len(bad) == 0, (name, op[bad[:5]], idx[bad[:5]], got[bad[:5]], res[bad[:5]])
+
+
+def test_mixed_batch_all_ops():
+    """3D, coplanar and inexact-orientation queries interleaved in one mixed
+    batch, against the per-family batches and the oracle."""
+    rng = np.random.default_rng(31)
+    pts = np.vstack([W.coplanar_points("ring_tilted", None), W.lattice(4, 4, 4),
+                     rng.random((200, 3))])
+    n = 60000
+    op = rng.choice(np.array([0, 1, 2] + [16 + k for k in range(9)] + [3, 15, 25, 255], np.uint8),
+                    size=n)
+    idx = rng.integers(0, len(pts), size=(n, 5)).astype(np.int32)
+    say("mixed batch", n)
+    got = T_replay(pts, op, idx)
+    want = np.zeros(n, np.int8)
+    m3 = op <= 2
+    want[m3] = O.pred_batch(pts, op[m3], idx[m3], "oracle")
+    m2 = (op >= 16) & (op <= 24)
+    want[m2] = O.pred2_batch(pts, op[m2] - 16, idx[m2], "oracle")
+    assert np.array_equal(got, want)
+    assert np.array_equal(got[m2], run2(pts, op[m2] - 16, idx[m2]))
+
+
+def T_replay(pts, op, idx):
+    from del3d_amd import trace as T
+    return T.replay(pts, op, idx)
+
+
+def test_inexact_orientation_matches_driver():
+    """The driver's plain-double orientation on near-coplanar quadruples
+    (signs of rounded determinants) against the oracle and, where built, the
+    driver's own code."""
+    import test_coplanar as TC
+    rng = np.random.default_rng(8)
+    pts = TC.inexact_points(rng)
+    n = 20000
+    idx = rng.integers(0, len(pts), size=(n, 5)).astype(np.int32)
+    idx[: n // 2, :4] = rng.integers(300, 600, size=(n // 2, 4))
+    op = np.full(n, D.INEXACT_ORIENTATION, np.uint8)
+    got = run2(pts, op, idx)
+    assert np.array_equal(got, O.pred2_batch(pts, op, idx, "oracle"))
+    if O.REF_DRIVER_SO.exists():
+        assert np.array_equal(got, O.pred2_batch(pts, op, idx, "ref"))
