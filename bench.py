@@ -21,9 +21,10 @@ other_configs (skipped with --headline-only): the large configuration (10^7
 points, 2^28 gathered queries), the batched one (4096 point sets, one
 autotuned plan replayed as a graph, against one call per set) and the served
 one (a Poisson job stream through the serving loop, p50/p99 latency, against
-plain per-request calls) and the coplanar predicates (2^24 queries on a
-tilted planar grid), each with its own bit-exactness checks; --config runs
-one of them alone.
+plain per-request calls), the coplanar predicates (2^24 queries on a
+tilted planar grid) and the circumcentres (2^22 cells, lazy and exact
+forms), each with its own bit-exactness checks; --config runs one of them
+alone.
 """
 from __future__ import annotations
 
@@ -412,6 +413,46 @@ def run_coplanar(args) -> dict:
             "bit_exact_sample": bool(np.array_equal(got[s], want)), "sample": len(s)}
 
 
+def run_circumcenters(args) -> dict:
+    """Circumcentres of 2^22 cells over 10^6 points (uniform in the unit
+    cube, every fourth cell a sliver of a few ulps' thickness): the lazy form
+    and the exact form, one batched call per step; samples against the
+    oracle, bit for bit."""
+    import del3d_amd as D
+    rng = np.random.default_rng(7)
+    n_pts, n_c = 1_000_000, 1 << 22
+    pts = rng.random((n_pts, 3))
+    cells = rng.integers(0, n_pts, size=(n_c, 4)).astype(np.int32)
+    # slivers: the fourth point moved onto the plane of the first three
+    sl = np.arange(0, n_c, 4)[: n_pts // 4]
+    a, b, c = (pts[cells[sl, k]] for k in range(3))
+    u, v = rng.random((len(sl), 1)), rng.random((len(sl), 1))
+    pts_sl = a + u * (b - a) + v * (c - a)
+    ids = np.arange(len(sl)) * 4 + 3
+    pts[ids] = pts_sl
+    cells[sl, 3] = ids
+    stream = D.Stream()
+    xyz = D.DeviceArray.from_host(pts)
+    dc = D.DeviceArray.from_host(cells)
+    out = D.DeviceArray((n_c, 3), np.float64)
+    res = {"config": "circumcenters", "points": n_pts, "cells": n_c, "slivers": len(sl)}
+    sys.path.insert(0, str(ROOT / "tests"))
+    import oracle_lib as O
+    s = np.sort(rng.choice(n_c, size=1 << 14, replace=False))
+    for lazy in (True, False):
+        for _ in range(args.warmup + 1):
+            D.circumcenters(xyz, dc, out, lazy, stream)
+        ms = _events_ms(D, stream, lambda: D.circumcenters(xyz, dc, out, lazy, stream), args.steps)
+        got = out.to_host()[s]
+        want = O.circumcenters(pts, cells[s], int(lazy), "ref" if O.have_ref() else "oracle")
+        key = "lazy" if lazy else "exact"
+        res[f"{key}_gcells_per_s"] = round(n_c / (ms / args.steps / 1e3) / 1e9, 3)
+        res[f"{key}_ms_per_step"] = round(ms / args.steps, 3)
+        res[f"{key}_bit_exact_sample"] = bool(np.array_equal(got.view(np.uint64), want.view(np.uint64)))
+    res["sample"] = len(s)
+    return res
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -423,7 +464,7 @@ def main() -> int:
                     help="only the degenerate-lattice line, without the other configurations")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--config", default="lattice",
-                    choices=["lattice", "large", "batched", "served", "coplanar"],
+                    choices=["lattice", "large", "batched", "served", "coplanar", "circumcenters"],
                     help="lattice: the headline line; large / batched: the other configurations")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="only run the calibration copy kernel (for counter scaling)")
@@ -452,7 +493,7 @@ def main() -> int:
 
     if args.config != "lattice":
         res = {"large": run_large, "batched": run_batched, "served": run_served,
-               "coplanar": run_coplanar}[args.config](args)
+               "coplanar": run_coplanar, "circumcenters": run_circumcenters}[args.config](args)
         print(json.dumps(res), flush=True)
         return 0
 
@@ -543,7 +584,7 @@ def main() -> int:
         sub = argparse.Namespace(steps=max(3, args.steps // 5), warmup=1)
         others = {}
         for name, fn in (("large", run_large), ("batched", run_batched), ("served", run_served),
-                         ("coplanar", run_coplanar)):
+                         ("coplanar", run_coplanar), ("circumcenters", run_circumcenters)):
             t0 = time.perf_counter()
             try:
                 others[name] = fn(sub)

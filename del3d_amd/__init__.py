@@ -9,6 +9,7 @@ reference's interface:
   spatial_sort(points, order)          include/del3d/spatial_sort.h:37
   orient2d, coplanar_orientation, collinear, equal, collinear_position,
   coplanar_in_circle(_unperturbed)     include/del3d/predicates2.h:26-69
+  circumcenter, circumcenter_lazy      include/del3d/delaunay.h:29-49
 
 with the same sign convention (NEGATIVE=-1, ZERO=0, POSITIVE=1) and the same
 results, plus batched forms over device buffers (``predicates``,
@@ -79,6 +80,10 @@ def lib() -> C.CDLL:
         L.del3d_predicates2.argtypes = [vp, i64, vp, vp, i64, vp, vp]
         L.del3d_predicates2_host.argtypes = [vp, i64, vp, vp, i64, vp]
         L.del3d_predicates_mixed.argtypes = [vp, i64, vp, vp, i64, vp, vp]
+        L.del3d_circumcenters.argtypes = [vp, i64, vp, i64, C.c_int, vp, vp]
+        L.del3d_circumcenters_host.argtypes = [vp, i64, vp, i64, C.c_int, vp]
+        L.del3d_circumcenter.argtypes = [dp, dp, dp, dp, C.c_int, dp]
+        L.del3d_circumcenter.restype = None
         L.del3d_orient2d.argtypes = [C.c_double] * 6
         for name, n in (("del3d_coplanar_orientation", 3), ("del3d_coplanar_orientation4", 4),
                         ("del3d_collinear", 3), ("del3d_equal", 2), ("del3d_collinear_position", 3),
@@ -135,6 +140,15 @@ def predicates_mixed(xyz: DeviceArray, op: DeviceArray, idx: DeviceArray, out: D
     n_q = int(op.shape[0])
     _check(lib().del3d_predicates_mixed(xyz.ptr, int(xyz.shape[0]), op.ptr, idx.ptr, n_q, out.ptr,
                                         _stream_handle(stream)), "del3d_predicates_mixed")
+
+
+def circumcenters(xyz: DeviceArray, cells: DeviceArray, out: DeviceArray, lazy: bool = False,
+                  stream: Stream | None = None) -> None:
+    """Circumcentres of cells (n, 4) int32 into out (n, 3) f64: the exact
+    circumcenter, or circumcenter_lazy with lazy=True. Enqueued on `stream`."""
+    _check(lib().del3d_circumcenters(xyz.ptr, int(xyz.shape[0]), cells.ptr, int(cells.shape[0]),
+                                     1 if lazy else 0, out.ptr, _stream_handle(stream)),
+           "del3d_circumcenters")
 
 
 STAT_NAMES = ("filter", "certified", "exact_regs", "exact_wide", "perturbed", "bad_index")
@@ -338,6 +352,17 @@ def predicates2_np(points, op, idx) -> np.ndarray:
     return out
 
 
+def circumcenters_np(points, cells, lazy: bool = False) -> np.ndarray:
+    """Circumcentres of cells on host arrays; returns (n, 3)."""
+    xyz = np.ascontiguousarray(np.asarray(points, dtype=np.float64).reshape(-1, 3))
+    cells = np.ascontiguousarray(np.asarray(cells, dtype=np.int32).reshape(-1, 4))
+    out = np.zeros((len(cells), 3))
+    _check(lib().del3d_circumcenters_host(xyz.ctypes.data, len(xyz), cells.ctypes.data, len(cells),
+                                          1 if lazy else 0, out.ctypes.data),
+           "del3d_circumcenters_host")
+    return out
+
+
 def spatial_sort(points, order=None) -> np.ndarray:
     """The reference's spatial_sort(points, order) on host arrays; returns the
     permuted order (a new array)."""
@@ -424,6 +449,22 @@ def coplanar_in_circle_unperturbed(p0, p1, p2, p) -> int:
     return int(lib().del3d_coplanar_in_circle_unperturbed(*ptr))
 
 
+def circumcenter(p0, p1, p2, p3) -> tuple:
+    """The exact circumcentre (one GPU evaluation); p0 if coplanar."""
+    keep, p = _pts(p0, p1, p2, p3)
+    out = (C.c_double * 3)()
+    lib().del3d_circumcenter(*p, 0, out)
+    return tuple(out)
+
+
+def circumcenter_lazy(p, q, r, s) -> tuple:
+    """The lazy circumcentre, bit-compatible with the reference's."""
+    keep, ptr = _pts(p, q, r, s)
+    out = (C.c_double * 3)()
+    lib().del3d_circumcenter(*ptr, 1, out)
+    return tuple(out)
+
+
 __all__ = [
     "NEGATIVE", "ZERO", "POSITIVE", "ORIENT3D", "IN_SPHERE", "IN_SPHERE_UNPERTURBED",
     "DeviceArray", "Stream", "Event", "device_count", "synchronize", "lib", "Del3dError",
@@ -433,5 +474,5 @@ __all__ = [
     "collinear_position", "coplanar_in_circle", "coplanar_in_circle_unperturbed",
     "COPLANAR_ORIENTATION", "COPLANAR_ORIENTATION4", "COLLINEAR", "EQUAL", "COLLINEAR_POSITION",
     "IN_CIRCLE", "IN_CIRCLE_UNPERTURBED", "ORIENT2D_XY", "INEXACT_ORIENTATION", "MIXED_OP2_BASE",
-    "predicates_mixed",
+    "predicates_mixed", "circumcenters", "circumcenters_np", "circumcenter", "circumcenter_lazy",
 ]

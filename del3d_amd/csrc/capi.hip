@@ -351,6 +351,37 @@ int del3d_predicates_mixed(const double* xyz, int64_t n_pts, const uint8_t* op, 
     });
 }
 
+int del3d_circumcenters(const double* xyz, int64_t n_pts, const int32_t* cells, int64_t n_cells,
+                        int lazy, double* out, void* stream) {
+    return guarded([&] {
+        if (n_cells < 0 || n_pts < 0) return set_error(DEL3D_ERR_INVALID, "negative size");
+        if (n_cells > 0 && (!xyz || !cells || !out || n_pts == 0))
+            return set_error(DEL3D_ERR_INVALID, "null pointer or empty point set with cells");
+        launch_circumcenters(xyz, n_pts, cells, n_cells, lazy ? 1 : 0, out, (hipStream_t)stream);
+        return DEL3D_OK;
+    });
+}
+
+int del3d_circumcenters_host(const double* xyz, int64_t n_pts, const int32_t* cells, int64_t n_cells,
+                             int lazy, double* out) {
+    return guarded([&] {
+        if (n_cells < 0 || n_pts < 0) return set_error(DEL3D_ERR_INVALID, "negative size");
+        if (n_cells == 0) return DEL3D_OK;
+        if (!xyz || !cells || !out || n_pts == 0)
+            return set_error(DEL3D_ERR_INVALID, "null pointer or empty point set with cells");
+        hipStream_t s = thread_stream();
+        Scratch sc(s);
+        double* dx = sc.get<double>((size_t)n_pts * 3);
+        int32_t* dc = sc.get<int32_t>((size_t)n_cells * 4);
+        double* dout = sc.get<double>((size_t)n_cells * 3);
+        copy_h2d(dx, xyz, (size_t)n_pts * 24, s);
+        copy_h2d(dc, cells, (size_t)n_cells * 16, s);
+        launch_circumcenters(dx, n_pts, dc, n_cells, lazy ? 1 : 0, dout, s);
+        copy_d2h(out, dout, (size_t)n_cells * 24, s);
+        return DEL3D_OK;
+    });
+}
+
 int del3d_predicates2_host(const double* xyz, int64_t n_pts, const uint8_t* op, const int32_t* idx,
                            int64_t n_q, int8_t* out) {
     return guarded([&] {
@@ -478,6 +509,17 @@ int del3d_coplanar_in_circle_unperturbed(const double* p0, const double* p1, con
                                          const double* p) {
     const double* v[4] = {p0, p1, p2, p};
     return scalar_or_abort(DEL3D_OP2_IN_CIRCLE_UNPERTURBED, v, 4, true);
+}
+
+void del3d_circumcenter(const double* p0, const double* p1, const double* p2, const double* p3,
+                        int lazy, double* out) {
+    const double xyz[12] = {p0[0], p0[1], p0[2], p1[0], p1[1], p1[2],
+                            p2[0], p2[1], p2[2], p3[0], p3[1], p3[2]};
+    const int32_t cell[4] = {0, 1, 2, 3};
+    if (del3d_circumcenters_host(xyz, 4, cell, 1, lazy, out) != DEL3D_OK) {
+        std::fprintf(stderr, "del3d_amd: circumcenter failed: %s\n", last_error_slot().c_str());
+        std::abort();
+    }
 }
 
 }  // extern "C"

@@ -8,6 +8,7 @@
 #include <cstdint>
 #include <vector>
 
+#include "circ_core.h"
 #include "pred2_core.h"
 #include "pred_core.h"
 
@@ -128,5 +129,38 @@ extern "C" void predcore2_host_batch(const double* xyz, int64_t n_pts, const uin
             s = pred2_eval(o, p, id, ws.data());
         }
         out[q] = (int8_t)s;
+    }
+}
+
+// Host build of the circumcentres (circ_core.h): lazy or exact, per cell.
+static int64_t g_circ_overflows = 0;
+
+// Cells of the last batches that outgrew the small workspace.
+extern "C" int64_t circcore_overflows(void) { return g_circ_overflows; }
+
+// The small workspace first, the full one for cells that overflow it (as the
+// GPU's two exact passes do).
+extern "C" void circcore_host_batch(const double* xyz, const int32_t* cells, int64_t n, int lazy,
+                                    double* out) {
+    std::vector<double> ws((size_t)kCircFull.total());
+    const Buf b{ws.data(), 1};
+    for (int64_t c = 0; c < n; ++c) {
+        P3 p[4];
+        for (int k = 0; k < 4; ++k) p[k] = load_p3(xyz, cells[4 * c + k]);
+        double o[3];
+        int need = 7;
+        if (lazy) need = circ_lazy_pass(p[0], p[1], p[2], p[3], o);
+        if (need) {
+            bool over = false;
+            P3 e = circ_exact(p[0], p[1], p[2], p[3], b, kCircSmall, &over);
+            if (over) {
+                ++g_circ_overflows;
+                e = circ_exact(p[0], p[1], p[2], p[3], b, kCircFull, &over);
+            }
+            if (need & 1) o[0] = e.x;
+            if (need & 2) o[1] = e.y;
+            if (need & 4) o[2] = e.z;
+        }
+        for (int k = 0; k < 3; ++k) out[3 * c + k] = o[k];
     }
 }

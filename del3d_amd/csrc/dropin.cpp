@@ -57,6 +57,18 @@ Sign coplanar_in_circle_unperturbed(const double* p0, const double* p1, const do
     return static_cast<Sign>(del3d_coplanar_in_circle_unperturbed(p0, p1, p2, p));
 }
 
+Point3 circumcenter(const Point3& p0, const Point3& p1, const Point3& p2, const Point3& p3) {
+    Point3 r;
+    del3d_circumcenter(&p0.x, &p1.x, &p2.x, &p3.x, 0, &r.x);
+    return r;
+}
+
+Point3 circumcenter_lazy(const Point3& p, const Point3& q, const Point3& r, const Point3& s) {
+    Point3 o;
+    del3d_circumcenter(&p.x, &q.x, &r.x, &s.x, 1, &o.x);
+    return o;
+}
+
 void spatial_sort(const std::vector<Point3>& points, std::vector<int>& order) {
     if (order.size() < 2) return;
     static_assert(sizeof(Point3) == 24, "Point3 must be three packed doubles");

@@ -99,4 +99,9 @@ void launch_predicates2(const double* xyz, int64_t n_pts, const uint8_t* op, con
 // One coplanar query of a ScalarQuery record (points by value, op DEL3D_OP2_*).
 void launch_single2(ScalarQuery* qd, hipStream_t s);
 
+// Circumcentres (circumcenter.hip): cells 4 int32 each, out 3 doubles per
+// cell; lazy selects circumcenter_lazy, else the exact circumcenter.
+void launch_circumcenters(const double* xyz, int64_t n_pts, const int32_t* cells, int64_t n_cells,
+                          int lazy, double* out, hipStream_t s);
+
 }  // namespace d3

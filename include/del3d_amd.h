@@ -291,12 +291,33 @@ int del3d_predicates2(const double* xyz, int64_t n_pts, const uint8_t* op, const
 int del3d_predicates_mixed(const double* xyz, int64_t n_pts, const uint8_t* op,
                            const int32_t* idx, int64_t n_q, int8_t* out, void* stream);
 
+/*
+ * Batched circumcentres of tetrahedra over device-resident data.
+ *
+ * Replaces, per cell c, one call of
+ *   circumcenter(p0,p1,p2,p3)        include/del3d/delaunay.h:29-30
+ *                                    (src/triangulation.cpp:805)      lazy = 0
+ *   circumcenter_lazy(p,q,r,s)       include/del3d/delaunay.h:48-49
+ *                                    (src/triangulation.cpp:874)      lazy != 0
+ * with the points at xyz + 3*cells[4c+k]; out[3c .. 3c+2] receives the
+ * result's x, y, z, bit for bit the reference's (the exact form's expansion
+ * components and the lazy form's upward-rounded intervals are reproduced; see
+ * the range note in circ_core.h). A cell with an index outside [0, n_pts)
+ * gets NaN coordinates.
+ *   xyz device, 3*n_pts doubles; cells device, 4*n_cells int32;
+ *   out device, 3*n_cells doubles.
+ */
+int del3d_circumcenters(const double* xyz, int64_t n_pts, const int32_t* cells, int64_t n_cells,
+                        int lazy, double* out, void* stream);
+
 /* Host-memory forms: copy in, compute on the current device, copy out,
  * synchronously. For callers without device buffers. */
 int del3d_predicates_host(const double* xyz, int64_t n_pts, const uint8_t* op,
                           const int32_t* idx, int64_t n_q, int8_t* sign_out);
 int del3d_predicates2_host(const double* xyz, int64_t n_pts, const uint8_t* op,
                            const int32_t* idx, int64_t n_q, int8_t* out);
+int del3d_circumcenters_host(const double* xyz, int64_t n_pts, const int32_t* cells,
+                             int64_t n_cells, int lazy, double* out);
 int del3d_spatial_sort_host(const double* xyz, int64_t n_pts, int32_t* order, int64_t n_order);
 
 /*
@@ -324,6 +345,10 @@ int del3d_coplanar_in_circle(const double* p0, const double* p1, const double* p
                              const double* p);
 int del3d_coplanar_in_circle_unperturbed(const double* p0, const double* p1, const double* p2,
                                          const double* p);
+
+/* One circumcentre (lazy or exact), computed on the GPU; out: x, y, z. */
+void del3d_circumcenter(const double* p0, const double* p1, const double* p2, const double* p3,
+                        int lazy, double* out);
 
 #ifdef __cplusplus
 }

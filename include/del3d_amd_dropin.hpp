@@ -9,6 +9,8 @@
 //   the coplanar predicates          include/del3d/predicates2.h:26-69
 //     (orient2d, coplanar_orientation x2, collinear, equal,
 //      collinear_position, coplanar_in_circle(_unperturbed))
+//   Point3 circumcenter(...)         include/del3d/delaunay.h:29-30
+//   Point3 circumcenter_lazy(...)    include/del3d/delaunay.h:48-49
 // The enum and struct layouts are the reference's (predicates.h:21,
 // predicates2.h:50, spatial_sort.h:31), which the mangled symbols and the
 // data layout depend on.
@@ -42,6 +44,9 @@ CollinearPosition collinear_position(const double* p, const double* q, const dou
 Sign coplanar_in_circle(const double* p0, const double* p1, const double* p2, const double* p);
 Sign coplanar_in_circle_unperturbed(const double* p0, const double* p1, const double* p2,
                                     const double* p);
+
+Point3 circumcenter(const Point3& p0, const Point3& p1, const Point3& p2, const Point3& p3);
+Point3 circumcenter_lazy(const Point3& p, const Point3& q, const Point3& r, const Point3& s);
 
 }  // namespace del3d
 

@@ -9,6 +9,7 @@
  * What it restates (reference paths relative to the del3d source tree):
  *   - predicates   src/predicates.cpp:22-232 (filter, expansion exact stage,
  *                  symbolic perturbation) with src/expansion.h:33-178
+ *   - circumcentres src/triangulation.cpp:805-930 with src/interval.h:27-186
  *   - coplanar     src/predicates2.cpp:22-259 (orient2d, coplanar orientation,
  *                  collinear, equal, collinear_position, in-circle) and the
  *                  driver's plain-double orientation, src/triangulation.cpp:296-308
@@ -80,6 +81,12 @@ int orc_pred2_one(uint8_t op, const double* a, const double* b, const double* c,
                   const double* d);
 void orc_pred2_batch(const double* xyz, int64_t n_pts, const uint8_t* op,
                      const int32_t* idx5, int64_t n_q, int8_t* sign_out);
+
+/* ---- circumcentres (restating src/triangulation.cpp:805-930) ----------- */
+/* cells: 4 point indices per cell; out: 3 doubles per cell. lazy != 0:
+ * circumcenter_lazy (intervals under FE_UPWARD, src/interval.h), else the
+ * exact circumcenter. */
+void orc_circumcenters(const double* xyz, const int32_t* cells, int64_t n, int lazy, double* out);
 
 /* ---- big-integer oracle (method of tests/exact.h) ---------------------- */
 int orc_int_orient3d(const double* a, const double* b, const double* c, const double* d);

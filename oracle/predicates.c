@@ -636,3 +636,182 @@ void orc_pred2_batch(const double* xyz, int64_t n_pts, const uint8_t* op,
         sign_out[q] = (int8_t)orc_pred2_one(op[q], p[0], p[1], p[2], p[3]);
     }
 }
+
+/* ====================================================================== */
+/* Circumcentres (restating src/triangulation.cpp:805-930 with             */
+/* src/interval.h:27-186).                                                 */
+/* ====================================================================== */
+#include <fenv.h>
+
+/* The exact closed form: numerator and denominator as expansions, one
+ * rounding per coordinate at the end. */
+static void circumcenter_exact(const double* p0, const double* p1, const double* p2,
+                               const double* p3, double out[3]) {
+    Exp a[3], b[3], c[3];
+    for (int k = 0; k < 3; ++k) {
+        a[k] = exp_diff2(p1[k], p0[k]);
+        b[k] = exp_diff2(p2[k], p0[k]);
+        c[k] = exp_diff2(p3[k], p0[k]);
+    }
+    /* u x v, coordinate k = u[k+1] v[k+2] - u[k+2] v[k+1] */
+    Exp bc[3], ca[3], ab[3];
+    for (int k = 0; k < 3; ++k) {
+        const int k1 = (k + 1) % 3, k2 = (k + 2) % 3;
+        bc[k] = exp_sub(exp_mul(b[k1], c[k2]), exp_mul(b[k2], c[k1]));
+        ca[k] = exp_sub(exp_mul(c[k1], a[k2]), exp_mul(c[k2], a[k1]));
+        ab[k] = exp_sub(exp_mul(a[k1], b[k2]), exp_mul(a[k2], b[k1]));
+    }
+    const Exp a2 = exp_add(exp_add(exp_mul(a[0], a[0]), exp_mul(a[1], a[1])), exp_mul(a[2], a[2]));
+    const Exp b2 = exp_add(exp_add(exp_mul(b[0], b[0]), exp_mul(b[1], b[1])), exp_mul(b[2], b[2]));
+    const Exp c2 = exp_add(exp_add(exp_mul(c[0], c[0]), exp_mul(c[1], c[1])), exp_mul(c[2], c[2]));
+    double num[3];
+    for (int k = 0; k < 3; ++k) {
+        const Exp n = exp_add(exp_add(exp_mul(a2, bc[k]), exp_mul(b2, ca[k])), exp_mul(c2, ab[k]));
+        double s = 0.0;
+        for (int i = 0; i < n.n; ++i) s += n.c[i];
+        num[k] = s;
+    }
+    Exp two;
+    two.n = 1;
+    two.c = arena_alloc(1);
+    two.c[0] = 2.0;
+    const Exp den = exp_mul(exp_add(exp_add(exp_mul(a[0], bc[0]), exp_mul(a[1], bc[1])),
+                                    exp_mul(a[2], bc[2])),
+                            two);
+    if (den.n == 0) {
+        out[0] = p0[0]; out[1] = p0[1]; out[2] = p0[2];
+        return;
+    }
+    double d = 0.0;
+    for (int i = 0; i < den.n; ++i) d += den.c[i];
+    for (int k = 0; k < 3; ++k) out[k] = p0[k] + num[k] / d;
+}
+
+/* Intervals under FE_UPWARD, the lower bound stored negated; every
+ * operation forced through memory so nothing is folded or reassociated. */
+typedef struct { double ni, s; } Iv;
+static double opq(double x) { volatile double v = x; return v; }
+static double iv_lo(Iv a) { return -a.ni; }
+static Iv iv_make(double i, double s) { Iv r; r.ni = -i; r.s = s; return r; }
+static Iv iv_pt(double d) { return iv_make(d, d); }
+static double up_mul(double a, double b) { return opq(opq(a) * opq(b)); }
+static double up_div(double a, double b) { return opq(opq(a) / opq(b)); }
+static double dmax(double a, double b) { return a < b ? b : a; }
+
+static Iv iv_add(Iv a, Iv b) {
+    return iv_make(-opq(opq(-iv_lo(a)) + opq(-iv_lo(b))), opq(opq(a.s) + opq(b.s)));
+}
+static Iv iv_sub(Iv a, Iv b) {
+    return iv_make(-opq(opq(-iv_lo(a)) + opq(b.s)), opq(opq(a.s) + opq(-iv_lo(b))));
+}
+static Iv iv_mul(Iv a, Iv b) {
+    if (iv_lo(a) >= 0.0) {
+        double aa = iv_lo(a), bb = a.s;
+        if (bb <= 0.0) return iv_pt(0.0);
+        if (iv_lo(b) < 0.0) {
+            aa = bb;
+            if (b.s < 0.0) bb = iv_lo(a);
+        }
+        const double r = (b.s == 0) ? 0.0 : up_mul(bb, b.s);
+        return iv_make(-up_mul(aa, -iv_lo(b)), r);
+    }
+    if (a.s <= 0.0) {
+        double aa = a.s, bb = iv_lo(a);
+        if (iv_lo(b) < 0.0) {
+            aa = bb;
+            if (b.s <= 0.0) bb = a.s;
+        } else if (b.s <= 0) {
+            return iv_pt(0.0);
+        }
+        return iv_make(-up_mul(-bb, b.s), up_mul(-aa, -iv_lo(b)));
+    }
+    if (iv_lo(b) >= 0.0) {
+        if (b.s <= 0.0) return iv_pt(0.0);
+        return iv_make(-up_mul(-iv_lo(a), b.s), up_mul(a.s, b.s));
+    }
+    if (b.s <= 0.0) return iv_make(-up_mul(a.s, -iv_lo(b)), up_mul(-iv_lo(a), -iv_lo(b)));
+    const double t1 = up_mul(-iv_lo(a), b.s), t2 = up_mul(a.s, -iv_lo(b));
+    const double t3 = up_mul(-iv_lo(a), -iv_lo(b)), t4 = up_mul(a.s, b.s);
+    return iv_make(-dmax(t1, t2), dmax(t3, t4));
+}
+static Iv iv_div(Iv a, Iv b) {
+    if (iv_lo(b) > 0.0) {
+        double aa = b.s, bb = iv_lo(b);
+        if (iv_lo(a) < 0.0) {
+            aa = bb;
+            if (a.s < 0.0) bb = b.s;
+        }
+        return iv_make(-up_div(-iv_lo(a), aa), up_div(a.s, bb));
+    }
+    if (b.s < 0.0) {
+        double aa = b.s, bb = iv_lo(b);
+        if (iv_lo(a) < 0.0) {
+            bb = aa;
+            if (a.s < 0.0) aa = iv_lo(b);
+        }
+        return iv_make(-up_div(a.s, -aa), up_div(iv_lo(a), bb));
+    }
+    return iv_make(-INFINITY, INFINITY);
+}
+static Iv iv_sq(Iv d) {
+    if (iv_lo(d) >= 0.0) return iv_make(-up_mul(-iv_lo(d), iv_lo(d)), up_mul(d.s, d.s));
+    if (d.s <= 0.0) return iv_make(-up_mul(d.s, -d.s), up_mul(-iv_lo(d), -iv_lo(d)));
+    const double m = dmax(-iv_lo(d), d.s);
+    return iv_make(0.0, up_mul(m, m));
+}
+static Iv iv_det3(Iv a00, Iv a01, Iv a02, Iv a10, Iv a11, Iv a12, Iv a20, Iv a21, Iv a22) {
+    const Iv m01 = iv_sub(iv_mul(a00, a11), iv_mul(a10, a01));
+    const Iv m02 = iv_sub(iv_mul(a00, a21), iv_mul(a20, a01));
+    const Iv m12 = iv_sub(iv_mul(a10, a21), iv_mul(a20, a11));
+    return iv_add(iv_sub(iv_mul(m01, a22), iv_mul(m02, a12)), iv_mul(m12, a02));
+}
+
+static void circumcenter_lazy(const double* p, const double* q, const double* r, const double* s,
+                              double out[3]) {
+    int need[3] = {0, 0, 0};
+    const int saved = fegetround();
+    fesetround(FE_UPWARD);
+    {
+        const Iv px = iv_pt(p[0]), py = iv_pt(p[1]), pz = iv_pt(p[2]);
+        const Iv qpx = iv_sub(iv_pt(q[0]), px), qpy = iv_sub(iv_pt(q[1]), py), qpz = iv_sub(iv_pt(q[2]), pz);
+        const Iv qp2 = iv_add(iv_add(iv_sq(qpx), iv_sq(qpy)), iv_sq(qpz));
+        const Iv rpx = iv_sub(iv_pt(r[0]), px), rpy = iv_sub(iv_pt(r[1]), py), rpz = iv_sub(iv_pt(r[2]), pz);
+        const Iv rp2 = iv_add(iv_add(iv_sq(rpx), iv_sq(rpy)), iv_sq(rpz));
+        const Iv spx = iv_sub(iv_pt(s[0]), px), spy = iv_sub(iv_pt(s[1]), py), spz = iv_sub(iv_pt(s[2]), pz);
+        const Iv sp2 = iv_add(iv_add(iv_sq(spx), iv_sq(spy)), iv_sq(spz));
+        const Iv nx = iv_det3(qpy, qpz, qp2, rpy, rpz, rp2, spy, spz, sp2);
+        const Iv ny = iv_det3(qpx, qpz, qp2, rpx, rpz, rp2, spx, spz, sp2);
+        const Iv nz = iv_det3(qpx, qpy, qp2, rpx, rpy, rp2, spx, spy, sp2);
+        const Iv den = iv_det3(qpx, qpy, qpz, rpx, rpy, rpz, spx, spy, spz);
+        const Iv inv = iv_div(iv_pt(1.0), iv_mul(iv_pt(2.0), den));
+        const Iv c[3] = {iv_add(px, iv_mul(nx, inv)), iv_sub(py, iv_mul(ny, inv)),
+                         iv_add(pz, iv_mul(nz, inv))};
+        const double prec = 0.00001;
+        for (int i = 0; i < 3; ++i) {
+            const double lo = iv_lo(c[i]), hi = c[i].s;
+            const double mag = dmax(fabs(lo), fabs(hi));
+            const double radius = opq(opq(opq(hi) - opq(lo)) / 2);
+            if (hi == lo) out[i] = lo;
+            else if (mag == 0 || radius < opq(prec * mag)) out[i] = opq(opq(hi + lo) * 0.5);
+            else need[i] = 1;
+        }
+    }
+    fesetround(saved);
+    if (need[0] || need[1] || need[2]) {
+        double ex[3];
+        circumcenter_exact(p, q, r, s, ex);
+        for (int i = 0; i < 3; ++i)
+            if (need[i]) out[i] = ex[i];
+    }
+}
+
+void orc_circumcenters(const double* xyz, const int32_t* cells, int64_t n, int lazy, double* out) {
+    for (int64_t c = 0; c < n; ++c) {
+        const int32_t* v = cells + 4 * c;
+        const double* p[4];
+        for (int k = 0; k < 4; ++k) p[k] = xyz + 3 * (int64_t)v[k];
+        if (lazy) circumcenter_lazy(p[0], p[1], p[2], p[3], out + 3 * c);
+        else circumcenter_exact(p[0], p[1], p[2], p[3], out + 3 * c);
+        arena_reset();
+    }
+}

@@ -8,6 +8,7 @@
 //   include/del3d/predicates.h:26,49-50,54-55  orient3d / in_sphere(_unperturbed)
 //   include/del3d/predicates2.h:26-69          the coplanar predicates
 //   include/del3d/spatial_sort.h:31,37         Point3 / spatial_sort
+//   include/del3d/delaunay.h:29-49             circumcenter / circumcenter_lazy
 //   include/del3d/delaunay.h:62-92             Delaunay (build, finite_cells)
 #include "del3d/delaunay.h"
 #include "del3d/predicates.h"
@@ -84,6 +85,23 @@ void ref_pred2_batch(const double* xyz, int64_t n_pts, const uint8_t* op, const 
             default: break;
         }
         out[q] = static_cast<int8_t>(s);
+    }
+}
+
+// Circumcentres of cells (4 point indices each): lazy != 0 selects
+// circumcenter_lazy (include/del3d/delaunay.h:48-49), else circumcenter (:29-30).
+void ref_circumcenters(const double* xyz, const int32_t* cells, int64_t n, int lazy, double* out) {
+    for (int64_t c = 0; c < n; ++c) {
+        del3d::Point3 p[4];
+        for (int k = 0; k < 4; ++k) {
+            const double* v = xyz + 3 * int64_t(cells[4 * c + k]);
+            p[k] = {v[0], v[1], v[2]};
+        }
+        const del3d::Point3 r = lazy ? del3d::circumcenter_lazy(p[0], p[1], p[2], p[3])
+                                     : del3d::circumcenter(p[0], p[1], p[2], p[3]);
+        out[3 * c] = r.x;
+        out[3 * c + 1] = r.y;
+        out[3 * c + 2] = r.z;
     }
 }
 

@@ -59,6 +59,7 @@ def main():
             t.write_text(f'#include "{CSRC}/common.h"\n' + rewrite(p.read_text()).replace(
                 '#include "common.h"', "").replace('#include "pred_core.h"', f'#include "{CSRC}/pred_core.h"').replace(
                 '#include "pred2_core.h"', f'#include "{CSRC}/pred2_core.h"').replace(
+                '#include "circ_core.h"', f'#include "{CSRC}/circ_core.h"').replace(
                 '#include "predicates.h"', f'#include "{CSRC}/predicates.h"'))
             if p.name == "predicates.hip":
                 # two sources define ScalarQuery; keep the translation units apart

@@ -57,3 +57,8 @@ extern "C" int emu_predicates2(const double* xyz, int64_t n_pts, const uint8_t* 
                                const int32_t* idx, int64_t n_q, int8_t* out) {
     return del3d_predicates2_host(xyz, n_pts, op, idx, n_q, out);
 }
+
+extern "C" int emu_circumcenters(const double* xyz, int64_t n_pts, const int32_t* cells, int64_t n,
+                                 int lazy, double* out) {
+    return del3d_circumcenters_host(xyz, n_pts, cells, n, lazy, out);
+}

@@ -5,6 +5,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <cstring>
 #include <vector>
 
 #include "../../oracle/oracle.h"
@@ -242,6 +243,47 @@ static int check_coplanar() {
     return ok ? 0 : 1;
 }
 
+extern "C" int emu_circumcenters(const double* xyz, int64_t n_pts, const int32_t* cells, int64_t n,
+                                 int lazy, double* out);
+
+// Circumcentres, lazy and exact, on random cells and slivers (the lazy pass
+// refuses those and queues them for the exact kernel), plus a bad index.
+static int check_circumcenters() {
+    std::vector<double> xyz;
+    for (int i = 0; i < 200; ++i)
+        for (int k = 0; k < 3; ++k) xyz.push_back((double)(rnd() >> 11) * 0x1p-53);
+    for (int i = 0; i < 50; ++i) {  // near-coplanar quadruples: z = x/4 + y/2 + tiny
+        for (int j = 0; j < 4; ++j) {
+            const double x = (double)(rnd() >> 11) * 0x1p-53, y = (double)(rnd() >> 11) * 0x1p-53;
+            xyz.insert(xyz.end(), {x, y, 0.25 * x + 0.5 * y + (j == 3 ? 1e-13 : 0.0)});
+        }
+    }
+    for (int i = 0; i < 100; ++i)  // mixed scales: expansions outgrow the small workspace
+        for (int k = 0; k < 3; ++k)
+            xyz.push_back(std::ldexp((double)(rnd() >> 11) * 0x1p-53 + 0.5, (int)(rnd() % 120) - 60));
+    const int64_t n = (int64_t)xyz.size() / 3;
+    std::vector<int32_t> cells;
+    for (int c = 0; c < 600; ++c)
+        for (int k = 0; k < 4; ++k) cells.push_back((int32_t)(rnd() % 200));
+    for (int c = 0; c < 300; ++c)
+        for (int k = 0; k < 4; ++k) cells.push_back((int32_t)(400 + rnd() % 100));
+    for (int c = 0; c < 50; ++c)
+        for (int k = 0; k < 4; ++k) cells.push_back(200 + 4 * c + k);
+    cells.insert(cells.end(), {0, 1, 2, (int32_t)n});  // out of range: NaN
+    const int64_t nc = (int64_t)cells.size() / 4;
+    int fails = 0;
+    for (int lazy = 0; lazy < 2; ++lazy) {
+        std::vector<double> got(3 * nc), want(3 * nc);
+        if (emu_circumcenters(xyz.data(), n, cells.data(), nc, lazy, got.data()) != 0) return 1;
+        orc_circumcenters(xyz.data(), cells.data(), nc - 1, lazy, want.data());
+        bool ok = std::memcmp(got.data(), want.data(), sizeof(double) * 3 * (nc - 1)) == 0;
+        for (int k = 0; k < 3; ++k) ok = ok && std::isnan(got[3 * (nc - 1) + k]);
+        std::printf("circumcenters %s %s\n", lazy ? "lazy" : "exact", ok ? "ok" : "MISMATCH");
+        fails += ok ? 0 : 1;
+    }
+    return fails;
+}
+
 int main(int argc, char** argv) {
     int fails = 0;
     {
@@ -265,6 +307,7 @@ int main(int argc, char** argv) {
         fails += check_paths("wide", wide, 2500);
         fails += check_sets();
         fails += check_coplanar();
+        fails += check_circumcenters();
         if (!std::getenv("EMU_DEFER") || std::atoi(std::getenv("EMU_DEFER")) == 0) fails += check_server();
         if (std::getenv("EMU_PRED_ONLY")) return fails ? 1 : 0;
     }

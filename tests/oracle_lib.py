@@ -73,6 +73,7 @@ def oracle() -> C.CDLL:
         lib.orc_pts_lattice.restype = C.c_int64
         lib.orc_pts_random_cube.argtypes = [C.c_int64, C.c_uint32, _dp]
         lib.orc_pts_random_cube.restype = C.c_int64
+        lib.orc_circumcenters.argtypes = [_dp, _i32p, C.c_int64, C.c_int, _dp]
         lib.orc_mt19937_words.argtypes = [C.c_uint32, C.c_int64, _u32p]
         lib.orc_rng_below.argtypes = [C.c_uint32, C.c_uint64, C.c_int64, _i64p]
         _oracle = lib
@@ -94,6 +95,7 @@ def ref():
         lib.ref_in_sphere_unperturbed.restype = C.c_int
         lib.ref_pred_batch.argtypes = [_dp, C.c_int64, _u8p, _i32p, C.c_int64, _i8p]
         lib.ref_pred2_batch.argtypes = [_dp, C.c_int64, _u8p, _i32p, C.c_int64, _i8p]
+        lib.ref_circumcenters.argtypes = [_dp, _i32p, C.c_int64, C.c_int, _dp]
         lib.ref_pred_batch_mt.argtypes = [_dp, C.c_int64, _u8p, _i32p, C.c_int64, _i8p, C.c_int]
         lib.ref_spatial_sort.argtypes = [_dp, C.c_int64, _i32p]
         lib.ref_delaunay_cells.argtypes = [_dp, C.c_int64, _i32p, C.c_int64]
@@ -174,6 +176,37 @@ def ref_driver():
         lib.ref_inexact_orientation_batch.argtypes = [_dp, _i32p, C.c_int64, _i8p]
         _ref_driver = lib
     return _ref_driver
+
+
+def circumcenters(pts, cells, lazy: int, which: str = "oracle") -> np.ndarray:
+    """(n, 3) circumcentres of cells (4 point indices each) from the
+    restatement ("oracle") or the compiled reference ("ref")."""
+    xyz = _xyz(pts)
+    cells = np.ascontiguousarray(np.asarray(cells, dtype=np.int32).reshape(-1, 4))
+    out = np.zeros((len(cells), 3))
+    args = (_ptr(xyz, _dp), _ptr(cells, _i32p), len(cells), int(lazy), _ptr(out, _dp))
+    if which == "oracle":
+        oracle().orc_circumcenters(*args)
+    elif which == "ref":
+        r = ref()
+        if r is None:
+            raise RuntimeError("reference library not built")
+        r.ref_circumcenters(*args)
+    else:
+        raise ValueError(which)
+    return out
+
+
+def delaunay_cells(pts) -> np.ndarray:
+    """The finite cells of the reference's Delaunay triangulation, in its order."""
+    r = ref()
+    if r is None:
+        raise RuntimeError("reference library not built")
+    xyz = _xyz(pts)
+    n = r.ref_delaunay_cells(_ptr(xyz, _dp), len(xyz), None, 0)
+    cells = np.zeros((n, 4), np.int32)
+    r.ref_delaunay_cells(_ptr(xyz, _dp), len(xyz), _ptr(cells, _i32p), n)
+    return cells
 
 
 def stage_counts() -> tuple[int, int, int]:

@@ -26,6 +26,9 @@ DROPIN_SYMBOLS = [
     "_ZN5del3d18collinear_positionEPKdS1_S1_",
     "_ZN5del3d18coplanar_in_circleEPKdS1_S1_S1_",
     "_ZN5del3d30coplanar_in_circle_unperturbedEPKdS1_S1_S1_",
+    # include/del3d/delaunay.h:29-49
+    "_ZN5del3d12circumcenterERKNS_6Point3ES2_S2_S2_",
+    "_ZN5del3d17circumcenter_lazyERKNS_6Point3ES2_S2_S2_",
 ]
 
 

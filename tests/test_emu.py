@@ -25,7 +25,7 @@ def emu_exe():
     csrc = HERE.parent / "del3d_amd" / "csrc"
     subprocess.run(["python", str(HERE / "emu" / "build_emu.py"), str(EXE),
                     str(csrc / "capi.hip"), str(csrc / "predicates.hip"), str(csrc / "predicates2.hip"),
-                    str(csrc / "batch.hip"), str(csrc / "server.hip"), str(csrc / "spatial_sort.hip")],
+                    str(csrc / "circumcenter.hip"), str(csrc / "batch.hip"), str(csrc / "server.hip"), str(csrc / "spatial_sort.hip")],
                    check=True)
     return EXE
 
@@ -45,7 +45,7 @@ def test_emulated_server_under_thread_sanitizer():
     csrc = HERE.parent / "del3d_amd" / "csrc"
     subprocess.run(["python", str(HERE / "emu" / "build_emu.py"), str(exe),
                     str(csrc / "capi.hip"), str(csrc / "predicates.hip"), str(csrc / "predicates2.hip"),
-                    str(csrc / "batch.hip"), str(csrc / "server.hip"), str(csrc / "spatial_sort.hip")],
+                    str(csrc / "circumcenter.hip"), str(csrc / "batch.hip"), str(csrc / "server.hip"), str(csrc / "spatial_sort.hip")],
                    check=True,
                    env=dict(os.environ, EMU_SANITIZER="thread"))
     env = dict(os.environ, EMU_PRED_ONLY="1", TSAN_OPTIONS="halt_on_error=1")
