@@ -510,6 +510,21 @@ __global__ __launch_bounds__(256) void k_grid_points(BatchArgs a, uint32_t n_set
     s_hi[threadIdx.x] = hi;
     s_set[threadIdx.x] = cur;
     __syncthreads();
+    // common case: the whole workgroup saw one set (thread 0 holds the
+    // range's first points, so its set is the reference): a tree reduction
+    // and one atomic pair
+    const int32_t ref = s_set[0];
+    if (__syncthreads_and(cur < 0 || cur == ref)) {
+        for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+            if ((int)threadIdx.x < w) {
+                s_lo[threadIdx.x] = min(s_lo[threadIdx.x], s_lo[threadIdx.x + w]);
+                s_hi[threadIdx.x] = max(s_hi[threadIdx.x], s_hi[threadIdx.x + w]);
+            }
+            __syncthreads();
+        }
+        if (threadIdx.x == 0 && ref >= 0) grid_flush(grid, ref, s_lo[0], s_hi[0]);
+        return;
+    }
     if (threadIdx.x == 0) {
         int32_t c = -1, l = 0x7FFFFFFF, h = -0x7FFFFFFF;
         for (int t = 0; t < (int)blockDim.x; ++t) {
@@ -581,13 +596,13 @@ void enqueue_batch(const BatchArgs& base, const BatchShape& sh, hipStream_t s, h
         D3_HIP(hipEventRecord(ev[1], s));
         D3_HIP(hipEventRecord(ev[2], s));
     }
-    pred_exact<<<cus * 4, 256, 0, s>>>(base);
+    pred_exact<<<cus * 2, 256, 0, s>>>(base);
     D3_LAUNCH_CHECK();
     if (ev) {
         D3_HIP(hipEventRecord(ev[3], s));
         D3_HIP(hipEventRecord(ev[4], s));
     }
-    pred_wide<<<cus * 4, kWideBlock, 0, s>>>(base);
+    pred_wide<<<cus * 2, kWideBlock, 0, s>>>(base);
     D3_LAUNCH_CHECK();
     if (ev) D3_HIP(hipEventRecord(ev[5], s));
 }

@@ -52,6 +52,8 @@ struct Wave {
 struct Block {
     std::unique_ptr<std::barrier<>> bar;
     std::vector<Wave> waves;
+    std::atomic<int> and_count{0};  // __syncthreads_and: threads whose predicate was false
+    int and_result = 0;
 };
 inline thread_local dim3 tl_tid, tl_bid;
 inline dim3 g_bdim, g_gdim;
@@ -65,6 +67,19 @@ inline int g_cus = 4;
 #define gridDim (::emu::g_gdim)
 
 inline void __syncthreads() { emu::tl_block->bar->arrive_and_wait(); }
+// barrier, then the AND of every thread's predicate: count the false ones,
+// read the count after a second barrier, reset it after a third
+inline int __syncthreads_and(int pred) {
+    emu::Block* b = emu::tl_block;
+    b->bar->arrive_and_wait();
+    if (!pred) b->and_count.fetch_add(1);
+    b->bar->arrive_and_wait();
+    const int r = b->and_count.load() == 0;
+    b->bar->arrive_and_wait();
+    if (threadIdx.x == 0) b->and_count.store(0);
+    b->bar->arrive_and_wait();
+    return r;
+}
 inline int __lane_id() { return (int)(threadIdx.x & 63u); }
 inline uint64_t __lanemask_lt() { return (1ull << __lane_id()) - 1ull; }
 inline int __popcll(uint64_t v) { return __builtin_popcountll(v); }
