@@ -84,6 +84,40 @@ D3_HD Filt orient_filter(const P3& a, const P3& b, const P3& c, const P3& d) {
     return f;
 }
 
+// The determinants alone, with exactly the filters' operations (so the same
+// bits): for queries whose T1 certificate is tried before the filter.
+D3_HD double orient_det(const P3& a, const P3& b, const P3& c, const P3& d) {
+    const double adx = a.x - d.x, bdx = b.x - d.x, cdx = c.x - d.x;
+    const double ady = a.y - d.y, bdy = b.y - d.y, cdy = c.y - d.y;
+    const double adz = a.z - d.z, bdz = b.z - d.z, cdz = c.z - d.z;
+    const double bdxcdy = bdx * cdy, cdxbdy = cdx * bdy;
+    const double cdxady = cdx * ady, adxcdy = adx * cdy;
+    const double adxbdy = adx * bdy, bdxady = bdx * ady;
+    return adz * (bdxcdy - cdxbdy) + bdz * (cdxady - adxcdy) + cdz * (adxbdy - bdxady);
+}
+
+D3_HD double insphere_det(const P3& a, const P3& b, const P3& c, const P3& d, const P3& e) {
+    const double aex = a.x - e.x, aey = a.y - e.y, aez = a.z - e.z;
+    const double bex = b.x - e.x, bey = b.y - e.y, bez = b.z - e.z;
+    const double cex = c.x - e.x, cey = c.y - e.y, cez = c.z - e.z;
+    const double dex = d.x - e.x, dey = d.y - e.y, dez = d.z - e.z;
+    const double ab = aex * bey - bex * aey;
+    const double bc = bex * cey - cex * bey;
+    const double cd = cex * dey - dex * cey;
+    const double da = dex * aey - aex * dey;
+    const double ac = aex * cey - cex * aey;
+    const double bd = bex * dey - dex * bey;
+    const double abc = aez * bc - bez * ac + cez * ab;
+    const double bcd = bez * cd - cez * bd + dez * bc;
+    const double cda = cez * da + dez * ac + aez * cd;
+    const double dab = dez * ab + aez * bd + bez * da;
+    const double alift = aex * aex + aey * aey + aez * aez;
+    const double blift = bex * bex + bey * bey + bez * bez;
+    const double clift = cex * cex + cey * cey + cez * cez;
+    const double dlift = dex * dex + dey * dey + dez * dez;
+    return (dlift * abc - clift * dab) + (blift * cda - alift * bcd);
+}
+
 // src/predicates.cpp:153-198
 D3_HD Filt insphere_filter(const P3& a, const P3& b, const P3& c, const P3& d, const P3& e) {
     const double aex = a.x - e.x, aey = a.y - e.y, aez = a.z - e.z;
@@ -213,6 +247,13 @@ D3_HD bool t1_certify(const P3* p, int np, double maxd, int k, int qmin_allowed)
     // maxd == 0: all points identical, every difference is exactly 0
     return (maxd == 0.0) | ok;
 }
+
+// Whether a set's summary allows the set-level certificate at all: it needs
+// some q with mexp - 51 <= q <= gmin. When it does, the fast kernels try the
+// certificate before the filter (a certified determinant's sign is the exact
+// sign, so the filter's error bound need not be formed); sets of arbitrary
+// doubles fail this once per task and keep the filter-first order.
+D3_HD bool set_certifiable(SetGrid g) { return g.mexp - g.gmin <= 51; }
 
 // ---- the perturbation walk (src/predicates.cpp:211-231) ---------------------
 

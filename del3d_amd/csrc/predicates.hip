@@ -83,15 +83,43 @@ struct LdsPts {
 // perturbation. Returns the public sign, or 2 when the exact tiers must decide.
 // kReload: fetch the points again for T1 instead of keeping them live across
 // the filter (cheap from LDS; it lowers register pressure on the common path).
+// cert_first (uniform per task, from set_certifiable): the set-level T1
+// certificate on the bare determinant first; a certified determinant is exact,
+// so its sign is the one the filter or the exact tiers would return.
 template <bool kReload, class Fetch>
 __device__ __forceinline__ int fast_decide(const Fetch& F, uint8_t o, const int32_t id[5], SetGrid sg,
-                                           int& tier, bool& perturbed) {
+                                           bool cert_first, int& tier, bool& perturbed) {
     const bool ins = o != DEL3D_OP_ORIENT3D;
     P3 p[5];
     p[0] = F(id[0]);
     p[1] = F(id[1]);
     p[2] = F(id[2]);
     p[3] = F(id[3]);
+    if (cert_first) {
+        if (!ins) {
+            const double md = max_diff(p, 4);
+            if (t1_certify_set(md, 15, -358, sg)) {
+                tier = kTierCert;
+                return -sgn(orient_det(p[0], p[1], p[2], p[3]));
+            }
+        } else {
+            p[4] = F(id[4]);
+            const double md = max_diff(p, 5);
+            if (t1_certify_set(md, 8, -214, sg)) {
+                tier = kTierCert;
+                int sign = -sgn(insphere_det(p[0], p[1], p[2], p[3], p[4]));
+                if (sign == 0 && o == DEL3D_OP_IN_SPHERE) {
+                    perturbed = true;
+                    sign = perturbation_walk(p, id, [&](int k) {
+                        P3 t[4];
+                        subst_tet(p, k, t);
+                        return -sgn(orient_det(t[0], t[1], t[2], t[3]));
+                    });
+                }
+                return sign;
+            }
+        }
+    }
     double det;
     if (!ins) {
         const Filt f = orient_filter(p[0], p[1], p[2], p[3]);
@@ -219,6 +247,7 @@ __device__ __forceinline__ void pred_batch_body(const BatchArgs& a, BatchLds<kSt
             sg.gmin = __builtin_amdgcn_readfirstlane(a.grid[2 * set]);
             sg.mexp = __builtin_amdgcn_readfirstlane(a.grid[2 * set + 1]);
         }
+        const bool cert_first = set_certifiable(sg);
         if (kLds && set != loaded_set) {
             // np <= kLdsPts is guaranteed by the host, which routes larger sets
             // to the global kernel
@@ -287,8 +316,8 @@ __device__ __forceinline__ void pred_batch_body(const BatchArgs& a, BatchLds<kSt
             int sign = 0, tier = kTierFilter;
             bool perturbed = false;
             if (act && valid) {
-                if (kLds) sign = fast_decide<true>(LdsPts{s_pts}, o, id, sg, tier, perturbed);
-                else sign = fast_decide<false>(GlobalPts{setpts}, o, id, sg, tier, perturbed);
+                if (kLds) sign = fast_decide<true>(LdsPts{s_pts}, o, id, sg, cert_first, tier, perturbed);
+                else sign = fast_decide<false>(GlobalPts{setpts}, o, id, sg, cert_first, tier, perturbed);
             }
             const bool queued = act && valid && sign == 2;
             if (act && !queued) a.out[q] = (int8_t)sign;
