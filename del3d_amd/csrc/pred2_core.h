@@ -324,13 +324,15 @@ D3_HD_CALL int in_circle_exact(const P3& p0, const P3& p1, const P3& p2, const P
 //   in-circle (degree 7 for the |v|^2 term): k = 5, a = 2^6 bounds the
 //     differences, and the terms sum to below 432 a^6 + 1152 a^7 < 2^52.2;
 //     7q >= -1074 needs q >= -153, and 53 + 7q < 1024 needs q <= 138 (e <= 140).
-D3_HD bool t1_certify_values(const double* v, int n, double maxd, int k, int qmin, int emax) {
+template <int N>
+D3_HD bool t1_certify_values(const double (&v)[N], double maxd, int k, int qmin, int emax) {
     const int e = exponent_of(maxd);
     const int q = e - k;
     const bool in_range = (q >= qmin) & (e <= emax);
     const double inv = pow2i(in_range ? -q : 0);
     bool ok = in_range;
-    for (int i = 0; i < n; ++i) ok = ok & on_grid(v[i], inv);
+#pragma unroll
+    for (int i = 0; i < N; ++i) ok = ok & on_grid(v[i], inv);
     return (maxd == 0.0) | ok;
 }
 
@@ -346,7 +348,7 @@ D3_HD int orient2d_fast(double px, double py, double qx, double qy, double rx, d
     if ((det > errbound) | (-det > errbound)) return sgn(det);
     const double maxd = fmax(fmax(fabs(qpx), fabs(qpy)), fmax(fabs(rpx), fabs(rpy)));
     const double v[6] = {px, py, qx, qy, rx, ry};
-    return t1_certify_values(v, 6, maxd, 25, -537, 180) ? sgn(det) : kPending2;
+    return t1_certify_values(v, maxd, 25, -537, 180) ? sgn(det) : kPending2;
 }
 
 // The in-circle determinant's sign without the exact stage, else kPending2.
@@ -357,7 +359,7 @@ D3_HD int in_circle_fast(const P3& p0, const P3& p1, const P3& p2, const P3& p) 
     const P3 pts[4] = {p0, p1, p2, p};
     const double maxd = max_diff(pts, 4);
     const double v[12] = {p0.x, p0.y, p0.z, p1.x, p1.y, p1.z, p2.x, p2.y, p2.z, p.x, p.y, p.z};
-    return t1_certify_values(v, 12, maxd, 5, -153, 140) ? sgn(det) : kPending2;
+    return t1_certify_values(v, maxd, 5, -153, 140) ? sgn(det) : kPending2;
 }
 
 // ---- the predicates -----------------------------------------------------------
@@ -425,13 +427,18 @@ D3_HD bool equal3(const P3& p, const P3& q) { return p.x == q.x && p.y == q.y &&
 D3_HD int collinear_position3(const P3& p, const P3& q, const P3& r) {
     if (equal3(q, p)) return 1;
     if (equal3(q, r)) return 3;
-    int k;
-    if (p.x != r.x) k = 0;
-    else if (p.y != r.y) k = 1;
-    else k = 2;
-    const double a = k == 0 ? p.x : k == 1 ? p.y : p.z;
-    const double b = k == 0 ? q.x : k == 1 ? q.y : q.z;
-    const double c = k == 0 ? r.x : k == 1 ? r.y : r.z;
+    // the first coordinate in which p and r differ (x, else y, else z),
+    // picked with bit masks: a select between the fields would let the
+    // compiler select their addresses instead, which keeps the points in
+    // scratch memory
+    const bool kx = p.x != r.x, ky = !kx && p.y != r.y;
+    const uint64_t mx = 0ull - (uint64_t)kx, my = 0ull - (uint64_t)ky, mz = ~(mx | my);
+    auto pick = [&](const P3& v) {
+        return __builtin_bit_cast(double, (__builtin_bit_cast(uint64_t, v.x) & mx) |
+                                              (__builtin_bit_cast(uint64_t, v.y) & my) |
+                                              (__builtin_bit_cast(uint64_t, v.z) & mz));
+    };
+    const double a = pick(p), b = pick(q), c = pick(r);
     if (a < c) {
         if (b < a) return 0;
         if (b > c) return 4;
@@ -447,21 +454,31 @@ D3_HD int collinear_position3(const P3& p, const P3& q, const P3& r) {
 template <class O>
 D3_HD int in_circle_perturb(const P3 pt[4], const int32_t id[4], const O& o2) {
     // stable lexicographic order of the four slots (std::sort on four
-    // elements is an insertion sort, so equal points keep argument order)
-    int ord[4] = {0, 1, 2, 3};
-    for (int i = 1; i < 4; ++i) {
-        const int v = ord[i];
-        int j = i;
-        while (j > 0 && lex_less(pt[v], pt[ord[j - 1]])) {
-            ord[j] = ord[j - 1];
-            --j;
+    // elements is an insertion sort, so equal points keep argument order):
+    // slot s ends at rank = points strictly below it + equal points before it.
+    // Ranks and the selects below use only constant indices, so the points
+    // stay in registers.
+    int rank[4];
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+        int r = 0;
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            if (t == s) continue;
+            const bool lt = lex_less(pt[t], pt[s]);
+            const bool eq = !lt && !lex_less(pt[s], pt[t]);
+            r += (lt || (eq && t < s)) ? 1 : 0;
         }
-        ord[j] = v;
+        rank[s] = r;
     }
     const int local = coplanar_orientation3(pt[0], pt[1], pt[2], o2);
     if (local == kPending2) return kPending2;
+#pragma unroll
     for (int i = 3; i > 0; --i) {
-        const int32_t v = id[ord[i]];
+        int32_t v = id[0];
+        v = rank[1] == i ? id[1] : v;
+        v = rank[2] == i ? id[2] : v;
+        v = rank[3] == i ? id[3] : v;
         if (v == id[3]) return -1;
         int o;
         if (v == id[2] && (o = coplanar_orientation3(pt[0], pt[1], pt[3], o2)) != 0)

@@ -30,17 +30,18 @@ def emu_exe():
     return EXE
 
 
-@pytest.mark.parametrize("defer,order", [("1", "2"), ("0", "1")])
-def test_emulated_sort_matches_oracle(emu_exe, defer, order):
+@pytest.mark.parametrize("defer,order,sizes", [("1", "2", ["2", "65", "1100"]),
+                                               ("0", "1", ["2", "65", "1100", "2100"])])
+def test_emulated_sort_matches_oracle(emu_exe, defer, order, sizes):
     env = dict(os.environ, EMU_DEFER=defer, EMU_BLOCK_ORDER=order, ASAN_OPTIONS="detect_leaks=0")
-    r = subprocess.run([str(emu_exe), "2", "65", "1100", "2100"],
-                       capture_output=True, text=True, timeout=900, env=env)
+    r = subprocess.run([str(emu_exe)] + sizes, capture_output=True, text=True, timeout=900, env=env)
     assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-4000:]
 
 
 def test_emulated_server_under_thread_sanitizer():
     """The serving loop (scheduler thread, one worker thread per slot, two
-    client threads) and the predicate pipeline under ThreadSanitizer."""
+    client threads, and the predicate pipeline it drives) under
+    ThreadSanitizer."""
     exe = Path("/tmp") / f"del3d_emu_tsan_{os.getuid()}"
     csrc = HERE.parent / "del3d_amd" / "csrc"
     subprocess.run(["python", str(HERE / "emu" / "build_emu.py"), str(exe),
@@ -48,7 +49,7 @@ def test_emulated_server_under_thread_sanitizer():
                     str(csrc / "circumcenter.hip"), str(csrc / "batch.hip"), str(csrc / "server.hip"), str(csrc / "spatial_sort.hip")],
                    check=True,
                    env=dict(os.environ, EMU_SANITIZER="thread"))
-    env = dict(os.environ, EMU_PRED_ONLY="1", TSAN_OPTIONS="halt_on_error=1")
+    env = dict(os.environ, EMU_SERVER_ONLY="1", TSAN_OPTIONS="halt_on_error=1")
     r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=900, env=env)
     assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-4000:]
     assert "server jobs" in r.stdout and "MISMATCH" not in r.stdout
