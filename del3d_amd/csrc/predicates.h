@@ -26,6 +26,13 @@ struct PredCounters {
     uint32_t next_task[2];  // task cursors of the global [0] and LDS [1] kernels
 };
 
+// A point stored as four floats (16 bytes, one aligned load): the compact
+// copy of a point set whose coordinates are all exactly representable in
+// float (set_f32_exact).
+struct alignas(16) F4 {
+    float x, y, z, w;
+};
+
 // Kernel arguments of one launch sequence (device pointers).
 struct BatchArgs {
     const double* xyz;       // all points
@@ -51,6 +58,9 @@ struct BatchArgs {
     PredCounters* ctr;
     unsigned long long* stats;  // 6 counters or nullptr
     const int32_t* grid;        // per set {gmin, mexp} (SetGrid), or nullptr
+    // single-set launches: a float copy of the points, valid when the set's
+    // summary says every coordinate is exact in float (else nullptr or unused)
+    const F4* pts32;
 };
 
 // How a batch is split between the two fast kernels.
