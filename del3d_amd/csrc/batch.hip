@@ -33,6 +33,7 @@ struct del3d_pred_plan_s {
     uint32_t* wide = nullptr;
     d3::PredCounters* ctr = nullptr;
     int32_t* grid = nullptr;  // per set SetGrid summary, recomputed by every run
+    d3::F4* pts32 = nullptr;  // float copy of the float-exact sets, rewritten by every run
     uint32_t queue_cap = 0;
     struct Chunk {
         uint32_t lds0, n_lds, glob0, n_glob;  // task ranges in d_tasks
@@ -135,6 +136,7 @@ void run_plan(del3d_pred_plan_s* p, const double* xyz, const uint8_t* op, const 
         g.pt_off = p->d_pt_off;
         g.ctr = p->ctr;
         enqueue_set_grids(g, (uint32_t)p->n_sets, p->n_pts, p->grid, s);
+        enqueue_points_f32(xyz, p->d_pt_off, (uint32_t)p->n_sets, p->n_pts, p->grid, p->pts32, s);
     }
     bool first_chunk = true;
     for (const auto& c : p->chunks) {
@@ -153,6 +155,7 @@ void run_plan(del3d_pred_plan_s* p, const double* xyz, const uint8_t* op, const 
         a.ctr = p->ctr;
         a.stats = stats;
         a.grid = p->grid;
+        a.pts32 = p->pts32;
         BatchShape sh;
         sh.lds_tasks = p->d_tasks + c.lds0;
         sh.n_lds_tasks = c.n_lds;
@@ -234,6 +237,7 @@ int del3d_pred_plan_create(const int64_t* pt_off, const int64_t* q_off, int64_t 
         D3_HIP(hipMalloc((void**)&p->wide, (size_t)p->queue_cap * 4));
         D3_HIP(hipMalloc((void**)&p->ctr, sizeof(PredCounters)));
         D3_HIP(hipMalloc((void**)&p->grid, (size_t)std::max<int64_t>(n_sets, 1) * 8));
+        D3_HIP(hipMalloc((void**)&p->pts32, (size_t)std::max<int64_t>(p->n_pts, 1) * sizeof(F4)));
         build_tasks(p, s);
         guard.p = nullptr;
         *plan_out = p;
@@ -378,6 +382,7 @@ int del3d_pred_plan_destroy(del3d_pred_plan p) {
     (void)hipFree(p->wide);
     (void)hipFree(p->ctr);
     (void)hipFree(p->grid);
+    (void)hipFree(p->pts32);
     delete p;
     return DEL3D_OK;
 }

@@ -58,8 +58,9 @@ struct BatchArgs {
     PredCounters* ctr;
     unsigned long long* stats;  // 6 counters or nullptr
     const int32_t* grid;        // per set {gmin, mexp} (SetGrid), or nullptr
-    // single-set launches: a float copy of the points, valid when the set's
-    // summary says every coordinate is exact in float (else nullptr or unused)
+    // a float copy of the points (indexed like xyz + pt_off), valid for the
+    // sets whose summary says every coordinate is exact in float; nullptr
+    // when not made (the per-set pointer layout)
     const F4* pts32;
 };
 
@@ -82,6 +83,10 @@ struct ScalarQuery {
 // Writes the SetGrid summary of every set of `a` into grid (2 int32 per set,
 // n_sets sets; one set when a.pt_off and a.set_ptr are null), on s; also
 // zeroes a.ctr when it is set.
+// Writes the float copy of the points of the sets that are exact in float
+// (per the summaries in grid), for the global kernel's gathers.
+void enqueue_points_f32(const double* xyz, const int64_t* pt_off, uint32_t n_sets, int64_t total,
+                        const int32_t* grid, F4* out, hipStream_t s);
 void enqueue_set_grids(const BatchArgs& a, uint32_t n_sets, int64_t total_pts, int32_t* grid,
                        hipStream_t s);
 

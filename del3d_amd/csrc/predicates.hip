@@ -259,6 +259,7 @@ __device__ __forceinline__ void pred_batch_body(const BatchArgs& a, BatchLds<kSt
         }
         const bool cert_first = set_certifiable(sg);
         const bool use32 = !kLds && a.pts32 != nullptr && set_f32_exact(sg);
+        const F4* set_pts32 = use32 ? a.pts32 + (a.pt_off ? a.pt_off[set] : 0) : nullptr;
         if (kLds && set != loaded_set) {
             // np <= kLdsPts is guaranteed by the host, which routes larger sets
             // to the global kernel
@@ -328,8 +329,8 @@ __device__ __forceinline__ void pred_batch_body(const BatchArgs& a, BatchLds<kSt
             bool perturbed = false;
             if (act && valid) {
                 if (kLds) sign = fast_decide<true>(LdsPts{s_pts}, o, id, sg, cert_first, tier, perturbed);
-                else sign = fast_decide<false>(GlobalPts{setpts, a.pts32, use32}, o, id, sg, cert_first, tier,
-                                               perturbed);
+                else sign = fast_decide<false>(GlobalPts{setpts, set_pts32, use32}, o, id, sg, cert_first,
+                                               tier, perturbed);
             }
             const bool queued = act && valid && sign == 2;
             if (act && !queued) a.out[q] = (int8_t)sign;
@@ -585,14 +586,38 @@ __global__ __launch_bounds__(256) void k_grid_points(BatchArgs a, uint32_t n_set
     }
 }
 
-// The float copy of a single set's points, written only if the set's
-// summary (computed just before, on the same stream) says it is exact.
-__global__ __launch_bounds__(256) void k_points_f32(const double* __restrict__ xyz, int64_t n,
-                                                    const int32_t* __restrict__ grid, F4* __restrict__ out) {
-    const SetGrid g{grid[0], grid[1]};
-    if (!set_f32_exact(g)) return;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-        out[i] = F4{(float)xyz[3 * i], (float)xyz[3 * i + 1], (float)xyz[3 * i + 2], 0.0f};
+// The float copy of the points of every set whose summary (computed just
+// before, on the same stream) says it is exact in float; other sets' entries
+// are left unwritten (the kernels do not read them). pt_off: n_sets+1 point
+// offsets, or nullptr for one set.
+__global__ __launch_bounds__(256) void k_points_f32(const double* __restrict__ xyz,
+                                                    const int64_t* __restrict__ pt_off, uint32_t n_sets,
+                                                    int64_t total, const int32_t* __restrict__ grid,
+                                                    F4* __restrict__ out) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        uint32_t set = 0;
+        if (pt_off) {  // last s with pt_off[s] <= i
+            uint32_t lo = 0, hi = n_sets;
+            while (hi - lo > 1) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if (pt_off[mid] <= i) lo = mid;
+                else hi = mid;
+            }
+            set = lo;
+        }
+        if (set_f32_exact(SetGrid{grid[2 * set], grid[2 * set + 1]}))
+            out[i] = F4{(float)xyz[3 * i], (float)xyz[3 * i + 1], (float)xyz[3 * i + 2], 0.0f};
+    }
+}
+
+void enqueue_points_f32(const double* xyz, const int64_t* pt_off, uint32_t n_sets, int64_t total,
+                        const int32_t* grid, F4* out, hipStream_t s) {
+    if (total <= 0) return;
+    const int cus = device_cu_count();
+    const unsigned g = (unsigned)std::min<int64_t>((total + 255) / 256, (int64_t)cus * 8);
+    k_points_f32<<<g, 256, 0, s>>>(xyz, pt_off, n_sets, total, grid, out);
+    D3_LAUNCH_CHECK();
 }
 
 void enqueue_set_grids(const BatchArgs& a, uint32_t n_sets, int64_t total_pts, int32_t* grid,
@@ -695,11 +720,7 @@ void launch_predicates(const double* xyz, int64_t n_pts, const uint8_t* op, cons
         if (!grids_done) {
             // the set's grid summary, and this chunk's counters, in one prepass
             enqueue_set_grids(a, 1, n_pts, grid, s);
-            if (pts32) {
-                const unsigned g = (unsigned)std::min<int64_t>((n_pts + 255) / 256, (int64_t)cus * 8);
-                k_points_f32<<<g, 256, 0, s>>>(xyz, n_pts, grid, pts32);
-                D3_LAUNCH_CHECK();
-            }
+            if (pts32) enqueue_points_f32(xyz, nullptr, 1, n_pts, grid, pts32, s);
             grids_done = true;
             sh.counters_cleared = true;
         }
