@@ -471,7 +471,7 @@ def main() -> int:
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: every box here has one GPU, so --gpus {args.gpus} "
-              "is not run (the multi-GPU design is described in DESIGN.md)", file=sys.stderr)
+              "is not run (predicate batches would shard by query across GPUs, with no exchange)", file=sys.stderr)
         return 2
 
     import del3d_amd as D
