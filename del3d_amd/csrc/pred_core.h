@@ -255,10 +255,11 @@ D3_HD bool t1_certify(const P3* p, int np, double maxd, int k, int qmin_allowed)
 // doubles fail this once per task and keep the filter-first order.
 D3_HD bool set_certifiable(SetGrid g) { return g.mexp - g.gmin <= 51; }
 
-// Whether every coordinate of a set is exactly a float: its nonzero values
-// span at most 24 significant bits (mexp - gmin <= 23) within float's range
-// (multiples of 2^-149, below 2^128). An all-zero set passes too.
-D3_HD bool set_f32_exact(SetGrid g) { return g.gmin >= -149 && g.mexp <= 127 && g.mexp - g.gmin <= 23; }
+// Whether every coordinate of a set is exactly a normal float (or zero): its
+// nonzero values span at most 24 significant bits (mexp - gmin <= 23), none
+// below 2^-126 (so no float subnormal, whatever the device's float denormal
+// mode) and none at 2^128 or above. An all-zero set passes too.
+D3_HD bool set_f32_exact(SetGrid g) { return g.gmin >= -126 && g.mexp <= 127 && g.mexp - g.gmin <= 23; }
 
 // ---- the perturbation walk (src/predicates.cpp:211-231) ---------------------
 

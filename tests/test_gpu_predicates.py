@@ -279,15 +279,19 @@ def test_sets_call_and_graph_replays():
     plan.close()
 
 
-@pytest.mark.parametrize("kind", ["float_exact_lattice", "wide_integers", "random_doubles"])
+@pytest.mark.parametrize("kind", ["float_exact_lattice", "lattice_2^-120", "lattice_2^-135",
+                                  "wide_integers", "random_doubles"])
 def test_global_kernel_point_formats(kind):
     """Single sets above the LDS threshold, so the global kernel gathers them:
-    a half-integer lattice (exact in float: gathered from the 16-byte float
-    copy), integers up to 2^30 (certifiable, not exact in float: doubles)
-    and random doubles (neither) — each against the oracle."""
+    a half-integer lattice and one scaled by 2^-120 (exact normal floats:
+    gathered from the 16-byte float copy), one scaled by 2^-135 (float
+    subnormals: doubles), integers up to 2^30 (certifiable, not exact in
+    float: doubles) and random doubles (neither) — each against the oracle."""
     rng = np.random.default_rng(77)
-    if kind == "float_exact_lattice":
+    if "lattice" in kind:
         pts = W.lattice(24, 24, 24) * np.array([0.5, 0.25, 1.0]) - 3.0
+        if kind != "float_exact_lattice":
+            pts = pts * 2.0 ** (-120 if "120" in kind else -135)
     elif kind == "wide_integers":
         pts = rng.integers(-(1 << 30), 1 << 30, size=(9000, 3)).astype(np.float64)
         pts[::3] = np.round(pts[::3] / 1024.0)  # some close neighbours as well
@@ -296,7 +300,7 @@ def test_global_kernel_point_formats(kind):
     n = 1 << 17
     op = rng.integers(0, 3, size=n).astype(np.uint8)
     idx = rng.integers(0, len(pts), size=(n, 5)).astype(np.int32)
-    if kind == "float_exact_lattice":  # local cells: many degenerate queries
+    if "lattice" in kind:  # local cells: many degenerate queries
         base = rng.integers(0, 22, size=(n, 3))
         off = rng.integers(0, 3, size=(n, 5, 3))
         ijk = base[:, None, :] + off
@@ -304,5 +308,5 @@ def test_global_kernel_point_formats(kind):
     say("global kernel", kind, len(pts), "points")
     got, st = predicates_dev(pts, op, idx, stats=True)
     assert np.array_equal(got, O.pred_batch(pts, op, idx, "oracle"))
-    if kind == "float_exact_lattice":
+    if "lattice" in kind:
         assert st["perturbed"] > 1000
