@@ -243,6 +243,8 @@ def coplanar_points(kind: str, rng, n: int = 400) -> np.ndarray:
     tilted       the same grid on the plane z = x + y
     ring         the 12 lattice points of x^2 + y^2 = 25, its centre and (9,9)
     ring_tilted  the ring mapped by an integer similarity onto a tilted plane
+    ring_offset  ring_tilted scaled by 1024 and translated by a 2^-10-grid vector
+                 (still exactly cocircular, too wide for the certified FP64 tier)
     dyadic_plane random 2^-20-grid points on z = 3x/4 - 5y/8 + 1/2 (exact)
     line         points p + t d on one line, t small integers (collinear input)
     scaled       2^-20-grid points on z = 3x/4 - 5y/8, half scaled by 2^-60, half
@@ -254,9 +256,14 @@ def coplanar_points(kind: str, rng, n: int = 400) -> np.ndarray:
         if kind == "tilted":
             g[:, 2] = g[:, 0] + g[:, 1]
         return g
-    if kind in ("ring", "ring_tilted"):
+    if kind in ("ring", "ring_tilted", "ring_offset"):
         r = np.array([(x, y, 0.0) for x, y in _RING25] + [(0, 0, 0.0), (9, 9, 0.0)])
-        return r @ _TILT.T if kind == "ring_tilted" else r
+        if kind == "ring":
+            return r
+        r = r @ _TILT.T
+        if kind == "ring_offset":  # scaled and moved off the small grid: still cocircular
+            r = r * 1024.0 + np.array([0.5, -0.25, 0.125]) + 3 * 2.0 ** -10
+        return r
     if kind in ("dyadic_plane", "scaled"):
         xy = rng.integers(-(1 << 20), 1 << 20, size=(n, 2)) * 2.0 ** -20
         z = 0.75 * xy[:, 0] - 0.625 * xy[:, 1] + (0.5 if kind == "dyadic_plane" else 0.0)
@@ -274,8 +281,8 @@ def coplanar_points(kind: str, rng, n: int = 400) -> np.ndarray:
     raise ValueError(kind)
 
 
-COPLANAR_KINDS = ("grid_z0", "tilted", "ring", "ring_tilted", "dyadic_plane", "line", "scaled",
-                  "random")
+COPLANAR_KINDS = ("grid_z0", "tilted", "ring", "ring_tilted", "ring_offset", "dyadic_plane", "line",
+                  "scaled", "random")
 
 
 def coplanar_queries(n_pts: int, n_q: int, rng, ops=tuple(range(8))):

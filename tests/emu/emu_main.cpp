@@ -211,6 +211,10 @@ static int check_coplanar() {
         for (int j = -3; j <= 3; ++j) xyz.insert(xyz.end(), {(double)i, (double)j, (double)(i + j)});
     for (int i = 0; i < 40; ++i)
         for (int k = 0; k < 3; ++k) xyz.push_back((double)(rnd() >> 11) * 0x1p-53);
+    // a wide exponent span (more limbs than the lane-parallel exact pass takes)
+    for (int i = 0; i < 20; ++i)
+        for (int k = 0; k < 3; ++k)
+            xyz.push_back(std::ldexp((double)(rnd() >> 11) * 0x1p-53, (int)(rnd() % 200) - 100));
     const int64_t n = (int64_t)xyz.size() / 3, nq = 30000;
     std::vector<uint8_t> op(nq);
     std::vector<int32_t> idx(5 * nq, 0);
@@ -218,8 +222,8 @@ static int check_coplanar() {
         op[q] = (uint8_t)(rnd() % 9);
         // mostly within one group (ring, grid or random) so the queries meet
         // the predicates' preconditions; some across groups
-        const int64_t g = rnd() % 4, lo = g == 0 ? 0 : g == 1 ? 12 : g == 2 ? 61 : 0;
-        const int64_t w = g == 0 ? 12 : g == 1 ? 49 : g == 2 ? 40 : n;
+        const int64_t g = rnd() % 5, lo = g == 0 ? 0 : g == 1 ? 12 : g == 2 ? 61 : g == 3 ? 101 : 0;
+        const int64_t w = g == 0 ? 12 : g == 1 ? 49 : g == 2 ? 40 : g == 3 ? 20 : n;
         for (int k = 0; k < 4; ++k) idx[5 * q + k] = (int32_t)(lo + rnd() % w);
         if (q % 9 == 0) idx[5 * q + 3] = idx[5 * q + (rnd() % 3)];
     }
