@@ -22,9 +22,10 @@ points, 2^28 gathered queries), the batched one (4096 point sets, one
 autotuned plan replayed as a graph, against one call per set) and the served
 one (a Poisson job stream through the serving loop, p50/p99 latency, against
 plain per-request calls), the coplanar predicates (2^24 queries on a
-tilted planar grid) and the circumcentres (2^22 cells, lazy and exact
-forms), each with its own bit-exactness checks; --config runs one of them
-alone.
+tilted planar grid), the circumcentres (2^22 cells, lazy and exact
+forms) and the replay of the reference driver's predicate calls on 10,000
+rand48 points, each with its own bit-exactness checks; --config runs one of
+them alone.
 """
 from __future__ import annotations
 
@@ -453,6 +454,48 @@ def run_circumcenters(args) -> dict:
     return res
 
 
+def run_c1_trace(args) -> dict:
+    """The 10,000-point rand48 build of the reference's Delaunay driver,
+    replayed: every exact predicate call it makes (recorded from the
+    reference's own objects by oracle/_ref/trace_recorder), one mixed batch
+    per step, against the recorded results; the reference's time for the same
+    calls, one core, beside it."""
+    import subprocess
+    import tempfile
+    import del3d_amd as D
+    from del3d_amd import trace as T
+    sys.path.insert(0, str(ROOT / "tests"))
+    import oracle_lib as O
+    if not O.TRACE_RECORDER.exists():
+        return {"config": "c1_trace", "skipped": "trace recorder not built"}
+    with tempfile.TemporaryDirectory() as td:
+        path = Path(td) / "c1.trace"
+        subprocess.run([str(O.TRACE_RECORDER), "10000", "1", str(path)], check=True, timeout=300,
+                       capture_output=True)
+        pts, op, res, idx = T.read(path)
+    stream = D.Stream()
+    xyz = D.DeviceArray.from_host(np.ascontiguousarray(pts))
+    dop = D.DeviceArray.from_host(np.ascontiguousarray(op))
+    didx = D.DeviceArray.from_host(np.ascontiguousarray(idx))
+    out = D.DeviceArray((len(op),), np.int8)
+    for _ in range(args.warmup + 1):
+        D.predicates_mixed(xyz, dop, didx, out, stream)
+    ms = _events_ms(D, stream, lambda: D.predicates_mixed(xyz, dop, didx, out, stream), args.steps)
+    got = out.to_host()
+    line = {"config": "c1_trace", "points": len(pts), "records": len(op),
+            "gpred_per_s": round(len(op) / (ms / args.steps / 1e3) / 1e9, 3),
+            "ms_per_step": round(ms / args.steps, 4),
+            "bit_exact_vs_recorded": bool(np.array_equal(got, res))}
+    if O.have_ref():
+        m3 = op < 16
+        i3 = idx[m3].copy()
+        i3[op[m3] == 0, 4] = i3[op[m3] == 0, 3]
+        t0 = time.perf_counter()
+        O.pred_batch(pts, op[m3], i3, "ref")
+        line["cpu_reference_s_3d_calls"] = round(time.perf_counter() - t0, 4)
+    return line
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -464,7 +507,8 @@ def main() -> int:
                     help="only the degenerate-lattice line, without the other configurations")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--config", default="lattice",
-                    choices=["lattice", "large", "batched", "served", "coplanar", "circumcenters"],
+                    choices=["lattice", "large", "batched", "served", "coplanar", "circumcenters",
+                             "c1_trace"],
                     help="lattice: the headline line; large / batched: the other configurations")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="only run the calibration copy kernel (for counter scaling)")
@@ -493,7 +537,8 @@ def main() -> int:
 
     if args.config != "lattice":
         res = {"large": run_large, "batched": run_batched, "served": run_served,
-               "coplanar": run_coplanar, "circumcenters": run_circumcenters}[args.config](args)
+               "coplanar": run_coplanar, "circumcenters": run_circumcenters,
+               "c1_trace": run_c1_trace}[args.config](args)
         print(json.dumps(res), flush=True)
         return 0
 
@@ -584,7 +629,8 @@ def main() -> int:
         sub = argparse.Namespace(steps=max(3, args.steps // 5), warmup=1)
         others = {}
         for name, fn in (("large", run_large), ("batched", run_batched), ("served", run_served),
-                         ("coplanar", run_coplanar), ("circumcenters", run_circumcenters)):
+                         ("coplanar", run_coplanar), ("circumcenters", run_circumcenters),
+                         ("c1_trace", run_c1_trace)):
             t0 = time.perf_counter()
             try:
                 others[name] = fn(sub)
