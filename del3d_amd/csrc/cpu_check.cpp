@@ -100,6 +100,56 @@ extern "C" void predcore_host_t1(const double* pts, int np, int32_t gmin, int32_
     out[1] = t1_certify_set(md, k, qmin, SetGrid{gmin, mexp}) ? 1 : 0;
 }
 
+// The packed small-grid format of pred_core.h on a set with summary
+// (gmin, mexp): returns whether the set qualifies; if it does, words[i] is
+// point i packed and back[3i..3i+2] its unpacked coordinates.
+extern "C" int predcore_pack_points(const double* xyz, int64_t n, int32_t gmin, int32_t mexp,
+                                    uint32_t* words, double* back) {
+    const SetGrid g{gmin, mexp};
+    if (!set_packable(g)) return 0;
+    const double inv = pow2i(-gmin), scale = pow2i(gmin);
+    for (int64_t i = 0; i < n; ++i) {
+        words[i] = pack_point(xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2], inv);
+        const P3 p = unpack_point(words[i], scale);
+        back[3 * i] = p.x;
+        back[3 * i + 1] = p.y;
+        back[3 * i + 2] = p.z;
+    }
+    return 1;
+}
+
+// The packed kernel's per-query decision on a packable set with summary
+// (gmin, mexp): packed_decide on the packed words, the exact tiers on the
+// set's doubles where it returns kPackedGeneric (the kernel queues those
+// queries for the exact kernel). Returns 0 (nothing written)
+// when the set is not packable; else out[q] = sign, used[q] = 1 where the
+// grid-unit decision decided.
+extern "C" int predcore_packed_batch(const double* xyz, int64_t n_pts, int32_t gmin, int32_t mexp,
+                                     const uint8_t* op, const int32_t* idx, int64_t n_q, int8_t* out,
+                                     int8_t* used) {
+    if (!set_packable(SetGrid{gmin, mexp})) return 0;
+    const double inv = pow2i(-gmin);
+    std::vector<uint32_t> words((size_t)n_pts);
+    for (int64_t i = 0; i < n_pts; ++i) words[i] = pack_point(xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2], inv);
+    for (int64_t q = 0; q < n_q; ++q) {
+        const int32_t* id = idx + 5 * q;
+        const int np = op[q] == DEL3D_OP_ORIENT3D ? 4 : 5;
+        uint32_t w[5] = {0, 0, 0, 0, 0};
+        for (int k = 0; k < np; ++k) w[k] = words[id[k]];
+        int tier = -1;
+        bool perturbed = false;
+        int s = packed_decide(w, op[q], id, tier, perturbed);
+        used[q] = s != kPackedGeneric;
+        if (s == kPackedGeneric) {
+            P3 p[5];
+            for (int k = 0; k < 5; ++k) p[k] = k < np ? load_p3(xyz, id[k]) : P3{0.0, 0.0, 0.0};
+            s = eval_one(op[q], p, id, 1, &tier);
+        }
+        out[q] = (int8_t)s;
+    }
+    return 1;
+}
+
 // Host build of the coplanar predicates (pred2_core.h). mode 0: the full
 // evaluation (filters, exact tiers, perturbation); 1: the fast kernel's
 // result (9 where it leaves the query to the exact kernel);

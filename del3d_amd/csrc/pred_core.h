@@ -35,6 +35,8 @@
 #include <cmath>
 #include <cstdint>
 
+#include "../../include/del3d_amd.h"
+
 namespace d3 {
 
 #define D3_HD __host__ __device__ inline
@@ -272,7 +274,9 @@ D3_HD bool lex_less(const P3& p, const P3& q) {
 // Slots 0..4 are (a, b, c, d, e). The walk needs the slots at sorted ranks 4
 // and 3 of the stable lexicographic order; with ties broken by slot, the rank
 // of slot s is the number of slots that come before it.
-D3_HD void top_two_slots(const P3 p[5], int& s4, int& s3) {
+// less(t, s): slot t's point is lexicographically below slot s's.
+template <class Less>
+D3_HD void top_two_slots_by(Less&& less, int& s4, int& s3) {
     s4 = 0;
     s3 = 0;
 #pragma unroll
@@ -281,7 +285,7 @@ D3_HD void top_two_slots(const P3 p[5], int& s4, int& s3) {
 #pragma unroll
         for (int t = 0; t < 5; ++t) {
             if (t == s) continue;
-            const bool before = lex_less(p[t], p[s]) || (t < s && !lex_less(p[s], p[t]));
+            const bool before = less(t, s) || (t < s && !less(s, t));
             r += before ? 1 : 0;
         }
         if (r == 4) s4 = s;
@@ -289,13 +293,15 @@ D3_HD void top_two_slots(const P3 p[5], int& s4, int& s3) {
     }
 }
 
+D3_HD void top_two_slots(const P3 p[5], int& s4, int& s3) {
+    top_two_slots_by([&](int t, int s) { return lex_less(p[t], p[s]); }, s4, s3);
+}
+
 // `orient_subst(k)` returns the public orient3d sign of the tetrahedron
 // (a, b, c, d) with slot k (0..3) replaced by e; it is only called for the
 // substitutions the walk reaches.
 template <class F>
-D3_HD int perturbation_walk(const P3 p[5], const int32_t id[5], F&& orient_subst) {
-    int s4, s3;
-    top_two_slots(p, s4, s3);
+D3_HD int perturbation_walk_from(int s4, int s3, const int32_t id[5], F&& orient_subst) {
 #pragma unroll
     for (int it = 0; it < 2; ++it) {
         // a select chain rather than id[slot]: a runtime index would put the
@@ -319,12 +325,138 @@ D3_HD int perturbation_walk(const P3 p[5], const int32_t id[5], F&& orient_subst
     return -1;
 }
 
+template <class F>
+D3_HD int perturbation_walk(const P3 p[5], const int32_t id[5], F&& orient_subst) {
+    int s4, s3;
+    top_two_slots(p, s4, s3);
+    return perturbation_walk_from(s4, s3, id, orient_subst);
+}
+
 D3_HD void subst_tet(const P3 p[5], int k, P3 t[4]) {
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
         const bool sub = i == k;
         t[i] = P3{sub ? p[4].x : p[i].x, sub ? p[4].y : p[i].y, sub ? p[4].z : p[i].z};
     }
+}
+
+// ---- packed small-grid sets ---------------------------------------------------
+// A set whose nonzero coordinates are all c = k * 2^gmin with |k| <= 511 (its
+// values span at most 9 significant bits: mexp - gmin <= 8), with
+// -206 <= gmin <= 171, is stored as one 32-bit word per point: three 10-bit
+// fields k + 512, x in bits 20-29, y in 10-19, z in 0-9, so that comparing
+// words is comparing points lexicographically. -0.0 is stored as 0.0: no
+// fast tier depends on the sign of a zero coordinate (a zero operand adds or
+// multiplies to the same value either way, only the sign of a zero result can
+// differ, and every test on such a result, the filter's comparisons, the
+// grid test, sgn() and the lexicographic order, treats -0.0 as 0.0).
+// An all-zero set does not qualify.
+D3_HD bool set_packable(SetGrid g) {
+    return g.gmin <= g.mexp && g.gmin >= -206 && g.gmin <= 171 && g.mexp - g.gmin <= 8;
+}
+
+// inv_scale = 2^-gmin
+D3_HD uint32_t pack_point(double x, double y, double z, double inv_scale) {
+    const uint32_t fx = (uint32_t)((int32_t)(x * inv_scale) + 512);
+    const uint32_t fy = (uint32_t)((int32_t)(y * inv_scale) + 512);
+    const uint32_t fz = (uint32_t)((int32_t)(z * inv_scale) + 512);
+    return (fx << 20) | (fy << 10) | fz;
+}
+// scale = 2^gmin: the set's doubles (zeros as +0.0)
+D3_HD P3 unpack_point(uint32_t w, double scale) {
+    return P3{(double)((int32_t)((w >> 20) & 1023u) - 512) * scale,
+              (double)((int32_t)((w >> 10) & 1023u) - 512) * scale,
+              (double)((int32_t)(w & 1023u) - 512) * scale};
+}
+
+// The difference a - b of two packed points in grid units (exact doubles).
+struct DV {
+    double x, y, z;
+};
+D3_HD double pfield_diff(uint32_t a, uint32_t b, int shift) {
+    return (double)((int32_t)((a >> shift) & 1023u) - (int32_t)((b >> shift) & 1023u));
+}
+D3_HD DV pdiff(uint32_t a, uint32_t b) { return DV{pfield_diff(a, b, 20), pfield_diff(a, b, 10), pfield_diff(a, b, 0)}; }
+D3_HD DV dv_sub(const DV& a, const DV& b) { return DV{a.x - b.x, a.y - b.y, a.z - b.z}; }
+
+// orient_det and insphere_det on the differences to the last point.
+D3_HD double orient_det_dv(const DV& ad, const DV& bd, const DV& cd) {
+    const double bdxcdy = bd.x * cd.y, cdxbdy = cd.x * bd.y;
+    const double cdxady = cd.x * ad.y, adxcdy = ad.x * cd.y;
+    const double adxbdy = ad.x * bd.y, bdxady = bd.x * ad.y;
+    return ad.z * (bdxcdy - cdxbdy) + bd.z * (cdxady - adxcdy) + cd.z * (adxbdy - bdxady);
+}
+D3_HD double insphere_det_dv(const DV& ae, const DV& be, const DV& ce, const DV& de) {
+    const double ab = ae.x * be.y - be.x * ae.y;
+    const double bc = be.x * ce.y - ce.x * be.y;
+    const double cd = ce.x * de.y - de.x * ce.y;
+    const double da = de.x * ae.y - ae.x * de.y;
+    const double ac = ae.x * ce.y - ce.x * ae.y;
+    const double bd = be.x * de.y - de.x * be.y;
+    const double abc = ae.z * bc - be.z * ac + ce.z * ab;
+    const double bcd = be.z * cd - ce.z * bd + de.z * bc;
+    const double cda = ce.z * da + de.z * ac + ae.z * cd;
+    const double dab = de.z * ab + ae.z * bd + be.z * da;
+    const double alift = ae.x * ae.x + ae.y * ae.y + ae.z * ae.z;
+    const double blift = be.x * be.x + be.y * be.y + be.z * be.z;
+    const double clift = ce.x * ce.x + ce.y * ce.y + ce.z * ce.z;
+    const double dlift = de.x * de.x + de.y * de.y + de.z * de.z;
+    return (dlift * abc - clift * dab) + (blift * cda - alift * bcd);
+}
+
+constexpr int kPackedGeneric = 3;  // packed_decide: not decided on the grid
+
+// The fast tiers of one query of a packed set, on the integer grid: the
+// certified branch of the set-level certificate, which is what the generic
+// path takes for these sets. With |k| <= 511 and -206 <= gmin <= 171, the
+// orient3d certificate (k = 15) holds for every query, and the in_sphere one
+// (k = 8) exactly when every difference to e is below 512 grid units; a
+// certified determinant is exact, and on the grid it is the set's
+// determinant divided by a positive power of two, so its sign is the same,
+// as is every substituted orient3d of the perturbation and the points'
+// lexicographic order. w: the query's packed points (w[4] for in_sphere).
+// Returns kPackedGeneric for an in_sphere query whose certificate fails.
+D3_HD int packed_decide(const uint32_t w[5], uint8_t o, const int32_t id[5], int& tier, bool& perturbed) {
+    if (o == DEL3D_OP_ORIENT3D) {
+        tier = kTierCert;
+        return -sgn(orient_det_dv(pdiff(w[0], w[3]), pdiff(w[1], w[3]), pdiff(w[2], w[3])));
+    }
+    const DV d0 = pdiff(w[0], w[4]), d1 = pdiff(w[1], w[4]), d2 = pdiff(w[2], w[4]), d3 = pdiff(w[3], w[4]);
+    double md = fmax(fmax(fabs(d0.x), fabs(d0.y)), fabs(d0.z));
+    md = fmax(md, fmax(fmax(fabs(d1.x), fabs(d1.y)), fabs(d1.z)));
+    md = fmax(md, fmax(fmax(fabs(d2.x), fabs(d2.y)), fabs(d2.z)));
+    md = fmax(md, fmax(fmax(fabs(d3.x), fabs(d3.y)), fabs(d3.z)));
+    if (!(md < 512.0)) return kPackedGeneric;
+    tier = kTierCert;
+    int sign = -sgn(insphere_det_dv(d0, d1, d2, d3));
+    if (sign == 0 && o == DEL3D_OP_IN_SPHERE) {
+        perturbed = true;
+        // the slots of ranks 4 and 3 in the stable order (ties by slot): the
+        // two largest of (w[s], s), kept while scanning s upwards
+        int s4 = 0, s3 = 0;
+        uint32_t w4 = w[0], w3 = 0;
+#pragma unroll
+        for (int s = 1; s < 5; ++s) {
+            const bool top = w[s] >= w4;
+            const bool second = !top && (s == 1 || w[s] >= w3);
+            s3 = top ? s4 : second ? s : s3;
+            w3 = top ? w4 : second ? w[s] : w3;
+            s4 = top ? s : s4;
+            w4 = top ? w[s] : w4;
+        }
+        sign = perturbation_walk_from(s4, s3, id, [&](int k) {
+            // slot k replaced by e; differences to the fourth point, which is
+            // e for k = 3 and d otherwise: x - d = (x - e) - (d - e)
+            const bool kd = k != 3;
+            const DV base{kd ? d3.x : 0.0, kd ? d3.y : 0.0, kd ? d3.z : 0.0};
+            const DV nd{-d3.x, -d3.y, -d3.z};
+            const DV a = k == 0 ? nd : dv_sub(d0, base);
+            const DV b = k == 1 ? nd : dv_sub(d1, base);
+            const DV c = k == 2 ? nd : dv_sub(d2, base);
+            return -sgn(orient_det_dv(a, b, c));
+        });
+    }
+    return sign;
 }
 
 // ---- exact integer arithmetic: two's complement, 32-bit limbs --------------

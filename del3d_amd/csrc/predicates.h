@@ -11,6 +11,7 @@ namespace d3 {
 constexpr int kLdsThreads = 1024;   // workgroup of the LDS kernel (one per CU)
 constexpr int kGlobalThreads = 256; // workgroup of the global-gather kernel
 constexpr int kLdsPts = 5632;       // largest point set held in LDS (132 KB)
+constexpr int kPackPts = 33792;     // largest packed point set held in LDS (132 KB, 4 bytes a point)
 constexpr uint32_t kChunk = 1u << 26;  // queries per launch sequence (bounds the queues)
 constexpr int kStats = 6;           // decision counters (see del3d_predicates_stats)
 
@@ -62,6 +63,11 @@ struct BatchArgs {
     // sets whose summary says every coordinate is exact in float; nullptr
     // when not made (the per-set pointer layout)
     const F4* pts32;
+    // the packed copy of a single set (set_packable), and whether the call
+    // runs both the packed LDS kernel and the global one, each of which
+    // returns at once when the set's summary routes it to the other
+    const uint32_t* pk;
+    int32_t pack_route;
 };
 
 // How a batch is split between the two fast kernels.
@@ -71,6 +77,7 @@ struct BatchShape {
     const PredTask* global_tasks = nullptr;
     uint32_t n_global_tasks = 0, global_chunk = 0, global_blocks_per_cu = 8;
     bool counters_cleared = false;  // the grid prepass already zeroed the counters
+    bool packed = false;            // the LDS tasks run the packed-point kernel
 };
 
 struct ScalarQuery {
@@ -84,9 +91,10 @@ struct ScalarQuery {
 // n_sets sets; one set when a.pt_off and a.set_ptr are null), on s; also
 // zeroes a.ctr when it is set.
 // Writes the float copy of the points of the sets that are exact in float
-// (per the summaries in grid), for the global kernel's gathers.
+// (per the summaries in grid), for the global kernel's gathers; with pk (one
+// set only), the packed copy instead when the set is packable.
 void enqueue_points_f32(const double* xyz, const int64_t* pt_off, uint32_t n_sets, int64_t total,
-                        const int32_t* grid, F4* out, hipStream_t s);
+                        const int32_t* grid, F4* out, hipStream_t s, uint32_t* pk = nullptr);
 void enqueue_set_grids(const BatchArgs& a, uint32_t n_sets, int64_t total_pts, int32_t* grid,
                        hipStream_t s);
 
@@ -96,6 +104,9 @@ void enqueue_batch(const BatchArgs& a, const BatchShape& sh, hipStream_t s, hipE
                    hipEvent_t* ev2 = nullptr);
 
 // One point set; force_path 0 = choose, 1 = LDS kernel, 2 = global kernel.
+// The choice: the LDS kernel up to kLdsPts points; then up to kPackPts the
+// packed LDS kernel beside the global one (the set's summary runs one of
+// them); the global kernel above.
 // ev: 6 events (fast, exact, wide brackets) or nullptr; ev2: 2 events around
 // the fast kernels only, or nullptr.
 void launch_predicates(const double* xyz, int64_t n_pts, const uint8_t* op, const int32_t* idx,

@@ -89,6 +89,7 @@ struct LdsPts {
     __device__ P3 operator()(int32_t i) const { return P3{p[3 * i], p[3 * i + 1], p[3 * i + 2]}; }
 };
 
+
 // The fast tiers of one query: T0 filter, T1 certificate, in-register
 // perturbation. Returns the public sign, or 2 when the exact tiers must decide.
 // kReload: fetch the points again for T1 instead of keeping them live across
@@ -194,22 +195,29 @@ __device__ inline PredTask task_at(const BatchArgs& a, uint32_t t) {
     return r;
 }
 
-// Body of both fast kernels. kLds: the workgroup loads each task's point
-// set into LDS first. Workgroups take tasks dynamically (one atomic per task).
-template <bool kStats, bool kLds>
+// Body of the fast kernels. kMode: where the points come from (kGlobal:
+// gathers from global memory; kLdsF64: the workgroup loads each task's point
+// set into LDS first; kLdsPacked: the same with the set's packed copy).
+// Workgroups take tasks dynamically (one atomic per task).
+enum { kGlobal = 0, kLdsF64 = 1, kLdsPacked = 2 };
+
+template <bool kStats, int kMode>
 struct BatchLds {
+    static constexpr bool kLds = kMode != kGlobal;
     static constexpr int kT = kLds ? kLdsThreads : kGlobalThreads;  // queries per tile
     int32_t idx[kT * 5];
     uint8_t op[kT];
     uint16_t perm[kT];
     uint32_t wave_ins[kT / 64];
     int64_t task[4];
-    double pts[kLds ? 3 * kLdsPts : 1];
+    double pts[kMode == kLdsF64 ? 3 * kLdsPts : 1];
+    uint32_t pk[kMode == kLdsPacked ? kPackPts : 1];
 };
 
-template <bool kStats, bool kLds>
-__device__ __forceinline__ void pred_batch_body(const BatchArgs& a, BatchLds<kStats, kLds>& L) {
-    constexpr int kT = BatchLds<kStats, kLds>::kT;
+template <bool kStats, int kMode>
+__device__ __forceinline__ void pred_batch_body(const BatchArgs& a, BatchLds<kStats, kMode>& L) {
+    constexpr bool kLds = kMode != kGlobal;
+    constexpr int kT = BatchLds<kStats, kMode>::kT;
     constexpr int kW = kT / 64;
     int32_t* s_idx = L.idx;
     uint8_t* s_op = L.op;
@@ -217,6 +225,14 @@ __device__ __forceinline__ void pred_batch_body(const BatchArgs& a, BatchLds<kSt
     uint32_t* s_wave_ins = L.wave_ins;
     int64_t* s_task = L.task;
     double* s_pts = L.pts;
+    uint32_t* s_pk = L.pk;
+
+    // a single set run by both the packed kernel and the global one: the set's
+    // summary picks one, the other returns at once (a.grid holds the summary)
+    if (kMode != kLdsF64 && a.pack_route) {
+        const bool packable = set_packable(SetGrid{a.grid[0], a.grid[1]});
+        if (packable != (kMode == kLdsPacked)) return;
+    }
 
     const int tid = threadIdx.x;
     const int wave = tid >> 6;
@@ -248,8 +264,11 @@ __device__ __forceinline__ void pred_batch_body(const BatchArgs& a, BatchLds<kSt
         if (q0 < 0) break;
         const int64_t q1 = uniform64(s_task[1]);
         const int32_t set = (int32_t)uniform64(s_task[2]);
-        // points addressable by this task's indices (the LDS copy holds at most kLdsPts)
-        const int64_t np = kLds ? min(uniform64(s_task[3]), (int64_t)kLdsPts) : uniform64(s_task[3]);
+        // points addressable by this task's indices (the LDS copy holds at most
+        // kLdsPts, the packed one kPackPts)
+        const int64_t np = kMode == kLdsF64      ? min(uniform64(s_task[3]), (int64_t)kLdsPts)
+                           : kMode == kLdsPacked ? min(uniform64(s_task[3]), (int64_t)kPackPts)
+                                                 : uniform64(s_task[3]);
         const double* setpts = set_points(a, set);
         // no summary: an impossible grid, so only the per-coordinate test runs
         SetGrid sg{-100000, 100000};
@@ -260,11 +279,17 @@ __device__ __forceinline__ void pred_batch_body(const BatchArgs& a, BatchLds<kSt
         const bool cert_first = set_certifiable(sg);
         const bool use32 = !kLds && a.pts32 != nullptr && set_f32_exact(sg);
         const F4* set_pts32 = use32 ? a.pts32 + (a.pt_off ? a.pt_off[set] : 0) : nullptr;
-        if (kLds && set != loaded_set) {
+        if (kMode == kLdsF64 && set != loaded_set) {
             // np <= kLdsPts is guaranteed by the host, which routes larger sets
             // to the global kernel
             const int n3 = 3 * (int)np;
             for (int i = tid; i < n3; i += kT) s_pts[i] = setpts[i];
+            loaded_set = set;
+        }
+        // the packed kernel runs one set of at most kPackPts points, which the
+        // host and the pack_route test above guarantee
+        if (kMode == kLdsPacked && set != loaded_set) {
+            for (int i = tid; i < (int)np; i += kT) s_pk[i] = a.pk[i];
             loaded_set = set;
         }
 
@@ -328,7 +353,19 @@ __device__ __forceinline__ void pred_batch_body(const BatchArgs& a, BatchLds<kSt
             int sign = 0, tier = kTierFilter;
             bool perturbed = false;
             if (act && valid) {
-                if (kLds) sign = fast_decide<true>(LdsPts{s_pts}, o, id, sg, cert_first, tier, perturbed);
+                if (kMode == kLdsF64)
+                    sign = fast_decide<true>(LdsPts{s_pts}, o, id, sg, cert_first, tier, perturbed);
+                else if (kMode == kLdsPacked) {
+                    uint32_t w[5];
+#pragma unroll
+                    for (int k = 0; k < 5; ++k) w[k] = k < nchk ? s_pk[id[k]] : 0u;
+                    // an in_sphere query whose grid certificate fails goes to
+                    // the exact kernel, whose sign is the one the filter or the
+                    // exact tiers would give (these sets neither underflow nor
+                    // overflow FP64, so a deciding filter's sign is exact)
+                    sign = packed_decide(w, o, id, tier, perturbed);
+                    if (sign == kPackedGeneric) sign = 2;
+                }
                 else sign = fast_decide<false>(GlobalPts{setpts, set_pts32, use32}, o, id, sg, cert_first,
                                                tier, perturbed);
             }
@@ -350,15 +387,21 @@ __device__ __forceinline__ void pred_batch_body(const BatchArgs& a, BatchLds<kSt
 // from its 1024-thread workgroup.
 template <bool kStats>
 __global__ __launch_bounds__(kLdsThreads) void pred_batch_lds(BatchArgs a) {
-    __shared__ BatchLds<kStats, true> L;
-    pred_batch_body<kStats, true>(a, L);
+    __shared__ BatchLds<kStats, kLdsF64> L;
+    pred_batch_body<kStats, kLdsF64>(a, L);
+}
+
+template <bool kStats>
+__global__ __launch_bounds__(kLdsThreads) void pred_batch_packed(BatchArgs a) {
+    __shared__ BatchLds<kStats, kLdsPacked> L;
+    pred_batch_body<kStats, kLdsPacked>(a, L);
 }
 
 template <bool kStats>
 __global__ __launch_bounds__(kGlobalThreads) __attribute__((amdgpu_waves_per_eu(4)))
 void pred_batch_global(BatchArgs a) {
-    __shared__ BatchLds<kStats, false> L;
-    pred_batch_body<kStats, false>(a, L);
+    __shared__ BatchLds<kStats, kGlobal> L;
+    pred_batch_body<kStats, kGlobal>(a, L);
 }
 
 // ---------------------------------------------------------------------------
@@ -589,11 +632,22 @@ __global__ __launch_bounds__(256) void k_grid_points(BatchArgs a, uint32_t n_set
 // The float copy of the points of every set whose summary (computed just
 // before, on the same stream) says it is exact in float; other sets' entries
 // are left unwritten (the kernels do not read them). pt_off: n_sets+1 point
-// offsets, or nullptr for one set.
+// offsets, or nullptr for one set. pk (one set only, or nullptr): when the
+// set is packable, its packed copy is written instead of the float copy.
 __global__ __launch_bounds__(256) void k_points_f32(const double* __restrict__ xyz,
                                                     const int64_t* __restrict__ pt_off, uint32_t n_sets,
                                                     int64_t total, const int32_t* __restrict__ grid,
-                                                    F4* __restrict__ out) {
+                                                    F4* __restrict__ out, uint32_t* __restrict__ pk) {
+    if (pk) {
+        const SetGrid g{grid[0], grid[1]};
+        if (set_packable(g)) {
+            const double inv = pow2i(-g.gmin);
+            for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+                 i += (int64_t)gridDim.x * blockDim.x)
+                pk[i] = pack_point(xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2], inv);
+            return;
+        }
+    }
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
          i += (int64_t)gridDim.x * blockDim.x) {
         uint32_t set = 0;
@@ -612,11 +666,11 @@ __global__ __launch_bounds__(256) void k_points_f32(const double* __restrict__ x
 }
 
 void enqueue_points_f32(const double* xyz, const int64_t* pt_off, uint32_t n_sets, int64_t total,
-                        const int32_t* grid, F4* out, hipStream_t s) {
+                        const int32_t* grid, F4* out, hipStream_t s, uint32_t* pk) {
     if (total <= 0) return;
     const int cus = device_cu_count();
     const unsigned g = (unsigned)std::min<int64_t>((total + 255) / 256, (int64_t)cus * 8);
-    k_points_f32<<<g, 256, 0, s>>>(xyz, pt_off, n_sets, total, grid, out);
+    k_points_f32<<<g, 256, 0, s>>>(xyz, pt_off, n_sets, total, grid, out, pk);
     D3_LAUNCH_CHECK();
 }
 
@@ -652,8 +706,13 @@ void enqueue_batch(const BatchArgs& base, const BatchShape& sh, hipStream_t s, h
         a.n_tasks = sh.n_lds_tasks;
         a.chunk = sh.lds_chunk;
         const unsigned g = (unsigned)std::min<uint64_t>(sh.n_lds_tasks, (uint64_t)cus * sh.lds_blocks_per_cu);
-        if (a.stats) pred_batch_lds<true><<<g, kLdsThreads, 0, s>>>(a);
-        else pred_batch_lds<false><<<g, kLdsThreads, 0, s>>>(a);
+        if (sh.packed) {
+            if (a.stats) pred_batch_packed<true><<<g, kLdsThreads, 0, s>>>(a);
+            else pred_batch_packed<false><<<g, kLdsThreads, 0, s>>>(a);
+        } else {
+            if (a.stats) pred_batch_lds<true><<<g, kLdsThreads, 0, s>>>(a);
+            else pred_batch_lds<false><<<g, kLdsThreads, 0, s>>>(a);
+        }
         D3_LAUNCH_CHECK();
     }
     if (sh.n_global_tasks) {
@@ -683,9 +742,11 @@ void enqueue_batch(const BatchArgs& base, const BatchShape& sh, hipStream_t s, h
     if (ev) D3_HIP(hipEventRecord(ev[5], s));
 }
 
-// One point set: the LDS kernel when the set fits, else the global one; the
-// tasks are implicit equal chunks. Queries are processed in launch chunks of
-// at most kChunk (which bounds the queues).
+// One point set: the LDS kernel when the set fits; else, up to kPackPts
+// points, the packed LDS kernel and the global one, of which the set's
+// summary runs one (packable sets: the packed kernel); else the global one.
+// The tasks are implicit equal chunks. Queries are processed in launch chunks
+// of at most kChunk (which bounds the queues).
 void launch_predicates(const double* xyz, int64_t n_pts, const uint8_t* op, const int32_t* idx,
                        int64_t n_q, int8_t* out, hipStream_t s, unsigned long long* stats_dev,
                        hipEvent_t* ev, int force_path, hipEvent_t* ev2) {
@@ -698,8 +759,10 @@ void launch_predicates(const double* xyz, int64_t n_pts, const uint8_t* op, cons
     int32_t* grid = sc.get<int32_t>(2);
     const int cus = device_cu_count();
     const bool lds = force_path == 1 || (force_path == 0 && n_pts <= kLdsPts);
+    const bool packed = force_path == 0 && n_pts > kLdsPts && n_pts <= kPackPts;
     // the global kernel's compact float copy (used only if the set is exact in float)
     F4* pts32 = lds ? nullptr : sc.get<F4>((size_t)n_pts);
+    uint32_t* pk = packed ? sc.get<uint32_t>((size_t)n_pts) : nullptr;
     bool grids_done = false;
     for (int64_t off = 0; off < n_q; off += chunk) {
         const int64_t n = std::min<int64_t>(chunk, n_q - off);
@@ -716,21 +779,25 @@ void launch_predicates(const double* xyz, int64_t n_pts, const uint8_t* op, cons
         a.stats = stats_dev;
         a.grid = grid;
         a.pts32 = pts32;
+        a.pk = pk;
+        a.pack_route = packed ? 1 : 0;
         BatchShape sh{};
         if (!grids_done) {
             // the set's grid summary, and this chunk's counters, in one prepass
             enqueue_set_grids(a, 1, n_pts, grid, s);
-            if (pts32) enqueue_points_f32(xyz, nullptr, 1, n_pts, grid, pts32, s);
+            if (pts32) enqueue_points_f32(xyz, nullptr, 1, n_pts, grid, pts32, s, pk);
             grids_done = true;
             sh.counters_cleared = true;
         }
-        if (lds) {
+        if (lds || packed) {
             // about two tasks per CU, at least 8 tiles each
             const int64_t per = std::max<int64_t>(8 * kLdsThreads, (n + 2 * cus - 1) / (2 * cus));
             sh.lds_chunk = (uint32_t)std::min<int64_t>(per, n);
             sh.n_lds_tasks = (uint32_t)((n + sh.lds_chunk - 1) / sh.lds_chunk);
             sh.lds_blocks_per_cu = 1;
-        } else {
+            sh.packed = packed;
+        }
+        if (!lds) {
             // 4 workgroups of 4 wavefronts per CU (4 wavefronts per SIMD, the
             // kernel's occupancy), about two tasks each, tasks of 4..64 tiles
             // (measured on 2^24 lattice queries: 32 tiles x 4 per CU is 1.3%

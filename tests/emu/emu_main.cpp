@@ -309,12 +309,19 @@ int main(int argc, char** argv) {
         for (int i = 0; i < 200; ++i)
             for (int k = 0; k < 3; ++k) q48.push_back((double)(rnd() >> 16) * 3.552713678800501e-15);
         fails += check_predicates("grid48", q48, 2000);
-        // above the LDS threshold and exact in float: the global kernel's float copy
+        // above the LDS threshold, packable: the packed LDS kernel
         std::vector<double> lat20;
         for (int i = 0; i < 20; ++i)
             for (int j = 0; j < 20; ++j)
                 for (int k = 0; k < 20; ++k) lat20.insert(lat20.end(), {i * 0.5, j * 0.25, k - 10.0});
-        fails += check_predicates("lattice20 (float copy)", lat20, 3000);
+        fails += check_predicates("lattice20 (packed)", lat20, 3000);
+        // above the packed kernel's capacity and exact in float: the global
+        // kernel's float copy
+        std::vector<double> lat34;
+        for (int i = 0; i < 34; ++i)
+            for (int j = 0; j < 34; ++j)
+                for (int k = 0; k < 34; ++k) lat34.insert(lat34.end(), {i * 0.5, j * 0.25, k - 10.0});
+        fails += check_predicates("lattice34 (float copy)", lat34, 3000);
         fails += check_paths("lattice", lat, 5000);
         fails += check_paths("wide", wide, 2500);
         fails += check_sets();

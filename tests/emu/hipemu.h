@@ -193,8 +193,15 @@ inline bool is_pinned(const void* p) {
     return false;
 }
 }  // namespace emu
+// Device memory starts with garbage, as on a GPU (large host allocations
+// would otherwise read as zeros): every byte is EMU_MALLOC_FILL (default 0xA5).
 inline hipError_t hipMallocAsync(void** p, size_t n, hipStream_t) {
+    static const int fill = [] {
+        const char* e = std::getenv("EMU_MALLOC_FILL");
+        return e ? (int)std::strtol(e, nullptr, 0) : 0xA5;
+    }();
     *p = std::malloc(n ? n : 1);
+    if (*p) std::memset(*p, fill, n ? n : 1);
     return *p ? hipSuccess : hipErrorOutOfMemory;
 }
 inline hipError_t hipMalloc(void** p, size_t n) { return hipMallocAsync(p, n, nullptr); }

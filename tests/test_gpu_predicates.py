@@ -282,31 +282,85 @@ def test_sets_call_and_graph_replays():
 @pytest.mark.parametrize("kind", ["float_exact_lattice", "lattice_2^-120", "lattice_2^-135",
                                   "wide_integers", "random_doubles"])
 def test_global_kernel_point_formats(kind):
-    """Single sets above the LDS threshold, so the global kernel gathers them:
-    a half-integer lattice and one scaled by 2^-120 (exact normal floats:
-    gathered from the 16-byte float copy), one scaled by 2^-135 (float
-    subnormals: doubles), integers up to 2^30 (certifiable, not exact in
-    float: doubles) and random doubles (neither) — each against the oracle."""
+    """Single sets above the packed LDS kernel's capacity (34^3 = 39,304
+    points), so the global kernel gathers them: a half-integer lattice and one
+    scaled by 2^-120 (exact normal floats: gathered from the 16-byte float
+    copy), one scaled by 2^-135 (float subnormals: doubles), integers up to
+    2^30 (certifiable, not exact in float: doubles) and random doubles
+    (neither) — each against the oracle."""
     rng = np.random.default_rng(77)
+    m = 34
     if "lattice" in kind:
-        pts = W.lattice(24, 24, 24) * np.array([0.5, 0.25, 1.0]) - 3.0
+        pts = W.lattice(m, m, m) * np.array([0.5, 0.25, 1.0]) - 3.0
         if kind != "float_exact_lattice":
             pts = pts * 2.0 ** (-120 if "120" in kind else -135)
     elif kind == "wide_integers":
-        pts = rng.integers(-(1 << 30), 1 << 30, size=(9000, 3)).astype(np.float64)
+        pts = rng.integers(-(1 << 30), 1 << 30, size=(40000, 3)).astype(np.float64)
         pts[::3] = np.round(pts[::3] / 1024.0)  # some close neighbours as well
     else:
-        pts = rng.random((9000, 3))
+        pts = rng.random((40000, 3))
     n = 1 << 17
     op = rng.integers(0, 3, size=n).astype(np.uint8)
     idx = rng.integers(0, len(pts), size=(n, 5)).astype(np.int32)
     if "lattice" in kind:  # local cells: many degenerate queries
-        base = rng.integers(0, 22, size=(n, 3))
+        base = rng.integers(0, m - 2, size=(n, 3))
         off = rng.integers(0, 3, size=(n, 5, 3))
         ijk = base[:, None, :] + off
-        idx = ((ijk[..., 0] * 24 + ijk[..., 1]) * 24 + ijk[..., 2]).astype(np.int32)
+        idx = ((ijk[..., 0] * m + ijk[..., 1]) * m + ijk[..., 2]).astype(np.int32)
     say("global kernel", kind, len(pts), "points")
     got, st = predicates_dev(pts, op, idx, stats=True)
     assert np.array_equal(got, O.pred_batch(pts, op, idx, "oracle"))
     if "lattice" in kind:
+        assert st["perturbed"] > 1000
+
+
+def _packed_case(kind, rng):
+    """(points, lattice side or 0) of a set routed to the packed LDS kernel or,
+    when its values span too many bits, to the global kernel beside it."""
+    if kind == "lattice32":  # the headline set
+        return W.lattice(32, 32, 32), 32
+    if kind == "lattice32_2^-200":  # near the smallest packable grid, 2^-206
+        return W.lattice(32, 32, 32) * 2.0 ** -200, 32
+    if kind == "lattice32_2^-1000":
+        return W.lattice(32, 32, 32) * 2.0 ** -1000, 32
+    if kind == "centered_signed_zeros":  # -0.0 where the lattice has 0
+        pts = W.lattice(24, 24, 24) - 12.0
+        return np.where(pts == 0.0, -0.0, pts), 24
+    if kind == "span511":  # the widest packable span, random integers
+        return rng.integers(-511, 512, size=(20000, 3)).astype(np.float64), 0
+    if kind == "span512":  # one bit too wide: the global kernel runs
+        pts = rng.integers(-511, 512, size=(20000, 3)).astype(np.float64)
+        pts[7, 1] = 512.0
+        return pts, 0
+    if kind == "smallest":  # kLdsPts + 1 points
+        return rng.integers(-8, 8, size=(5633, 3)).astype(np.float64) * 0.125, 0
+    if kind == "largest":  # kPackPts points
+        return rng.integers(-20, 20, size=(33792, 3)).astype(np.float64), 0
+    return rng.random((20000, 3)), 0  # random doubles: the global kernel runs
+
+
+@pytest.mark.parametrize("kind", ["lattice32", "lattice32_2^-200", "lattice32_2^-1000", "centered_signed_zeros", "span511",
+                                  "span512", "smallest", "largest", "random_doubles"])
+def test_packed_kernel_point_formats(kind):
+    """Single sets of 5,633..33,792 points run the packed LDS kernel (one
+    32-bit word a point, the whole set in LDS) when their values span at most
+    9 bits on a grid 2^gmin with -206 <= gmin <= 171, else the global kernel:
+    each against the oracle, with the perturbation exercised on the lattices.
+    The lattice at 2^-1000 (global kernel) has products that underflow FP64,
+    where the reference's expansions are not exact and the library returns
+    the true sign: that case is checked against the big-integer oracle."""
+    rng = np.random.default_rng(91)
+    pts, m = _packed_case(kind, rng)
+    n = 1 << 17
+    op = rng.integers(0, 3, size=n).astype(np.uint8)
+    idx = rng.integers(0, len(pts), size=(n, 5)).astype(np.int32)
+    if m:  # local cells: many degenerate queries
+        base = rng.integers(0, m - 2, size=(n, 3))
+        off = rng.integers(0, 3, size=(n, 5, 3))
+        ijk = base[:, None, :] + off
+        idx = ((ijk[..., 0] * m + ijk[..., 1]) * m + ijk[..., 2]).astype(np.int32)
+    say("packed route", kind, len(pts), "points")
+    got, st = predicates_dev(pts, op, idx, stats=True)
+    assert np.array_equal(got, O.pred_batch(pts, op, idx, "int" if "1000" in kind else "oracle"))
+    if m:
         assert st["perturbed"] > 1000

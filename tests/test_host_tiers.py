@@ -106,3 +106,94 @@ def test_set_level_certificate_implies_per_coordinate(scale):
             accepted += int(both[1])
         if span <= 5 and trial % 4 != 1:
             assert accepted > 0
+
+
+
+def _pack(pts):
+    lib = C.CDLL(str(LIB))
+    pts = np.ascontiguousarray(pts, dtype=np.float64)
+    gmin, mexp = _set_grid(pts) if np.any(pts != 0) else (1 << 30, -(1 << 30))
+    words = np.zeros(len(pts), np.uint32)
+    back = np.zeros_like(pts)
+    ok = lib.predcore_pack_points(pts.ctypes.data_as(C.c_void_p), C.c_int64(len(pts)), C.c_int32(gmin),
+                                  C.c_int32(mexp), words.ctypes.data_as(C.c_void_p),
+                                  back.ctypes.data_as(C.c_void_p))
+    return bool(ok), words, back
+
+
+def _packed_sets():
+    rng = np.random.default_rng(7)
+    lat = O.lattice(32, 32, 32)
+    return rng, {
+        "lattice32": lat,
+        "lattice32_2^-40": lat * 2.0 ** -40,
+        "lattice32_2^-200": lat * 2.0 ** -200,
+        "lattice32_2^140": lat * 2.0 ** 140,
+        "centered_span511": rng.integers(-511, 512, size=(3000, 3)).astype(np.float64),
+        "signed_zeros": np.where(rng.random((2000, 3)) < 0.3, -0.0,
+                                 rng.integers(-300, 300, size=(2000, 3)).astype(np.float64)),
+        "span512": np.vstack([lat, [[512.0, 0.0, 0.0]]]),
+        "offset_lattice": lat + 1000.0,
+        "lattice32_2^-300": lat * 2.0 ** -300,
+        "lattice32_2^200": lat * 2.0 ** 200,
+        "all_zero": np.zeros((100, 3)),
+        "random": rng.normal(size=(500, 3)),
+    }
+
+
+@pytest.mark.parametrize("kind,packable", [
+    ("lattice32", True), ("lattice32_2^-40", True), ("lattice32_2^-200", True), ("lattice32_2^140", True),
+    ("centered_span511", True), ("signed_zeros", True), ("span512", False), ("offset_lattice", False),
+    ("lattice32_2^-300", False), ("lattice32_2^200", False), ("all_zero", False), ("random", False)])
+def test_packed_point_format_round_trip(kind, packable):
+    """The packed small-grid copy (one 32-bit word a point) that the packed
+    LDS kernel reads: a set qualifies exactly when its values span at most 9
+    significant bits on a grid 2^gmin with -206 <= gmin <= 171; then every
+    coordinate comes back (-0.0 as 0.0) and comparing words is comparing
+    points lexicographically."""
+    _, sets = _packed_sets()
+    pts = sets[kind]
+    ok, words, back = _pack(pts)
+    assert ok == packable
+    if ok:
+        assert np.array_equal(pts, back)  # == : -0.0 equals 0.0
+        assert np.all(words >> 30 == 0)
+        i = np.random.default_rng(1).integers(0, len(pts), size=(4000, 2))
+        a, b = pts[i[:, 0]], pts[i[:, 1]]
+        lex = (a[:, 0] < b[:, 0]) | ((a[:, 0] == b[:, 0]) & ((a[:, 1] < b[:, 1]) |
+                                                              ((a[:, 1] == b[:, 1]) & (a[:, 2] < b[:, 2]))))
+        assert np.array_equal(lex, words[i[:, 0]] < words[i[:, 1]])
+
+
+@pytest.mark.parametrize("kind", ["lattice32", "lattice32_2^-40", "lattice32_2^-200", "lattice32_2^140",
+                                  "centered_span511", "signed_zeros"])
+def test_packed_decision_matches_oracle(kind):
+    """The packed kernel's decision on the integer grid (the certified
+    branch, with the generic tiers where an in_sphere certificate fails),
+    built for the host, against the oracle on local-cell queries (most of
+    them degenerate) and random ones."""
+    rng, sets = _packed_sets()
+    pts = np.ascontiguousarray(sets[kind], dtype=np.float64)
+    gmin, mexp = _set_grid(pts)
+    n = 20000
+    op = rng.integers(0, 3, size=n).astype(np.uint8)
+    idx = rng.integers(0, len(pts), size=(n, 5)).astype(np.int32)
+    if "lattice" in kind:
+        base = rng.integers(0, 30, size=(n // 2, 3))
+        off = rng.integers(0, 3, size=(n // 2, 5, 3))
+        ijk = base[:, None, :] + off
+        idx[: n // 2] = ((ijk[..., 0] * 32 + ijk[..., 1]) * 32 + ijk[..., 2]).astype(np.int32)
+    out = np.zeros(n, np.int8)
+    used = np.zeros(n, np.int8)
+    lib = C.CDLL(str(LIB))
+    ok = lib.predcore_packed_batch(pts.ctypes.data_as(C.c_void_p), C.c_int64(len(pts)), C.c_int32(gmin),
+                                   C.c_int32(mexp), op.ctypes.data_as(C.c_void_p),
+                                   idx.ctypes.data_as(C.c_void_p), C.c_int64(n),
+                                   out.ctypes.data_as(C.c_void_p), used.ctypes.data_as(C.c_void_p))
+    assert ok == 1
+    assert np.array_equal(out, O.pred_batch(pts, op, idx, "oracle"))
+    assert np.all(used[op == 0] == 1)  # every orient3d query is certified on the grid
+    if kind != "centered_span511":
+        assert used.sum() > n // 2
+    else:  # wide in_sphere queries take the generic tiers
+        assert (used == 0).sum() > 100
