@@ -73,7 +73,7 @@ def cpu_baseline(pts, op, idx, gpu_signs, sample: int):
 def load_traffic():
     """HBM bytes per fast-path launch from the committed rocprofv3 counter
     summary (FETCH_SIZE + WRITE_SIZE, calibrated on a copy kernel), if any."""
-    f = ROOT / "profiles" / "pred_batch_global_lattice_traffic.json"
+    f = ROOT / "profiles" / "pred_batch_packed_lattice_traffic.json"
     if not f.exists():
         return None
     try:
@@ -596,7 +596,7 @@ def main() -> int:
                    "points": len(pts), "queries": n_q, "in_sphere": n_ins, "seed": 2},
         "roofline": {
             "bound": "mfma",
-            "kernel": "pred_batch_global",
+            "kernel": "pred_batch_packed",
             "achieved": round(achieved, 4),
             "peak": FP64_PEAK_TFLOPS,
             "unit": "TFLOP/s",
