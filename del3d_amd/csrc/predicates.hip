@@ -208,7 +208,7 @@ struct BatchLds {
     int32_t idx[kT * 5];
     uint8_t op[kT];
     uint16_t perm[kT];
-    uint32_t wave_ins[kT / 64];
+    uint32_t wave_ins[kT / 256];  // in_sphere count of each wavefront, one byte each
     int64_t task[4];
     double pts[kMode == kLdsF64 ? 3 * kLdsPts : 1];
     uint32_t pk[kMode == kLdsPacked ? kPackPts : 1];
@@ -222,7 +222,7 @@ __device__ __forceinline__ void pred_batch_body(const BatchArgs& a, BatchLds<kSt
     int32_t* s_idx = L.idx;
     uint8_t* s_op = L.op;
     uint16_t* s_perm = L.perm;
-    uint32_t* s_wave_ins = L.wave_ins;
+    uint8_t* s_wave_ins = reinterpret_cast<uint8_t*>(L.wave_ins);
     int64_t* s_task = L.task;
     double* s_pts = L.pts;
     uint32_t* s_pk = L.pk;
@@ -235,7 +235,7 @@ __device__ __forceinline__ void pred_batch_body(const BatchArgs& a, BatchLds<kSt
     }
 
     const int tid = threadIdx.x;
-    const int wave = tid >> 6;
+    const int wave = (int)__builtin_amdgcn_readfirstlane((uint32_t)tid >> 6);
     const int lane = __lane_id();
     int32_t loaded_set = -1;
     unsigned long long* stats = a.stats;
@@ -301,6 +301,12 @@ __device__ __forceinline__ void pred_batch_body(const BatchArgs& a, BatchLds<kSt
         auto prefetch = [&](int64_t b) {
             const int c = (int)min((int64_t)kT, q1 - b);
             const int32_t* src = a.idx + b * 5;
+            if (c == kT) {  // a full tile (uniform): no per-lane guards
+#pragma unroll
+                for (int k = 0; k < 5; ++k) pf_idx[k] = src[tid + k * kT];
+                pf_op = a.op[b + tid];
+                return;
+            }
 #pragma unroll
             for (int k = 0; k < 5; ++k) {
                 const int j = tid + k * kT;
@@ -322,14 +328,20 @@ __device__ __forceinline__ void pred_batch_body(const BatchArgs& a, BatchLds<kSt
             // query order, so wavefronts mostly run one code path
             const bool is_ins = live && my_op != DEL3D_OP_ORIENT3D;
             const uint64_t ins_mask = __ballot(is_ins);
-            if (lane == 0) s_wave_ins[wave] = (uint32_t)__popcll(ins_mask);
+            if (lane == 0) s_wave_ins[wave] = (uint8_t)__popcll(ins_mask);  // <= 64
             __syncthreads();
+            // sums of the byte counts, four wavefronts per dot product: all of
+            // them, and those of the wavefronts before this one
             uint32_t ins_before = 0, ins_total = 0;
 #pragma unroll
-            for (int w = 0; w < kW; ++w) {
-                const uint32_t c = s_wave_ins[w];
-                ins_total += c;
-                ins_before += (w < wave) ? c : 0u;
+            for (int j = 0; j < kW / 4; ++j) {
+                const uint32_t d = L.wave_ins[j];
+                const int below = wave - 4 * j;  // bytes 0 .. below-1 of d come before
+                const uint32_t m = below >= 4 ? 0x01010101u
+                                   : below <= 0 ? 0u
+                                                : (0x01010101u >> (8 * (4 - below)));
+                ins_total = __builtin_amdgcn_udot4(d, 0x01010101u, ins_total, false);
+                ins_before = __builtin_amdgcn_udot4(d, m, ins_before, false);
             }
             const uint32_t lt = (uint32_t)__popcll(ins_mask & lanemask_lt());
             const uint32_t pos = is_ins ? ins_before + lt
