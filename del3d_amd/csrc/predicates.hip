@@ -269,6 +269,9 @@ __device__ __forceinline__ void pred_batch_body(const BatchArgs& a, BatchLds<kSt
         const int64_t np = kMode == kLdsF64      ? min(uniform64(s_task[3]), (int64_t)kLdsPts)
                            : kMode == kLdsPacked ? min(uniform64(s_task[3]), (int64_t)kPackPts)
                                                  : uniform64(s_task[3]);
+        // the same bound for 32-bit compares: an index as uint32 is below np
+        // exactly when it is below min(np, 2^31) (negative ones never are)
+        const uint32_t np32 = (uint32_t)min(np, (int64_t)0x80000000);
         const double* setpts = set_points(a, set);
         // no summary: an impossible grid, so only the per-coordinate test runs
         SetGrid sg{-100000, 100000};
@@ -360,7 +363,7 @@ __device__ __forceinline__ void pred_batch_body(const BatchArgs& a, BatchLds<kSt
             const int nchk = (o == DEL3D_OP_ORIENT3D) ? 4 : 5;
             bool valid = o <= DEL3D_OP_IN_SPHERE_UNPERTURBED;
 #pragma unroll
-            for (int k = 0; k < 5; ++k) valid = valid && (k >= nchk || (uint32_t)id[k] < (uint64_t)np);
+            for (int k = 0; k < 5; ++k) valid = valid && (k >= nchk || (uint32_t)id[k] < np32);
 
             int sign = 0, tier = kTierFilter;
             bool perturbed = false;
