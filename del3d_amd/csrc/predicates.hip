@@ -318,14 +318,14 @@ __device__ __forceinline__ void pred_batch_body(const BatchArgs& a, BatchLds<kSt
             pf_op = tid < c ? a.op[b + tid] : (uint8_t)DEL3D_OP_ORIENT3D;
         };
         prefetch(q0);
+        // Two barriers per tile. The counts' byte is written before the first;
+        // its readers all passed the previous tile's second barrier. The
+        // records, s_perm and s_op are written between the two: every thread
+        // that reaches the first has finished the previous tile.
         for (int64_t base = q0; base < q1; base += kT) {
             const uint32_t cnt = (uint32_t)min((int64_t)kT, q1 - base);
-            __syncthreads();  // the previous tile is done with s_idx / s_op / s_perm
-#pragma unroll
-            for (int k = 0; k < 5; ++k) s_idx[tid + k * kT] = pf_idx[k];
             const bool live = (uint32_t)tid < cnt;
             const uint8_t my_op = live ? pf_op : (uint8_t)DEL3D_OP_ORIENT3D;
-            if (base + kT < q1) prefetch(base + kT);
 
             // regroup: in_sphere-type queries first, orient3d after, each in
             // query order, so wavefronts mostly run one code path
@@ -333,6 +333,9 @@ __device__ __forceinline__ void pred_batch_body(const BatchArgs& a, BatchLds<kSt
             const uint64_t ins_mask = __ballot(is_ins);
             if (lane == 0) s_wave_ins[wave] = (uint8_t)__popcll(ins_mask);  // <= 64
             __syncthreads();
+#pragma unroll
+            for (int k = 0; k < 5; ++k) s_idx[tid + k * kT] = pf_idx[k];
+            if (base + kT < q1) prefetch(base + kT);
             // sums of the byte counts, four wavefronts per dot product: all of
             // them, and those of the wavefronts before this one
             uint32_t ins_before = 0, ins_total = 0;

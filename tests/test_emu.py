@@ -38,10 +38,8 @@ def test_emulated_sort_matches_oracle(emu_exe, defer, order, sizes):
     assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-4000:]
 
 
-def test_emulated_server_under_thread_sanitizer():
-    """The serving loop (scheduler thread, one worker thread per slot, two
-    client threads, and the predicate pipeline it drives) under
-    ThreadSanitizer."""
+@pytest.fixture(scope="module")
+def tsan_exe():
     exe = Path("/tmp") / f"del3d_emu_tsan_{os.getuid()}"
     csrc = HERE.parent / "del3d_amd" / "csrc"
     subprocess.run(["python", str(HERE / "emu" / "build_emu.py"), str(exe),
@@ -49,6 +47,25 @@ def test_emulated_server_under_thread_sanitizer():
                     str(csrc / "circumcenter.hip"), str(csrc / "batch.hip"), str(csrc / "server.hip"), str(csrc / "spatial_sort.hip")],
                    check=True,
                    env=dict(os.environ, EMU_SANITIZER="thread"))
+    return exe
+
+
+def test_emulated_predicates_under_thread_sanitizer(tsan_exe):
+    """The 3D predicate pipeline (LDS, packed and global kernels over several
+    tiles per workgroup, plans) under ThreadSanitizer: a missing barrier
+    between a tile's LDS writes and another thread's reads is a data race
+    between the emulated threads."""
+    env = dict(os.environ, EMU_PRED_ONLY="1", TSAN_OPTIONS="halt_on_error=1")
+    r = subprocess.run([str(tsan_exe)], capture_output=True, text=True, timeout=900, env=env)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-4000:]
+    assert "pred lattice20 (packed) ok" in r.stdout and "MISMATCH" not in r.stdout
+
+
+def test_emulated_server_under_thread_sanitizer(tsan_exe):
+    """The serving loop (scheduler thread, one worker thread per slot, two
+    client threads, and the predicate pipeline it drives) under
+    ThreadSanitizer."""
+    exe = tsan_exe
     env = dict(os.environ, EMU_SERVER_ONLY="1", TSAN_OPTIONS="halt_on_error=1")
     r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=900, env=env)
     assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-4000:]
