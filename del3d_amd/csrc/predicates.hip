@@ -6,9 +6,13 @@
 //  pred_batch_lds     sets of at most kLdsPts points: one 1024-thread
 //                     workgroup per CU holds the whole set in LDS (up to
 //                     132 KB) and walks its queries in 1024-query tiles.
+//  pred_batch_packed  single sets of kLdsPts+1 .. kPackPts points on a small
+//                     integer grid (pack_point): the same, with the set's
+//                     packed copy (4 bytes a point) in LDS and every query
+//                     decided on the grid (packed_decide).
 //  pred_batch_global  larger sets: 256-thread workgroups gather points from
 //                     global memory (L2 / Infinity Cache / HBM).
-//     Both: the tile's index records are staged through LDS with coalesced
+//     All: the tile's index records are staged through LDS with coalesced
 //     dword loads, the tile is regrouped so in_sphere and orient3d queries
 //     occupy separate wavefronts, and each lane runs, in registers, the FP64
 //     filter in the reference's operation order, then for undecided queries
@@ -88,7 +92,6 @@ struct LdsPts {
     const double* p;  // points to the __shared__ array
     __device__ P3 operator()(int32_t i) const { return P3{p[3 * i], p[3 * i + 1], p[3 * i + 2]}; }
 };
-
 
 // The fast tiers of one query: T0 filter, T1 certificate, in-register
 // perturbation. Returns the public sign, or 2 when the exact tiers must decide.
