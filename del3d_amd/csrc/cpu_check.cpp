@@ -119,14 +119,10 @@ extern "C" int predcore_pack_points(const double* xyz, int64_t n, int32_t gmin, 
 }
 
 // The packed kernel's per-query decision on a packable set with summary
-// (gmin, mexp): packed_decide on the packed words, the exact tiers on the
-// set's doubles where it returns kPackedGeneric (the kernel queues those
-// queries for the exact kernel). Returns 0 (nothing written)
-// when the set is not packable; else out[q] = sign, used[q] = 1 where the
-// grid-unit decision decided.
+// (gmin, mexp): packed_decide on the packed words. Returns 0 (nothing
+// written) when the set is not packable; else out[q] = sign.
 extern "C" int predcore_packed_batch(const double* xyz, int64_t n_pts, int32_t gmin, int32_t mexp,
-                                     const uint8_t* op, const int32_t* idx, int64_t n_q, int8_t* out,
-                                     int8_t* used) {
+                                     const uint8_t* op, const int32_t* idx, int64_t n_q, int8_t* out) {
     if (!set_packable(SetGrid{gmin, mexp})) return 0;
     const double inv = pow2i(-gmin);
     std::vector<uint32_t> words((size_t)n_pts);
@@ -138,14 +134,7 @@ extern "C" int predcore_packed_batch(const double* xyz, int64_t n_pts, int32_t g
         for (int k = 0; k < np; ++k) w[k] = words[id[k]];
         int tier = -1;
         bool perturbed = false;
-        int s = packed_decide(w, op[q], id, tier, perturbed);
-        used[q] = s != kPackedGeneric;
-        if (s == kPackedGeneric) {
-            P3 p[5];
-            for (int k = 0; k < 5; ++k) p[k] = k < np ? load_p3(xyz, id[k]) : P3{0.0, 0.0, 0.0};
-            s = eval_one(op[q], p, id, 1, &tier);
-        }
-        out[q] = (int8_t)s;
+        out[q] = (int8_t)packed_decide(w, op[q], id, tier, perturbed);
     }
     return 1;
 }

@@ -341,8 +341,8 @@ D3_HD void subst_tet(const P3 p[5], int k, P3 t[4]) {
 }
 
 // ---- packed small-grid sets ---------------------------------------------------
-// A set whose nonzero coordinates are all c = k * 2^gmin with |k| <= 511 (its
-// values span at most 9 significant bits: mexp - gmin <= 8), with
+// A set whose nonzero coordinates are all c = k * 2^gmin with |k| <= 255 (its
+// values span at most 8 significant bits: mexp - gmin <= 7), with
 // -206 <= gmin <= 171, is stored as one 32-bit word per point: three 10-bit
 // fields k + 512, x in bits 20-29, y in 10-19, z in 0-9, so that comparing
 // words is comparing points lexicographically. -0.0 is stored as 0.0: no
@@ -352,7 +352,7 @@ D3_HD void subst_tet(const P3 p[5], int k, P3 t[4]) {
 // grid test, sgn() and the lexicographic order, treats -0.0 as 0.0).
 // An all-zero set does not qualify.
 D3_HD bool set_packable(SetGrid g) {
-    return g.gmin <= g.mexp && g.gmin >= -206 && g.gmin <= 171 && g.mexp - g.gmin <= 8;
+    return g.gmin <= g.mexp && g.gmin >= -206 && g.gmin <= 171 && g.mexp - g.gmin <= 7;
 }
 
 // inv_scale = 2^-gmin
@@ -404,30 +404,20 @@ D3_HD double insphere_det_dv(const DV& ae, const DV& be, const DV& ce, const DV&
     return (dlift * abc - clift * dab) + (blift * cda - alift * bcd);
 }
 
-constexpr int kPackedGeneric = 3;  // packed_decide: not decided on the grid
-
 // The fast tiers of one query of a packed set, on the integer grid: the
 // certified branch of the set-level certificate, which is what the generic
-// path takes for these sets. With |k| <= 511 and -206 <= gmin <= 171, the
-// orient3d certificate (k = 15) holds for every query, and the in_sphere one
-// (k = 8) exactly when every difference to e is below 512 grid units; a
-// certified determinant is exact, and on the grid it is the set's
-// determinant divided by a positive power of two, so its sign is the same,
-// as is every substituted orient3d of the perturbation and the points'
-// lexicographic order. w: the query's packed points (w[4] for in_sphere).
-// Returns kPackedGeneric for an in_sphere query whose certificate fails.
+// path takes for every query of these sets. With |k| <= 255 every difference
+// is below 512 grid units, so with -206 <= gmin <= 171 the orient3d (k = 15)
+// and in_sphere (k = 8) certificates hold for every query; a certified
+// determinant is exact, and on the grid it is the set's determinant divided
+// by a positive power of two, so its sign is the same, as is every
+// substituted orient3d of the perturbation and the points' lexicographic
+// order. w: the query's packed points (w[4] for in_sphere) of a packable set.
 D3_HD int packed_decide(const uint32_t w[5], uint8_t o, const int32_t id[5], int& tier, bool& perturbed) {
-    if (o == DEL3D_OP_ORIENT3D) {
-        tier = kTierCert;
-        return -sgn(orient_det_dv(pdiff(w[0], w[3]), pdiff(w[1], w[3]), pdiff(w[2], w[3])));
-    }
-    const DV d0 = pdiff(w[0], w[4]), d1 = pdiff(w[1], w[4]), d2 = pdiff(w[2], w[4]), d3 = pdiff(w[3], w[4]);
-    double md = fmax(fmax(fabs(d0.x), fabs(d0.y)), fabs(d0.z));
-    md = fmax(md, fmax(fmax(fabs(d1.x), fabs(d1.y)), fabs(d1.z)));
-    md = fmax(md, fmax(fmax(fabs(d2.x), fabs(d2.y)), fabs(d2.z)));
-    md = fmax(md, fmax(fmax(fabs(d3.x), fabs(d3.y)), fabs(d3.z)));
-    if (!(md < 512.0)) return kPackedGeneric;
     tier = kTierCert;
+    if (o == DEL3D_OP_ORIENT3D)
+        return -sgn(orient_det_dv(pdiff(w[0], w[3]), pdiff(w[1], w[3]), pdiff(w[2], w[3])));
+    const DV d0 = pdiff(w[0], w[4]), d1 = pdiff(w[1], w[4]), d2 = pdiff(w[2], w[4]), d3 = pdiff(w[3], w[4]);
     int sign = -sgn(insphere_det_dv(d0, d1, d2, d3));
     if (sign == 0 && o == DEL3D_OP_IN_SPHERE) {
         perturbed = true;

@@ -380,12 +380,7 @@ __device__ __forceinline__ void pred_batch_body(const BatchArgs& a, BatchLds<kSt
                     uint32_t w[5];
 #pragma unroll
                     for (int k = 0; k < 5; ++k) w[k] = k < nchk ? s_pk[id[k]] : 0u;
-                    // an in_sphere query whose grid certificate fails goes to
-                    // the exact kernel, whose sign is the one the filter or the
-                    // exact tiers would give (these sets neither underflow nor
-                    // overflow FP64, so a deciding filter's sign is exact)
                     sign = packed_decide(w, o, id, tier, perturbed);
-                    if (sign == kPackedGeneric) sign = 2;
                 }
                 else sign = fast_decide<false>(GlobalPts{setpts, set_pts32, use32}, o, id, sg, cert_first,
                                                tier, perturbed);

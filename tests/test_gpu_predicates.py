@@ -326,11 +326,11 @@ def _packed_case(kind, rng):
     if kind == "centered_signed_zeros":  # -0.0 where the lattice has 0
         pts = W.lattice(24, 24, 24) - 12.0
         return np.where(pts == 0.0, -0.0, pts), 24
-    if kind == "span511":  # the widest packable span, random integers
-        return rng.integers(-511, 512, size=(20000, 3)).astype(np.float64), 0
-    if kind == "span512":  # one bit too wide: the global kernel runs
-        pts = rng.integers(-511, 512, size=(20000, 3)).astype(np.float64)
-        pts[7, 1] = 512.0
+    if kind == "span255":  # the widest packable span, random integers
+        return rng.integers(-255, 256, size=(20000, 3)).astype(np.float64), 0
+    if kind == "span256":  # one bit too wide: the global kernel runs
+        pts = rng.integers(-255, 256, size=(20000, 3)).astype(np.float64)
+        pts[7, 1] = 256.0
         return pts, 0
     if kind == "smallest":  # kLdsPts + 1 points
         return rng.integers(-8, 8, size=(5633, 3)).astype(np.float64) * 0.125, 0
@@ -339,12 +339,12 @@ def _packed_case(kind, rng):
     return rng.random((20000, 3)), 0  # random doubles: the global kernel runs
 
 
-@pytest.mark.parametrize("kind", ["lattice32", "lattice32_2^-200", "lattice32_2^-1000", "centered_signed_zeros", "span511",
-                                  "span512", "smallest", "largest", "random_doubles"])
+@pytest.mark.parametrize("kind", ["lattice32", "lattice32_2^-200", "lattice32_2^-1000", "centered_signed_zeros", "span255",
+                                  "span256", "smallest", "largest", "random_doubles"])
 def test_packed_kernel_point_formats(kind):
     """Single sets of 5,633..33,792 points run the packed LDS kernel (one
     32-bit word a point, the whole set in LDS) when their values span at most
-    9 bits on a grid 2^gmin with -206 <= gmin <= 171, else the global kernel:
+    8 bits on a grid 2^gmin with -206 <= gmin <= 171, else the global kernel:
     each against the oracle, with the perturbation exercised on the lattices.
     The lattice at 2^-1000 (global kernel) has products that underflow FP64,
     where the reference's expansions are not exact and the library returns
