@@ -364,3 +364,27 @@ def test_packed_kernel_point_formats(kind):
     assert np.array_equal(got, O.pred_batch(pts, op, idx, "int" if "1000" in kind else "oracle"))
     if m:
         assert st["perturbed"] > 1000
+
+
+@pytest.mark.parametrize("kind", ["lattice32", "random9000"])
+def test_more_queries_than_one_launch_sequence(kind):
+    """2^26 + 2^18 queries in one call, above the 2^26 of one launch sequence
+    (so two sequences with their own queues and counters), on the 32^3
+    lattice (packed LDS kernel) and on 9,000 random points (the global kernel
+    beside it): equal to the same queries in calls of 2^24."""
+    if kind == "lattice32":
+        pts, op, idx = W.c2(1 << 24, seed=2)
+    else:
+        rng = np.random.default_rng(33)
+        pts = rng.random((9000, 3))
+        op = rng.integers(0, 3, size=1 << 24).astype(np.uint8)
+        idx = rng.integers(0, len(pts), size=(1 << 24, 5)).astype(np.int32)
+    idx = np.ascontiguousarray(idx, dtype=np.int32)
+    one, _ = predicates_dev(pts, op, idx)
+    tail = 1 << 18
+    say("two launch sequences", kind, 4 * len(op) + tail, "queries")
+    got, _ = predicates_dev(pts, np.concatenate([op] * 4 + [op[:tail]]),
+                            np.concatenate([idx] * 4 + [idx[:tail]]))
+    assert np.array_equal(got, np.concatenate([one] * 4 + [one[:tail]]))
+    s = np.random.default_rng(1).choice(len(op), 1 << 14, replace=False)
+    assert np.array_equal(one[s], O.pred_batch(pts, op[s], idx[s], "oracle"))
