@@ -78,6 +78,7 @@ struct BatchShape {
     uint32_t n_global_tasks = 0, global_chunk = 0, global_blocks_per_cu = 8;
     bool counters_cleared = false;  // the grid prepass already zeroed the counters
     bool packed = false;            // the LDS tasks run the packed-point kernel
+    bool global_packed = false;     // the global tasks also run the global packed variant
 };
 
 struct ScalarQuery {

@@ -11,7 +11,9 @@
 //                     packed copy (4 bytes a point) in LDS and every query
 //                     decided on the grid (packed_decide).
 //  pred_batch_global  larger sets: 256-thread workgroups gather points from
-//                     global memory (L2 / Infinity Cache / HBM).
+//                     global memory (L2 / Infinity Cache / HBM); for a
+//                     single packable set, pred_batch_global_packed gathers
+//                     its 4-byte words and decides on the grid.
 //     All: the tile's index records are staged through LDS with coalesced
 //     dword loads, the tile is regrouped so in_sphere and orient3d queries
 //     occupy separate wavefronts, and each lane runs, in registers, the FP64
@@ -200,13 +202,14 @@ __device__ inline PredTask task_at(const BatchArgs& a, uint32_t t) {
 
 // Body of the fast kernels. kMode: where the points come from (kGlobal:
 // gathers from global memory; kLdsF64: the workgroup loads each task's point
-// set into LDS first; kLdsPacked: the same with the set's packed copy).
+// set into LDS first; kLdsPacked: the same with the set's packed copy;
+// kGlobalPacked: gathers of the packed copy's words from global memory).
 // Workgroups take tasks dynamically (one atomic per task).
-enum { kGlobal = 0, kLdsF64 = 1, kLdsPacked = 2 };
+enum { kGlobal = 0, kLdsF64 = 1, kLdsPacked = 2, kGlobalPacked = 3 };
 
 template <bool kStats, int kMode>
 struct BatchLds {
-    static constexpr bool kLds = kMode != kGlobal;
+    static constexpr bool kLds = kMode == kLdsF64 || kMode == kLdsPacked;
     static constexpr int kT = kLds ? kLdsThreads : kGlobalThreads;  // queries per tile
     int32_t idx[kT * 5];
     uint8_t op[kT];
@@ -219,7 +222,8 @@ struct BatchLds {
 
 template <bool kStats, int kMode>
 __device__ __forceinline__ void pred_batch_body(const BatchArgs& a, BatchLds<kStats, kMode>& L) {
-    constexpr bool kLds = kMode != kGlobal;
+    constexpr bool kLds = BatchLds<kStats, kMode>::kLds;
+    constexpr bool kPacked = kMode == kLdsPacked || kMode == kGlobalPacked;
     constexpr int kT = BatchLds<kStats, kMode>::kT;
     constexpr int kW = kT / 64;
     int32_t* s_idx = L.idx;
@@ -230,11 +234,11 @@ __device__ __forceinline__ void pred_batch_body(const BatchArgs& a, BatchLds<kSt
     double* s_pts = L.pts;
     uint32_t* s_pk = L.pk;
 
-    // a single set run by both the packed kernel and the global one: the set's
+    // a single set run by both a packed kernel and the global one: the set's
     // summary picks one, the other returns at once (a.grid holds the summary)
     if (kMode != kLdsF64 && a.pack_route) {
         const bool packable = set_packable(SetGrid{a.grid[0], a.grid[1]});
-        if (packable != (kMode == kLdsPacked)) return;
+        if (packable != kPacked) return;
     }
 
     const int tid = threadIdx.x;
@@ -283,7 +287,7 @@ __device__ __forceinline__ void pred_batch_body(const BatchArgs& a, BatchLds<kSt
             sg.mexp = __builtin_amdgcn_readfirstlane(a.grid[2 * set + 1]);
         }
         const bool cert_first = set_certifiable(sg);
-        const bool use32 = !kLds && a.pts32 != nullptr && set_f32_exact(sg);
+        const bool use32 = kMode == kGlobal && a.pts32 != nullptr && set_f32_exact(sg);
         const F4* set_pts32 = use32 ? a.pts32 + (a.pt_off ? a.pt_off[set] : 0) : nullptr;
         if (kMode == kLdsF64 && set != loaded_set) {
             // np <= kLdsPts is guaranteed by the host, which routes larger sets
@@ -382,8 +386,15 @@ __device__ __forceinline__ void pred_batch_body(const BatchArgs& a, BatchLds<kSt
                     for (int k = 0; k < 5; ++k) w[k] = k < nchk ? s_pk[id[k]] : 0u;
                     sign = packed_decide(w, o, id, tier, perturbed);
                 }
-                else sign = fast_decide<false>(GlobalPts{setpts, set_pts32, use32}, o, id, sg, cert_first,
-                                               tier, perturbed);
+                else if (kMode == kGlobalPacked) {
+                    uint32_t w[5];
+#pragma unroll
+                    for (int k = 0; k < 5; ++k) w[k] = k < nchk ? a.pk[id[k]] : 0u;
+                    sign = packed_decide(w, o, id, tier, perturbed);
+                } else {
+                    sign = fast_decide<false>(GlobalPts{setpts, set_pts32, use32}, o, id, sg, cert_first,
+                                              tier, perturbed);
+                }
             }
             const bool queued = act && valid && sign == 2;
             if (act && !queued) a.out[q] = (int8_t)sign;
@@ -418,6 +429,13 @@ __global__ __launch_bounds__(kGlobalThreads) __attribute__((amdgpu_waves_per_eu(
 void pred_batch_global(BatchArgs a) {
     __shared__ BatchLds<kStats, kGlobal> L;
     pred_batch_body<kStats, kGlobal>(a, L);
+}
+
+template <bool kStats>
+__global__ __launch_bounds__(kGlobalThreads) __attribute__((amdgpu_waves_per_eu(4)))
+void pred_batch_global_packed(BatchArgs a) {
+    __shared__ BatchLds<kStats, kGlobalPacked> L;
+    pred_batch_body<kStats, kGlobalPacked>(a, L);
 }
 
 // ---------------------------------------------------------------------------
@@ -741,6 +759,11 @@ void enqueue_batch(const BatchArgs& base, const BatchShape& sh, hipStream_t s, h
         if (a.stats) pred_batch_global<true><<<g, kGlobalThreads, 0, s>>>(a);
         else pred_batch_global<false><<<g, kGlobalThreads, 0, s>>>(a);
         D3_LAUNCH_CHECK();
+        if (sh.global_packed) {
+            if (a.stats) pred_batch_global_packed<true><<<g, kGlobalThreads, 0, s>>>(a);
+            else pred_batch_global_packed<false><<<g, kGlobalThreads, 0, s>>>(a);
+            D3_LAUNCH_CHECK();
+        }
     }
     if (ev2) D3_HIP(hipEventRecord(ev2[1], s));
     if (ev) {
@@ -760,7 +783,8 @@ void enqueue_batch(const BatchArgs& base, const BatchShape& sh, hipStream_t s, h
 
 // One point set: the LDS kernel when the set fits; else, up to kPackPts
 // points, the packed LDS kernel and the global one, of which the set's
-// summary runs one (packable sets: the packed kernel); else the global one.
+// summary runs one (packable sets: the packed kernel); else the global
+// kernel and its packed variant, again one of them by the summary.
 // The tasks are implicit equal chunks. Queries are processed in launch chunks
 // of at most kChunk (which bounds the queues).
 void launch_predicates(const double* xyz, int64_t n_pts, const uint8_t* op, const int32_t* idx,
@@ -776,9 +800,13 @@ void launch_predicates(const double* xyz, int64_t n_pts, const uint8_t* op, cons
     const int cus = device_cu_count();
     const bool lds = force_path == 1 || (force_path == 0 && n_pts <= kLdsPts);
     const bool packed = force_path == 0 && n_pts > kLdsPts && n_pts <= kPackPts;
+    const bool global_packed = force_path == 0 && n_pts > kPackPts;
     // the global kernel's compact float copy (used only if the set is exact in float)
     F4* pts32 = lds ? nullptr : sc.get<F4>((size_t)n_pts);
-    uint32_t* pk = packed ? sc.get<uint32_t>((size_t)n_pts) : nullptr;
+    // the packed copy (written and used only if the set is packable, in place
+    // of the float copy): held in LDS by the packed kernel, or gathered from
+    // global memory by the global one's packed variant for larger sets
+    uint32_t* pk = (packed || global_packed) ? sc.get<uint32_t>((size_t)n_pts) : nullptr;
     bool grids_done = false;
     for (int64_t off = 0; off < n_q; off += chunk) {
         const int64_t n = std::min<int64_t>(chunk, n_q - off);
@@ -796,7 +824,7 @@ void launch_predicates(const double* xyz, int64_t n_pts, const uint8_t* op, cons
         a.grid = grid;
         a.pts32 = pts32;
         a.pk = pk;
-        a.pack_route = packed ? 1 : 0;
+        a.pack_route = (packed || global_packed) ? 1 : 0;
         BatchShape sh{};
         if (!grids_done) {
             // the set's grid summary, and this chunk's counters, in one prepass
@@ -823,6 +851,7 @@ void launch_predicates(const double* xyz, int64_t n_pts, const uint8_t* op, cons
             const int64_t per = std::min<int64_t>(64, std::max<int64_t>(4, tiles / (2 * 4 * (int64_t)cus)));
             sh.global_chunk = (uint32_t)(per * kGlobalThreads);
             sh.n_global_tasks = (uint32_t)((n + sh.global_chunk - 1) / sh.global_chunk);
+            sh.global_packed = global_packed;
         }
         enqueue_batch(a, sh, s, ev, ev2);
     }

@@ -315,13 +315,18 @@ int main(int argc, char** argv) {
             for (int j = 0; j < 20; ++j)
                 for (int k = 0; k < 20; ++k) lat20.insert(lat20.end(), {i * 0.5, j * 0.25, k - 10.0});
         fails += check_predicates("lattice20 (packed)", lat20, 3000);
-        // above the packed kernel's capacity and exact in float: the global
-        // kernel's float copy
-        std::vector<double> lat34;
+        // above the packed kernel's capacity, packable: the global kernel's
+        // packed variant; moved by 1000, exact in float but too wide to pack:
+        // the global kernel's float copy
+        std::vector<double> lat34, lat34f;
         for (int i = 0; i < 34; ++i)
             for (int j = 0; j < 34; ++j)
-                for (int k = 0; k < 34; ++k) lat34.insert(lat34.end(), {i * 0.5, j * 0.25, k - 10.0});
-        fails += check_predicates("lattice34 (float copy)", lat34, 3000);
+                for (int k = 0; k < 34; ++k) {
+                    lat34.insert(lat34.end(), {i * 0.5, j * 0.25, k - 10.0});
+                    lat34f.insert(lat34f.end(), {i * 0.5 + 1000.0, j * 0.25, k - 10.0});
+                }
+        fails += check_predicates("lattice34 (global packed)", lat34, 3000);
+        fails += check_predicates("lattice34+1000 (float copy)", lat34f, 3000);
         fails += check_paths("lattice", lat, 5000);
         fails += check_paths("wide", wide, 2500);
         fails += check_sets();

@@ -280,20 +280,27 @@ def test_sets_call_and_graph_replays():
 
 
 @pytest.mark.parametrize("kind", ["float_exact_lattice", "lattice_2^-120", "lattice_2^-135",
-                                  "wide_integers", "random_doubles"])
+                                  "wide_integers", "random_doubles", "packed_lattice",
+                                  "packed_lattice_2^-200"])
 def test_global_kernel_point_formats(kind):
     """Single sets above the packed LDS kernel's capacity (34^3 = 39,304
-    points), so the global kernel gathers them: a half-integer lattice and one
-    scaled by 2^-120 (exact normal floats: gathered from the 16-byte float
-    copy), one scaled by 2^-135 (float subnormals: doubles), integers up to
-    2^30 (certifiable, not exact in float: doubles) and random doubles
-    (neither) — each against the oracle."""
+    points), so the global kernel gathers them: a half-integer lattice moved
+    by 1000 (too wide to pack) and the same scaled by 2^-120 (exact normal
+    floats: gathered from the 16-byte float copy), scaled by 2^-135 (float
+    subnormals: doubles), integers up to 2^30 (certifiable, not exact in
+    float: doubles), random doubles (neither), and the unmoved lattice at 2^0
+    and 2^-200 (packable: the global kernel's packed variant gathers 4-byte
+    words and decides on the grid) — each against the oracle."""
     rng = np.random.default_rng(77)
     m = 34
     if "lattice" in kind:
         pts = W.lattice(m, m, m) * np.array([0.5, 0.25, 1.0]) - 3.0
-        if kind != "float_exact_lattice":
+        if not kind.startswith("packed"):
+            pts = pts + 1000.0
+        if "120" in kind or "135" in kind:
             pts = pts * 2.0 ** (-120 if "120" in kind else -135)
+        if "200" in kind:
+            pts = pts * 2.0 ** -200
     elif kind == "wide_integers":
         pts = rng.integers(-(1 << 30), 1 << 30, size=(40000, 3)).astype(np.float64)
         pts[::3] = np.round(pts[::3] / 1024.0)  # some close neighbours as well
