@@ -133,20 +133,6 @@ void fill_u32(uint32_t* p, uint32_t v, size_t n, hipStream_t s) {
     D3_LAUNCH_CHECK();
 }
 
-// The GPU spatial sort faulted the device in two runs, for a cause not yet
-// shown; until it is, the sort entry points refuse to run unless the caller
-// opts in explicitly.
-static bool gpu_sort_enabled() {
-    const char* e = std::getenv("DEL3D_ENABLE_GPU_SORT");
-    return e && std::strcmp(e, "1") == 0;
-}
-
-static int sort_disabled() {
-    return set_error(DEL3D_ERR_DEVICE,
-                     "the GPU spatial sort is disabled pending a device-fault investigation; "
-                     "set DEL3D_ENABLE_GPU_SORT=1 to run it");
-}
-
 // A non-blocking stream per thread and device for the synchronous forms.
 hipStream_t thread_stream() {
     thread_local std::vector<hipStream_t> streams;
@@ -410,7 +396,6 @@ int del3d_spatial_sort(const double* xyz, int64_t n_pts, int32_t* order, int64_t
         if (n_order > 0x7FFFFFFF) return set_error(DEL3D_ERR_INVALID, "more than 2^31-1 indices");
         if (n_order >= 2 && (!xyz || !order || n_pts == 0))
             return set_error(DEL3D_ERR_INVALID, "null pointer or no points");
-        if (n_order >= 2 && !gpu_sort_enabled()) return sort_disabled();
         spatial_sort_device(xyz, n_pts, order, n_order, (hipStream_t)stream);
         return DEL3D_OK;
     });
@@ -427,7 +412,6 @@ int del3d_spatial_sort_sets(const double* xyz, const int64_t* off, int64_t n_set
             if (off[s + 1] - off[s] > 0x7FFFFFFF) return set_error(DEL3D_ERR_INVALID, "set too large");
         }
         if (off[n_sets] > 0 && (!xyz || !order)) return set_error(DEL3D_ERR_INVALID, "null pointer");
-        if (off[n_sets] >= 2 && !gpu_sort_enabled()) return sort_disabled();
         spatial_sort_sets_device(xyz, off, n_sets, order, (hipStream_t)stream);
         return DEL3D_OK;
     });
@@ -439,7 +423,6 @@ int del3d_spatial_sort_host(const double* xyz, int64_t n_pts, int32_t* order, in
         if (n_order > 0x7FFFFFFF) return set_error(DEL3D_ERR_INVALID, "more than 2^31-1 indices");
         if (n_order < 2) return DEL3D_OK;
         if (!xyz || !order || n_pts == 0) return set_error(DEL3D_ERR_INVALID, "null pointer or no points");
-        if (!gpu_sort_enabled()) return sort_disabled();
         hipStream_t s = thread_stream();
         Scratch sc(s);
         double* dx = sc.get<double>((size_t)n_pts * 3);

@@ -4,37 +4,45 @@
 // rand48-driven Fisher-Yates loop (src/rand48.h:113-153), then runs the
 // multiscale pass (:130-137) and, on each multiscale segment, the Hilbert
 // median recursion (:95-117) whose median splits are a middle-pivot Lomuto
-// quickselect (:40-79). The result depends on the exact draw sequence and on
-// the exact arrangement every Lomuto round leaves behind, so the GPU does not
-// approximate either; it evaluates them in closed, parallel form.
+// quickselect (:40-79). The permutation depends on the exact draw sequence
+// and on the exact arrangement every Lomuto round leaves behind, so neither
+// is approximated: both are evaluated in closed, parallel form.
 //
-// Shuffle. The accept/reject decision for Fisher-Yates step k depends only on
-// k and the draw, never on n, so the steps of a shuffle of n elements are a
-// prefix of one universal step sequence j_1, j_2, ... A draw u can only be
-// rejected when u >= 2^31 - (K-1), so the few such candidates are compacted
-// and resolved by one wavefront (speculative prefix of rejections, iterated to
-// its fixed point); every other draw maps to its step in parallel. The final
-// position of each element follows from the steps by two relations: F(p), the
-// first step after p that selects position p, and G(k), the next step after
-// k selecting the same position as k. Element k ends at terminal(F-chain
-// from G(k)) if G(k) exists, else at j_k; element 0 at terminal(F(0)) or 0.
-// Terminals come from pointer jumping. Truncating F and G at n gives the
-// shuffle of any n <= K from the same tables.
+// Shuffle. Fisher-Yates step k (k = 1 .. n-1) swaps a[k] with a[j_k],
+// j_k in [0, k]. Whether a draw is rejected depends on k and the draw only,
+// so the steps of a shuffle of n are a prefix of one universal sequence
+// j_1, j_2, ... Only a draw u >= 2^31 - (K-1) can be rejected; those few
+// candidates are compacted in draw order and resolved by one wavefront, and
+// every other draw maps to its step in parallel. Element k of the input ends
+// at j_k if no later step selects j_k, otherwise it follows the chain
+// G(k) -> F(.) -> F(.) ..., where G(k) is the next step selecting the same
+// position as step k and F(p) the first step after p selecting position p;
+// element 0 follows F from 0. Truncating F and G at n gives the shuffle of
+// any n <= K from the same tables.
 //
-// Median splits. Per Lomuto round on [L, R] with pivot slot pv: after the
-// pivot swap, the elements less than the pivot move, in order, to L, L+1, ...
-// An element that is not less, starting at p, is carried along the chain
-// q <- Lpos[q] (Lpos[L+r] = position of the r-th smaller element) while
-// q < L + nless; the one that ends at L + nless is swapped with the pivot at
-// R. So a round is: flags, a per-range prefix, the Lpos table, one scatter.
-//   - ranges longer than kWave run these rounds device-wide (tiles of
-//     kTileElems records, several ranges batched per launch);
-//   - ranges of at most kWave elements finish in one wavefront with keys in
-//     LDS; whole Hilbert subtrees of at most kWave elements are sorted by one
-//     wavefront from start to finish.
-// The recursion plan (split positions, axes, directions) does not depend on
-// the data; only the arrangement inside each range does. The host walks the
-// plan level by level and batches every range of a phase into one launch.
+// Median splits. A Lomuto round on [L, R] with pivot slot pv swaps a[pv]
+// with a[R], then moves the elements less than the pivot, in order, to L,
+// L+1, ...; an element that is not less, starting at p, is carried along the
+// chain q <- Lpos[q - L] (Lpos[r] = position of the r-th smaller element)
+// while q < lim = L + nless; the one whose chain ends at lim goes to R, and
+// the pivot to lim. A round is therefore: flags, a per-range prefix, the
+// Lpos table, one scatter. The recursion plan (every range, axis and
+// direction) does not depend on the data, so the host builds it once and
+// uploads it in one copy; only the arrangement inside each range is data.
+//   - a select on more than kFin elements runs device-wide rounds: tiles of
+//     kTile elements, records in structure-of-arrays form ping-ponged between
+//     two buffers, round control kept on the device (the host only reads a
+//     counter of still-active selects every few rounds);
+//   - a select on at most kFin elements (or a big one once its range has
+//     shrunk to that) is finished by one 256-thread workgroup in LDS;
+//   - a Hilbert subtree of at most kSub elements is sorted from start to
+//     finish by one wavefront in LDS.
+//
+// Every global-memory index in these kernels is checked against the size of
+// the buffer it addresses, and every loop has a bound; a failed check skips
+// the access and sets a bit in a device error word, which the call returns
+// as DEL3D_ERR_INTERNAL. Device memory outside the call's buffers is never
+// touched, whatever a table holds.
 #include <algorithm>
 #include <cmath>
 #include <vector>
@@ -42,6 +50,20 @@
 #include "common.h"
 
 namespace d3 {
+namespace {
+
+// ---------------------------------------------------------------------------
+// error word
+// ---------------------------------------------------------------------------
+enum : uint32_t {
+    kErrShuffle = 1u,   // a shuffle table entry out of range
+    kErrIndex = 2u,     // an order[] entry outside the set's points
+    kErrRound = 4u,     // a device-round index out of range
+    kErrLds = 8u,       // an LDS-pass range or index out of range
+    kErrLoop = 16u,     // a loop hit its bound
+};
+
+__device__ inline void raise_err(uint32_t* err, uint32_t bit) { atomicOr(err, bit); }
 
 // ---------------------------------------------------------------------------
 // rand48 with jump-ahead (src/rand48.h:26-50)
@@ -55,30 +77,29 @@ struct Aff {
     uint64_t a, c;  // x -> a*x + c (mod 2^48)
 };
 
-__host__ __device__ inline Aff aff_then(Aff f, Aff g) {  // g after f
+__host__ __device__ inline Aff aff_compose(Aff f, Aff g) {  // g after f
     return Aff{(g.a * f.a) & kMask48, (g.a * f.c + g.c) & kMask48};
 }
 
-__host__ __device__ inline Aff aff_steps(uint64_t t) {
+__host__ __device__ inline Aff aff_pow(uint64_t t) {  // t steps of the generator
     Aff r{1, 0}, b{kA48, kC48};
     while (t) {
-        if (t & 1) r = aff_then(r, b);
-        b = aff_then(b, b);
+        if (t & 1) r = aff_compose(r, b);
+        b = aff_compose(b, b);
         t >>= 1;
     }
     return r;
 }
 
-constexpr int kDrawsPerThread = 64;
+constexpr int kDrawRun = 64;
 
-// u[t] = (state after t+1 steps) >> 17, for t in [0, count)
-__global__ __launch_bounds__(256) void k_draws(uint32_t* __restrict__ u, uint64_t count,
-                                               uint64_t x0) {
-    const uint64_t t0 = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) * kDrawsPerThread;
+// u[t] = (state after t+1 steps) >> 17, t in [0, count)
+__global__ __launch_bounds__(256) void k_draws(uint32_t* __restrict__ u, uint64_t count, uint64_t x0) {
+    const uint64_t t0 = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) * kDrawRun;
     if (t0 >= count) return;
-    const Aff f = aff_steps(t0 + 1);
+    const Aff f = aff_pow(t0 + 1);
     uint64_t x = (f.a * x0 + f.c) & kMask48;
-    const uint64_t end = min(count, t0 + kDrawsPerThread);
+    const uint64_t end = t0 + kDrawRun < count ? t0 + kDrawRun : count;
     for (uint64_t t = t0; t < end; ++t) {
         u[t] = (uint32_t)(x >> 17);
         x = (kA48 * x + kC48) & kMask48;
@@ -86,16 +107,17 @@ __global__ __launch_bounds__(256) void k_draws(uint32_t* __restrict__ u, uint64_
 }
 
 // ---------------------------------------------------------------------------
-// Ordered compaction of draw indices with u >= thr (candidates for rejection)
+// scans
 // ---------------------------------------------------------------------------
-constexpr int kScanBlock = 256;
+constexpr int kScanThreads = 256;
 constexpr int kScanItems = 16;
-constexpr int kScanTile = kScanBlock * kScanItems;
+constexpr int kScanTile = kScanThreads * kScanItems;
 
-__device__ inline uint32_t block_exclusive_scan(uint32_t v, uint32_t* s_wave, uint32_t& total) {
-    const int lane = __lane_id();
+// Exclusive prefix of v over the workgroup; *total = the workgroup's sum.
+// Every thread of the workgroup must call it.
+__device__ inline uint32_t wg_exclusive(uint32_t v, uint32_t* s_wave, uint32_t* total) {
+    const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
-    // inclusive scan within the wavefront
     uint32_t x = v;
 #pragma unroll
     for (int d = 1; d < 64; d <<= 1) {
@@ -104,118 +126,121 @@ __device__ inline uint32_t block_exclusive_scan(uint32_t v, uint32_t* s_wave, ui
     }
     if (lane == 63) s_wave[wave] = x;
     __syncthreads();
-    uint32_t before = 0;
-    total = 0;
-    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) {
+    uint32_t before = 0, all = 0;
+    const int nw = (int)(blockDim.x >> 6);
+    for (int w = 0; w < nw; ++w) {
         const uint32_t c = s_wave[w];
-        before += (w < wave) ? c : 0u;
-        total += c;
+        before += w < wave ? c : 0u;
+        all += c;
     }
     __syncthreads();
+    *total = all;
     return before + x - v;
 }
 
-__global__ __launch_bounds__(kScanBlock) void k_cand_count(const uint32_t* __restrict__ u,
-                                                           uint64_t count, uint32_t thr,
-                                                           uint32_t* __restrict__ blk) {
-    __shared__ uint32_t s_wave[kScanBlock / 64];
+// blk[b] = number of u[t] >= thr in tile b
+__global__ __launch_bounds__(kScanThreads) void k_cand_count(const uint32_t* __restrict__ u, uint64_t count,
+                                                             uint32_t thr, uint32_t* __restrict__ blk) {
+    __shared__ uint32_t s_wave[kScanThreads / 64];
     const uint64_t base = (uint64_t)blockIdx.x * kScanTile;
     uint32_t c = 0;
     for (int i = 0; i < kScanItems; ++i) {
-        const uint64_t t = base + (uint64_t)i * kScanBlock + threadIdx.x;
+        const uint64_t t = base + (uint64_t)i * kScanThreads + threadIdx.x;
         c += (t < count && u[t] >= thr) ? 1u : 0u;
     }
     uint32_t total;
-    (void)block_exclusive_scan(c, s_wave, total);
+    (void)wg_exclusive(c, s_wave, &total);
     if (threadIdx.x == 0) blk[blockIdx.x] = total;
 }
 
-// exclusive scan of a small array in one block (in place), total to *sum
-__global__ __launch_bounds__(1024) void k_scan_small(uint32_t* __restrict__ a, uint32_t n,
-                                                     uint32_t* __restrict__ sum) {
+// in-place exclusive scan of a[0..n) by one 1024-thread workgroup; *sum = total
+__global__ __launch_bounds__(1024) void k_scan_one(uint32_t* __restrict__ a, uint32_t n,
+                                                   uint32_t* __restrict__ sum) {
     __shared__ uint32_t s_wave[16];
-    __shared__ uint32_t s_carry;
-    if (threadIdx.x == 0) s_carry = 0;
-    __syncthreads();
+    uint32_t carry = 0;
     for (uint32_t base = 0; base < n; base += blockDim.x) {
         const uint32_t i = base + threadIdx.x;
         const uint32_t v = i < n ? a[i] : 0u;
         uint32_t total;
-        const uint32_t ex = block_exclusive_scan(v, s_wave, total);
-        const uint32_t carry = s_carry;
+        const uint32_t ex = wg_exclusive(v, s_wave, &total);
         if (i < n) a[i] = carry + ex;
-        __syncthreads();
-        if (threadIdx.x == 0) s_carry = carry + total;
-        __syncthreads();
+        carry += total;
     }
-    if (threadIdx.x == 0) *sum = s_carry;
+    if (threadIdx.x == 0) *sum = carry;
 }
 
-__global__ __launch_bounds__(kScanBlock) void k_cand_write(const uint32_t* __restrict__ u,
-                                                           uint64_t count, uint32_t thr,
-                                                           const uint32_t* __restrict__ blk,
-                                                           uint32_t* __restrict__ cand) {
-    __shared__ uint32_t s_wave[kScanBlock / 64];
+// draw indices with u >= thr, in draw order, into cand[0 .. cap)
+__global__ __launch_bounds__(kScanThreads) void k_cand_write(const uint32_t* __restrict__ u, uint64_t count,
+                                                             uint32_t thr, const uint32_t* __restrict__ blk,
+                                                             uint32_t* __restrict__ cand, uint32_t cap,
+                                                             uint32_t* __restrict__ err) {
+    __shared__ uint32_t s_wave[kScanThreads / 64];
     const uint64_t base = (uint64_t)blockIdx.x * kScanTile;
     uint32_t off = blk[blockIdx.x];
     for (int i = 0; i < kScanItems; ++i) {
-        const uint64_t t = base + (uint64_t)i * kScanBlock + threadIdx.x;
+        const uint64_t t = base + (uint64_t)i * kScanThreads + threadIdx.x;
         const bool f = t < count && u[t] >= thr;
         uint32_t total;
-        const uint32_t ex = block_exclusive_scan(f ? 1u : 0u, s_wave, total);
-        if (f) cand[off + ex] = (uint32_t)t;
+        const uint32_t ex = wg_exclusive(f ? 1u : 0u, s_wave, &total);
+        if (f) {
+            if (off + ex < cap) cand[off + ex] = (uint32_t)t;
+            else raise_err(err, kErrShuffle);
+        }
         off += total;
     }
 }
 
-// One wavefront walks the candidates in draw order. The step of candidate c
-// is t_c + 1 - (rejections before it); within a chunk of 64 the number of
+// One wavefront walks the candidates in draw order. Candidate c belongs to
+// step t_c + 1 - (rejections before it); inside a chunk of 64 the number of
 // earlier rejections is found by iterating prefix -> decisions -> prefix to
 // its fixed point (lane i is final after at most i+1 iterations).
 // out[0] = number of rejections; rej[] = rejected draw indices, ascending.
-__global__ __launch_bounds__(64) void k_resolve(const uint32_t* __restrict__ u,
+__global__ __launch_bounds__(64) void k_resolve(const uint32_t* __restrict__ u, uint64_t count,
                                                 const uint32_t* __restrict__ cand,
-                                                const uint32_t* __restrict__ ncand_p,
+                                                const uint32_t* __restrict__ ncand_p, uint32_t cap,
                                                 uint32_t last_step, uint32_t* __restrict__ rej,
-                                                uint32_t* __restrict__ out) {
-    const uint32_t ncand = *ncand_p;
-    const int lane = __lane_id();
-    const uint64_t lt = __lanemask_lt();
+                                                uint32_t* __restrict__ out, uint32_t* __restrict__ err) {
+    uint32_t ncand = *ncand_p;
+    if (ncand > cap) {
+        if (threadIdx.x == 0) raise_err(err, kErrShuffle);
+        ncand = cap;
+    }
+    const int lane = threadIdx.x;
+    const uint64_t lt = (1ull << lane) - 1ull;
     uint32_t s_in = 0;
     bool finished = false;
     for (uint32_t base = 0; base < ncand && !finished; base += 64) {
         const uint32_t c = base + lane;
         const bool valid = c < ncand;
         const uint32_t t = valid ? cand[c] : 0u;
-        const uint32_t uv = valid ? u[t] : 0u;
+        const bool t_ok = t < count;
+        const uint32_t uv = (valid && t_ok) ? u[t] : 0u;
         uint32_t r = 0;
         uint64_t m = 0;
         bool in_range = false;
-        for (int it = 0; it < 65; ++it) {
+        for (int it = 0; it < 66; ++it) {
             const uint32_t k = t + 1u - (s_in + r);
-            in_range = valid && k <= last_step;
+            in_range = valid && t_ok && k <= last_step;
             bool rj = false;
-            if (in_range) {
-                const uint32_t b = 0x80000000u / (k + 1u);
-                rj = (uv / b) > k;
-            }
+            if (in_range) rj = (uv / (0x80000000u / (k + 1u))) > k;
             m = __ballot(rj);
             const uint32_t rn = (uint32_t)__popcll(m & lt);
             const bool changed = rn != r;
             r = rn;
             if (__ballot(changed) == 0) break;
         }
-        const uint32_t k = t + 1u - (s_in + r);
-        const bool rj = in_range && ((uv / (0x80000000u / (k + 1u))) > k);
-        if (rj) rej[s_in + r] = t;
+        if ((m >> lane) & 1ull) {
+            if (s_in + r < cap) rej[s_in + r] = t;
+            else raise_err(err, kErrShuffle);
+        }
         s_in += (uint32_t)__popcll(m);
         if (__ballot(valid && !in_range) != 0) finished = true;
     }
     if (lane == 0) out[0] = s_in;
 }
 
-__device__ inline uint32_t lower_bound_u32(const uint32_t* a, uint32_t n, uint32_t v) {
-    uint32_t lo = 0, hi = n;
+__device__ inline uint32_t count_below(const uint32_t* a, uint32_t n, uint32_t v) {
+    uint32_t lo = 0, hi = n;  // number of a[i] < v, a ascending
     while (lo < hi) {
         const uint32_t mid = (lo + hi) >> 1;
         if (a[mid] < v) lo = mid + 1;
@@ -224,81 +249,96 @@ __device__ inline uint32_t lower_bound_u32(const uint32_t* a, uint32_t n, uint32
     return lo;
 }
 
-// j[k] for steps 1..last_step from the accepted draws
+// j[k] for steps 1..last_step from the accepted draws; gcount[p] = #{k : j_k = p}
 __global__ __launch_bounds__(256) void k_steps(const uint32_t* __restrict__ u, uint64_t count,
                                                const uint32_t* __restrict__ rej,
-                                               const uint32_t* __restrict__ nrej_p,
-                                               uint32_t last_step, uint32_t* __restrict__ j,
-                                               uint32_t* __restrict__ gcount) {
-    const uint32_t nrej = *nrej_p;
+                                               const uint32_t* __restrict__ nrej_p, uint32_t cap,
+                                               uint32_t last_step, uint32_t K, uint32_t* __restrict__ j,
+                                               uint32_t* __restrict__ gcount, uint32_t* __restrict__ err) {
+    uint32_t nrej = *nrej_p;
+    if (nrej > cap) nrej = cap;  // k_resolve raised the error
     for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < count;
          t += (uint64_t)gridDim.x * blockDim.x) {
-        const uint32_t lb = lower_bound_u32(rej, nrej, (uint32_t)t);
-        if (lb < nrej && rej[lb] == (uint32_t)t) continue;
-        const uint64_t k = t + 1 - lb;
+        const uint32_t before = count_below(rej, nrej, (uint32_t)t);
+        if (before < nrej && rej[before] == (uint32_t)t) continue;  // rejected draw
+        const uint64_t k = t + 1 - before;
         if (k < 1 || k > last_step) continue;
-        const uint32_t b = 0x80000000u / (uint32_t)(k + 1);
-        const uint32_t jk = u[t] / b;
+        const uint32_t jk = u[t] / (0x80000000u / (uint32_t)(k + 1));
+        if (jk > k || k >= K) {
+            raise_err(err, kErrShuffle);
+            continue;
+        }
         j[k] = jk;
         atomicAdd(&gcount[jk], 1u);
     }
 }
 
-// block-level tile scan of a large array: per-block sums, then offsets
-__global__ __launch_bounds__(kScanBlock) void k_tile_sum(const uint32_t* __restrict__ a, uint64_t n,
-                                                         uint32_t* __restrict__ blk) {
-    __shared__ uint32_t s_wave[kScanBlock / 64];
+// per-tile sums of a[0..n) (tiles of kScanTile)
+__global__ __launch_bounds__(kScanThreads) void k_tile_sum(const uint32_t* __restrict__ a, uint64_t n,
+                                                           uint32_t* __restrict__ blk) {
+    __shared__ uint32_t s_wave[kScanThreads / 64];
     const uint64_t base = (uint64_t)blockIdx.x * kScanTile;
     uint32_t c = 0;
     for (int i = 0; i < kScanItems; ++i) {
-        const uint64_t t = base + (uint64_t)i * kScanBlock + threadIdx.x;
+        const uint64_t t = base + (uint64_t)i * kScanThreads + threadIdx.x;
         c += t < n ? a[t] : 0u;
     }
     uint32_t total;
-    (void)block_exclusive_scan(c, s_wave, total);
+    (void)wg_exclusive(c, s_wave, &total);
     if (threadIdx.x == 0) blk[blockIdx.x] = total;
 }
 
-__global__ __launch_bounds__(kScanBlock) void k_tile_scan(const uint32_t* __restrict__ a, uint64_t n,
-                                                          const uint32_t* __restrict__ blk,
-                                                          uint32_t* __restrict__ out) {
-    __shared__ uint32_t s_wave[kScanBlock / 64];
+// out = exclusive prefix of a, given the exclusive prefix of the tile sums
+__global__ __launch_bounds__(kScanThreads) void k_tile_scan(const uint32_t* __restrict__ a, uint64_t n,
+                                                            const uint32_t* __restrict__ blk,
+                                                            uint32_t* __restrict__ out) {
+    __shared__ uint32_t s_wave[kScanThreads / 64];
     const uint64_t base = (uint64_t)blockIdx.x * kScanTile;
     uint32_t off = blk[blockIdx.x];
     for (int i = 0; i < kScanItems; ++i) {
-        const uint64_t t = base + (uint64_t)i * kScanBlock + threadIdx.x;
+        const uint64_t t = base + (uint64_t)i * kScanThreads + threadIdx.x;
         const uint32_t v = t < n ? a[t] : 0u;
         uint32_t total;
-        const uint32_t ex = block_exclusive_scan(v, s_wave, total);
+        const uint32_t ex = wg_exclusive(v, s_wave, &total);
         if (t < n) out[t] = off + ex;
         off += total;
     }
 }
 
-// scatter step k into its group (unordered within the group)
-__global__ __launch_bounds__(256) void k_group_scatter(const uint32_t* __restrict__ j,
-                                                       uint32_t last_step,
-                                                       const uint32_t* __restrict__ gstart,
+// step k into its group's slots (unordered within the group)
+__global__ __launch_bounds__(256) void k_group_scatter(const uint32_t* __restrict__ j, uint32_t last_step,
+                                                       uint32_t K, const uint32_t* __restrict__ gstart,
                                                        uint32_t* __restrict__ gfill,
-                                                       uint32_t* __restrict__ grp) {
+                                                       uint32_t* __restrict__ grp,
+                                                       uint32_t* __restrict__ err) {
     for (uint64_t k = 1 + (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k <= last_step;
          k += (uint64_t)gridDim.x * blockDim.x) {
         const uint32_t p = j[k];
-        grp[gstart[p] + atomicAdd(&gfill[p], 1u)] = (uint32_t)k;
+        if (p >= K) {
+            raise_err(err, kErrShuffle);
+            continue;
+        }
+        const uint64_t slot = (uint64_t)gstart[p] + atomicAdd(&gfill[p], 1u);
+        if (slot < K) grp[slot] = (uint32_t)k;
+        else raise_err(err, kErrShuffle);
     }
 }
 
-// sort each group ascending; F(p) = first member > p; G(member) = next member
+// sorts each group ascending; F(p) = first member > p; G(member) = next member
 __global__ __launch_bounds__(256) void k_group_links(uint32_t K, const uint32_t* __restrict__ gstart,
                                                      const uint32_t* __restrict__ gcount,
-                                                     uint32_t* __restrict__ grp,
-                                                     uint32_t* __restrict__ F,
-                                                     uint32_t* __restrict__ G) {
+                                                     uint32_t* __restrict__ grp, uint32_t* __restrict__ F,
+                                                     uint32_t* __restrict__ G, uint32_t* __restrict__ err) {
     for (uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; p < K;
          p += (uint64_t)gridDim.x * blockDim.x) {
         const uint32_t s = gstart[p], c = gcount[p];
+        if ((uint64_t)s + c > K) {
+            raise_err(err, kErrShuffle);
+            F[p] = kNone;
+            continue;
+        }
         uint32_t* g = grp + s;
-        for (uint32_t i = 1; i < c; ++i) {
+        for (uint32_t i = 1; i < c; ++i) {  // insertion sort; groups hold ~ln(K/p) steps
             const uint32_t v = g[i];
             uint32_t k = i;
             while (k > 0 && g[k - 1] > v) {
@@ -315,13 +355,38 @@ __global__ __launch_bounds__(256) void k_group_links(uint32_t K, const uint32_t*
             }
         }
         F[p] = f;
-        for (uint32_t i = 0; i < c; ++i) G[g[i]] = (i + 1 < c) ? g[i + 1] : kNone;
+        for (uint32_t i = 0; i < c; ++i) {
+            if (g[i] < K) G[g[i]] = (i + 1 < c) ? g[i + 1] : kNone;
+            else raise_err(err, kErrShuffle);
+        }
     }
 }
 
 // ---------------------------------------------------------------------------
-// Per-set part of the shuffle over concatenated sets
+// Records: structure of arrays, two buffers (0 = "A", 1 = "B")
 // ---------------------------------------------------------------------------
+struct Bufs {
+    double* x[2];
+    double* y[2];
+    double* z[2];
+    int32_t* id[2];
+    int64_t n;  // elements in each array
+};
+
+// Buffer selection by a runtime parity is written as selects, not array
+// indexing, so the pointers stay in registers.
+__device__ inline double* bx(const Bufs& b, int par) { return par ? b.x[1] : b.x[0]; }
+__device__ inline double* by(const Bufs& b, int par) { return par ? b.y[1] : b.y[0]; }
+__device__ inline double* bz(const Bufs& b, int par) { return par ? b.z[1] : b.z[0]; }
+__device__ inline int32_t* bid(const Bufs& b, int par) { return par ? b.id[1] : b.id[0]; }
+
+__device__ inline const double* axis_of(const Bufs& b, int par, int axis) {
+    return axis == 0 ? bx(b, par) : (axis == 1 ? by(b, par) : bz(b, par));
+}
+
+// comparator of src/spatial_sort.cpp:33-35: up ? key(b) < key(a) : key(a) < key(b)
+__device__ inline bool key_less(double a, double b, int up) { return up ? (b < a) : (a < b); }
+
 __device__ inline uint32_t set_of(const int64_t* off, uint32_t nsets, uint64_t g) {
     uint32_t lo = 0, hi = nsets;  // last s with off[s] <= g
     while (hi - lo > 1) {
@@ -332,354 +397,595 @@ __device__ inline uint32_t set_of(const int64_t* off, uint32_t nsets, uint64_t g
     return lo;
 }
 
-// nxt[g] = global position of F(p) if it lies inside the set, else g
-__global__ __launch_bounds__(256) void k_chain_init(const int64_t* __restrict__ off, uint32_t nsets,
-                                                    uint64_t total, const uint32_t* __restrict__ F,
-                                                    uint32_t* __restrict__ nxt) {
-    for (uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; g < total;
-         g += (uint64_t)gridDim.x * blockDim.x) {
-        const uint32_t s = set_of(off, nsets, g);
-        const uint64_t b = (uint64_t)off[s], n = (uint64_t)off[s + 1] - b;
-        const uint32_t p = (uint32_t)(g - b);
-        const uint32_t f = F[p];
-        nxt[g] = (f != kNone && f < n) ? (uint32_t)(b + f) : (uint32_t)g;
+// Final position of element k of a set of n >= 2 elements under the shuffle.
+__device__ inline uint64_t shuffle_dest(uint32_t k, uint64_t n, const uint32_t* j, const uint32_t* F,
+                                        const uint32_t* G, uint32_t K, uint32_t* err) {
+    uint32_t start;
+    if (k == 0) {
+        start = F[0];
+        if (start == kNone || start >= n) return 0;
+    } else {
+        const uint32_t gk = G[k];
+        if (gk == kNone || gk >= n) return j[k];
+        start = gk;
     }
-}
-
-__global__ __launch_bounds__(256) void k_jump(const uint32_t* __restrict__ a, uint64_t total,
-                                              uint32_t* __restrict__ b) {
-    for (uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; g < total;
-         g += (uint64_t)gridDim.x * blockDim.x)
-        b[g] = a[a[g]];
-}
-
-// shuffled[final position of k] = order[k]; records are built at once
-struct Rec {
-    double x, y, z;
-    int32_t idx;
-    int32_t pad;
-};
-
-__global__ __launch_bounds__(256) void k_shuffle_place(const int64_t* __restrict__ off,
-                                                       const int64_t* __restrict__ poff,
-                                                       uint32_t nsets, uint64_t total,
-                                                       const uint32_t* __restrict__ j,
-                                                       const uint32_t* __restrict__ F,
-                                                       const uint32_t* __restrict__ G,
-                                                       const uint32_t* __restrict__ term,
-                                                       const int32_t* __restrict__ order,
-                                                       const double* __restrict__ xyz,
-                                                       Rec* __restrict__ A) {
-    for (uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; g < total;
-         g += (uint64_t)gridDim.x * blockDim.x) {
-        const uint32_t s = set_of(off, nsets, g);
-        const uint64_t b = (uint64_t)off[s], n = (uint64_t)off[s + 1] - b;
-        const uint32_t k = (uint32_t)(g - b);
-        uint64_t pos;
-        if (n < 2) {
-            pos = k;
-        } else if (k == 0) {
-            const uint32_t f = F[0];
-            pos = (f != kNone && f < n) ? (uint64_t)term[b + f] - b : 0;
-        } else {
-            const uint32_t gk = G[k];
-            pos = (gk != kNone && gk < n) ? (uint64_t)term[b + gk] - b : (uint64_t)j[k];
+    uint32_t q = start;
+    for (uint64_t hop = 0; hop < n; ++hop) {  // F strictly increases along the chain
+        if (q >= K) {
+            raise_err(err, kErrShuffle);
+            return 0;
         }
-        const int32_t id = order[g];
-        const double* p = xyz + 3 * ((uint64_t)poff[s] + (uint64_t)id);
-        Rec r;
-        r.x = p[0];
-        r.y = p[1];
-        r.z = p[2];
-        r.idx = id;
-        r.pad = 0;
-        A[b + pos] = r;
+        const uint32_t f = F[q];
+        if (f == kNone || f >= n) return q;
+        q = f;
+    }
+    raise_err(err, kErrLoop);
+    return 0;
+}
+
+// Shuffles each set's order[] and gathers the points into buffer 0.
+__global__ __launch_bounds__(256) void k_place(const int64_t* __restrict__ off, const int64_t* __restrict__ poff,
+                                               uint32_t nsets, uint64_t total, int64_t n_pts,
+                                               const uint32_t* __restrict__ j, const uint32_t* __restrict__ F,
+                                               const uint32_t* __restrict__ G, uint32_t K,
+                                               const int32_t* __restrict__ order,
+                                               const double* __restrict__ xyz, Bufs bufs,
+                                               uint32_t* __restrict__ err) {
+    for (uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; g < total;
+         g += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t s = set_of(off, nsets, g);
+        const uint64_t b = (uint64_t)off[s], n = (uint64_t)off[s + 1] - b;
+        const uint64_t k = g - b;
+        if (k >= n || n > K) {
+            raise_err(err, kErrShuffle);
+            continue;
+        }
+        const uint64_t pos = n < 2 ? k : shuffle_dest((uint32_t)k, n, j, F, G, K, err);
+        if (pos >= n) {
+            raise_err(err, kErrShuffle);
+            continue;
+        }
+        const int64_t id = order[g];
+        const int64_t pb = poff[s], pn = poff[s + 1] - pb;
+        const uint64_t dst = b + pos;
+        if (id < 0 || id >= pn || pb + id >= n_pts || dst >= (uint64_t)bufs.n) {
+            raise_err(err, kErrIndex);
+            continue;
+        }
+        const double* p = xyz + 3 * (pb + id);
+        bufs.x[0][dst] = p[0];
+        bufs.y[0][dst] = p[1];
+        bufs.z[0][dst] = p[2];
+        bufs.id[0][dst] = (int32_t)id;
     }
 }
 
 // ---------------------------------------------------------------------------
-// Device-wide Lomuto rounds (ranges longer than kWave)
+// Device-wide Lomuto rounds for selects on more than kFin elements
 // ---------------------------------------------------------------------------
-constexpr int kWave = 1024;          // ranges up to this size are finished by one wavefront
+constexpr int kFin = 4096;          // selects up to this size finish in LDS
 constexpr int kTileThreads = 256;
-constexpr int kTileItems = 16;
-constexpr int kTileElems = kTileThreads * kTileItems;
+constexpr int kTileIters = 16;
+constexpr int kTile = kTileThreads * kTileIters;  // 4096 elements, 64 mask words
+constexpr int kTileWords = kTile / 64;
+constexpr int kRing = 4;            // active-select counters, indexed by round
 
-struct SelTask {
-    int64_t L, R, pv;  // current inclusive range and this round's pivot slot
+struct RTask {           // a big select of one phase (host-built)
+    int64_t b, e;        // original range [b, e)
+    int64_t nth;
+    int64_t tile0, ntiles;
     int32_t axis, up;
-    int64_t nless;     // output of the round
 };
 
-struct SelTile {
-    int32_t task;
-    int32_t len;
-    int64_t start;
-    int64_t first_tile;  // index of the task's first tile (for the per-task offset)
+struct RState {          // per task, double-buffered by round parity
+    int64_t L, R;        // current inclusive range
+    int64_t lim;         // L + number of smaller elements, from k_rd_scan
+    int64_t m0lo, m0hi;  // margins [lo, hi) finalized by the last round
+    int64_t m1lo, m1hi;
+    int32_t par;         // buffer holding [L, R]
+    int32_t active;      // device rounds continue
+    int32_t fin;         // [L, R] is left for the LDS finish
+    int32_t mpar;        // the margins are in buffer 1 and are copied to buffer 0
 };
 
-__device__ inline double rec_key(const Rec& r, int axis) {
-    return axis == 0 ? r.x : (axis == 1 ? r.y : r.z);
+struct RTile {
+    int64_t start, len;
+    int32_t task, pad;
+};
+
+__device__ inline bool rd_range_ok(const RState& S, int64_t n) {
+    return S.L >= 0 && S.L < S.R && S.R < n;
 }
 
-// comparator of src/spatial_sort.cpp:33-35: up ? key(b) < key(a) : key(a) < key(b)
-__device__ inline bool key_less(double a, double b, int up) { return up ? (b < a) : (a < b); }
-
-// the arrangement after the pivot swap of src/spatial_sort.cpp:41
-__device__ inline const Rec& rec_at(const Rec* A, const SelTask& t, int64_t p) {
-    return p == t.pv ? A[t.R] : A[p];
+__device__ inline void copy_rec(const Bufs& b, int from, int to, int64_t src, int64_t dst) {
+    bx(b, to)[dst] = bx(b, from)[src];
+    by(b, to)[dst] = by(b, from)[src];
+    bz(b, to)[dst] = bz(b, from)[src];
+    bid(b, to)[dst] = bid(b, from)[src];
 }
 
-__global__ __launch_bounds__(kTileThreads) void k_sel_count(const Rec* __restrict__ A,
-                                                            const SelTask* __restrict__ tasks,
-                                                            const SelTile* __restrict__ tiles,
-                                                            uint32_t* __restrict__ cnt) {
-    __shared__ uint32_t s_wave[kTileThreads / 64];
-    const SelTile tl = tiles[blockIdx.x];
-    const SelTask t = tasks[tl.task];
-    const double pk = rec_key(A[t.pv], t.axis);
+__global__ __launch_bounds__(256) void k_rd_init(const RTask* __restrict__ tasks, uint32_t ntasks,
+                                                 RState* __restrict__ st) {
+    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < ntasks; t += gridDim.x * blockDim.x) {
+        const RTask T = tasks[t];
+        RState S;
+        S.L = T.b;
+        S.R = T.e - 1;
+        S.lim = T.b;
+        S.m0lo = S.m0hi = S.m1lo = S.m1hi = 0;
+        S.par = 0;
+        S.active = 1;
+        S.fin = 0;
+        S.mpar = 0;
+        st[t] = S;
+    }
+}
+
+// Round part 1 (per tile): copies the margins the previous round left in
+// buffer 1, then flags the tile's elements that are less than the pivot
+// (one bit each) and counts them.
+__global__ __launch_bounds__(kTileThreads) void k_rd_count(Bufs bufs, const RTask* __restrict__ tasks,
+                                                           uint32_t ntasks, const RTile* __restrict__ tiles,
+                                                           uint32_t ntiles, const RState* __restrict__ st,
+                                                           uint64_t* __restrict__ mask,
+                                                           uint32_t* __restrict__ tcount,
+                                                           uint32_t* __restrict__ ring, uint32_t round,
+                                                           uint32_t* __restrict__ err) {
+    __shared__ uint32_t s_cnt[kTileThreads / 64];
+    const uint32_t tile = blockIdx.x;
+    if (tile == 0 && threadIdx.x == 0) ring[(round + 1) % kRing] = 0;
+    if (tile >= ntiles) return;
+    const RTile tl = tiles[tile];
+    if (tl.task < 0 || (uint32_t)tl.task >= ntasks || tl.len < 0 || tl.len > kTile || tl.start < 0 ||
+        tl.start + tl.len > bufs.n) {
+        if (threadIdx.x == 0) raise_err(err, kErrRound);
+        return;
+    }
+    const RTask T = tasks[tl.task];
+    const RState S = st[tl.task];
+    const int64_t t0 = tl.start, t1 = tl.start + tl.len;
+    if (S.mpar) {
+        for (int m = 0; m < 2; ++m) {
+            const int64_t mlo = m ? S.m1lo : S.m0lo, mhi = m ? S.m1hi : S.m0hi;
+            const int64_t lo = mlo > t0 ? mlo : t0, hi = mhi < t1 ? mhi : t1;
+            if (lo < hi && (lo < 0 || hi > bufs.n)) {
+                if (threadIdx.x == 0) raise_err(err, kErrRound);
+                continue;
+            }
+            for (int64_t p = lo + threadIdx.x; p < hi; p += kTileThreads) copy_rec(bufs, 1, 0, p, p);
+        }
+    }
+    if (!S.active) {
+        if (threadIdx.x == 0) tcount[tile] = 0;
+        return;
+    }
+    if (!rd_range_ok(S, bufs.n) || (S.par & ~1)) {
+        if (threadIdx.x == 0) raise_err(err, kErrRound);
+        if (threadIdx.x == 0) tcount[tile] = 0;
+        return;
+    }
+    const double* key = axis_of(bufs, S.par, T.axis);
+    const int64_t pv = S.L + (S.R - S.L) / 2;
+    const double pk = key[pv];
+    const double kr = key[S.R];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     uint32_t c = 0;
-    for (int i = 0; i < kTileItems; ++i) {
-        const int o = i * kTileThreads + threadIdx.x;
-        if (o < tl.len) c += key_less(rec_key(rec_at(A, t, tl.start + o), t.axis), pk, t.up) ? 1u : 0u;
-    }
-    uint32_t total;
-    (void)block_exclusive_scan(c, s_wave, total);
-    if (threadIdx.x == 0) cnt[blockIdx.x] = total;
-}
-
-// per task: exclusive offsets over its tiles, nless
-__global__ __launch_bounds__(256) void k_sel_offsets(SelTask* __restrict__ tasks, uint32_t ntasks,
-                                                     const int64_t* __restrict__ task_tile0,
-                                                     const uint32_t* __restrict__ cnt,
-                                                     uint32_t* __restrict__ toff) {
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < ntasks; i += gridDim.x * blockDim.x) {
-        int64_t acc = 0;
-        for (int64_t k = task_tile0[i]; k < task_tile0[i + 1]; ++k) {
-            toff[k] = (uint32_t)acc;
-            acc += cnt[k];
-        }
-        tasks[i].nless = acc;
-    }
-}
-
-__global__ __launch_bounds__(kTileThreads) void k_sel_flagged(const Rec* __restrict__ A,
-                                                              const SelTask* __restrict__ tasks,
-                                                              const SelTile* __restrict__ tiles,
-                                                              const uint32_t* __restrict__ toff,
-                                                              int64_t* __restrict__ Lpos,
-                                                              Rec* __restrict__ B) {
-    __shared__ uint32_t s_wave[kTileThreads / 64];
-    const SelTile tl = tiles[blockIdx.x];
-    const SelTask t = tasks[tl.task];
-    const double pk = rec_key(A[t.pv], t.axis);
-    int64_t rank = toff[blockIdx.x];
-    for (int i = 0; i < kTileItems; ++i) {
-        const int o = i * kTileThreads + threadIdx.x;
-        const int64_t p = tl.start + o;
+    for (int it = 0; it < kTileIters; ++it) {
+        const int w = it * (kTileThreads / 64) + wave;
+        const int64_t p = t0 + (int64_t)w * 64 + lane;
         bool f = false;
-        Rec r;
-        if (o < tl.len) {
-            r = rec_at(A, t, p);
-            f = key_less(rec_key(r, t.axis), pk, t.up);
-        }
+        if (p < t1 && p >= S.L && p < S.R) f = key_less(p == pv ? kr : key[p], pk, T.up);
+        const uint64_t m = __ballot(f);
+        if (lane == 0) mask[(uint64_t)tile * kTileWords + w] = m;
+        c += (uint32_t)__popcll(m);
+    }
+    if (lane == 0) s_cnt[wave] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t s = 0;
+        for (int w = 0; w < kTileThreads / 64; ++w) s += s_cnt[w];
+        tcount[tile] = s;
+    }
+}
+
+// Round part 2 (per task): exclusive prefix of the task's tile counts, lim.
+__global__ __launch_bounds__(256) void k_rd_scan(const RTask* __restrict__ tasks, uint32_t ntasks,
+                                                 uint32_t ntiles, RState* __restrict__ st,
+                                                 const uint32_t* __restrict__ tcount,
+                                                 uint32_t* __restrict__ toff, uint32_t* __restrict__ err) {
+    __shared__ uint32_t s_wave[4];
+    const uint32_t t = blockIdx.x;
+    if (t >= ntasks) return;
+    const RTask T = tasks[t];
+    const RState S = st[t];
+    if (!S.active) return;
+    if (T.tile0 < 0 || T.ntiles < 0 || T.tile0 + T.ntiles > (int64_t)ntiles) {
+        if (threadIdx.x == 0) raise_err(err, kErrRound);
+        return;
+    }
+    uint32_t carry = 0;
+    for (int64_t base = 0; base < T.ntiles; base += blockDim.x) {
+        const int64_t i = base + threadIdx.x;
+        const uint32_t v = i < T.ntiles ? tcount[T.tile0 + i] : 0u;
         uint32_t total;
-        const uint32_t ex = block_exclusive_scan(f ? 1u : 0u, s_wave, total);
-        if (f) {
-            Lpos[t.L + rank + ex] = p;
-            B[t.L + rank + ex] = r;
+        const uint32_t ex = wg_exclusive(v, s_wave, &total);
+        if (i < T.ntiles) toff[T.tile0 + i] = carry + ex;
+        carry += total;
+    }
+    if (threadIdx.x == 0) st[t].lim = S.L + (int64_t)carry;
+}
+
+// Round part 3 (per tile): elements less than the pivot go, in order, to
+// L + rank of the other buffer; Lpos[L + rank] = their position - L.
+__global__ __launch_bounds__(kTileThreads) void k_rd_flagged(Bufs bufs, const RTask* __restrict__ tasks,
+                                                             uint32_t ntasks, const RTile* __restrict__ tiles,
+                                                             uint32_t ntiles, const RState* __restrict__ st,
+                                                             const uint64_t* __restrict__ mask,
+                                                             const uint32_t* __restrict__ toff,
+                                                             uint32_t* __restrict__ lpos,
+                                                             uint32_t* __restrict__ err) {
+    __shared__ uint32_t s_pre[kTileWords];
+    const uint32_t tile = blockIdx.x;
+    if (tile >= ntiles) return;
+    const RTile tl = tiles[tile];
+    if (tl.task < 0 || (uint32_t)tl.task >= ntasks) return;  // k_rd_count raised it
+    const RState S = st[tl.task];
+    if (!S.active || !rd_range_ok(S, bufs.n) || (S.par & ~1)) return;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (wave == 0) {  // exclusive prefix of the 64 words' popcounts
+        const uint32_t c = (uint32_t)__popcll(mask[(uint64_t)tile * kTileWords + lane]);
+        uint32_t x = c;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const uint32_t y = __shfl_up(x, d);
+            if (lane >= d) x += y;
         }
-        rank += total;
+        s_pre[lane] = x - c;
+    }
+    __syncthreads();
+    const int src = S.par, dst = S.par ^ 1;
+    const int64_t pv = S.L + (S.R - S.L) / 2;
+    const int64_t base = (int64_t)toff[tile];
+    const uint64_t lt = (1ull << lane) - 1ull;
+    for (int it = 0; it < kTileIters; ++it) {
+        const int w = it * (kTileThreads / 64) + wave;
+        const uint64_t m = mask[(uint64_t)tile * kTileWords + w];
+        if (!((m >> lane) & 1ull)) continue;
+        const int64_t p = tl.start + (int64_t)w * 64 + lane;
+        const int64_t d = S.L + base + s_pre[w] + __popcll(m & lt);
+        if (d < S.L || d >= S.lim || S.lim > S.R || p < S.L || p >= S.R) {
+            raise_err(err, kErrRound);
+            continue;
+        }
+        copy_rec(bufs, src, dst, p == pv ? S.R : p, d);
+        lpos[d] = (uint32_t)(p - S.L);
     }
 }
 
-__global__ __launch_bounds__(kTileThreads) void k_sel_unflagged(const Rec* __restrict__ A,
-                                                                const SelTask* __restrict__ tasks,
-                                                                const SelTile* __restrict__ tiles,
-                                                                const int64_t* __restrict__ Lpos,
-                                                                Rec* __restrict__ B) {
-    const SelTile tl = tiles[blockIdx.x];
-    const SelTask t = tasks[tl.task];
-    const double pk = rec_key(A[t.pv], t.axis);
-    const int64_t lim = t.L + t.nless;
-    for (int i = 0; i < kTileItems; ++i) {
-        const int o = i * kTileThreads + threadIdx.x;
-        if (o >= tl.len) continue;
-        const int64_t p = tl.start + o;
-        const Rec r = rec_at(A, t, p);
-        if (key_less(rec_key(r, t.axis), pk, t.up)) continue;
-        int64_t q = p;
-        while (q < lim) q = Lpos[q];
-        B[q == lim ? t.R : q] = r;
+// Round part 4 (per tile): the other elements follow their Lpos chains; the
+// pivot goes to lim; the task's first tile writes the next round's state.
+__global__ __launch_bounds__(kTileThreads) void k_rd_rest(Bufs bufs, const RTask* __restrict__ tasks,
+                                                          uint32_t ntasks, const RTile* __restrict__ tiles,
+                                                          uint32_t ntiles, const RState* __restrict__ st,
+                                                          RState* __restrict__ st_next,
+                                                          const uint64_t* __restrict__ mask,
+                                                          const uint32_t* __restrict__ lpos,
+                                                          uint32_t* __restrict__ ring, uint32_t round,
+                                                          uint32_t* __restrict__ err) {
+    const uint32_t tile = blockIdx.x;
+    if (tile >= ntiles) return;
+    const RTile tl = tiles[tile];
+    if (tl.task < 0 || (uint32_t)tl.task >= ntasks) return;
+    const RTask T = tasks[tl.task];
+    const RState S = st[tl.task];
+    const bool first = (int64_t)tile == T.tile0;
+    const bool ok = S.active && rd_range_ok(S, bufs.n) && !(S.par & ~1) && S.lim >= S.L && S.lim <= S.R;
+    if (S.active && !ok && first && threadIdx.x == 0) raise_err(err, kErrRound);
+    const int64_t pv = S.L + (S.R - S.L) / 2;
+    if (ok) {
+        const int src = S.par, dst = S.par ^ 1;
+        const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+        const int64_t span = S.R - S.L;
+        for (int it = 0; it < kTileIters; ++it) {
+            const int w = it * (kTileThreads / 64) + wave;
+            const int64_t p = tl.start + (int64_t)w * 64 + lane;
+            if (p >= tl.start + tl.len || p < S.L || p >= S.R) continue;
+            if ((mask[(uint64_t)tile * kTileWords + w] >> lane) & 1ull) continue;
+            int64_t q = p;
+            int64_t hops = 0;
+            while (q < S.lim && hops <= span) {  // q strictly increases along the chain
+                q = S.L + (int64_t)lpos[q];
+                ++hops;
+            }
+            const int64_t d = q == S.lim ? S.R : q;
+            if (hops > span || d <= S.lim || d > S.R) {
+                raise_err(err, hops > span ? kErrLoop : kErrRound);
+                continue;
+            }
+            copy_rec(bufs, src, dst, p == pv ? S.R : p, d);
+        }
+        if (first && threadIdx.x == 0) copy_rec(bufs, src, dst, pv, S.lim);  // the pivot
     }
-    if (tl.start == t.L && threadIdx.x == 0) B[lim] = A[t.pv];  // the pivot
-}
-
-__global__ __launch_bounds__(kTileThreads) void k_sel_copy(Rec* __restrict__ A,
-                                                           const SelTask* __restrict__ tasks,
-                                                           const SelTile* __restrict__ tiles,
-                                                           const Rec* __restrict__ B) {
-    const SelTile tl = tiles[blockIdx.x];
-    const SelTask t = tasks[tl.task];
-    for (int i = 0; i < kTileItems; ++i) {
-        const int o = i * kTileThreads + threadIdx.x;
-        if (o < tl.len) A[tl.start + o] = B[tl.start + o];
+    if (!(first && threadIdx.x == 0)) return;
+    RState N = S;
+    N.mpar = 0;
+    N.m0lo = N.m0hi = N.m1lo = N.m1hi = 0;
+    if (ok) {
+        const int dst = S.par ^ 1;
+        const int64_t pos = S.lim;
+        N.par = dst;
+        N.mpar = dst;  // margins are final in the buffer just written
+        N.active = 0;
+        N.fin = 0;
+        if (pos == T.nth) {
+            N.m0lo = S.L;
+            N.m0hi = S.R + 1;
+            N.L = N.R = pos;
+        } else {
+            int64_t L = S.L, R = S.R;
+            if (T.nth < pos) R = pos - 1;
+            else L = pos + 1;
+            N.m0lo = S.L;
+            N.m0hi = L;
+            N.m1lo = R + 1;
+            N.m1hi = S.R + 1;
+            N.L = L;
+            N.R = R;
+            const int64_t size = R - L + 1;
+            if (size > kFin) {
+                N.active = 1;
+                atomicAdd(&ring[(round + 1) % kRing], 1u);
+            } else if (size >= 2) {
+                N.fin = 1;
+            }
+            // size 1: the element at nth is in place; its slot is in the
+            // new range, which is still in buffer dst: copy it as a margin too
+            if (size == 1) {
+                N.m0lo = S.L;
+                N.m0hi = S.R + 1;
+                N.m1lo = N.m1hi = 0;
+            }
+        }
+    } else if (S.active) {
+        N.active = 0;  // an invalid state stops (the error is raised)
     }
-    if (tl.start == t.L && threadIdx.x == 0) A[t.R] = B[t.R];
+    st_next[tl.task] = N;
 }
 
 // ---------------------------------------------------------------------------
-// One wavefront per range: quickselect and whole Hilbert subtrees in LDS
+// LDS passes: Lomuto rounds over local slots, keys read through loc[]
 // ---------------------------------------------------------------------------
-struct WaveTask {
-    int64_t L, R, nth;  // inclusive range and target (quickselect finish)
-    int32_t axis, up;
-};
-
-struct LeafNode {
-    int64_t lo, hi;  // [lo, hi)
-    int32_t x, ux, uy, uz;
-};
-
-// One Lomuto round over slots [l, r] (local) of loc[], comparing key[loc[.]],
-// pivot at slot pv. Returns the pivot's final slot. 64 threads (one wavefront).
-__device__ inline int wave_partition(const double* key, uint16_t* loc, uint16_t* lp, uint16_t* tmp,
-                                     int l, int r, int pv, int up) {
-    const int lane = threadIdx.x;
-    const uint64_t lt = __lanemask_lt();
-    if (lane == 0) {
+// One round over local slots [l, r] with pivot slot pv, by a workgroup of NW
+// wavefronts; returns the pivot's final slot. loc[] holds the arrangement
+// (indices into key[]); lp[] and tmp[] are scratch of the same size.
+template <int NW>
+__device__ int lds_partition(const double* key, uint16_t* loc, uint16_t* lp, uint16_t* tmp,
+                             uint32_t* s_cnt, int l, int r, int pv, int up, uint32_t* err) {
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const uint64_t lt = (1ull << lane) - 1ull;
+    if (tid == 0) {
         const uint16_t v = loc[pv];
         loc[pv] = loc[r];
         loc[r] = v;
     }
     __syncthreads();
     const double pk = key[loc[r]];
-    int nless = 0;
-    for (int c = l; c < r; c += 64) {
+    const int span = r - l;
+    const int cw = (span + NW - 1) / NW;
+    const int wb = l + w * cw;
+    const int we = wb + cw < r ? wb + cw : r;
+    int cnt = 0;
+    for (int c = wb; c < we; c += 64) {
         const int p = c + lane;
-        const bool f = p < r && key_less(key[loc[p]], pk, up);
-        const uint64_t m = __ballot(f);
-        if (f) lp[nless + __popcll(m & lt)] = (uint16_t)p;
-        nless += __popcll(m);
+        const bool f = p < we && key_less(key[loc[p]], pk, up);
+        cnt += __popcll(__ballot(f));
     }
+    if (lane == 0) s_cnt[w] = (uint32_t)cnt;
     __syncthreads();
+    int before = 0, nless = 0;
+    for (int i = 0; i < NW; ++i) {
+        const int v = (int)s_cnt[i];
+        before += i < w ? v : 0;
+        nless += v;
+    }
     const int lim = l + nless;
-    int rank = 0;
-    for (int c = l; c < r; c += 64) {
+    int rank = before;
+    for (int c = wb; c < we; c += 64) {
         const int p = c + lane;
-        const bool valid = p < r;
-        const bool f = valid && key_less(key[loc[p]], pk, up);
+        const bool f = p < we && key_less(key[loc[p]], pk, up);
         const uint64_t m = __ballot(f);
-        if (valid) {
-            int dest;
-            if (f) {
-                dest = l + rank + __popcll(m & lt);
-            } else {
-                int q = p;
-                while (q < lim) q = lp[q - l];
-                dest = (q == lim) ? r : q;
-            }
-            tmp[dest] = loc[p];
+        if (f) {
+            const int rk = rank + __popcll(m & lt);
+            lp[rk] = (uint16_t)p;
+            tmp[l + rk] = loc[p];
         }
         rank += __popcll(m);
     }
-    if (lane == 0) tmp[lim] = loc[r];
     __syncthreads();
-    for (int p = l + lane; p <= r; p += 64) loc[p] = tmp[p];
+    for (int c = wb; c < we; c += 64) {
+        const int p = c + lane;
+        if (p >= we || key_less(key[loc[p]], pk, up)) continue;
+        int q = p, hops = 0;
+        while (q < lim && hops <= span) {
+            q = lp[q - l];
+            ++hops;
+        }
+        if (hops > span || (q != lim && (q < lim || q > r))) {
+            raise_err(err, kErrLds);
+            continue;
+        }
+        tmp[q == lim ? r : q] = loc[p];
+    }
+    if (tid == 0) tmp[lim] = loc[r];
+    __syncthreads();
+    for (int p = l + tid; p <= r; p += NW * 64) loc[p] = tmp[p];
     __syncthreads();
     return lim;
 }
 
 // quickselect of src/spatial_sort.cpp:60-71 over local slots [l, r]
-__device__ inline void wave_select(const double* key, uint16_t* loc, uint16_t* lp, uint16_t* tmp,
-                                   int l, int r, int nth, int up) {
-    for (;;) {
+template <int NW>
+__device__ void lds_select(const double* key, uint16_t* loc, uint16_t* lp, uint16_t* tmp, uint32_t* s_cnt,
+                           int l, int r, int nth, int up, uint32_t* err) {
+    const int bound = r - l + 1;  // every round removes at least the pivot's slot
+    for (int round = 0; round <= bound; ++round) {
         const int pv = l + (r - l) / 2;
-        const int pos = wave_partition(key, loc, lp, tmp, l, r, pv, up);
+        const int pos = lds_partition<NW>(key, loc, lp, tmp, s_cnt, l, r, pv, up, err);
         if (pos == nth) return;
         if (nth < pos) r = pos - 1;
         else l = pos + 1;
+        if (l > r) break;
     }
+    if (threadIdx.x == 0) raise_err(err, kErrLoop);
 }
 
 // hilbert_split of src/spatial_sort.cpp:74-79 on local slots [b, e)
-__device__ inline int wave_split(const double* key, uint16_t* loc, uint16_t* lp, uint16_t* tmp,
-                                 int b, int e, int up) {
+template <int NW>
+__device__ int lds_split(const double* key, uint16_t* loc, uint16_t* lp, uint16_t* tmp, uint32_t* s_cnt,
+                         int b, int e, int up, uint32_t* err) {
     if (b >= e) return b;
     const int mid = b + (e - b) / 2;
-    if (e - b >= 2) wave_select(key, loc, lp, tmp, b, e - 1, mid, up);
+    if (e - b >= 2) lds_select<NW>(key, loc, lp, tmp, s_cnt, b, e - 1, mid, up, err);
     return mid;
 }
 
-__global__ __launch_bounds__(64) void k_wave_select(Rec* __restrict__ A, Rec* __restrict__ B,
-                                                    const WaveTask* __restrict__ tasks,
-                                                    uint32_t ntasks) {
-    __shared__ double s_key[kWave];
-    __shared__ uint16_t s_loc[kWave], s_lp[kWave], s_tmp[kWave];
-    for (uint32_t ti = blockIdx.x; ti < ntasks; ti += gridDim.x) {
-        const WaveTask t = tasks[ti];
-        const int m = (int)(t.R - t.L + 1);
-        __syncthreads();
-        for (int i = threadIdx.x; i < m; i += 64) {
-            s_key[i] = rec_key(A[t.L + i], t.axis);
-            s_loc[i] = (uint16_t)i;
+struct FTask {           // a select finished in LDS
+    int64_t L, R, nth;   // (small tasks; big ones read L, R from their state)
+    int32_t axis, up;
+    int32_t big;         // index of the big task's state, or -1
+    int32_t pad;
+};
+
+// One workgroup per select: keys of [L, R] into LDS, the quickselect, then
+// every field of the records permuted into buffer 0.
+__global__ __launch_bounds__(256) void k_lds_select(Bufs bufs, const FTask* __restrict__ ft, uint32_t nft,
+                                                    const RState* __restrict__ st_big, uint32_t nbig,
+                                                    uint32_t* __restrict__ err) {
+    __shared__ double s_key[kFin];
+    __shared__ uint16_t s_loc[kFin], s_lp[kFin], s_tmp[kFin];
+    __shared__ uint32_t s_cnt[4];
+    const uint32_t ti = blockIdx.x;
+    if (ti >= nft) return;
+    const FTask T = ft[ti];
+    int64_t L = T.L, R = T.R;
+    int par = 0;
+    if (T.big >= 0) {
+        if ((uint32_t)T.big >= nbig) {
+            if (threadIdx.x == 0) raise_err(err, kErrLds);
+            return;
         }
+        const RState S = st_big[T.big];
+        if (!S.fin) return;
+        L = S.L;
+        R = S.R;
+        par = S.par;
+    }
+    const int64_t m = R - L + 1;
+    if (m < 2 || m > kFin || L < 0 || R >= bufs.n || T.nth < L || T.nth > R || (par & ~1)) {
+        if (threadIdx.x == 0) raise_err(err, kErrLds);
+        return;
+    }
+    const int mi = (int)m;
+    const double* key = axis_of(bufs, par, T.axis);
+    for (int i = threadIdx.x; i < mi; i += 256) {
+        s_key[i] = key[L + i];
+        s_loc[i] = (uint16_t)i;
+    }
+    __syncthreads();
+    lds_select<4>(s_key, s_loc, s_lp, s_tmp, s_cnt, 0, mi - 1, (int)(T.nth - L), T.up, err);
+    // apply the arrangement to every field (staged through s_key: the source
+    // may be buffer 0 itself)
+    for (int f = 0; f < 3; ++f) {
+        const double* src = axis_of(bufs, par, f);
+        double* dst = f == 0 ? bufs.x[0] : (f == 1 ? bufs.y[0] : bufs.z[0]);
         __syncthreads();
-        wave_select(s_key, s_loc, s_lp, s_tmp, 0, m - 1, (int)(t.nth - t.L), t.up);
-        for (int i = threadIdx.x; i < m; i += 64) B[t.L + i] = A[t.L + s_loc[i]];
+        for (int i = threadIdx.x; i < mi; i += 256) s_key[i] = src[L + i];
         __syncthreads();
-        for (int i = threadIdx.x; i < m; i += 64) A[t.L + i] = B[t.L + i];
+        for (int i = threadIdx.x; i < mi; i += 256) {
+            const int k = s_loc[i];
+            if (k < mi) dst[L + i] = s_key[k];
+        }
+    }
+    int32_t* s_id = reinterpret_cast<int32_t*>(s_key);
+    __syncthreads();
+    const int32_t* src_id = bid(bufs, par);
+    for (int i = threadIdx.x; i < mi; i += 256) s_id[i] = src_id[L + i];
+    __syncthreads();
+    for (int i = threadIdx.x; i < mi; i += 256) {
+        const int k = s_loc[i];
+        if (k < mi) bufs.id[0][L + i] = s_id[k];
     }
 }
 
-__global__ __launch_bounds__(64) void k_wave_subtree(const Rec* __restrict__ A,
-                                                     const LeafNode* __restrict__ nodes,
-                                                     uint32_t nnodes, int32_t* __restrict__ order_out) {
-    __shared__ double s_key[3][kWave];
-    __shared__ uint16_t s_loc[kWave], s_lp[kWave], s_tmp[kWave];
-    __shared__ int32_t s_stack[64][6];
-    for (uint32_t ni = blockIdx.x; ni < nnodes; ni += gridDim.x) {
-        const LeafNode nd = nodes[ni];
-        const int m = (int)(nd.hi - nd.lo);
+// ---------------------------------------------------------------------------
+// Whole Hilbert subtrees of at most kSub elements, one wavefront each
+// ---------------------------------------------------------------------------
+constexpr int kSub = 1024;
+constexpr int kStack = 64;
+
+struct Leaf {
+    int64_t lo, hi;  // [lo, hi)
+    int32_t x, ux, uy, uz;
+};
+
+__global__ __launch_bounds__(64) void k_subtree(Bufs bufs, const Leaf* __restrict__ leaves, uint32_t nleaves,
+                                                int32_t* __restrict__ order, uint32_t* __restrict__ err) {
+    __shared__ double s_key[3][kSub];
+    __shared__ uint16_t s_loc[kSub], s_lp[kSub], s_tmp[kSub];
+    __shared__ int32_t s_stack[kStack][6];
+    __shared__ uint32_t s_cnt[1];
+    for (uint32_t li = blockIdx.x; li < nleaves; li += gridDim.x) {
+        const Leaf nd = leaves[li];
+        const int64_t m = nd.hi - nd.lo;
+        if (m < 1 || m > kSub || nd.lo < 0 || nd.hi > bufs.n) {
+            if (threadIdx.x == 0) raise_err(err, kErrLds);
+            continue;
+        }
+        const int mi = (int)m;
         __syncthreads();
-        for (int i = threadIdx.x; i < m; i += 64) {
-            const Rec r = A[nd.lo + i];
-            s_key[0][i] = r.x;
-            s_key[1][i] = r.y;
-            s_key[2][i] = r.z;
+        for (int i = threadIdx.x; i < mi; i += 64) {
+            s_key[0][i] = bufs.x[0][nd.lo + i];
+            s_key[1][i] = bufs.y[0][nd.lo + i];
+            s_key[2][i] = bufs.z[0][nd.lo + i];
             s_loc[i] = (uint16_t)i;
         }
         if (threadIdx.x == 0) {
             s_stack[0][0] = 0;
-            s_stack[0][1] = m;
+            s_stack[0][1] = mi;
             s_stack[0][2] = nd.x;
             s_stack[0][3] = nd.ux;
             s_stack[0][4] = nd.uy;
             s_stack[0][5] = nd.uz;
         }
         __syncthreads();
-        int sp = 1;  // uniform across the wavefront
-        while (sp > 0) {
+        int sp = 1;  // the same in every lane
+        for (int guard = 0; sp > 0 && guard <= mi; ++guard) {  // at most one pop per node
             --sp;
             const int b = s_stack[sp][0], e = s_stack[sp][1];
             const int x = s_stack[sp][2], ux = s_stack[sp][3], uy = s_stack[sp][4], uz = s_stack[sp][5];
             __syncthreads();
-            if (e - b <= 8) continue;
+            if (e - b <= 8 || b < 0 || e > mi || x < 0 || x > 2) continue;
             const int y = (x + 1) % 3, z = (x + 2) % 3;
             const int m0 = b, m8 = e;
-            const int m4 = wave_split(s_key[x], s_loc, s_lp, s_tmp, m0, m8, ux);
-            const int m2 = wave_split(s_key[y], s_loc, s_lp, s_tmp, m0, m4, uy);
-            const int m1 = wave_split(s_key[z], s_loc, s_lp, s_tmp, m0, m2, uz);
-            const int m3 = wave_split(s_key[z], s_loc, s_lp, s_tmp, m2, m4, !uz);
-            const int m6 = wave_split(s_key[y], s_loc, s_lp, s_tmp, m4, m8, !uy);
-            const int m5 = wave_split(s_key[z], s_loc, s_lp, s_tmp, m4, m6, uz);
-            const int m7 = wave_split(s_key[z], s_loc, s_lp, s_tmp, m6, m8, !uz);
-            // children of src/spatial_sort.cpp:109-116 (order of processing is
-            // irrelevant: the ranges are disjoint)
+            const int m4 = lds_split<1>(s_key[x], s_loc, s_lp, s_tmp, s_cnt, m0, m8, ux, err);
+            const int m2 = lds_split<1>(s_key[y], s_loc, s_lp, s_tmp, s_cnt, m0, m4, uy, err);
+            const int m1 = lds_split<1>(s_key[z], s_loc, s_lp, s_tmp, s_cnt, m0, m2, uz, err);
+            const int m3 = lds_split<1>(s_key[z], s_loc, s_lp, s_tmp, s_cnt, m2, m4, !uz, err);
+            const int m6 = lds_split<1>(s_key[y], s_loc, s_lp, s_tmp, s_cnt, m4, m8, !uy, err);
+            const int m5 = lds_split<1>(s_key[z], s_loc, s_lp, s_tmp, s_cnt, m4, m6, uz, err);
+            const int m7 = lds_split<1>(s_key[z], s_loc, s_lp, s_tmp, s_cnt, m6, m8, !uz, err);
+            // the recursion of src/spatial_sort.cpp:109-116 (disjoint ranges,
+            // so the order in which they are processed does not matter)
             const int ch[8][6] = {{m0, m1, z, uz, ux, uy},   {m1, m2, y, uy, uz, ux},
                                   {m2, m3, y, uy, uz, ux},   {m3, m4, x, ux, !uy, !uz},
                                   {m4, m5, x, ux, !uy, !uz}, {m5, m6, y, !uy, uz, !ux},
                                   {m6, m7, y, !uy, uz, !ux}, {m7, m8, z, !uz, !ux, uy}};
+            int add = 0;
+            for (int c = 0; c < 8; ++c) add += (ch[c][1] - ch[c][0] > 8) ? 1 : 0;
+            if (sp + add > kStack) {
+                if (threadIdx.x == 0) raise_err(err, kErrLds);
+                break;
+            }
             if (threadIdx.x == 0) {
                 int w = sp;
                 for (int c = 0; c < 8; ++c) {
@@ -688,70 +994,74 @@ __global__ __launch_bounds__(64) void k_wave_subtree(const Rec* __restrict__ A,
                     ++w;
                 }
             }
-            int add = 0;
-            for (int c = 0; c < 8; ++c) add += (ch[c][1] - ch[c][0] > 8) ? 1 : 0;
             sp += add;
             __syncthreads();
         }
-        for (int i = threadIdx.x; i < m; i += 64) order_out[nd.lo + i] = A[nd.lo + s_loc[i]].idx;
+        if (sp > 0 && threadIdx.x == 0) raise_err(err, kErrLoop);
+        __syncthreads();
+        for (int i = threadIdx.x; i < mi; i += 64) {
+            const int k = s_loc[i];
+            if (k < mi) order[nd.lo + i] = bufs.id[0][nd.lo + k];
+        }
     }
 }
 
-// ---------------------------------------------------------------------------
-// Host driver
-// ---------------------------------------------------------------------------
-struct Node {
-    int64_t lo, hi;
-    int x, ux, uy, uz;
-};
+__global__ __launch_bounds__(256) void k_write_order(const int32_t* __restrict__ id, uint64_t n,
+                                                     int32_t* __restrict__ order) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (uint64_t)gridDim.x * blockDim.x)
+        order[i] = id[i];
+}
 
-static inline int64_t mid_of(int64_t b, int64_t e) { return b >= e ? b : b + (e - b) / 2; }
-
-static int grid_for(uint64_t work, int per_block, int cus) {
+// ---------------------------------------------------------------------------
+// Host side
+// ---------------------------------------------------------------------------
+int grid_for(uint64_t work, int per_block, int cus) {
     const uint64_t blocks = (work + per_block - 1) / per_block;
     return (int)std::max<uint64_t>(1, std::min<uint64_t>(blocks, (uint64_t)cus * 32));
 }
 
-// Universal shuffle tables for steps 1..K-1: j, F, G (device arrays of K).
-struct ShuffleTables {
+struct Shuffle {
     uint32_t *j = nullptr, *F = nullptr, *G = nullptr;
 };
 
-static ShuffleTables build_shuffle_tables(uint64_t K, hipStream_t s, Scratch& sc) {
+// The universal shuffle tables for steps 1..K-1 (device arrays of K).
+Shuffle build_shuffle(uint64_t K, hipStream_t s, Scratch& sc, uint32_t* err) {
     const int cus = device_cu_count();
     const uint32_t last_step = (uint32_t)(K - 1);
-    // expected rejections ~ K^2 / 2^33; generous margin, re-run if exceeded
+    // expected rejections ~ K^2 / 2^33; a generous margin, redone if short
     const double er = (double)K * (double)K / 8589934592.0;
     uint64_t D = K - 1 + (uint64_t)(er + 12.0 * std::sqrt(er + 1.0)) + 256;
-    const uint64_t x0 = ((1ull << 16) | 0x330Eull) & kMask48;  // seed 1
-    for (;;) {
+    const uint64_t x0 = ((1ull << 16) | 0x330Eull) & kMask48;  // seed 1 (src/rand48.h:30-33)
+    for (int attempt = 0; attempt < 8; ++attempt) {
         uint32_t* u = sc.get<uint32_t>(D);
-        const uint64_t nthreads = (D + kDrawsPerThread - 1) / kDrawsPerThread;
-        k_draws<<<(unsigned)((nthreads + 255) / 256), 256, 0, s>>>(u, D, x0);
+        const uint64_t runs = (D + kDrawRun - 1) / kDrawRun;
+        k_draws<<<(unsigned)((runs + 255) / 256), 256, 0, s>>>(u, D, x0);
         D3_LAUNCH_CHECK();
-        const uint32_t thr = 0x80000000u - last_step;  // 2^31 - (K-1)
+        const uint32_t thr = 0x80000000u - last_step;  // only u >= 2^31 - (K-1) can be rejected
         const uint64_t nblk = (D + kScanTile - 1) / kScanTile;
-        uint32_t* blkp = sc.get<uint32_t>(nblk);
-        uint32_t* metap = sc.get<uint32_t>(4);  // [0] ncand, [1] nrej, [2] group total
-        k_cand_count<<<(unsigned)nblk, kScanBlock, 0, s>>>(u, D, thr, blkp);
+        uint32_t* blk = sc.get<uint32_t>(nblk);
+        uint32_t* meta = sc.get<uint32_t>(4);  // [0] candidates, [1] rejections, [2] steps
+        k_cand_count<<<(unsigned)nblk, kScanThreads, 0, s>>>(u, D, thr, blk);
         D3_LAUNCH_CHECK();
-        k_scan_small<<<1, 1024, 0, s>>>(blkp, (uint32_t)nblk, metap);
+        k_scan_one<<<1, 1024, 0, s>>>(blk, (uint32_t)nblk, meta);
         D3_LAUNCH_CHECK();
-        uint32_t ncand_h = 0;
-        copy_d2h(&ncand_h, metap, 4, s);
-        uint32_t* cand = sc.get<uint32_t>((uint64_t)ncand_h + 1);
-        uint32_t* rej = sc.get<uint32_t>((uint64_t)ncand_h + 1);
-        k_cand_write<<<(unsigned)nblk, kScanBlock, 0, s>>>(u, D, thr, blkp, cand);
+        uint32_t ncand = 0;
+        copy_d2h(&ncand, meta, 4, s);
+        const uint32_t cap = ncand + 1;
+        uint32_t* cand = sc.get<uint32_t>(cap);
+        uint32_t* rej = sc.get<uint32_t>(cap);
+        k_cand_write<<<(unsigned)nblk, kScanThreads, 0, s>>>(u, D, thr, blk, cand, cap, err);
         D3_LAUNCH_CHECK();
-        k_resolve<<<1, 64, 0, s>>>(u, cand, metap, last_step, rej, metap + 1);
+        k_resolve<<<1, 64, 0, s>>>(u, D, cand, meta, cap, last_step, rej, meta + 1, err);
         D3_LAUNCH_CHECK();
-        uint32_t nrej_h = 0;
-        copy_d2h(&nrej_h, metap + 1, 4, s);
-        if (D < (uint64_t)last_step + nrej_h) {  // not enough draws: redo with more
-            D = (uint64_t)last_step + nrej_h + 4096 + D / 64;
+        uint32_t nrej = 0;
+        copy_d2h(&nrej, meta + 1, 4, s);
+        if (D < (uint64_t)last_step + nrej) {  // too few draws: redo with more
+            D = (uint64_t)last_step + nrej + 4096 + D / 64;
             continue;
         }
-        ShuffleTables T;
+        Shuffle T;
         T.j = sc.get<uint32_t>(K);
         T.F = sc.get<uint32_t>(K);
         T.G = sc.get<uint32_t>(K);
@@ -762,235 +1072,310 @@ static ShuffleTables build_shuffle_tables(uint64_t K, hipStream_t s, Scratch& sc
         fill_u32(gcount, 0u, K, s);
         fill_u32(gfill, 0u, K, s);
         fill_u32(T.G, kNone, K, s);
-        k_steps<<<grid_for(D, 256, cus), 256, 0, s>>>(u, D, rej, metap + 1, last_step, T.j, gcount);
+        fill_u32(T.j, 0u, K, s);
+        k_steps<<<grid_for(D, 256, cus), 256, 0, s>>>(u, D, rej, meta + 1, cap, last_step, (uint32_t)K, T.j,
+                                                      gcount, err);
         D3_LAUNCH_CHECK();
         const uint64_t kb = (K + kScanTile - 1) / kScanTile;
         uint32_t* kblk = sc.get<uint32_t>(kb);
-        k_tile_sum<<<(unsigned)kb, kScanBlock, 0, s>>>(gcount, K, kblk);
+        k_tile_sum<<<(unsigned)kb, kScanThreads, 0, s>>>(gcount, K, kblk);
         D3_LAUNCH_CHECK();
-        k_scan_small<<<1, 1024, 0, s>>>(kblk, (uint32_t)kb, metap + 2);
+        k_scan_one<<<1, 1024, 0, s>>>(kblk, (uint32_t)kb, meta + 2);
         D3_LAUNCH_CHECK();
-        k_tile_scan<<<(unsigned)kb, kScanBlock, 0, s>>>(gcount, K, kblk, gstart);
+        k_tile_scan<<<(unsigned)kb, kScanThreads, 0, s>>>(gcount, K, kblk, gstart);
         D3_LAUNCH_CHECK();
-        k_group_scatter<<<grid_for(K, 256, cus), 256, 0, s>>>(T.j, last_step, gstart, gfill, grp);
+        k_group_scatter<<<grid_for(K, 256, cus), 256, 0, s>>>(T.j, last_step, (uint32_t)K, gstart, gfill, grp,
+                                                              err);
         D3_LAUNCH_CHECK();
-        k_group_links<<<grid_for(K, 256, cus), 256, 0, s>>>((uint32_t)K, gstart, gcount, grp, T.F, T.G);
+        k_group_links<<<grid_for(K, 256, cus), 256, 0, s>>>((uint32_t)K, gstart, gcount, grp, T.F, T.G, err);
         D3_LAUNCH_CHECK();
         return T;
     }
+    set_error(DEL3D_ERR_INTERNAL, "spatial sort: could not draw enough shuffle steps");
+    throw HipError{DEL3D_ERR_INTERNAL};
 }
 
-// Runs the median splits `tasks` (begin, end, axis, up) on A, exactly as
-// hilbert_split would, all ranges disjoint.
-struct SplitReq {
+struct Node {
+    int64_t lo, hi;
+    int x, ux, uy, uz;
+};
+
+struct Split {
     int64_t b, e;
     int axis, up;
 };
 
-static void run_splits(Rec* A, Rec* B, int64_t* Lpos, const std::vector<SplitReq>& reqs,
-                       hipStream_t s, Scratch& sc) {
-    const int cus = device_cu_count();
-    std::vector<SelTask> big;
-    std::vector<int64_t> big_nth;
-    std::vector<WaveTask> small;
-    for (const SplitReq& r : reqs) {
-        if (r.e - r.b < 2) continue;  // empty or single: nth_element returns at once
-        const int64_t nth = r.b + (r.e - r.b) / 2;
-        if (r.e - r.b <= kWave) {
-            small.push_back(WaveTask{r.b, r.e - 1, nth, r.axis, r.up});
-        } else {
-            big.push_back(SelTask{r.b, r.e - 1, 0, r.axis, r.up, 0});
-            big_nth.push_back(nth);
-        }
-    }
-    // device-wide rounds while a range is longer than one wavefront's share
-    while (!big.empty()) {
-        std::vector<SelTile> tiles;
-        std::vector<int64_t> tile0;
-        for (size_t i = 0; i < big.size(); ++i) {
-            SelTask& t = big[i];
-            t.pv = t.L + (t.R - t.L) / 2;
-            tile0.push_back((int64_t)tiles.size());
-            for (int64_t st = t.L; st < t.R; st += kTileElems) {
-                const int64_t len = std::min<int64_t>(kTileElems, t.R - st);
-                tiles.push_back(SelTile{(int32_t)i, (int32_t)len, st, tile0.back()});
-            }
-        }
-        tile0.push_back((int64_t)tiles.size());
-        SelTask* dt = sc.get<SelTask>(big.size());
-        SelTile* dtl = sc.get<SelTile>(tiles.size());
-        int64_t* dt0 = sc.get<int64_t>(tile0.size());
-        uint32_t* dcnt = sc.get<uint32_t>(tiles.size());
-        uint32_t* doff = sc.get<uint32_t>(tiles.size());
-        copy_h2d(dt, big.data(), big.size() * sizeof(SelTask), s);
-        copy_h2d(dtl, tiles.data(), tiles.size() * sizeof(SelTile), s);
-        copy_h2d(dt0, tile0.data(), tile0.size() * 8, s);
-        const unsigned nt = (unsigned)tiles.size();
-        k_sel_count<<<nt, kTileThreads, 0, s>>>(A, dt, dtl, dcnt);
-        D3_LAUNCH_CHECK();
-        k_sel_offsets<<<grid_for(big.size(), 256, cus), 256, 0, s>>>(dt, (uint32_t)big.size(), dt0,
-                                                                     dcnt, doff);
-        D3_LAUNCH_CHECK();
-        k_sel_flagged<<<nt, kTileThreads, 0, s>>>(A, dt, dtl, doff, Lpos, B);
-        D3_LAUNCH_CHECK();
-        k_sel_unflagged<<<nt, kTileThreads, 0, s>>>(A, dt, dtl, Lpos, B);
-        D3_LAUNCH_CHECK();
-        k_sel_copy<<<nt, kTileThreads, 0, s>>>(A, dt, dtl, B);
-        D3_LAUNCH_CHECK();
-        std::vector<SelTask> back(big.size());
-        copy_d2h(back.data(), dt, big.size() * sizeof(SelTask), s);
-        std::vector<SelTask> next;
-        std::vector<int64_t> next_nth;
-        for (size_t i = 0; i < big.size(); ++i) {
-            SelTask t = big[i];
-            const int64_t pos = t.L + back[i].nless;
-            const int64_t nth = big_nth[i];
-            if (pos == nth) continue;
-            if (nth < pos) t.R = pos - 1;
-            else t.L = pos + 1;
-            if (t.R - t.L + 1 <= kWave) {
-                small.push_back(WaveTask{t.L, t.R, nth, t.axis, t.up});
-            } else {
-                next.push_back(t);
-                next_nth.push_back(nth);
-            }
-        }
-        big.swap(next);
-        big_nth.swap(next_nth);
-    }
-    if (!small.empty()) {
-        WaveTask* dw = sc.get<WaveTask>(small.size());
-        copy_h2d(dw, small.data(), small.size() * sizeof(WaveTask), s);
-        const unsigned g = (unsigned)std::min<size_t>(small.size(), (size_t)cus * 8);
-        k_wave_select<<<g, 64, 0, s>>>(A, B, dw, (uint32_t)small.size());
-        D3_LAUNCH_CHECK();
-    }
-}
+inline int64_t mid_of(int64_t b, int64_t e) { return b >= e ? b : b + (e - b) / 2; }
 
-// Sorts every set: order[off[i] .. off[i+1]) holds indices into the set's
-// points, which start at xyz + 3*poff[i] (poff == nullptr: poff = off).
-static void sort_sets(const double* xyz, const int64_t* poff, const int64_t* off, int64_t n_sets,
-                      int32_t* order, hipStream_t s) {
-    const uint64_t total = (uint64_t)off[n_sets];
-    if (total == 0) return;
-    const int cus = device_cu_count();
-    uint64_t K = 2;
-    for (int64_t i = 0; i < n_sets; ++i) K = std::max<uint64_t>(K, (uint64_t)(off[i + 1] - off[i]));
+// One phase of the plan: its big selects (device rounds) and the selects
+// finished in LDS, as offsets into the uploaded tables.
+struct Phase {
+    size_t task0 = 0, ntasks = 0;   // RTask range
+    size_t tile0 = 0, ntiles = 0;   // RTile range
+    size_t fin0 = 0, nfin = 0;      // FTask range
+    int64_t max_big = 0;            // largest big select
+};
 
-    // every buffer of this call lives until the end of the call (see Scratch)
-    Scratch sc(s);
-    const ShuffleTables T = build_shuffle_tables(K, s, sc);
+struct Plan {
+    std::vector<RTask> tasks;
+    std::vector<RTile> tiles;
+    std::vector<FTask> fins;
+    std::vector<Leaf> leaves;
+    std::vector<Phase> phases;
+};
 
-    int64_t* doff = sc.get<int64_t>(n_sets + 1);
-    int64_t* dpoff = sc.get<int64_t>(n_sets + 1);
-    copy_h2d(doff, off, (n_sets + 1) * 8, s);
-    copy_h2d(dpoff, poff ? poff : off, (n_sets + 1) * 8, s);
-    uint32_t* ta = sc.get<uint32_t>(total);
-    uint32_t* tb = sc.get<uint32_t>(total);
-    k_chain_init<<<grid_for(total, 256, cus), 256, 0, s>>>(doff, (uint32_t)n_sets, total, T.F, ta);
-    D3_LAUNCH_CHECK();
-    int rounds = 1;
-    while ((1ull << rounds) < K) ++rounds;
-    uint32_t* cur = ta;
-    uint32_t* nxt = tb;
-    for (int r = 0; r < rounds; ++r) {
-        k_jump<<<grid_for(total, 256, cus), 256, 0, s>>>(cur, total, nxt);
-        D3_LAUNCH_CHECK();
-        std::swap(cur, nxt);
-    }
-    Rec* A = sc.get<Rec>(total);
-    Rec* B = sc.get<Rec>(total);
-    int64_t* lpos = sc.get<int64_t>(total);
-    k_shuffle_place<<<grid_for(total, 256, cus), 256, 0, s>>>(doff, dpoff, (uint32_t)n_sets, total,
-                                                              T.j, T.F, T.G, cur, order, xyz, A);
-    D3_LAUNCH_CHECK();
-
-    // multiscale segments (src/spatial_sort.cpp:130-137) of every set
-    std::vector<Node> level, leaves;
-    for (int64_t si = 0; si < n_sets; ++si) {
-        int64_t b = off[si], e = off[si + 1];
-        if (e - b < 2) {
-            if (e - b == 1) leaves.push_back(Node{b, e, 0, 0, 0, 0});
+void add_phase(Plan& P, const std::vector<Split>& splits) {
+    Phase ph;
+    ph.task0 = P.tasks.size();
+    ph.tile0 = P.tiles.size();
+    ph.fin0 = P.fins.size();
+    for (const Split& sp : splits) {
+        const int64_t m = sp.e - sp.b;
+        if (m < 2) continue;  // nth_element returns at once (src/spatial_sort.cpp:62-64)
+        const int64_t nth = sp.b + m / 2;
+        if (m <= kFin) {
+            P.fins.push_back(FTask{sp.b, sp.e - 1, nth, sp.axis, sp.up, -1, 0});
             continue;
         }
-        for (;;) {
+        RTask t;
+        t.b = sp.b;
+        t.e = sp.e;
+        t.nth = nth;
+        t.axis = sp.axis;
+        t.up = sp.up;
+        t.tile0 = (int64_t)(P.tiles.size() - ph.tile0);
+        const int32_t local = (int32_t)(P.tasks.size() - ph.task0);
+        for (int64_t st = sp.b; st < sp.e; st += kTile)
+            P.tiles.push_back(RTile{st, std::min<int64_t>(kTile, sp.e - st), local, 0});
+        t.ntiles = (int64_t)(P.tiles.size() - ph.tile0) - t.tile0;
+        P.tasks.push_back(t);
+        P.fins.push_back(FTask{0, 0, nth, sp.axis, sp.up, local, 0});
+        ph.max_big = std::max(ph.max_big, m);
+    }
+    ph.ntasks = P.tasks.size() - ph.task0;
+    ph.ntiles = P.tiles.size() - ph.tile0;
+    ph.nfin = P.fins.size() - ph.fin0;
+    if (ph.ntasks || ph.nfin) P.phases.push_back(ph);
+}
+
+// The data-independent recursion plan of every set
+// (src/spatial_sort.cpp:95-137).
+Plan make_plan(const int64_t* off, int64_t n_sets) {
+    Plan P;
+    std::vector<Node> level;
+    auto place = [&](const Node& nd, std::vector<Node>& next) {
+        const int64_t m = nd.hi - nd.lo;
+        if (m <= 8) return;  // kHilbertLimit: left as it is
+        if (m <= kSub) P.leaves.push_back(Leaf{nd.lo, nd.hi, nd.x, nd.ux, nd.uy, nd.uz});
+        else next.push_back(nd);
+    };
+    for (int64_t si = 0; si < n_sets; ++si) {
+        int64_t b = off[si], e = off[si + 1];
+        if (e - b < 2) continue;  // src/spatial_sort.cpp:142
+        for (;;) {  // multiscale segments: [mid, e) sorted, [b, mid) recursed
             int64_t mid = b;
             if (e - b >= 64) mid = b + (int64_t)((double)(e - b) * 0.125);
-            level.push_back(Node{mid, e, 0, 0, 0, 0});
+            place(Node{mid, e, 0, 0, 0, 0}, level);
             if (mid == b) break;
             e = mid;
         }
     }
-    // level-synchronous Hilbert recursion for ranges longer than a wavefront's share
     while (!level.empty()) {
-        std::vector<Node> big;
-        for (const Node& nd : level) {
-            if (nd.hi - nd.lo > kWave) big.push_back(nd);
-            else leaves.push_back(nd);
-        }
-        if (big.empty()) break;
-        std::vector<SplitReq> px, py, pz;
-        for (const Node& nd : big) px.push_back(SplitReq{nd.lo, nd.hi, nd.x, nd.ux});
-        run_splits(A, B, lpos, px, s, sc);
-        for (const Node& nd : big) {
-            const int y = (nd.x + 1) % 3;
-            const int64_t m0 = nd.lo, m8 = nd.hi, m4 = mid_of(m0, m8);
-            py.push_back(SplitReq{m0, m4, y, nd.uy});
-            py.push_back(SplitReq{m4, m8, y, !nd.uy});
-        }
-        run_splits(A, B, lpos, py, s, sc);
+        std::vector<Split> px, py, pz;
         std::vector<Node> next;
-        for (const Node& nd : big) {
+        for (const Node& nd : level) {
             const int x = nd.x, y = (x + 1) % 3, z = (x + 2) % 3;
             const int ux = nd.ux, uy = nd.uy, uz = nd.uz;
             const int64_t m0 = nd.lo, m8 = nd.hi, m4 = mid_of(m0, m8);
             const int64_t m2 = mid_of(m0, m4), m6 = mid_of(m4, m8);
             const int64_t m1 = mid_of(m0, m2), m3 = mid_of(m2, m4);
             const int64_t m5 = mid_of(m4, m6), m7 = mid_of(m6, m8);
-            pz.push_back(SplitReq{m0, m2, z, uz});
-            pz.push_back(SplitReq{m2, m4, z, !uz});
-            pz.push_back(SplitReq{m4, m6, z, uz});
-            pz.push_back(SplitReq{m6, m8, z, !uz});
-            next.push_back(Node{m0, m1, z, uz, ux, uy});
-            next.push_back(Node{m1, m2, y, uy, uz, ux});
-            next.push_back(Node{m2, m3, y, uy, uz, ux});
-            next.push_back(Node{m3, m4, x, ux, !uy, !uz});
-            next.push_back(Node{m4, m5, x, ux, !uy, !uz});
-            next.push_back(Node{m5, m6, y, !uy, uz, !ux});
-            next.push_back(Node{m6, m7, y, !uy, uz, !ux});
-            next.push_back(Node{m7, m8, z, !uz, !ux, uy});
+            px.push_back(Split{m0, m8, x, ux});
+            py.push_back(Split{m0, m4, y, uy});
+            py.push_back(Split{m4, m8, y, !uy});
+            pz.push_back(Split{m0, m2, z, uz});
+            pz.push_back(Split{m2, m4, z, !uz});
+            pz.push_back(Split{m4, m6, z, uz});
+            pz.push_back(Split{m6, m8, z, !uz});
+            place(Node{m0, m1, z, uz, ux, uy}, next);
+            place(Node{m1, m2, y, uy, uz, ux}, next);
+            place(Node{m2, m3, y, uy, uz, ux}, next);
+            place(Node{m3, m4, x, ux, !uy, !uz}, next);
+            place(Node{m4, m5, x, ux, !uy, !uz}, next);
+            place(Node{m5, m6, y, !uy, uz, !ux}, next);
+            place(Node{m6, m7, y, !uy, uz, !ux}, next);
+            place(Node{m7, m8, z, !uz, !ux, uy}, next);
         }
-        run_splits(A, B, lpos, pz, s, sc);
+        add_phase(P, px);
+        add_phase(P, py);
+        add_phase(P, pz);
         level.swap(next);
     }
-    std::vector<LeafNode> lv;
-    lv.reserve(leaves.size());
-    for (const Node& nd : leaves)
-        if (nd.hi > nd.lo) lv.push_back(LeafNode{nd.lo, nd.hi, nd.x, nd.ux, nd.uy, nd.uz});
-    if (!lv.empty()) {
-        LeafNode* dl = sc.get<LeafNode>(lv.size());
-        copy_h2d(dl, lv.data(), lv.size() * sizeof(LeafNode), s);
-        const unsigned g = (unsigned)std::min<size_t>(lv.size(), (size_t)cus * 8);
-        k_wave_subtree<<<g, 64, 0, s>>>(A, dl, (uint32_t)lv.size(), order);
-        D3_LAUNCH_CHECK();
-    }
-    // the scratch arena is released after this: nothing queued may still use it
-    D3_HIP(hipStreamSynchronize(s));
+    return P;
 }
 
-void spatial_sort_sets_device(const double* xyz, const int64_t* off, int64_t n_sets,
-                              int32_t* order, hipStream_t s) {
-    sort_sets(xyz, nullptr, off, n_sets, order, s);
+template <class T>
+size_t align_up_bytes(size_t n) {
+    return (n * sizeof(T) + 255) & ~size_t(255);
+}
+
+// Sorts every set: order[off[i] .. off[i+1]) holds indices into the set's
+// points, which start at xyz + 3*poff[i] (poff == nullptr: poff = off).
+void sort_sets(const double* xyz, int64_t n_pts, const int64_t* poff, const int64_t* off, int64_t n_sets,
+               int32_t* order, hipStream_t s) {
+    const uint64_t total = (uint64_t)off[n_sets];
+    if (total == 0) return;
+    const int cus = device_cu_count();
+    uint64_t K = 2;
+    for (int64_t i = 0; i < n_sets; ++i) K = std::max<uint64_t>(K, (uint64_t)(off[i + 1] - off[i]));
+
+    Scratch sc(s);
+    uint32_t* err = sc.get<uint32_t>(1);
+    fill_u32(err, 0u, 1, s);
+    const Shuffle T = build_shuffle(K, s, sc, err);
+
+    // the plan and the set tables, uploaded in one copy
+    const Plan P = make_plan(off, n_sets);
+    const size_t b_off = 0;
+    const size_t b_poff = b_off + align_up_bytes<int64_t>(n_sets + 1);
+    const size_t b_tasks = b_poff + align_up_bytes<int64_t>(n_sets + 1);
+    const size_t b_tiles = b_tasks + align_up_bytes<RTask>(P.tasks.size());
+    const size_t b_fins = b_tiles + align_up_bytes<RTile>(P.tiles.size());
+    const size_t b_leaves = b_fins + align_up_bytes<FTask>(P.fins.size());
+    const size_t b_end = b_leaves + align_up_bytes<Leaf>(P.leaves.size());
+    std::vector<char> blob(b_end, 0);
+    std::memcpy(blob.data() + b_off, off, (n_sets + 1) * sizeof(int64_t));
+    std::memcpy(blob.data() + b_poff, poff ? poff : off, (n_sets + 1) * sizeof(int64_t));
+    if (!P.tasks.empty()) std::memcpy(blob.data() + b_tasks, P.tasks.data(), P.tasks.size() * sizeof(RTask));
+    if (!P.tiles.empty()) std::memcpy(blob.data() + b_tiles, P.tiles.data(), P.tiles.size() * sizeof(RTile));
+    if (!P.fins.empty()) std::memcpy(blob.data() + b_fins, P.fins.data(), P.fins.size() * sizeof(FTask));
+    if (!P.leaves.empty())
+        std::memcpy(blob.data() + b_leaves, P.leaves.data(), P.leaves.size() * sizeof(Leaf));
+    char* dblob = sc.get<char>(b_end);
+    copy_h2d(dblob, blob.data(), b_end, s);
+    const int64_t* doff = reinterpret_cast<const int64_t*>(dblob + b_off);
+    const int64_t* dpoff = reinterpret_cast<const int64_t*>(dblob + b_poff);
+    const RTask* dtasks = reinterpret_cast<const RTask*>(dblob + b_tasks);
+    const RTile* dtiles = reinterpret_cast<const RTile*>(dblob + b_tiles);
+    const FTask* dfins = reinterpret_cast<const FTask*>(dblob + b_fins);
+    const Leaf* dleaves = reinterpret_cast<const Leaf*>(dblob + b_leaves);
+
+    Bufs bufs;
+    for (int i = 0; i < 2; ++i) {
+        bufs.x[i] = sc.get<double>(total);
+        bufs.y[i] = sc.get<double>(total);
+        bufs.z[i] = sc.get<double>(total);
+        bufs.id[i] = sc.get<int32_t>(total);
+    }
+    bufs.n = (int64_t)total;
+    k_place<<<grid_for(total, 256, cus), 256, 0, s>>>(doff, dpoff, (uint32_t)n_sets, total, n_pts, T.j, T.F,
+                                                      T.G, (uint32_t)K, order, xyz, bufs, err);
+    D3_LAUNCH_CHECK();
+
+    // round buffers, sized for the largest phase
+    size_t max_tasks = 0, max_tiles = 0;
+    for (const Phase& ph : P.phases) {
+        max_tasks = std::max(max_tasks, ph.ntasks);
+        max_tiles = std::max(max_tiles, ph.ntiles);
+    }
+    RState* st[2] = {nullptr, nullptr};
+    uint64_t* mask = nullptr;
+    uint32_t *tcount = nullptr, *toff = nullptr, *lpos = nullptr, *ring = nullptr;
+    if (max_tasks) {
+        st[0] = sc.get<RState>(max_tasks);
+        st[1] = sc.get<RState>(max_tasks);
+        mask = sc.get<uint64_t>(max_tiles * kTileWords);
+        tcount = sc.get<uint32_t>(max_tiles);
+        toff = sc.get<uint32_t>(max_tiles);
+        lpos = sc.get<uint32_t>(total);
+        ring = sc.get<uint32_t>(kRing);
+        fill_u32(ring, 0u, kRing, s);
+    }
+    for (const Phase& ph : P.phases) {
+        const RState* final_state = st[0];
+        if (ph.ntasks) {
+            const RTask* pt = dtasks + ph.task0;
+            const RTile* ptl = dtiles + ph.tile0;
+            const uint32_t nt = (uint32_t)ph.ntasks, ntl = (uint32_t)ph.ntiles;
+            k_rd_init<<<grid_for(nt, 256, cus), 256, 0, s>>>(pt, nt, st[0]);
+            D3_LAUNCH_CHECK();
+            // A select's range shrinks by at least one slot per round; the
+            // expected count is ~1.5 log2(m / kFin) rounds. Check every two
+            // rounds once that many have run.
+            const int64_t bound = ph.max_big + 2;
+            int64_t check_at = 2;
+            while ((int64_t(kFin) << check_at) < ph.max_big && check_at < 40) ++check_at;
+            check_at = check_at + check_at / 2;
+            uint32_t round = 0;
+            for (;;) {
+                const int c = round & 1;
+                k_rd_count<<<ntl, kTileThreads, 0, s>>>(bufs, pt, nt, ptl, ntl, st[c], mask, tcount, ring, round,
+                                                        err);
+                D3_LAUNCH_CHECK();
+                k_rd_scan<<<nt, 256, 0, s>>>(pt, nt, ntl, st[c], tcount, toff, err);
+                D3_LAUNCH_CHECK();
+                k_rd_flagged<<<ntl, kTileThreads, 0, s>>>(bufs, pt, nt, ptl, ntl, st[c], mask, toff, lpos, err);
+                D3_LAUNCH_CHECK();
+                k_rd_rest<<<ntl, kTileThreads, 0, s>>>(bufs, pt, nt, ptl, ntl, st[c], st[c ^ 1], mask, lpos, ring,
+                                                       round, err);
+                D3_LAUNCH_CHECK();
+                ++round;
+                if ((int64_t)round >= check_at) {
+                    uint32_t active = 0;
+                    copy_d2h(&active, ring + (round % kRing), 4, s);
+                    if (active == 0) break;
+                    check_at = round + 2;
+                }
+                if ((int64_t)round > bound) {
+                    set_error(DEL3D_ERR_INTERNAL, "spatial sort: a select did not finish in %lld rounds",
+                              (long long)bound);
+                    throw HipError{DEL3D_ERR_INTERNAL};
+                }
+            }
+            // the margins of the last round
+            k_rd_count<<<ntl, kTileThreads, 0, s>>>(bufs, pt, nt, ptl, ntl, st[round & 1], mask, tcount, ring,
+                                                    round, err);
+            D3_LAUNCH_CHECK();
+            final_state = st[round & 1];
+        }
+        if (ph.nfin) {
+            k_lds_select<<<(unsigned)ph.nfin, 256, 0, s>>>(bufs, dfins + ph.fin0, (uint32_t)ph.nfin, final_state,
+                                                           (uint32_t)ph.ntasks, err);
+            D3_LAUNCH_CHECK();
+        }
+    }
+    k_write_order<<<grid_for(total, 256, cus), 256, 0, s>>>(bufs.id[0], total, order);
+    D3_LAUNCH_CHECK();
+    if (!P.leaves.empty()) {
+        const unsigned g = (unsigned)std::min<size_t>(P.leaves.size(), (size_t)cus * 16);
+        k_subtree<<<g, 64, 0, s>>>(bufs, dleaves, (uint32_t)P.leaves.size(), order, err);
+        D3_LAUNCH_CHECK();
+    }
+    uint32_t herr = 0;
+    copy_d2h(&herr, err, 4, s);  // also the final wait: the scratch is released after this
+    if (herr & kErrIndex) {
+        set_error(DEL3D_ERR_INVALID, "spatial sort: an order entry lies outside its set's points");
+        throw HipError{DEL3D_ERR_INVALID};
+    }
+    if (herr) {
+        set_error(DEL3D_ERR_INTERNAL, "spatial sort: internal check failed (code %u)", herr);
+        throw HipError{DEL3D_ERR_INTERNAL};
+    }
+}
+
+}  // namespace
+
+void spatial_sort_sets_device(const double* xyz, const int64_t* off, int64_t n_sets, int32_t* order,
+                              hipStream_t s) {
+    sort_sets(xyz, off[n_sets], nullptr, off, n_sets, order, s);
 }
 
 // The reference's (points, order) pair: n_order indices into n_pts points.
-void spatial_sort_device(const double* xyz, int64_t n_pts, int32_t* order, int64_t n_order,
-                         hipStream_t s) {
-    (void)n_pts;
+void spatial_sort_device(const double* xyz, int64_t n_pts, int32_t* order, int64_t n_order, hipStream_t s) {
     if (n_order < 2) return;
     const int64_t off[2] = {0, n_order};
     const int64_t poff[2] = {0, n_pts};
-    sort_sets(xyz, poff, off, 1, order, s);
+    sort_sets(xyz, n_pts, poff, off, 1, order, s);
 }
 
 }  // namespace d3

@@ -35,6 +35,7 @@ extern "C" {
 #define DEL3D_ERR_INVALID 1     /* bad argument (size, pointer, op code) */
 #define DEL3D_ERR_DEVICE 2      /* a HIP call failed */
 #define DEL3D_ERR_NOMEM 3       /* device allocation failed */
+#define DEL3D_ERR_INTERNAL 4    /* a device-side consistency check failed (no result written) */
 
 /* Predicate op codes, one byte per query. */
 #define DEL3D_OP_ORIENT3D 0              /* orient3d(r0, r1, r2, r3) */

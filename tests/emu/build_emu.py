@@ -66,9 +66,15 @@ def main():
                 t.write_text(t.read_text())
             srcs.append(str(t))
         srcs.append(str(HERE / "emu_driver.cpp"))
+        # EMU_SANITIZER=none builds without a sanitizer (the fiber mode, whose
+        # switched stacks a sanitizer's stack poisoning does not follow);
+        # EMU_OPT sets the optimisation level (default -O1)
         san = os.environ.get("EMU_SANITIZER", "address")
-        cmd = ["g++", "-std=c++20", "-g", "-O1", "-fPIC", "-pthread", "-ffp-contract=off",
-               f"-fsanitize={san}", "-fno-omit-frame-pointer", f"-I{HERE}", "-o", out] + srcs
+        opt = os.environ.get("EMU_OPT", "-O1")
+        cmd = ["g++", "-std=c++20", "-g", opt, "-fPIC", "-pthread", "-ffp-contract=off",
+               "-fno-omit-frame-pointer", f"-I{HERE}", "-o", out] + srcs
+        if san != "none":
+            cmd.insert(6, f"-fsanitize={san}")
         if out.endswith(".so"):
             cmd.insert(1, "-shared")
         else:

@@ -288,11 +288,72 @@ static int check_circumcenters() {
     return fails;
 }
 
+// One sort of n points (order = 0..n-1, or the given order) against the oracle.
+static int check_sort(const char* name, const std::vector<double>& xyz, int64_t n,
+                      const std::vector<int32_t>* order = nullptr) {
+    std::vector<int32_t> got, want;
+    if (order) {
+        got = want = *order;
+    } else {
+        got.resize(n);
+        for (int64_t i = 0; i < n; ++i) got[i] = (int32_t)i;
+        want = got;
+    }
+    const int64_t m = (int64_t)got.size();
+    if (emu_spatial_sort(xyz.data(), n, got.data(), m) != 0) {
+        std::printf("sort %s error: %s\n", name, del3d_last_error());
+        return 1;
+    }
+    orc_spatial_sort(xyz.data(), m, want.data());
+    const bool ok = got == want;
+    std::printf("sort %s %s\n", name, ok ? "ok" : "MISMATCH");
+    return ok ? 0 : 1;
+}
+
+// Tie-heavy inputs, an order that is a permuted subset, and a bad index.
+static int check_sort_cases() {
+    int fails = 0;
+    {
+        std::vector<double> same;
+        for (int i = 0; i < 4700; ++i) same.insert(same.end(), {0.25, 0.5, 0.75});
+        fails += check_sort("identical 4700", same, 4700);
+    }
+    {
+        std::vector<double> lat;
+        for (int i = 0; i < 20; ++i)
+            for (int j = 0; j < 20; ++j)
+                for (int k = 0; k < 20; ++k) lat.insert(lat.end(), {(double)i, (double)j, (double)k});
+        fails += check_sort("lattice 20^3", lat, 8000);
+    }
+    {
+        std::vector<double> few;
+        for (int i = 0; i < 12000; ++i)
+            for (int k = 0; k < 3; ++k) few.push_back((double)(rnd() % 3));
+        fails += check_sort("3 values per axis n=12000", few, 12000);
+    }
+    {
+        std::vector<double> pts(3 * 9000 + 3);
+        orc_pts_random_cube(9000, 3u, pts.data());
+        std::vector<int32_t> ord;
+        for (int i = 0; i < 9000; ++i)
+            if (rnd() % 3) ord.push_back(i);
+        for (size_t i = ord.size(); i > 1; --i) std::swap(ord[i - 1], ord[rnd() % i]);
+        fails += check_sort("subset order", pts, 9000, &ord);
+        std::vector<int32_t> bad = ord;
+        bad[17] = 9000;
+        const int rc = emu_spatial_sort(pts.data(), 9000, bad.data(), (int64_t)bad.size());
+        const bool ok = rc == DEL3D_ERR_INVALID;
+        std::printf("sort bad index %s (rc %d)\n", ok ? "ok" : "MISMATCH", rc);
+        fails += ok ? 0 : 1;
+    }
+    return fails;
+}
+
 int main(int argc, char** argv) {
     int fails = 0;
     // the serving loop alone (the ThreadSanitizer run)
     if (std::getenv("EMU_SERVER_ONLY")) return check_server() ? 1 : 0;
-    {
+    if (!std::getenv("EMU_SORT_ONLY")) {
         std::vector<double> lat;
         for (int i = 0; i < 5; ++i)
             for (int j = 0; j < 5; ++j)
@@ -342,22 +403,20 @@ int main(int argc, char** argv) {
         // every fourth point snapped to a coarse grid: ties on every axis
         for (int64_t i = 0; i < n; i += 4)
             for (int k = 0; k < 3; ++k) xyz[3 * i + k] = (double)(int)(xyz[3 * i + k] * 4) / 4;
-        std::vector<int32_t> got(n), want(n);
-        for (int64_t i = 0; i < n; ++i) got[i] = want[i] = (int32_t)i;
-        if (emu_spatial_sort(xyz.data(), n, got.data(), n) != 0) { std::printf("n=%lld error\n", (long long)n); ++fails; continue; }
-        orc_spatial_sort(xyz.data(), n, want.data());
-        const bool ok = got == want;
-        std::printf("n=%lld %s\n", (long long)n, ok ? "ok" : "MISMATCH");
-        fails += ok ? 0 : 1;
+        fails += check_sort(("random+grid n=" + std::to_string(n)).c_str(), xyz, n);
     }
-    // three sets in one call
-    const int64_t sizes[3] = {1500, 3, 700};
+    if (std::getenv("EMU_SORT_CASES")) fails += check_sort_cases();
+    // sets in one call, on both sides of the LDS and subtree thresholds
+    const int64_t sizes[8] = {1500, 3, 700, 0, 9, 5000, 1, 9100};
     std::vector<double> xyz;
     std::vector<int64_t> off = {0};
     std::vector<int32_t> got, want;
-    for (int s = 0; s < 3; ++s) {
+    for (int s = 0; s < 8; ++s) {
         std::vector<double> p(3 * (size_t)sizes[s] + 3);
         orc_pts_random_cube(sizes[s], 30u + s, p.data());
+        if (s == 7)  // a lattice: masses of equal keys in the device rounds
+            for (int64_t i = 0; i < sizes[s]; ++i)
+                for (int k = 0; k < 3; ++k) p[3 * i + k] = (double)((i >> (4 * k)) & 15);
         xyz.insert(xyz.end(), p.begin(), p.begin() + 3 * sizes[s]);
         std::vector<int32_t> o(sizes[s]);
         for (int64_t i = 0; i < sizes[s]; ++i) o[i] = (int32_t)i;
@@ -366,7 +425,7 @@ int main(int argc, char** argv) {
         want.insert(want.end(), o.begin(), o.end());
         off.push_back(off.back() + sizes[s]);
     }
-    const bool sets_ok = emu_spatial_sort_sets(xyz.data(), off.data(), 3, got.data()) == 0 && got == want;
+    const bool sets_ok = emu_spatial_sort_sets(xyz.data(), off.data(), 8, got.data()) == 0 && got == want;
     std::printf("sets %s\n", sets_ok ? "ok" : "MISMATCH");
     fails += sets_ok ? 0 : 1;
     return fails ? 1 : 0;

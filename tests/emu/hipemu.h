@@ -44,13 +44,67 @@ enum hipMemcpyKind { hipMemcpyHostToHost = 0, hipMemcpyHostToDevice = 1, hipMemc
 enum { hipStreamNonBlocking = 1, hipHostMallocDefault = 0, hipDeviceAttributeMultiprocessorCount = 63 };
 
 namespace emu {
+// EMU_FIBERS=1 runs the threads of a block as fibers on the launching OS
+// thread (cooperative switches at barriers and wavefront operations): the
+// same kernel code, many times faster than one OS thread per lane, but with
+// no concurrency for ThreadSanitizer to check.
+inline bool fiber_mode() {
+    static const bool f = [] {
+        const char* e = std::getenv("EMU_FIBERS");
+        return e && std::atoi(e) != 0;
+    }();
+    return f;
+}
+struct Fiber {
+    void* sp = nullptr;
+    char* stack = nullptr;
+    unsigned tid = 0;
+    bool done = false;
+};
+inline thread_local void* t_sched_sp = nullptr;
+inline thread_local Fiber* t_cur = nullptr;
+inline void fiber_yield();
+// A barrier over `count` participants; in fiber mode a generation counter
+// that waiting fibers poll between yields.
+struct Bar {
+    std::unique_ptr<std::barrier<>> sb;
+    unsigned count = 0, arrived = 0;
+    uint64_t gen = 0;
+    explicit Bar(unsigned n) : count(n) {
+        if (!fiber_mode()) sb = std::make_unique<std::barrier<>>(n);
+    }
+    void arrive_and_wait() {
+        if (sb) {
+            sb->arrive_and_wait();
+            return;
+        }
+        if (++arrived >= count) {
+            arrived = 0;
+            ++gen;
+            return;
+        }
+        const uint64_t g = gen;
+        while (gen == g) fiber_yield();
+    }
+    void arrive_and_drop() {
+        if (sb) {
+            sb->arrive_and_drop();
+            return;
+        }
+        --count;
+        if (count > 0 && arrived >= count) {
+            arrived = 0;
+            ++gen;
+        }
+    }
+};
 struct Wave {
-    std::unique_ptr<std::barrier<>> bar;
+    std::unique_ptr<Bar> bar;
     uint64_t vals[64];
     int64_t wide[64];
 };
 struct Block {
-    std::unique_ptr<std::barrier<>> bar;
+    std::unique_ptr<Bar> bar;
     std::vector<Wave> waves;
     std::atomic<int> and_count{0};  // __syncthreads_and: threads whose predicate was false
     int and_result = 0;
@@ -125,6 +179,8 @@ inline T __shfl_xor(T v, int m) { return emu_exchange(v, __lane_id() ^ m); }
 
 template <class T>
 inline T atomicAdd(T* p, T v) { return __atomic_fetch_add(p, v, __ATOMIC_SEQ_CST); }
+template <class T>
+inline T atomicOr(T* p, T v) { return __atomic_fetch_or(p, v, __ATOMIC_SEQ_CST); }
 template <class T>
 inline T atomicMin(T* p, T v) {
     T cur = __atomic_load_n(p, __ATOMIC_SEQ_CST);
@@ -345,6 +401,80 @@ inline Pool& pool() {
     return p;
 }
 
+extern "C" void emu_ctx_switch(void** save_sp, void* load_sp);
+asm(R"(
+.text
+.weak emu_ctx_switch
+.type emu_ctx_switch,@function
+emu_ctx_switch:
+    pushq %rbp
+    pushq %rbx
+    pushq %r12
+    pushq %r13
+    pushq %r14
+    pushq %r15
+    movq %rsp, (%rdi)
+    movq %rsi, %rsp
+    popq %r15
+    popq %r14
+    popq %r13
+    popq %r12
+    popq %rbx
+    popq %rbp
+    ret
+.size emu_ctx_switch, .-emu_ctx_switch
+)");
+
+inline thread_local std::function<void(unsigned)>* t_job = nullptr;
+
+inline void fiber_yield() { emu_ctx_switch(&t_cur->sp, t_sched_sp); }
+
+[[noreturn]] inline void fiber_entry() {
+    Fiber* f = t_cur;
+    (*t_job)(f->tid);
+    f->done = true;
+    emu_ctx_switch(&f->sp, t_sched_sp);
+    std::abort();  // a finished fiber is never resumed
+}
+
+// Runs job(t) for t in [0, n) as fibers, round robin, until all finish.
+inline void run_fibers(unsigned n, std::function<void(unsigned)> job) {
+    constexpr size_t kStack = size_t(256) << 10;
+    thread_local std::vector<Fiber> fibers;
+    if (fibers.size() < n) {
+        const size_t old = fibers.size();
+        fibers.resize(n);
+        for (size_t i = old; i < n; ++i) fibers[i].stack = static_cast<char*>(std::malloc(kStack));
+    }
+    t_job = &job;
+    for (unsigned t = 0; t < n; ++t) {
+        Fiber& f = fibers[t];
+        f.tid = t;
+        f.done = false;
+        // initial frame: six callee-saved registers, then fiber_entry as the
+        // return address, then a fake caller slot (entry sees rsp % 16 == 8)
+        uintptr_t top = (reinterpret_cast<uintptr_t>(f.stack) + kStack) & ~uintptr_t(15);
+        void** sp = reinterpret_cast<void**>(top);
+        *--sp = nullptr;
+        *--sp = reinterpret_cast<void*>(&fiber_entry);
+        for (int r = 0; r < 6; ++r) *--sp = nullptr;
+        f.sp = sp;
+    }
+    unsigned alive = n;
+    while (alive > 0) {
+        for (unsigned t = 0; t < n; ++t) {
+            Fiber& f = fibers[t];
+            if (f.done) continue;
+            t_cur = &f;
+            tl_tid = dim3(t);
+            emu_ctx_switch(&t_sched_sp, f.sp);
+            if (f.done) --alive;
+        }
+    }
+    t_cur = nullptr;
+    t_job = nullptr;
+}
+
 inline std::mutex& launch_mutex() {
     static std::mutex m;
     return m;
@@ -372,11 +502,21 @@ void run_launch(dim3 grid, dim3 block, K kernel, A... args) {
     for (unsigned bi = 0; bi < grid.x; ++bi) {
         const unsigned b = bo[bi];
         Block blk;
-        blk.bar = std::make_unique<std::barrier<>>(nt);
+        blk.bar = std::make_unique<Bar>(nt);
         const unsigned nw = (nt + 63) / 64;
         blk.waves.resize(nw);
         for (unsigned w = 0; w < nw; ++w)
-            blk.waves[w].bar = std::make_unique<std::barrier<>>(std::min(64u, nt - 64 * w));
+            blk.waves[w].bar = std::make_unique<Bar>(std::min(64u, nt - 64 * w));
+        if (fiber_mode()) {
+            run_fibers(nt, [&, b](unsigned t) {
+                tl_bid = dim3(b);
+                tl_block = &blk;
+                kernel(args...);
+                blk.waves[t >> 6].bar->arrive_and_drop();
+                blk.bar->arrive_and_drop();
+            });
+            continue;
+        }
         pool().run(nt, [&, b](unsigned t) {
             tl_tid = dim3(t);
             tl_bid = dim3(b);

@@ -13,16 +13,11 @@ pytestmark = pytest.mark.gpu
 REF_BIN = Path(__file__).resolve().parents[1] / "oracle" / "_ref"
 
 
-SORT_FAULT_REASON = (
-    "runs the GPU spatial sort, which faulted the device twice; the cause is not yet shown on the "
-    "CPU, so the sort stays off the GPU until it is")
-
-
 @pytest.mark.parametrize("name", [
     "predicates",
     "predicates2",
-    pytest.param("spatial_sort", marks=pytest.mark.skip(reason=SORT_FAULT_REASON)),
-    pytest.param("delaunay", marks=pytest.mark.skip(reason=SORT_FAULT_REASON)),
+    "spatial_sort",
+    "delaunay",
 ])
 def test_reference_test_program_on_gpu_library(name):
     exe = REF_BIN / f"dropin_test_{name}"

@@ -11,11 +11,7 @@ import oracle_lib as O
 from del3d_amd import workloads as W
 from gpu_util import require_gpu, say
 
-SORT_FAULT_REASON = (
-    "the GPU spatial sort faulted the device twice (dropin_test_spatial_sort); the cause is not "
-    "yet shown on the CPU, where the emulated kernels are bit-exact (tests/test_emu.py), so it stays "
-    "off the GPU until it is")
-pytestmark = [pytest.mark.gpu, pytest.mark.skip(reason=SORT_FAULT_REASON)]
+pytestmark = pytest.mark.gpu
 PINS = json.loads((Path(__file__).parent / "golden" / "reference_pins.json").read_text())
 
 
@@ -54,7 +50,8 @@ def _device_sort(pts, order=None):
     return d.to_host()
 
 
-@pytest.mark.parametrize("n", [2, 3, 9, 64, 65, 1023, 1024, 1025, 2049, 5000, 20000, 100000])
+@pytest.mark.parametrize("n", [2, 3, 9, 64, 65, 1023, 1024, 1025, 2049, 4681, 5000, 20000, 100000,
+                               1000000])
 def test_random_sizes_vs_oracle(n):
     pts = O.random_cube(n, 11)
     say("sort random", n)
@@ -109,3 +106,14 @@ def test_ten_million_points_vs_oracle():
     got = _device_sort(pts)
     want = O.spatial_sort(pts)
     assert np.array_equal(got, want)
+
+
+def test_bad_order_entry_is_an_error():
+    pts = O.random_cube(6000, 4)
+    order = np.arange(6000, dtype=np.int32)
+    order[123] = 6000
+    say("sort bad index")
+    with pytest.raises(RuntimeError, match="outside"):
+        D.spatial_sort(pts, order)
+    # the library still sorts afterwards
+    assert np.array_equal(D.spatial_sort(pts), O.spatial_sort(pts))
