@@ -1,31 +1,38 @@
 #!/usr/bin/env python3
-"""Headline benchmark: exact orient3d + in_sphere throughput (bit-exact) on MI355X.
+"""Headline benchmark: del3d's hot path on MI355X, bit-exact.
 
-Workload (the degenerate-lattice configuration): the 32x32x32 integer lattice
-and 2^24 mixed queries (half orient3d on lattice planes and unit-cube corners,
-half in_sphere on exactly cospherical unit-cube corners and lattice spheres,
-every in_sphere query positively oriented); see del3d_amd/workloads.py. A
-step is one batched call over all 2^24 device-resident queries.
+Workload (the large configuration): 10^7 points (even indices N(0,1)^3,
+odd indices uniform in [-1,1)^3, seed 3; del3d_amd/workloads.py), the Hilbert
+spatial sort of all of them, and 2^28 random orient3d / in_sphere queries
+(half each) over their indices, generated on the device. A step is one pass
+of the predicate half over all 2^28 device-resident queries (four calls of
+2^26, the library's launch-sequence size); the sort half is timed beside it,
+one del3d_spatial_sort call of all 10^7 points per step.
 
 Prints one JSON line. value = whole-job Gpred/s = queries * steps / timed
-seconds (HIP events on the work stream around K back-to-back calls, device
-synchronized on both sides, no synchronization inside). roofline: the
-dominant kernel (the fast path) with the FP64 work the predicates' filters
-define per query (111 ops per in_sphere, 32 per orient3d), over that
-kernel's average duration from HIP events recorded around it in every timed
-call. cpu_baseline: the reference (compiled from its own sources into
-oracle/_ref) or, where that was not built, the oracle port, on one core,
-over the same 2^24 queries; its signs are compared with the GPU's.
+seconds (HIP events on the work stream around the K back-to-back steps,
+device synchronized on both sides, no synchronization inside).
+roofline: the dominant kernel (the predicates' fast path, which gathers the
+points from HBM): algorithmic bytes per launch (20 B index record + 1 B op +
+1 B sign + 24 B per referenced point: 142 B per in_sphere, 118 B per
+orient3d) over that kernel's average duration from HIP events recorded
+around it in every timed launch; traffic = HBM bytes per launch from the
+committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary. cpu_baseline: the
+reference (compiled from its own sources into oracle/_ref), one core, on the
+first 2^24 queries of the same workload; cpu_baseline_all_cores the same
+library split over the host's cores (at most 16). The sort carries its own
+roofline (32 compulsory bytes per point) and the reference's single-core
+time for the same 10^7 points, whose permutation is compared with the GPU's.
 
-other_configs (skipped with --headline-only): the large configuration (10^7
-points, 2^28 gathered queries), the batched one (4096 point sets, one
-autotuned plan replayed as a graph, against one call per set) and the served
-one (a Poisson job stream through the serving loop, p50/p99 latency, against
-plain per-request calls), the coplanar predicates (2^24 queries on a
-tilted planar grid), the circumcentres (2^22 cells, lazy and exact
-forms) and the replay of the reference driver's predicate calls on 10,000
-rand48 points, each with its own bit-exactness checks; --config runs one of
-them alone.
+other_configs (skipped with --headline-only): the degenerate lattice (32^3
+lattice, 2^24 queries, exact-fallback and perturbation heavy), the batched
+configuration (4096 point sets: one autotuned plan replayed as a graph
+against one call per set, plus their sorts in one call against one call
+per set), the served one (a Poisson job stream through the serving loop,
+p50/p99 latency, against plain per-request calls), the coplanar predicates,
+the circumcentres, the exact-heavy cospherical stream, and the replay of the
+reference driver's predicate calls on 10,000 rand48 points, each with its
+own bit-exactness checks; --config runs one of them alone.
 """
 from __future__ import annotations
 
@@ -70,18 +77,6 @@ def cpu_baseline(pts, op, idx, gpu_signs, sample: int):
     }
 
 
-def load_traffic():
-    """HBM bytes per fast-path launch from the committed rocprofv3 counter
-    summary (FETCH_SIZE + WRITE_SIZE, calibrated on a copy kernel), if any."""
-    f = ROOT / "profiles" / "pred_batch_packed_lattice_traffic.json"
-    if not f.exists():
-        return None
-    try:
-        return json.loads(f.read_text()).get("bytes_per_launch")
-    except (OSError, ValueError):
-        return None
-
-
 def _workloads_lib():
     import ctypes as C
     lib = C.CDLL(str(ROOT / "del3d_amd" / "libdel3d_workloads.so"))
@@ -105,40 +100,6 @@ def _events_ms(D, stream, fn, steps):
     e1.record(stream)
     e1.synchronize()
     return e0.elapsed_ms(e1)
-
-
-def run_large(args) -> dict:
-    """10^7 points (half N(0,1)^3, half uniform [-1,1)^3) and 2^28 random
-    queries over them (half orient3d, half in_sphere), generated on the
-    device; one step = one batched call over all queries."""
-    import del3d_amd as D
-    from del3d_amd import workloads as W
-    n_pts, n_q, seed = 10_000_000, 1 << 28, 3
-    pts = W.c3_points(n_pts, seed=seed)
-    stream = D.Stream()
-    xyz = D.DeviceArray.from_host(pts)
-    po = np.array([0, n_pts], np.int64)
-    qo = np.array([0, n_q], np.int64)
-    dpo, dqo = D.DeviceArray.from_host(po), D.DeviceArray.from_host(qo)
-    op = D.DeviceArray((n_q,), np.uint8)
-    idx = D.DeviceArray((n_q, 5), np.int32)
-    out = D.DeviceArray((n_q,), np.int8)
-    G = _workloads_lib()
-    assert G.del3d_gen_queries(seed, dpo.ptr, dqo.ptr, 1, n_q, op.ptr, idx.ptr, stream.handle) == 0
-    st = D.predicates_stats(xyz, op, idx, out, stream)
-    for _ in range(args.warmup):
-        D.predicates(xyz, op, idx, out, stream)
-    ms = _events_ms(D, stream, lambda: D.predicates(xyz, op, idx, out, stream), args.steps)
-    sample = np.random.default_rng(0).choice(n_q, size=1 << 18, replace=False)
-    sample.sort()
-    sop, sidx = W.hashed_queries(seed, n_pts, sample)
-    got = out.to_host()[sample]
-    want = _oracle_sample(pts, sop, sidx)
-    return {"config": "large", "points": n_pts, "queries": n_q,
-            "gpred_per_s": round(n_q * args.steps / (ms / 1e3) / 1e9, 3),
-            "ms_per_step": round(ms / args.steps, 3), "decided_by": st,
-            "bit_exact_sample": bool(np.array_equal(got, want)), "sample": len(sample),
-            "note": "the spatial-sort half of this configuration is not run (GPU sort disabled)"}
 
 
 def run_batched(args) -> dict:
@@ -206,7 +167,45 @@ def run_batched(args) -> dict:
         checked += len(q)
     knobs = plan.knobs
     plan.close()
+    del op, idx, out, out1
+    # the sort half: every set's Hilbert sort in one call, against one call per set
+    H = D._hip._lib()
+    n_all = int(po[-1])
+    ident = D.DeviceArray.from_host(np.concatenate([np.arange(n, dtype=np.int32) for n in sizes]))
+    dord = D.DeviceArray((n_all,), np.int32)
+    dord1 = D.DeviceArray((n_all,), np.int32)
+
+    def reset(d):
+        D._hip.check(H.hipMemcpyAsync(d.ptr, ident.ptr, 4 * n_all, 3, stream.handle), "D2D")
+
+    def sort_all():
+        reset(dord)
+        D.spatial_sort_sets(xyz, po, dord, stream)
+
+    def sort_each():
+        reset(dord1)
+        for si in range(len(sizes)):
+            rc = lib.del3d_spatial_sort(xyz.addr + 24 * int(po[si]), int(sizes[si]),
+                                        dord1.addr + 4 * int(po[si]), int(sizes[si]), stream.handle)
+            assert rc == 0, lib.del3d_last_error()
+    sort_all()
+    ms_sort = _events_ms(D, stream, sort_all, 2) / 2
+    ms_sort1 = _events_ms(D, stream, sort_each, 1)
+    o_all, o_each = dord.to_host(), dord1.to_host()
+    sort_same = bool(np.array_equal(o_all, o_each))
+    sys.path.insert(0, str(ROOT / "tests"))
+    import oracle_lib as O
+    sort_exact = True
+    for si in pick[:4]:
+        a, b = int(po[si]), int(po[si + 1])
+        sort_exact &= bool(np.array_equal(o_all[a:b], O.spatial_sort(pts[a:b])))
     return {"config": "batched", "sets": len(sizes), "points": int(po[-1]), "queries": n_q,
+            "sort_mpts_per_s_one_call": round(n_all / (ms_sort / 1e3) / 1e6, 2),
+            "sort_ms_one_call": round(ms_sort, 2),
+            "sort_mpts_per_s_one_call_per_set": round(n_all / (ms_sort1 / 1e3) / 1e6, 2),
+            "sort_ms_one_call_per_set": round(ms_sort1, 2),
+            "sort_speedup_vs_one_call_per_set": round(ms_sort1 / ms_sort, 2),
+            "sort_same_batched_vs_per_set": sort_same, "sort_bit_exact_oracle_sets": sort_exact,
             "gpred_per_s_graph": round(n_q * args.steps / (ms_graph / 1e3) / 1e9, 3),
             "ms_per_step_graph": round(ms_graph / args.steps, 3),
             "gpred_per_s_one_call_per_set": round(n_q * steps1 / (ms_single / 1e3) / 1e9, 3),
@@ -216,8 +215,7 @@ def run_batched(args) -> dict:
             "autotuned_knobs": knobs, "autotune_best_ms": round(tuned_ms, 3),
             "autotune_s": round(t_tune, 2), "points_gen_s": round(t_gen, 1), "decided_by": st,
             "same_signs_batched_vs_per_set": same, "oracle_checked_queries": checked,
-            "bit_exact_oracle_sample": exact,
-            "note": "the spatial-sort half of this configuration is not run (GPU sort disabled)"}
+            "bit_exact_oracle_sample": exact}
 
 
 def run_served(args) -> dict:
@@ -366,6 +364,116 @@ def run_served(args) -> dict:
                     "pinned host buffers and the device"}
 
 
+def sphere_lattice_points(radius: int = 975) -> np.ndarray:
+    """Every integer point (x, y, z) with x^2 + y^2 + z^2 = radius^2 (9,750
+    points for radius 975): five of them are always exactly cospherical."""
+    x = np.arange(-radius, radius + 1)
+    X, Y = np.meshgrid(x, x, indexing="ij")
+    r = radius * radius - X * X - Y * Y
+    ok = r >= 0
+    Z = np.sqrt(np.where(ok, r, 0)).astype(np.int64)
+    ok &= Z * Z == r
+    pts = [np.stack([X[ok], Y[ok], Z[ok]], axis=1)]
+    neg = ok & (Z > 0)
+    pts.append(np.stack([X[neg], Y[neg], -Z[neg]], axis=1))
+    return np.concatenate(pts).astype(np.float64)
+
+
+def run_exact(args) -> dict:
+    """The exact-heavy stream: 2^22 in_sphere queries, each of five random
+    points of the 9,750 integer points on the sphere of radius 975. Every
+    query is exactly cospherical, so the FP64 filter cannot decide it; the
+    differences span up to 1,950 grid units, past the certified grid tier's
+    2^8, so the exact integer tier decides each query and runs the symbolic
+    perturbation. One batched call per step; the kernels' durations from the
+    timed form; a 2^18 sample against the compiled reference."""
+    import del3d_amd as D
+    sys.path.insert(0, str(ROOT / "tests"))
+    import oracle_lib as O
+    rng = np.random.default_rng(9)
+    pts = sphere_lattice_points(975)
+    n_pts, n_q = len(pts), 1 << 22
+    op = np.ones(n_q, np.uint8)
+    idx = rng.integers(0, n_pts, size=(n_q, 5)).astype(np.int32)
+    stream = D.Stream()
+    xyz = D.DeviceArray.from_host(pts)
+    dop, didx = D.DeviceArray.from_host(op), D.DeviceArray.from_host(idx)
+    out = D.DeviceArray((n_q,), np.int8)
+    st = D.predicates_stats(xyz, dop, didx, out, stream)
+    for _ in range(args.warmup):
+        D.predicates(xyz, dop, didx, out, stream)
+    ms = _events_ms(D, stream, lambda: D.predicates(xyz, dop, didx, out, stream), args.steps)
+    tiers = D.predicates_timed(xyz, dop, didx, out, stream)
+    got = out.to_host()
+    s = np.sort(rng.choice(n_q, size=1 << 18, replace=False))
+    kind = "ref" if O.have_ref() else "oracle"
+    t1 = time.perf_counter()
+    want = O.pred_batch(pts, op[s], idx[s], kind)
+    dt = time.perf_counter() - t1
+    t = ms / args.steps / 1e3
+    return {"config": "exact", "points": n_pts, "queries": n_q,
+            "gpred_per_s": round(n_q / t / 1e9, 4), "ms_per_step": round(ms / args.steps, 3),
+            "decided_by": st,
+            "kernel_ms": {"fast": round(tiers[0], 3), "exact_regs": round(tiers[1], 3),
+                          "exact_wide": round(tiers[2], 3)},
+            "exact_tier_gqueries_per_s": round((st["exact_regs"] + st["exact_wide"]) / (tiers[1] + tiers[2])
+                                               / 1e6, 4) if tiers[1] + tiers[2] > 0 else None,
+            "bit_exact_sample": bool(np.array_equal(got[s], want)), "sample": len(s),
+            "cpu_baseline": {"value": round(len(s) / dt / 1e9, 6), "unit": "Gpred/s", "cores": 1,
+                             "kind": "reference" if kind == "ref" else "port",
+                             "sample": f"{len(s)} of the queries, one thread"}}
+
+
+SORT_BYTES_PER_POINT = 32  # 24 B coordinates read, 4 B order read, 4 B order written
+
+
+def run_sort(args, n: int = 10_000_000, seed: int = 3, pts=None) -> dict:
+    """The Hilbert spatial sort of the large configuration's 10^7 points
+    (device-resident), one del3d_spatial_sort call per step, each on a fresh
+    copy of the identity order; HIP events on the work stream around each
+    call. The permutation is compared with the oracle's."""
+    import del3d_amd as D
+    from del3d_amd import workloads as W
+    H = D._hip._lib()
+    if pts is None:
+        pts = W.c3_points(n, seed=seed)
+    stream = D.Stream()
+    xyz = D.DeviceArray.from_host(pts)
+    ident = D.DeviceArray.from_host(np.arange(n, dtype=np.int32))
+    order = D.DeviceArray((n,), np.int32)
+
+    def one(ev=None):
+        D._hip.check(H.hipMemcpyAsync(order.ptr, ident.ptr, 4 * n, 3, stream.handle), "D2D")
+        if ev:
+            ev[0].record(stream)
+        D.spatial_sort_device(xyz, order, stream)
+        if ev:
+            ev[1].record(stream)
+
+    for _ in range(args.warmup + 1):
+        one()
+    evs = [(D.Event(), D.Event()) for _ in range(args.steps)]
+    w0 = time.perf_counter()
+    for ev in evs:
+        one(ev)
+    stream.synchronize()
+    wall = time.perf_counter() - w0
+    ms = [a.elapsed_ms(b) for a, b in evs]
+    got = order.to_host()
+    res = {"config": "sort", "points": n, "steps": args.steps,
+           "mpts_per_s": round(n / (float(np.mean(ms)) / 1e3) / 1e6, 2),
+           "ms_per_sort": round(float(np.mean(ms)), 3), "ms_min": round(float(np.min(ms)), 3),
+           "wall_ms_per_sort": round(1e3 * wall / args.steps, 3)}
+    if not getattr(args, "no_check", False):
+        sys.path.insert(0, str(ROOT / "tests"))
+        import oracle_lib as O
+        t0 = time.perf_counter()
+        want = O.spatial_sort(pts, which="ref" if O.have_ref() else "oracle")
+        res["cpu_seconds_one_core"] = round(time.perf_counter() - t0, 3)
+        res["bit_exact_vs_cpu"] = bool(np.array_equal(got, want))
+    return res
+
+
 # FP64 operations of the coplanar filters per query (src/predicates2.cpp:
 # 111-118 and 182-230): orient2d 9; in-circle 109 (differences 15, lifts 15,
 # normal 9 and its square 5, 2x2 minors 18, 3x3 minors 20, determinant 7,
@@ -496,52 +604,184 @@ def run_c1_trace(args) -> dict:
     return line
 
 
-def main() -> int:
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=50)
-    ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--queries", type=int, default=1 << 24)
-    ap.add_argument("--cpu-sample", type=int, default=1 << 24)
-    ap.add_argument("--headline-only", action="store_true",
-                    help="only the degenerate-lattice line, without the other configurations")
-    ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--config", default="lattice",
-                    choices=["lattice", "large", "batched", "served", "coplanar", "circumcenters",
-                             "c1_trace"],
-                    help="lattice: the headline line; large / batched: the other configurations")
-    ap.add_argument("--calib-copy-mib", type=int, default=0,
-                    help="only run the calibration copy kernel (for counter scaling)")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print(f"bench.py measures one GPU: every box here has one GPU, so --gpus {args.gpus} "
-              "is not run (predicate batches would shard by query across GPUs, with no exchange)", file=sys.stderr)
-        return 2
+PRED_BYTES_INSPHERE = 20 + 1 + 1 + 5 * 24   # index record, op, sign, five gathered points
+PRED_BYTES_ORIENT = 20 + 1 + 1 + 4 * 24     # the record is read whole; four points
 
+
+def load_traffic(name: str):
+    """HBM bytes per launch of a kernel from a committed rocprofv3 counter
+    summary (FETCH_SIZE + WRITE_SIZE, calibrated on a copy kernel), if any."""
+    f = ROOT / "profiles" / name
+    if not f.exists():
+        return None
+    try:
+        return json.loads(f.read_text()).get("bytes_per_launch")
+    except (OSError, ValueError):
+        return None
+
+
+def host_threads() -> int:
+    return max(1, min(16, os.cpu_count() or 1))
+
+
+def run_headline(args) -> dict:
+    """The large configuration: sort of 10^7 points and 2^28 queries."""
     import del3d_amd as D
     from del3d_amd import workloads as W
+    sys.path.insert(0, str(ROOT / "tests"))
+    import oracle_lib as O
+    n_pts, n_q, seed = 10_000_000, 1 << 28, 3
+    chunk = 1 << 26
+    t0 = time.perf_counter()
+    pts = W.c3_points(n_pts, seed=seed)
+    stream = D.Stream()
+    xyz = D.DeviceArray.from_host(pts)
+    po = np.array([0, n_pts], np.int64)
+    qo = np.array([0, n_q], np.int64)
+    dpo, dqo = D.DeviceArray.from_host(po), D.DeviceArray.from_host(qo)
+    op = D.DeviceArray((n_q,), np.uint8)
+    idx = D.DeviceArray((n_q, 5), np.int32)
+    out = D.DeviceArray((n_q,), np.int8)
+    G = _workloads_lib()
+    assert G.del3d_gen_queries(seed, dpo.ptr, dqo.ptr, 1, n_q, op.ptr, idx.ptr, stream.handle) == 0
+    stream.synchronize()
+    n_ins = int(np.count_nonzero(op.to_host()))
+    t_gen = time.perf_counter() - t0
+    st = D.predicates_stats(xyz, op, idx, out, stream)
+    lib = D.lib()
 
-    if args.calib_copy_mib:
-        import ctypes as C
-        lib = C.CDLL(str(ROOT / "del3d_amd" / "libdel3d_calib.so"))
-        lib.del3d_calib_copy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p]
-        nbytes = args.calib_copy_mib << 20
-        a = D.DeviceArray((nbytes,), np.uint8)
-        b = D.DeviceArray((nbytes,), np.uint8)
-        a.zero()
-        for _ in range(5):
-            lib.del3d_calib_copy(a.ptr, b.ptr, nbytes, None)
-        D.synchronize()
-        print(json.dumps({"calib_copy_bytes": nbytes, "launches": 5}))
-        return 0
+    def step(evs=None):
+        for c in range(n_q // chunk):
+            a = c * chunk
+            if evs is None:
+                rc = lib.del3d_predicates(xyz.ptr, n_pts, op.addr + a, idx.addr + 20 * a, chunk,
+                                          out.addr + a, stream.handle)
+            else:
+                e = evs[c]
+                rc = lib.del3d_predicates_ev(xyz.ptr, n_pts, op.addr + a, idx.addr + 20 * a, chunk,
+                                             out.addr + a, stream.handle, e[0].handle, e[1].handle)
+            assert rc == 0, lib.del3d_last_error()
 
-    if args.config != "lattice":
-        res = {"large": run_large, "batched": run_batched, "served": run_served,
-               "coplanar": run_coplanar, "circumcenters": run_circumcenters,
-               "c1_trace": run_c1_trace}[args.config](args)
-        print(json.dumps(res), flush=True)
-        return 0
+    for _ in range(args.warmup):
+        step()
+    evs = [[(D.Event(), D.Event()) for _ in range(n_q // chunk)] for _ in range(args.steps)]
+    e0, e1 = D.Event(), D.Event()
+    D.synchronize()
+    w0 = time.perf_counter()
+    e0.record(stream)
+    for ev in evs:
+        step(ev)
+    e1.record(stream)
+    e1.synchronize()
+    wall = time.perf_counter() - w0
+    total_ms = e0.elapsed_ms(e1)
+    kern_ms = [a.elapsed_ms(b) for ev in evs for a, b in ev]
+    value = n_q * args.steps / (total_ms / 1e3) / 1e9
+    kern_avg = float(np.mean(kern_ms))
+    bytes_launch = PRED_BYTES_INSPHERE * n_ins + PRED_BYTES_ORIENT * (n_q - n_ins)
+    bytes_launch //= n_q // chunk  # per 2^26-query launch
+    achieved = bytes_launch / (kern_avg / 1e3) / 1e9
+    signs = out.to_host()
+    # bit-exactness: a 2^18 sample of the queries against the compiled reference
+    sample = np.sort(np.random.default_rng(0).choice(n_q, size=1 << 18, replace=False))
+    sop, sidx = W.hashed_queries(seed, n_pts, sample)
+    ref_kind = "ref" if O.have_ref() else "oracle"
+    exact_sample = bool(np.array_equal(signs[sample], O.pred_batch(pts, sop, sidx, ref_kind)))
+    line = {
+        "metric": METRIC,
+        "value": round(value, 4),
+        "unit": "Gpred/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(total_ms / args.steps, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "large: 10^7 points (even indices N(0,1)^3, odd uniform in [-1,1)^3, "
+                               "seed 3), Hilbert sort of all points + 2^28 random orient3d/in_sphere "
+                               "queries (half each) over their indices",
+                   "points": n_pts, "queries": n_q, "in_sphere": n_ins, "seed": seed},
+        "roofline": {
+            "bound": "hbm",
+            "kernel": "pred_batch_global",
+            "achieved": round(achieved, 2),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": load_traffic("pred_batch_global_large_traffic.json"),
+            "algorithmic_bytes_per_launch": int(bytes_launch),
+            "queries_per_launch": chunk,
+            "kernel_ms_avg": round(kern_avg, 4),
+            "note": "142 B per in_sphere, 118 B per orient3d (20 B index record, op, sign, 24 B per "
+                    "gathered point); FP64 filter work on the same launch: "
+                    + str(round((OPS_INSPHERE * n_ins + OPS_ORIENT * (n_q - n_ins)) / (n_q // chunk)
+                                / (kern_avg / 1e3) / 1e12, 3)) + " TFLOP/s of 78.6",
+        },
+        "decided_by": st,
+        "bit_exact_vs_cpu_sample": exact_sample,
+        "wall_s_timed": round(wall, 4),
+        "input_gen_s": round(t_gen, 2),
+    }
+    if not args.no_cpu_baseline:
+        kind = "reference" if O.have_ref() else "port"
+        ns = min(n_q, args.cpu_sample)
+        q = np.arange(ns)
+        cop, cidx = W.hashed_queries(seed, n_pts, q)
+        t1 = time.perf_counter()
+        ref1 = O.pred_batch(pts, cop, cidx, ref_kind)
+        dt1 = time.perf_counter() - t1
+        th = host_threads()
+        ns_mt = min(n_q, 2 * args.cpu_sample)
+        cop2, cidx2 = (cop, cidx) if ns_mt == ns else W.hashed_queries(seed, n_pts, np.arange(ns_mt))
+        t1 = time.perf_counter()
+        refm = O.pred_batch(pts, cop2, cidx2, ref_kind, threads=th) if kind == "reference" else None
+        dtm = time.perf_counter() - t1
+        line["cpu_baseline"] = {
+            "value": round(ns / dt1 / 1e9, 6), "unit": "Gpred/s", "cores": 1, "kind": kind,
+            "sample": f"the first {ns} of the {n_q} queries of the same workload, one thread",
+            "seconds": round(dt1, 3)}
+        line["bit_exact_vs_cpu_sample"] = exact_sample and bool(np.array_equal(ref1, signs[:ns]))
+        if refm is not None:
+            line["cpu_baseline_all_cores"] = {
+                "value": round(ns_mt / dtm / 1e9, 6), "unit": "Gpred/s", "cores": th, "kind": kind,
+                "sample": f"the first {ns_mt} queries, split over {th} std::threads",
+                "seconds": round(dtm, 3)}
+            line["bit_exact_vs_cpu_sample"] &= bool(np.array_equal(refm, signs[:ns_mt]))
+        line["gpu_over_cpu"] = round(value / line["cpu_baseline"]["value"], 1)
+    else:
+        line["cpu_baseline"] = None
+    del op, idx, out
+    if getattr(args, "no_sort", False):
+        return line
+    # the sort half: one del3d_spatial_sort of all 10^7 points per step
+    sargs = argparse.Namespace(steps=max(3, args.steps // 2), warmup=1,
+                               no_check=args.no_cpu_baseline)
+    srt = run_sort(sargs, n=n_pts, seed=seed, pts=pts)
+    sort_ms = srt["ms_per_sort"]
+    sort_gbs = SORT_BYTES_PER_POINT * n_pts / (sort_ms / 1e3) / 1e9
+    srt["roofline"] = {"bound": "hbm", "achieved": round(sort_gbs, 2), "peak": HBM_PEAK_GBS,
+                       "unit": "GB/s", "frac": round(sort_gbs / HBM_PEAK_GBS, 5),
+                       "algorithmic_bytes": SORT_BYTES_PER_POINT * n_pts,
+                       "note": "32 compulsory bytes per point (24 B coordinates read, 4 B order "
+                               "read, 4 B order written); the Lomuto rounds visit each point about "
+                               "70 times, which this figure does not count"}
+    line["hilbert_sort_mpts_per_s"] = srt["mpts_per_s"]
+    line["sort"] = srt
+    line["step_ms_sort_plus_queries"] = round(sort_ms + total_ms / args.steps, 3)
+    return line
 
+
+def run_lattice(args) -> dict:
+    """The degenerate lattice: the 32^3 integer lattice and 2^24 mixed
+    queries (half orient3d on lattice planes and unit-cube corners, half
+    in_sphere on exactly cospherical unit-cube corners and lattice spheres),
+    one batched call per step; the fast path (the packed LDS kernel) timed
+    with events around every launch; every sign against the reference."""
+    import del3d_amd as D
+    from del3d_amd import workloads as W
     t_gen = time.perf_counter()
     pts, op, idx = W.c2(args.queries, seed=2)
     t_gen = time.perf_counter() - t_gen
@@ -579,29 +819,20 @@ def main() -> int:
     ops = OPS_INSPHERE * n_ins + OPS_ORIENT * (n_q - n_ins)
     achieved = ops / (fast_avg / 1e3) / 1e12
     line = {
-        "metric": METRIC,
-        "value": round(value, 4),
-        "unit": "Gpred/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
+        "config": "lattice",
+        "gpred_per_s": round(value, 4),
         "ms_per_step": round(total_ms / args.steps, 4),
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f64",
-        "data": "synthetic",
-        "config": {"workload": "degenerate-lattice: 32^3 integer lattice, 2^24 mixed "
-                               "orient3d/in_sphere queries, all in_sphere exactly cospherical",
-                   "points": len(pts), "queries": n_q, "in_sphere": n_ins, "seed": 2},
+        "workload": "32^3 integer lattice, 2^24 mixed orient3d/in_sphere queries, all in_sphere "
+                    "exactly cospherical",
+        "points": len(pts), "queries": n_q, "in_sphere": n_ins, "seed": 2,
         "roofline": {
-            "bound": "mfma",
+            "bound": "fp64-valu",
             "kernel": "pred_batch_packed",
             "achieved": round(achieved, 4),
             "peak": FP64_PEAK_TFLOPS,
             "unit": "TFLOP/s",
             "frac": round(achieved / FP64_PEAK_TFLOPS, 4),
-            "traffic": load_traffic(),
+            "traffic": load_traffic("pred_batch_packed_lattice_traffic.json"),
             "algorithmic_ops_per_launch": ops,
             "kernel_ms_avg": round(fast_avg, 4),
             "note": "FP64 VALU (no MFMA on this path): 111 FP64 ops per in_sphere and 32 per "
@@ -611,9 +842,6 @@ def main() -> int:
         "decided_by": st,
         "wall_s_timed": round(wall, 4),
         "input_gen_s": round(t_gen, 2),
-        "hilbert_sort_mpts_per_s": None,
-        "hilbert_sort_note": "not measured: the GPU spatial sort is switched off after device "
-                             "faults whose cause has not been shown",
     }
     if not args.no_cpu_baseline:
         cb = cpu_baseline(pts, op, idx, signs, args.cpu_sample)
@@ -622,22 +850,79 @@ def main() -> int:
         line["gpu_over_cpu"] = round(value / cb["value"], 1)
     else:
         line["cpu_baseline"] = None
-    del xyz, dop, didx, out
+    return line
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--queries", type=int, default=1 << 24,
+                    help="queries of the degenerate-lattice configuration")
+    ap.add_argument("--cpu-sample", type=int, default=1 << 24,
+                    help="queries the single-core CPU baseline runs (twice that on all cores)")
+    ap.add_argument("--headline-only", action="store_true",
+                    help="only the large configuration, without the other configurations")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-sort", action="store_true",
+                    help="headline without its sort half (for counter runs of the predicate kernel)")
+    ap.add_argument("--no-check", action="store_true",
+                    help="skip the CPU comparisons of a --config run (for profiler runs)")
+    ap.add_argument("--config", default="headline",
+                    choices=["headline", "lattice", "batched", "served", "coplanar", "circumcenters",
+                             "c1_trace", "sort", "exact"],
+                    help="headline: the large configuration (the bench line); the others alone")
+    ap.add_argument("--calib-copy-mib", type=int, default=0,
+                    help="only run the calibration copy kernel (for counter scaling)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: every box here has one GPU, so --gpus {args.gpus} "
+              "is not run (predicate batches would shard by query across GPUs, with no exchange)",
+              file=sys.stderr)
+        return 2
+
+    import del3d_amd as D
+
+    if args.calib_copy_mib:
+        import ctypes as C
+        lib = C.CDLL(str(ROOT / "del3d_amd" / "libdel3d_calib.so"))
+        lib.del3d_calib_copy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p]
+        nbytes = args.calib_copy_mib << 20
+        a = D.DeviceArray((nbytes,), np.uint8)
+        b = D.DeviceArray((nbytes,), np.uint8)
+        a.zero()
+        for _ in range(5):
+            lib.del3d_calib_copy(a.ptr, b.ptr, nbytes, None)
+        D.synchronize()
+        print(json.dumps({"calib_copy_bytes": nbytes, "launches": 5}))
+        return 0
+
+    configs = {"lattice": run_lattice, "batched": run_batched, "served": run_served,
+               "coplanar": run_coplanar, "circumcenters": run_circumcenters,
+               "c1_trace": run_c1_trace, "sort": run_sort, "exact": run_exact}
+    if args.config != "headline":
+        print(json.dumps(configs[args.config](args)), flush=True)
+        return 0
+
+    line = run_headline(args)
     if not args.headline_only:
         # the other configurations, reported beside the headline (each one's
         # own bit-exactness checks included); a failure is reported, not fatal
-        sub = argparse.Namespace(steps=max(3, args.steps // 5), warmup=1)
+        sub = argparse.Namespace(steps=max(3, args.steps // 4), warmup=1, queries=args.queries,
+                                 cpu_sample=args.cpu_sample, no_cpu_baseline=args.no_cpu_baseline,
+                                 no_check=args.no_cpu_baseline)
         others = {}
-        for name, fn in (("large", run_large), ("batched", run_batched), ("served", run_served),
-                         ("coplanar", run_coplanar), ("circumcenters", run_circumcenters),
-                         ("c1_trace", run_c1_trace)):
+        for name in ("lattice", "batched", "served", "exact", "coplanar", "circumcenters", "c1_trace"):
             t0 = time.perf_counter()
             try:
-                others[name] = fn(sub)
+                others[name] = configs[name](sub)
             except Exception as e:  # noqa: BLE001 - reported in the JSON line
                 others[name] = {"error": f"{type(e).__name__}: {e}"}
             others[name]["wall_s"] = round(time.perf_counter() - t0, 1)
         line["other_configs"] = others
+        served = others.get("served", {})
+        line["served_p99_ms"] = served.get("p99_ms")
     print(json.dumps(line), flush=True)
     return 0
 

@@ -22,12 +22,17 @@
 //
 // Median splits. A Lomuto round on [L, R] with pivot slot pv swaps a[pv]
 // with a[R], then moves the elements less than the pivot, in order, to L,
-// L+1, ...; an element that is not less, starting at p, is carried along the
-// chain q <- Lpos[q - L] (Lpos[r] = position of the r-th smaller element)
-// while q < lim = L + nless; the one whose chain ends at lim goes to R, and
-// the pivot to lim. A round is therefore: flags, a per-range prefix, the
-// Lpos table, one scatter. The recursion plan (every range, axis and
-// direction) does not depend on the data, so the host builds it once and
+// L+1, ...; the elements that are not less form a window that starts at the
+// next free front slot, and each later smaller element swaps with the
+// window's front. The pivot ends at lim = L + (number of smaller elements)
+// and what would have landed on lim goes to R. Two closed forms of one round
+// are used. Device-wide, destination-driven: a slot in [lim, R) keeps an
+// element that is not less and otherwise gathers from the backward chain
+// s <- L + rank(s), which needs only the flag bits (every write goes to
+// consecutive slots). In LDS, source-driven: an element that is not less,
+// at p, follows the forward chain q <- Lpos[q - L] (Lpos[r] = slot of the
+// r-th smaller element) while q < lim. The recursion plan (every range, axis
+// and direction) does not depend on the data, so the host builds it once and
 // uploads it in one copy; only the arrangement inside each range is data.
 //   - a select on more than kFin elements runs device-wide rounds: tiles of
 //     kTile elements, records in structure-of-arrays form ping-ponged between
@@ -522,16 +527,20 @@ __global__ __launch_bounds__(256) void k_rd_init(const RTask* __restrict__ tasks
 }
 
 // Round part 1 (per tile): copies the margins the previous round left in
-// buffer 1, then flags the tile's elements that are less than the pivot
-// (one bit each) and counts them.
+// buffer 1, then flags the tile's elements that are less than the pivot (one
+// bit each), counts them, and records each 64-element word's exclusive
+// prefix inside the tile. Wavefront w takes words 16w .. 16w+15.
 __global__ __launch_bounds__(kTileThreads) void k_rd_count(Bufs bufs, const RTask* __restrict__ tasks,
                                                            uint32_t ntasks, const RTile* __restrict__ tiles,
                                                            uint32_t ntiles, const RState* __restrict__ st,
                                                            uint64_t* __restrict__ mask,
+                                                           uint16_t* __restrict__ wloc,
                                                            uint32_t* __restrict__ tcount,
                                                            uint32_t* __restrict__ ring, uint32_t round,
                                                            uint32_t* __restrict__ err) {
+    constexpr int kWordsPerWave = kTileWords / (kTileThreads / 64);
     __shared__ uint32_t s_cnt[kTileThreads / 64];
+    __shared__ uint16_t s_wl[kTileWords];
     const uint32_t tile = blockIdx.x;
     if (tile == 0 && threadIdx.x == 0) ring[(round + 1) % kRing] = 0;
     if (tile >= ntiles) return;
@@ -560,8 +569,10 @@ __global__ __launch_bounds__(kTileThreads) void k_rd_count(Bufs bufs, const RTas
         return;
     }
     if (!rd_range_ok(S, bufs.n) || (S.par & ~1)) {
-        if (threadIdx.x == 0) raise_err(err, kErrRound);
-        if (threadIdx.x == 0) tcount[tile] = 0;
+        if (threadIdx.x == 0) {
+            raise_err(err, kErrRound);
+            tcount[tile] = 0;
+        }
         return;
     }
     const double* key = axis_of(bufs, S.par, T.axis);
@@ -570,21 +581,30 @@ __global__ __launch_bounds__(kTileThreads) void k_rd_count(Bufs bufs, const RTas
     const double kr = key[S.R];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     uint32_t c = 0;
-    for (int it = 0; it < kTileIters; ++it) {
-        const int w = it * (kTileThreads / 64) + wave;
+    for (int it = 0; it < kWordsPerWave; ++it) {
+        const int w = wave * kWordsPerWave + it;
         const int64_t p = t0 + (int64_t)w * 64 + lane;
         bool f = false;
         if (p < t1 && p >= S.L && p < S.R) f = key_less(p == pv ? kr : key[p], pk, T.up);
         const uint64_t m = __ballot(f);
-        if (lane == 0) mask[(uint64_t)tile * kTileWords + w] = m;
+        if (lane == 0) {
+            mask[(uint64_t)tile * kTileWords + w] = m;
+            s_wl[w] = (uint16_t)c;
+        }
         c += (uint32_t)__popcll(m);
     }
     if (lane == 0) s_cnt[wave] = c;
     __syncthreads();
+    if (threadIdx.x < kTileWords) {
+        const int w = threadIdx.x, ww = w / kWordsPerWave;
+        uint32_t before = 0;
+        for (int i = 0; i < ww; ++i) before += s_cnt[i];
+        wloc[(uint64_t)tile * kTileWords + w] = (uint16_t)(before + s_wl[w]);
+    }
     if (threadIdx.x == 0) {
-        uint32_t s = 0;
-        for (int w = 0; w < kTileThreads / 64; ++w) s += s_cnt[w];
-        tcount[tile] = s;
+        uint32_t sum = 0;
+        for (int i = 0; i < kTileThreads / 64; ++i) sum += s_cnt[i];
+        tcount[tile] = sum;
     }
 }
 
@@ -615,94 +635,91 @@ __global__ __launch_bounds__(256) void k_rd_scan(const RTask* __restrict__ tasks
     if (threadIdx.x == 0) st[t].lim = S.L + (int64_t)carry;
 }
 
-// Round part 3 (per tile): elements less than the pivot go, in order, to
-// L + rank of the other buffer; Lpos[L + rank] = their position - L.
-__global__ __launch_bounds__(kTileThreads) void k_rd_flagged(Bufs bufs, const RTask* __restrict__ tasks,
-                                                             uint32_t ntasks, const RTile* __restrict__ tiles,
-                                                             uint32_t ntiles, const RState* __restrict__ st,
-                                                             const uint64_t* __restrict__ mask,
-                                                             const uint32_t* __restrict__ toff,
-                                                             uint32_t* __restrict__ lpos,
-                                                             uint32_t* __restrict__ err) {
-    __shared__ uint32_t s_pre[kTileWords];
-    const uint32_t tile = blockIdx.x;
-    if (tile >= ntiles) return;
-    const RTile tl = tiles[tile];
-    if (tl.task < 0 || (uint32_t)tl.task >= ntasks) return;  // k_rd_count raised it
-    const RState S = st[tl.task];
-    if (!S.active || !rd_range_ok(S, bufs.n) || (S.par & ~1)) return;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    if (wave == 0) {  // exclusive prefix of the 64 words' popcounts
-        const uint32_t c = (uint32_t)__popcll(mask[(uint64_t)tile * kTileWords + lane]);
-        uint32_t x = c;
-#pragma unroll
-        for (int d = 1; d < 64; d <<= 1) {
-            const uint32_t y = __shfl_up(x, d);
-            if (lane >= d) x += y;
-        }
-        s_pre[lane] = x - c;
-    }
-    __syncthreads();
-    const int src = S.par, dst = S.par ^ 1;
-    const int64_t pv = S.L + (S.R - S.L) / 2;
-    const int64_t base = (int64_t)toff[tile];
-    const uint64_t lt = (1ull << lane) - 1ull;
-    for (int it = 0; it < kTileIters; ++it) {
-        const int w = it * (kTileThreads / 64) + wave;
-        const uint64_t m = mask[(uint64_t)tile * kTileWords + w];
-        if (!((m >> lane) & 1ull)) continue;
-        const int64_t p = tl.start + (int64_t)w * 64 + lane;
-        const int64_t d = S.L + base + s_pre[w] + __popcll(m & lt);
-        if (d < S.L || d >= S.lim || S.lim > S.R || p < S.L || p >= S.R) {
-            raise_err(err, kErrRound);
-            continue;
-        }
-        copy_rec(bufs, src, dst, p == pv ? S.R : p, d);
-        lpos[d] = (uint32_t)(p - S.L);
-    }
-}
+// Round part 3 (one slot per thread, kSlices blocks per tile): the whole
+// Lomuto round as a permutation written into the other buffer.
+//   - an element less than the pivot, at p, goes to L + rank(p), where
+//     rank(p) = number of such elements in [L, p) (order kept);
+//   - a slot d in [lim, R) whose element is not less keeps it (the window of
+//     not-less elements only ever starts below lim, so such an element is
+//     never at its front);
+//   - a slot d in [lim, R) whose element is less receives the element at the
+//     window's front when the scan reached d, the slot L + rank(d); if the
+//     element that slot started with is less as well, it had itself taken
+//     the front of its own time, so s <- L + rank(s) is followed (strictly
+//     down) until it reaches a slot whose element is not less;
+//   - what lands on lim goes to R, and the pivot to lim
+//     (src/spatial_sort.cpp:41-50).
+// rank() needs only the flag words and their prefixes, which are small and
+// stay in cache; every write of this kernel is to consecutive slots. The
+// chains are long only when nearly every element is less than the pivot;
+// one slot per thread spreads them over the whole device.
+constexpr int kSlices = kTile / kTileThreads;
 
-// Round part 4 (per tile): the other elements follow their Lpos chains; the
-// pivot goes to lim; the task's first tile writes the next round's state.
-__global__ __launch_bounds__(kTileThreads) void k_rd_rest(Bufs bufs, const RTask* __restrict__ tasks,
+__global__ __launch_bounds__(kTileThreads) void k_rd_move(Bufs bufs, const RTask* __restrict__ tasks,
                                                           uint32_t ntasks, const RTile* __restrict__ tiles,
                                                           uint32_t ntiles, const RState* __restrict__ st,
                                                           RState* __restrict__ st_next,
                                                           const uint64_t* __restrict__ mask,
-                                                          const uint32_t* __restrict__ lpos,
+                                                          const uint16_t* __restrict__ wloc,
+                                                          const uint32_t* __restrict__ toff,
                                                           uint32_t* __restrict__ ring, uint32_t round,
                                                           uint32_t* __restrict__ err) {
-    const uint32_t tile = blockIdx.x;
+    const uint32_t tile = blockIdx.x / kSlices, slice = blockIdx.x % kSlices;
     if (tile >= ntiles) return;
     const RTile tl = tiles[tile];
-    if (tl.task < 0 || (uint32_t)tl.task >= ntasks) return;
+    if (tl.task < 0 || (uint32_t)tl.task >= ntasks) return;  // k_rd_count raised it
     const RTask T = tasks[tl.task];
     const RState S = st[tl.task];
-    const bool first = (int64_t)tile == T.tile0;
-    const bool ok = S.active && rd_range_ok(S, bufs.n) && !(S.par & ~1) && S.lim >= S.L && S.lim <= S.R;
+    const bool first = (int64_t)tile == T.tile0 && slice == 0;
+    const bool ok = S.active && rd_range_ok(S, bufs.n) && !(S.par & ~1) && S.lim >= S.L && S.lim <= S.R &&
+                    T.tile0 >= 0 && T.tile0 + T.ntiles <= (int64_t)ntiles && S.L >= T.b && S.R < T.e;
     if (S.active && !ok && first && threadIdx.x == 0) raise_err(err, kErrRound);
     const int64_t pv = S.L + (S.R - S.L) / 2;
     if (ok) {
         const int src = S.par, dst = S.par ^ 1;
-        const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-        const int64_t span = S.R - S.L;
-        for (int it = 0; it < kTileIters; ++it) {
-            const int w = it * (kTileThreads / 64) + wave;
-            const int64_t p = tl.start + (int64_t)w * 64 + lane;
-            if (p >= tl.start + tl.len || p < S.L || p >= S.R) continue;
-            if ((mask[(uint64_t)tile * kTileWords + w] >> lane) & 1ull) continue;
-            int64_t q = p;
-            int64_t hops = 0;
-            while (q < S.lim && hops <= span) {  // q strictly increases along the chain
-                q = S.L + (int64_t)lpos[q];
-                ++hops;
+        const int lane = threadIdx.x & 63;
+        const int w = (int)slice * (kTileThreads / 64) + (threadIdx.x >> 6);
+        const int64_t p = tl.start + (int64_t)w * 64 + lane;
+        const uint64_t gw = (uint64_t)tile * kTileWords + w;
+        const uint64_t m = mask[gw];
+        if (p < tl.start + tl.len && p >= S.L && p < S.R) {
+            const bool f = (m >> lane) & 1ull;
+            if (f) {  // less: to the front, in order
+                const int64_t d = S.L + (int64_t)toff[tile] + wloc[gw] + __popcll(m & ((1ull << lane) - 1ull));
+                if (d >= S.L && d < S.lim) copy_rec(bufs, src, dst, p == pv ? S.R : p, d);
+                else raise_err(err, kErrRound);
             }
-            const int64_t d = q == S.lim ? S.R : q;
-            if (hops > span || d <= S.lim || d > S.R) {
-                raise_err(err, hops > span ? kErrLoop : kErrRound);
-                continue;
+            if (p >= S.lim) {
+                int64_t s = p;
+                bool less = f;
+                // the three words rank(s) needs, carried so a hop costs one load latency
+                uint64_t ms = m;
+                uint32_t ws = wloc[gw], ts_off = toff[tile];
+                int64_t os = p - T.b;
+                const int64_t span = S.R - S.L;
+                int64_t hops = 0;
+                bool bad = false;
+                while (less) {
+                    if (++hops > span) {
+                        bad = true;
+                        break;
+                    }
+                    s = S.L + (int64_t)ts_off + ws + __popcll(ms & ((1ull << (os & 63)) - 1ull));
+                    if (s < S.L || s >= S.lim) {
+                        bad = true;
+                        break;
+                    }
+                    os = s - T.b;
+                    const int64_t tsi = T.tile0 + os / kTile;
+                    const uint64_t gws = (uint64_t)tsi * kTileWords + (uint64_t)((os % kTile) >> 6);
+                    ms = mask[gws];
+                    ws = wloc[gws];
+                    ts_off = toff[tsi];
+                    less = (ms >> (os & 63)) & 1ull;
+                }
+                if (bad) raise_err(err, hops > span ? kErrLoop : kErrRound);
+                else copy_rec(bufs, src, dst, s == pv ? S.R : s, p == S.lim ? S.R : p);
             }
-            copy_rec(bufs, src, dst, p == pv ? S.R : p, d);
         }
         if (first && threadIdx.x == 0) copy_rec(bufs, src, dst, pv, S.lim);  // the pivot
     }
@@ -737,10 +754,7 @@ __global__ __launch_bounds__(kTileThreads) void k_rd_rest(Bufs bufs, const RTask
                 atomicAdd(&ring[(round + 1) % kRing], 1u);
             } else if (size >= 2) {
                 N.fin = 1;
-            }
-            // size 1: the element at nth is in place; its slot is in the
-            // new range, which is still in buffer dst: copy it as a margin too
-            if (size == 1) {
+            } else {  // one slot: the element at nth is in place; all of it is final
                 N.m0lo = S.L;
                 N.m0hi = S.R + 1;
                 N.m1lo = N.m1hi = 0;
@@ -1280,14 +1294,15 @@ void sort_sets(const double* xyz, int64_t n_pts, const int64_t* poff, const int6
     }
     RState* st[2] = {nullptr, nullptr};
     uint64_t* mask = nullptr;
-    uint32_t *tcount = nullptr, *toff = nullptr, *lpos = nullptr, *ring = nullptr;
+    uint16_t* wloc = nullptr;
+    uint32_t *tcount = nullptr, *toff = nullptr, *ring = nullptr;
     if (max_tasks) {
         st[0] = sc.get<RState>(max_tasks);
         st[1] = sc.get<RState>(max_tasks);
         mask = sc.get<uint64_t>(max_tiles * kTileWords);
         tcount = sc.get<uint32_t>(max_tiles);
         toff = sc.get<uint32_t>(max_tiles);
-        lpos = sc.get<uint32_t>(total);
+        wloc = sc.get<uint16_t>(max_tiles * kTileWords);
         ring = sc.get<uint32_t>(kRing);
         fill_u32(ring, 0u, kRing, s);
     }
@@ -1309,15 +1324,13 @@ void sort_sets(const double* xyz, int64_t n_pts, const int64_t* poff, const int6
             uint32_t round = 0;
             for (;;) {
                 const int c = round & 1;
-                k_rd_count<<<ntl, kTileThreads, 0, s>>>(bufs, pt, nt, ptl, ntl, st[c], mask, tcount, ring, round,
-                                                        err);
+                k_rd_count<<<ntl, kTileThreads, 0, s>>>(bufs, pt, nt, ptl, ntl, st[c], mask, wloc, tcount, ring,
+                                                        round, err);
                 D3_LAUNCH_CHECK();
                 k_rd_scan<<<nt, 256, 0, s>>>(pt, nt, ntl, st[c], tcount, toff, err);
                 D3_LAUNCH_CHECK();
-                k_rd_flagged<<<ntl, kTileThreads, 0, s>>>(bufs, pt, nt, ptl, ntl, st[c], mask, toff, lpos, err);
-                D3_LAUNCH_CHECK();
-                k_rd_rest<<<ntl, kTileThreads, 0, s>>>(bufs, pt, nt, ptl, ntl, st[c], st[c ^ 1], mask, lpos, ring,
-                                                       round, err);
+                k_rd_move<<<ntl * kSlices, kTileThreads, 0, s>>>(bufs, pt, nt, ptl, ntl, st[c], st[c ^ 1], mask, wloc,
+                                                                 toff, ring, round, err);
                 D3_LAUNCH_CHECK();
                 ++round;
                 if ((int64_t)round >= check_at) {
@@ -1333,8 +1346,8 @@ void sort_sets(const double* xyz, int64_t n_pts, const int64_t* poff, const int6
                 }
             }
             // the margins of the last round
-            k_rd_count<<<ntl, kTileThreads, 0, s>>>(bufs, pt, nt, ptl, ntl, st[round & 1], mask, tcount, ring,
-                                                    round, err);
+            k_rd_count<<<ntl, kTileThreads, 0, s>>>(bufs, pt, nt, ptl, ntl, st[round & 1], mask, wloc, tcount,
+                                                    ring, round, err);
             D3_LAUNCH_CHECK();
             final_state = st[round & 1];
         }

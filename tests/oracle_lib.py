@@ -111,9 +111,10 @@ def _xyz(pts) -> np.ndarray:
     return a
 
 
-def pred_batch(pts, op, idx5, which: str = "oracle") -> np.ndarray:
+def pred_batch(pts, op, idx5, which: str = "oracle", threads: int = 1) -> np.ndarray:
     """Signs of a query batch from the restatement ("oracle"), the big-integer
-    oracle ("int") or the compiled reference ("ref")."""
+    oracle ("int") or the compiled reference ("ref"; `threads` > 1 splits the
+    batch over that many std::threads)."""
     xyz = _xyz(pts)
     op = np.ascontiguousarray(op, dtype=np.uint8)
     idx5 = np.ascontiguousarray(np.asarray(idx5, dtype=np.int32).reshape(-1, 5))
@@ -128,8 +129,12 @@ def pred_batch(pts, op, idx5, which: str = "oracle") -> np.ndarray:
         r = ref()
         if r is None:
             raise RuntimeError("reference library not built")
-        r.ref_pred_batch(_ptr(xyz, _dp), len(xyz), _ptr(op, _u8p), _ptr(idx5, _i32p),
-                         len(op), _ptr(out, _i8p))
+        if threads > 1:
+            r.ref_pred_batch_mt(_ptr(xyz, _dp), len(xyz), _ptr(op, _u8p), _ptr(idx5, _i32p),
+                                len(op), _ptr(out, _i8p), int(threads))
+        else:
+            r.ref_pred_batch(_ptr(xyz, _dp), len(xyz), _ptr(op, _u8p), _ptr(idx5, _i32p),
+                             len(op), _ptr(out, _i8p))
     else:
         raise ValueError(which)
     return out
