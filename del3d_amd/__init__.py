@@ -54,6 +54,7 @@ def lib() -> C.CDLL:
         L.del3d_predicates.argtypes = [vp, i64, vp, vp, i64, vp, vp]
         L.del3d_predicates_stats.argtypes = [vp, i64, vp, vp, i64, vp, vp, C.POINTER(C.c_int64)]
         L.del3d_predicates_host.argtypes = [vp, i64, vp, vp, i64, vp]
+        L.del3d_predicates_noalloc.argtypes = [vp, i64, vp, vp, i64, vp, vp]
         L.del3d_predicates_timed.argtypes = [vp, i64, vp, vp, i64, vp, vp, C.POINTER(C.c_float)]
         L.del3d_predicates_ev.argtypes = [vp, i64, vp, vp, i64, vp, vp, vp, vp]
         i64p = C.POINTER(C.c_int64)

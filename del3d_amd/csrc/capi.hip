@@ -291,6 +291,21 @@ int del3d_predicates_ev(const double* xyz, int64_t n_pts, const uint8_t* op, con
     });
 }
 
+int del3d_predicates_noalloc(const double* xyz, int64_t n_pts, const uint8_t* op, const int32_t* idx,
+                             int64_t n_q, int8_t* sign_out, void* stream) {
+    return guarded([&] {
+        const int rc = check_pred_args(xyz, n_pts, op, idx, n_q, sign_out);
+        if (rc) return rc;
+        if (n_pts > kNoAllocPts)
+            return set_error(DEL3D_ERR_INVALID,
+                             "del3d_predicates_noalloc takes at most %lld points (its fixed scratch "
+                             "holds %zu bytes); use del3d_predicates",
+                             (long long)kNoAllocPts, kNoAllocScratchBytes);
+        launch_predicates_noalloc(xyz, n_pts, op, idx, n_q, sign_out, (hipStream_t)stream);
+        return DEL3D_OK;
+    });
+}
+
 int del3d_predicates_host(const double* xyz, int64_t n_pts, const uint8_t* op, const int32_t* idx,
                           int64_t n_q, int8_t* sign_out) {
     return guarded([&] {

@@ -114,6 +114,16 @@ void launch_predicates(const double* xyz, int64_t n_pts, const uint8_t* op, cons
                        int64_t n_q, int8_t* out, hipStream_t s, unsigned long long* stats_dev,
                        hipEvent_t* ev = nullptr, int force_path = 0, hipEvent_t* ev2 = nullptr);
 
+// The allocation-free form: at most kNoAllocPts points, queries in launch
+// sequences of kNoAllocChunk, all scratch in one file-scope device array of
+// kNoAllocScratchBytes (queues 2 x 4 MiB, float and packed copies 20 B per
+// point, counters and the grid summary).
+constexpr int64_t kNoAllocPts = int64_t(1) << 20;
+constexpr uint32_t kNoAllocChunk = 1u << 20;
+constexpr size_t kNoAllocScratchBytes = size_t(32) << 20;
+void launch_predicates_noalloc(const double* xyz, int64_t n_pts, const uint8_t* op, const int32_t* idx,
+                               int64_t n_q, int8_t* out, hipStream_t s);
+
 void launch_single(ScalarQuery* qd, hipStream_t s);
 
 // Coplanar batch (predicates2.hip): op codes DEL3D_OP2_* + op_base, idx 5

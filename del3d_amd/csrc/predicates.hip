@@ -36,6 +36,8 @@
 // exactly as the reference's (src/predicates.cpp:125-201) does.
 #include <algorithm>
 
+#include <mutex>
+
 #include "common.h"
 #include "pred_core.h"
 #include "predicates.h"
@@ -781,35 +783,44 @@ void enqueue_batch(const BatchArgs& base, const BatchShape& sh, hipStream_t s, h
     if (ev) D3_HIP(hipEventRecord(ev[5], s));
 }
 
+// The scratch one batch call uses: the exact tiers' queues (one entry per
+// query of a launch chunk), the counters, the set's grid summary, and the
+// float or packed copy of the points.
+struct PredScratch {
+    uint32_t* queue = nullptr;
+    uint32_t* wide = nullptr;
+    PredCounters* ctr = nullptr;
+    int32_t* grid = nullptr;
+    F4* pts32 = nullptr;
+    uint32_t* pk = nullptr;
+    uint32_t chunk = 0;  // queries per launch sequence
+};
+
+struct PredRoute {
+    bool lds, packed, global_packed;
+};
+
+static PredRoute route_for(int64_t n_pts, int force_path) {
+    PredRoute r;
+    r.lds = force_path == 1 || (force_path == 0 && n_pts <= kLdsPts);
+    r.packed = force_path == 0 && n_pts > kLdsPts && n_pts <= kPackPts;
+    r.global_packed = force_path == 0 && n_pts > kPackPts;
+    return r;
+}
+
 // One point set: the LDS kernel when the set fits; else, up to kPackPts
 // points, the packed LDS kernel and the global one, of which the set's
 // summary runs one (packable sets: the packed kernel); else the global
 // kernel and its packed variant, again one of them by the summary.
-// The tasks are implicit equal chunks. Queries are processed in launch chunks
-// of at most kChunk (which bounds the queues).
-void launch_predicates(const double* xyz, int64_t n_pts, const uint8_t* op, const int32_t* idx,
-                       int64_t n_q, int8_t* out, hipStream_t s, unsigned long long* stats_dev,
-                       hipEvent_t* ev, int force_path, hipEvent_t* ev2) {
-    if (n_q <= 0) return;
-    const uint32_t chunk = (uint32_t)std::min<int64_t>(n_q, kChunk);
-    Scratch sc(s);
-    uint32_t* queue = sc.get<uint32_t>(chunk);
-    uint32_t* wide = sc.get<uint32_t>(chunk);
-    PredCounters* ctr = sc.get<PredCounters>(1);
-    int32_t* grid = sc.get<int32_t>(2);
+// The tasks are implicit equal chunks. Queries are processed in launch
+// sequences of at most b.chunk (which bounds the queues).
+static void run_predicates(const PredScratch& b, const PredRoute& r, const double* xyz, int64_t n_pts,
+                           const uint8_t* op, const int32_t* idx, int64_t n_q, int8_t* out, hipStream_t s,
+                           unsigned long long* stats_dev, hipEvent_t* ev, hipEvent_t* ev2) {
     const int cus = device_cu_count();
-    const bool lds = force_path == 1 || (force_path == 0 && n_pts <= kLdsPts);
-    const bool packed = force_path == 0 && n_pts > kLdsPts && n_pts <= kPackPts;
-    const bool global_packed = force_path == 0 && n_pts > kPackPts;
-    // the global kernel's compact float copy (used only if the set is exact in float)
-    F4* pts32 = lds ? nullptr : sc.get<F4>((size_t)n_pts);
-    // the packed copy (written and used only if the set is packable, in place
-    // of the float copy): held in LDS by the packed kernel, or gathered from
-    // global memory by the global one's packed variant for larger sets
-    uint32_t* pk = (packed || global_packed) ? sc.get<uint32_t>((size_t)n_pts) : nullptr;
     bool grids_done = false;
-    for (int64_t off = 0; off < n_q; off += chunk) {
-        const int64_t n = std::min<int64_t>(chunk, n_q - off);
+    for (int64_t off = 0; off < n_q; off += b.chunk) {
+        const int64_t n = std::min<int64_t>(b.chunk, n_q - off);
         BatchArgs a{};
         a.xyz = xyz;
         a.n_pts = n_pts;
@@ -817,31 +828,31 @@ void launch_predicates(const double* xyz, int64_t n_pts, const uint8_t* op, cons
         a.idx = idx + off * 5;
         a.out = out + off;
         a.n_q = n;
-        a.queue = queue;
-        a.wide = wide;
-        a.ctr = ctr;
+        a.queue = b.queue;
+        a.wide = b.wide;
+        a.ctr = b.ctr;
         a.stats = stats_dev;
-        a.grid = grid;
-        a.pts32 = pts32;
-        a.pk = pk;
-        a.pack_route = (packed || global_packed) ? 1 : 0;
+        a.grid = b.grid;
+        a.pts32 = b.pts32;
+        a.pk = b.pk;
+        a.pack_route = (r.packed || r.global_packed) ? 1 : 0;
         BatchShape sh{};
         if (!grids_done) {
             // the set's grid summary, and this chunk's counters, in one prepass
-            enqueue_set_grids(a, 1, n_pts, grid, s);
-            if (pts32) enqueue_points_f32(xyz, nullptr, 1, n_pts, grid, pts32, s, pk);
+            enqueue_set_grids(a, 1, n_pts, b.grid, s);
+            if (b.pts32) enqueue_points_f32(xyz, nullptr, 1, n_pts, b.grid, b.pts32, s, b.pk);
             grids_done = true;
             sh.counters_cleared = true;
         }
-        if (lds || packed) {
+        if (r.lds || r.packed) {
             // about two tasks per CU, at least 8 tiles each
             const int64_t per = std::max<int64_t>(8 * kLdsThreads, (n + 2 * cus - 1) / (2 * cus));
             sh.lds_chunk = (uint32_t)std::min<int64_t>(per, n);
             sh.n_lds_tasks = (uint32_t)((n + sh.lds_chunk - 1) / sh.lds_chunk);
             sh.lds_blocks_per_cu = 1;
-            sh.packed = packed;
+            sh.packed = r.packed;
         }
-        if (!lds) {
+        if (!r.lds) {
             // 4 workgroups of 4 wavefronts per CU (4 wavefronts per SIMD, the
             // kernel's occupancy), about two tasks each, tasks of 4..64 tiles
             // (measured on 2^24 lattice queries: 32 tiles x 4 per CU is 1.3%
@@ -851,10 +862,88 @@ void launch_predicates(const double* xyz, int64_t n_pts, const uint8_t* op, cons
             const int64_t per = std::min<int64_t>(64, std::max<int64_t>(4, tiles / (2 * 4 * (int64_t)cus)));
             sh.global_chunk = (uint32_t)(per * kGlobalThreads);
             sh.n_global_tasks = (uint32_t)((n + sh.global_chunk - 1) / sh.global_chunk);
-            sh.global_packed = global_packed;
+            sh.global_packed = r.global_packed;
         }
         enqueue_batch(a, sh, s, ev, ev2);
     }
+}
+
+void launch_predicates(const double* xyz, int64_t n_pts, const uint8_t* op, const int32_t* idx,
+                       int64_t n_q, int8_t* out, hipStream_t s, unsigned long long* stats_dev,
+                       hipEvent_t* ev, int force_path, hipEvent_t* ev2) {
+    if (n_q <= 0) return;
+    const PredRoute r = route_for(n_pts, force_path);
+    PredScratch b;
+    b.chunk = (uint32_t)std::min<int64_t>(n_q, kChunk);
+    Scratch sc(s);
+    b.queue = sc.get<uint32_t>(b.chunk);
+    b.wide = sc.get<uint32_t>(b.chunk);
+    b.ctr = sc.get<PredCounters>(1);
+    b.grid = sc.get<int32_t>(2);
+    // the global kernel's compact float copy (used only if the set is exact in float)
+    b.pts32 = r.lds ? nullptr : sc.get<F4>((size_t)n_pts);
+    // the packed copy (written and used only if the set is packable, in place
+    // of the float copy): held in LDS by the packed kernel, or gathered from
+    // global memory by the global one's packed variant for larger sets
+    b.pk = (r.packed || r.global_packed) ? sc.get<uint32_t>((size_t)n_pts) : nullptr;
+    run_predicates(b, r, xyz, n_pts, op, idx, n_q, out, s, stats_dev, ev, ev2);
+}
+
+// ---------------------------------------------------------------------------
+// Allocation-free form (del3d_predicates_noalloc): the same pipeline with its
+// scratch carved from one file-scope device array, for callers that cannot
+// allocate device memory. Each device holds its own copy of the array until
+// the process ends; calls that overlap on one device share it and race.
+// ---------------------------------------------------------------------------
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+// The array's device address on the current device, looked up once per
+// device (the array's name in host code is not a device address).
+static unsigned char* noalloc_scratch_ptr() {
+    static std::mutex mu;
+    static std::vector<unsigned char*> ptrs;
+    int dev = 0;
+    D3_HIP(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lock(mu);
+    if ((int)ptrs.size() <= dev) ptrs.resize(dev + 1, nullptr);
+    if (!ptrs[dev]) {
+        void* p = nullptr;
+        D3_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) {
+            set_error(DEL3D_ERR_DEVICE, "hipGetSymbolAddress returned no address for the scratch array");
+            throw HipError{DEL3D_ERR_DEVICE};
+        }
+        ptrs[dev] = static_cast<unsigned char*>(p);
+    }
+    return ptrs[dev];
+}
+
+void launch_predicates_noalloc(const double* xyz, int64_t n_pts, const uint8_t* op, const int32_t* idx,
+                               int64_t n_q, int8_t* out, hipStream_t s) {
+    if (n_q <= 0) return;
+    unsigned char* base = noalloc_scratch_ptr();
+    const PredRoute r = route_for(n_pts, 0);
+    PredScratch b;
+    b.chunk = (uint32_t)std::min<int64_t>(n_q, kNoAllocChunk);
+    size_t at = 0;
+    auto carve = [&](size_t bytes) {
+        unsigned char* p = base + at;
+        at += (bytes + 255) & ~size_t(255);
+        return p;
+    };
+    b.queue = reinterpret_cast<uint32_t*>(carve(sizeof(uint32_t) * kNoAllocChunk));
+    b.wide = reinterpret_cast<uint32_t*>(carve(sizeof(uint32_t) * kNoAllocChunk));
+    b.ctr = reinterpret_cast<PredCounters*>(carve(sizeof(PredCounters)));
+    b.grid = reinterpret_cast<int32_t*>(carve(2 * sizeof(int32_t)));
+    b.pts32 = r.lds ? nullptr : reinterpret_cast<F4*>(carve(sizeof(F4) * (size_t)n_pts));
+    b.pk = (r.packed || r.global_packed) ? reinterpret_cast<uint32_t*>(carve(sizeof(uint32_t) * (size_t)n_pts))
+                                         : nullptr;
+    if (at > kNoAllocScratchBytes) {  // n_pts is checked by the caller; this is the layout's own bound
+        set_error(DEL3D_ERR_INVALID, "the allocation-free form's scratch (%zu bytes) cannot hold %zu bytes",
+                  kNoAllocScratchBytes, at);
+        throw HipError{DEL3D_ERR_INVALID};
+    }
+    run_predicates(b, r, xyz, n_pts, op, idx, n_q, out, s, nullptr, nullptr, nullptr);
 }
 
 // ---------------------------------------------------------------------------

@@ -128,6 +128,23 @@ int del3d_predicates_ev(const double* xyz, int64_t n_pts, const uint8_t* op, con
                         int64_t n_q, int8_t* sign_out, void* stream, void* ev_begin, void* ev_end);
 
 /*
+ * Allocation-free form of del3d_predicates (no counterpart in the reference;
+ * for callers that cannot allocate device memory: code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc). Same
+ * arguments, results and stream semantics as del3d_predicates, but every
+ * scratch buffer (the exact tiers' queues, counters, the set's grid summary
+ * and its float or packed copy) lives in one file-scope device array of
+ * 32 MiB that each device holds until the process ends; the call makes no
+ * device allocation. Limits: n_pts <= 1,048,576 (larger sets are refused
+ * with DEL3D_ERR_INVALID); any n_q (processed in launch sequences of 2^20).
+ * Calls that overlap on one device share the array and race: serialize them
+ * (one stream, or wait between calls). The array's contents are whatever the
+ * last call left; the kernels clear what they need.
+ */
+int del3d_predicates_noalloc(const double* xyz, int64_t n_pts, const uint8_t* op,
+                             const int32_t* idx, int64_t n_q, int8_t* sign_out, void* stream);
+
+/*
  * Many point sets in one call (the batched configuration). Set s owns the
  * points xyz[3*pt_off[s] .. 3*pt_off[s+1]) and the queries
  * [q_off[s], q_off[s+1]) of op/idx/sign_out, whose indices are local to the

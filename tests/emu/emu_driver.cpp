@@ -60,3 +60,9 @@ extern "C" int emu_circumcenters(const double* xyz, int64_t n_pts, const int32_t
                                  int lazy, double* out) {
     return del3d_circumcenters_host(xyz, n_pts, cells, n, lazy, out);
 }
+
+extern "C" int emu_predicates_noalloc(const double* xyz, int64_t n_pts, const uint8_t* op,
+                                      const int32_t* idx, int64_t n_q, int8_t* out) {
+    // device buffers are host memory in the emulation
+    return del3d_predicates_noalloc(xyz, n_pts, op, idx, n_q, out, nullptr);
+}

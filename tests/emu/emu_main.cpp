@@ -52,6 +52,28 @@ extern "C" int emu_predicates_sets(const double* xyz, const int64_t* pt_off, int
                                    const uint8_t* op, const int32_t* idx, const int64_t* q_off,
                                    int8_t* out);
 
+extern "C" int emu_predicates_noalloc(const double* xyz, int64_t n_pts, const uint8_t* op,
+                                      const int32_t* idx, int64_t n_q, int8_t* out);
+
+// The allocation-free form against the integer oracle, on consecutive calls
+// sharing its fixed scratch (the second on a smaller set).
+static int check_noalloc(const char* name, const std::vector<double>& xyz, int64_t nq) {
+    const int64_t n = (int64_t)xyz.size() / 3;
+    std::vector<uint8_t> op(nq);
+    std::vector<int32_t> idx(5 * nq);
+    for (int64_t q = 0; q < nq; ++q) {
+        op[q] = (uint8_t)(rnd() % 3);
+        for (int k = 0; k < 5; ++k) idx[5 * q + k] = (int32_t)(rnd() % n);
+        if (q % 7 == 0) idx[5 * q + 4] = idx[5 * q];
+    }
+    std::vector<int8_t> got(nq, 9), want(nq);
+    const int rc = emu_predicates_noalloc(xyz.data(), n, op.data(), idx.data(), nq, got.data());
+    orc_int_pred_batch(xyz.data(), n, op.data(), idx.data(), nq, want.data());
+    const bool ok = rc == 0 && got == want;
+    std::printf("noalloc %s %s\n", name, ok ? "ok" : "MISMATCH");
+    return ok ? 0 : 1;
+}
+
 // The same queries through the LDS kernel (path 1) and the global kernel
 // (path 2), each against the integer oracle.
 static int check_paths(const char* name, const std::vector<double>& xyz, int64_t nq) {
@@ -388,6 +410,9 @@ int main(int argc, char** argv) {
                 }
         fails += check_predicates("lattice34 (global packed)", lat34, 3000);
         fails += check_predicates("lattice34+1000 (float copy)", lat34f, 3000);
+        fails += check_noalloc("lattice34 (global packed)", lat34, 4000);
+        fails += check_noalloc("lattice20 (packed)", lat20, 3000);
+        fails += check_noalloc("grid48 (LDS)", q48, 2000);
         fails += check_paths("lattice", lat, 5000);
         fails += check_paths("wide", wide, 2500);
         fails += check_sets();

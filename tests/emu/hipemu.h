@@ -29,6 +29,8 @@
 #define __forceinline__ inline
 #define __launch_bounds__(...)
 #define __shared__ static
+#define __align__(n) alignas(n)
+#define HIP_SYMBOL(x) (&(x))
 
 struct dim3 {
     unsigned x = 1, y = 1, z = 1;
@@ -200,6 +202,10 @@ using std::min;
 inline const char* hipGetErrorString(hipError_t) { return "emulated error"; }
 inline hipError_t hipGetLastError() { return hipSuccess; }
 inline hipError_t hipGetDevice(int* d) { *d = 0; return hipSuccess; }
+inline hipError_t hipGetSymbolAddress(void** p, const void* sym) {
+    *p = const_cast<void*>(sym);
+    return hipSuccess;
+}
 inline hipError_t hipDeviceGetAttribute(int* v, int, int) { *v = emu::g_cus; return hipSuccess; }
 namespace emu {
 // EMU_DEFER=1 models the runtime's documented asynchrony: kernel launches and

@@ -81,6 +81,20 @@ def test_argument_errors_without_device(lib):
     assert b"null" in lib.del3d_last_error()
 
 
+def test_noalloc_refuses_sets_above_its_limit(lib):
+    """The allocation-free form checks the point count against its fixed
+    scratch before touching a device."""
+    lib.del3d_last_error.restype = ctypes.c_char_p
+    vp = ctypes.c_void_p
+    lib.del3d_predicates_noalloc.argtypes = [vp, ctypes.c_int64, vp, vp, ctypes.c_int64, vp, vp]
+    dummy = ctypes.create_string_buffer(64)
+    p = ctypes.cast(dummy, vp)
+    assert lib.del3d_predicates_noalloc(p, (1 << 20) + 1, p, p, 1, p, None) == 1
+    assert b"at most 1048576 points" in lib.del3d_last_error()
+    assert lib.del3d_predicates_noalloc(None, 0, None, None, 0, None, None) == 0  # nothing to do
+    assert lib.del3d_predicates_noalloc(None, 0, None, None, -1, None, None) == 1
+
+
 def test_version(lib):
     lib.del3d_version.restype = ctypes.c_char_p
     assert b"gfx950" in lib.del3d_version()
