@@ -8,6 +8,7 @@
 #include <vector>
 
 #include "common.h"
+#include "host_scalar.h"
 #include "predicates.h"
 
 namespace d3 {
@@ -151,44 +152,23 @@ static int check_pred_args(const double* xyz, int64_t n_pts, const uint8_t* op, 
     return DEL3D_OK;
 }
 
-// Scalar path: one pinned staging record and one device record per thread.
-struct ScalarSlot {
-    ScalarQuery* host = nullptr;
-    ScalarQuery* dev = nullptr;
-};
-
+// Scalar path: evaluated on the host (host_scalar.cpp). The slots' pointer
+// identities become ids: a slot gets the id of the first slot holding the same
+// pointer (the perturbation compares pointers, src/predicates.cpp:220-227).
 static int scalar_query(uint8_t op, const double* const* pts, int np, bool coplanar) {
-    thread_local std::vector<ScalarSlot> slots;
-    int dev = 0;
-    D3_HIP(hipGetDevice(&dev));
-    if ((int)slots.size() <= dev) slots.resize(dev + 1);
-    ScalarSlot& sl = slots[dev];
-    if (!sl.host) {
-        D3_HIP(hipHostMalloc((void**)&sl.host, sizeof(ScalarQuery), hipHostMallocDefault));
-        D3_HIP(hipMalloc((void**)&sl.dev, sizeof(ScalarQuery)));
-    }
-    ScalarQuery& q = *sl.host;
-    q.op = op;
+    P3 p[5];
+    int32_t id[5];
     for (int k = 0; k < 5; ++k) {
-        const double* p = pts[k < np ? k : np - 1];
-        q.xyz[3 * k] = p[0];
-        q.xyz[3 * k + 1] = p[1];
-        q.xyz[3 * k + 2] = p[2];
-        // pointer identity -> index identity: the first slot holding the same pointer
-        int ident = k;
+        const double* v = pts[k < np ? k : np - 1];
+        p[k] = P3{v[0], v[1], v[2]};
+        id[k] = k;
         for (int j = 0; j < k; ++j)
-            if (pts[j < np ? j : np - 1] == p) { ident = q.id[j]; break; }
-        q.id[k] = ident;
+            if (pts[j < np ? j : np - 1] == v) {
+                id[k] = id[j];
+                break;
+            }
     }
-    hipStream_t s = thread_stream();
-    D3_HIP(hipMemcpyAsync(sl.dev, sl.host, sizeof(ScalarQuery), hipMemcpyHostToDevice, s));
-    if (coplanar)
-        launch_single2(sl.dev, s);
-    else
-        launch_single(sl.dev, s);
-    D3_HIP(hipMemcpyAsync(sl.host, sl.dev, sizeof(ScalarQuery), hipMemcpyDeviceToHost, s));
-    D3_HIP(hipStreamSynchronize(s));
-    return q.sign;
+    return coplanar ? host_pred2(op, p, id) : host_pred3(op, p, id);
 }
 
 static int scalar_or_abort(uint8_t op, const double* const* pts, int np, bool coplanar = false) {
@@ -511,13 +491,9 @@ int del3d_coplanar_in_circle_unperturbed(const double* p0, const double* p1, con
 
 void del3d_circumcenter(const double* p0, const double* p1, const double* p2, const double* p3,
                         int lazy, double* out) {
-    const double xyz[12] = {p0[0], p0[1], p0[2], p1[0], p1[1], p1[2],
-                            p2[0], p2[1], p2[2], p3[0], p3[1], p3[2]};
-    const int32_t cell[4] = {0, 1, 2, 3};
-    if (del3d_circumcenters_host(xyz, 4, cell, 1, lazy, out) != DEL3D_OK) {
-        std::fprintf(stderr, "del3d_amd: circumcenter failed: %s\n", last_error_slot().c_str());
-        std::abort();
-    }
+    const P3 p[4] = {P3{p0[0], p0[1], p0[2]}, P3{p1[0], p1[1], p1[2]}, P3{p2[0], p2[1], p2[2]},
+                     P3{p3[0], p3[1], p3[2]}};
+    host_circumcenter(p, lazy, out);
 }
 
 }  // extern "C"

@@ -25,8 +25,9 @@
 
 namespace d3 {
 
-// The exact tiers are large and rarely run: compiled once, called.
-#define D3_HD_CALL __host__ __device__ __attribute__((noinline))
+// The exact tiers are large and rarely run: compiled once per translation
+// unit and called (inline only for linkage: several sources include this).
+#define D3_HD_CALL __host__ __device__ inline __attribute__((noinline))
 
 // src/predicates2.cpp:22-26
 constexpr double kOrient2dBound = (3.0 + 16.0 * kEps) * kEps;

@@ -81,13 +81,6 @@ struct BatchShape {
     bool global_packed = false;     // the global tasks also run the global packed variant
 };
 
-struct ScalarQuery {
-    double xyz[15];
-    int32_t id[5];
-    uint8_t op;
-    int8_t sign;
-};
-
 // Writes the SetGrid summary of every set of `a` into grid (2 int32 per set,
 // n_sets sets; one set when a.pt_off and a.set_ptr are null), on s; also
 // zeroes a.ctr when it is set.
@@ -124,8 +117,6 @@ constexpr size_t kNoAllocScratchBytes = size_t(32) << 20;
 void launch_predicates_noalloc(const double* xyz, int64_t n_pts, const uint8_t* op, const int32_t* idx,
                                int64_t n_q, int8_t* out, hipStream_t s);
 
-void launch_single(ScalarQuery* qd, hipStream_t s);
-
 // Coplanar batch (predicates2.hip): op codes DEL3D_OP2_* + op_base, idx 5
 // per query (the op's points in argument order), out = the predicate's
 // result. Queries with an op byte below op_base are left untouched (another
@@ -133,8 +124,6 @@ void launch_single(ScalarQuery* qd, hipStream_t s);
 // [0, n_pts) gives 0.
 void launch_predicates2(const double* xyz, int64_t n_pts, const uint8_t* op, const int32_t* idx,
                         int64_t n_q, int8_t* out, hipStream_t s, uint8_t op_base = 0);
-// One coplanar query of a ScalarQuery record (points by value, op DEL3D_OP2_*).
-void launch_single2(ScalarQuery* qd, hipStream_t s);
 
 // Circumcentres (circumcenter.hip): cells 4 int32 each, out 3 doubles per
 // cell; lazy selects circumcenter_lazy, else the exact circumcenter.

@@ -946,58 +946,4 @@ void launch_predicates_noalloc(const double* xyz, int64_t n_pts, const uint8_t* 
     run_predicates(b, r, xyz, n_pts, op, idx, n_q, out, s, nullptr, nullptr, nullptr);
 }
 
-// ---------------------------------------------------------------------------
-// One query, every tier in one thread: the scalar entry points (C++ drop-in).
-// ---------------------------------------------------------------------------
-__global__ __launch_bounds__(64) void pred_single(ScalarQuery* qd) {
-    __shared__ uint32_t ws[64 * kWideLimbsMax];
-    if (threadIdx.x != 0) return;
-    const uint8_t o = qd->op;
-    const int np = (o == DEL3D_OP_ORIENT3D) ? 4 : 5;
-    P3 p[5];
-    int32_t id[5];
-    for (int k = 0; k < 5; ++k) id[k] = qd->id[k];
-    // slots hold their own coordinates: xyz[3k..3k+2] is slot k's point, and
-    // id[] carries the caller's pointer identities
-    for (int k = 0; k < np; ++k) p[k] = P3{qd->xyz[3 * k], qd->xyz[3 * k + 1], qd->xyz[3 * k + 2]};
-    if (np == 4) p[4] = p[3];
-    const Grid g = grid_of(p, np);
-    const int L = limbs_for(g);
-    auto orient_pub = [&](const P3* t) -> int {
-        const Filt f = orient_filter(t[0], t[1], t[2], t[3]);
-        if (filter_decides(f, kOrientBound)) return -sgn(f.det);
-        if (L == 1) return orient_public_int<1>(t, g.q);
-        if (L == 2) return orient_public_int<2>(t, g.q);
-        return -rt_orient_raw(t[0], t[1], t[2], t[3], g.q, L, ws);
-    };
-    int s;
-    if (o == DEL3D_OP_ORIENT3D) {
-        s = orient_pub(p);
-    } else {
-        const Filt f = insphere_filter(p[0], p[1], p[2], p[3], p[4]);
-        if (filter_decides(f, kInSphereBound)) {
-            s = -sgn(f.det);
-        } else if (L <= 2) {
-            bool dummy = false;
-            s = L == 1 ? exact_query<1>(DEL3D_OP_IN_SPHERE_UNPERTURBED, p, id, g.q, dummy)
-                       : exact_query<2>(DEL3D_OP_IN_SPHERE_UNPERTURBED, p, id, g.q, dummy);
-        } else {
-            s = -rt_insphere_raw(p[0], p[1], p[2], p[3], p[4], g.q, L, ws);
-        }
-        if (s == 0 && o == DEL3D_OP_IN_SPHERE) {
-            s = perturbation_walk(p, id, [&](int k) {
-                P3 t[4];
-                subst_tet(p, k, t);
-                return orient_pub(t);
-            });
-        }
-    }
-    qd->sign = (int8_t)s;
-}
-
-void launch_single(ScalarQuery* qd, hipStream_t s) {
-    pred_single<<<1, 64, 0, s>>>(qd);
-    D3_LAUNCH_CHECK();
-}
-
 }  // namespace d3

@@ -180,22 +180,4 @@ void launch_predicates2(const double* xyz, int64_t n_pts, const uint8_t* op, con
     }
 }
 
-// One query given by its points' coordinates (scalar drop-in calls).
-__global__ __launch_bounds__(64) void pred2_single_k(ScalarQuery* qd) {
-    __shared__ uint32_t ws[kWs2];
-    if (threadIdx.x != 0) return;
-    P3 pt[4];
-    int32_t id[4];
-    for (int k = 0; k < 4; ++k) {
-        pt[k] = P3{qd->xyz[3 * k], qd->xyz[3 * k + 1], qd->xyz[3 * k + 2]};
-        id[k] = qd->id[k];
-    }
-    qd->sign = (int8_t)pred2_eval(qd->op, pt, id, ws);
-}
-
-void launch_single2(ScalarQuery* qd, hipStream_t s) {
-    pred2_single_k<<<1, 64, 0, s>>>(qd);
-    D3_LAUNCH_CHECK();
-}
-
 }  // namespace d3

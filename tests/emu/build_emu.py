@@ -72,7 +72,7 @@ def main():
         san = os.environ.get("EMU_SANITIZER", "address")
         opt = os.environ.get("EMU_OPT", "-O1")
         cmd = ["g++", "-std=c++20", "-g", opt, "-fPIC", "-pthread", "-ffp-contract=off",
-               "-fno-omit-frame-pointer", f"-I{HERE}", "-o", out] + srcs
+               "-fno-omit-frame-pointer", f"-I{HERE}", f"-I{CSRC}", "-o", out] + srcs
         if san != "none":
             cmd.insert(6, f"-fsanitize={san}")
         if out.endswith(".so"):

@@ -3,6 +3,8 @@
 #include <cstdint>
 #include <cstdlib>
 
+#include <hip/hip_runtime.h>
+
 #include "../../include/del3d_amd.h"
 
 extern "C" int emu_spatial_sort(const double* xyz, int64_t n_pts, int32_t* order, int64_t n_order) {
@@ -63,6 +65,9 @@ extern "C" int emu_circumcenters(const double* xyz, int64_t n_pts, const int32_t
 
 extern "C" int emu_predicates_noalloc(const double* xyz, int64_t n_pts, const uint8_t* op,
                                       const int32_t* idx, int64_t n_q, int8_t* out) {
-    // device buffers are host memory in the emulation
-    return del3d_predicates_noalloc(xyz, n_pts, op, idx, n_q, out, nullptr);
+    // device buffers are host memory in the emulation; the call is
+    // stream-ordered, so wait for it (the deferred mode runs kernels only here)
+    const int rc = del3d_predicates_noalloc(xyz, n_pts, op, idx, n_q, out, nullptr);
+    hipStreamSynchronize(nullptr);
+    return rc;
 }

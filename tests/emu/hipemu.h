@@ -29,7 +29,7 @@
 #define __forceinline__ inline
 #define __launch_bounds__(...)
 #define __shared__ static
-#define __align__(n) alignas(n)
+#define __align__(n) __attribute__((aligned(n)))
 #define HIP_SYMBOL(x) (&(x))
 
 struct dim3 {

@@ -25,17 +25,50 @@ def emu_exe():
     csrc = HERE.parent / "del3d_amd" / "csrc"
     subprocess.run(["python", str(HERE / "emu" / "build_emu.py"), str(EXE),
                     str(csrc / "capi.hip"), str(csrc / "predicates.hip"), str(csrc / "predicates2.hip"),
-                    str(csrc / "circumcenter.hip"), str(csrc / "batch.hip"), str(csrc / "server.hip"), str(csrc / "spatial_sort.hip")],
+                    str(csrc / "circumcenter.hip"), str(csrc / "batch.hip"), str(csrc / "server.hip"), str(csrc / "spatial_sort.hip"),
+                    str(csrc / "host_scalar.cpp")],
                    check=True)
     return EXE
 
 
-@pytest.mark.parametrize("defer,order,sizes", [("1", "2", ["2", "65", "1100"]),
-                                               ("0", "1", ["2", "65", "1100", "2100"])])
-def test_emulated_sort_matches_oracle(emu_exe, defer, order, sizes):
-    env = dict(os.environ, EMU_DEFER=defer, EMU_BLOCK_ORDER=order, ASAN_OPTIONS="detect_leaks=0")
+@pytest.mark.parametrize("defer,order,sizes,sort_only", [("1", "2", ["2", "65", "1100"], ""),
+                                                         ("0", "1", ["2", "65", "1100"], "1")])
+def test_emulated_sort_matches_oracle(emu_exe, defer, order, sizes, sort_only):
+    """Threads per lane under AddressSanitizer: the predicate pipelines and
+    small sorts (subtree and LDS passes); the second run sorts only."""
+    env = dict(os.environ, EMU_DEFER=defer, EMU_BLOCK_ORDER=order, ASAN_OPTIONS="detect_leaks=0",
+               EMU_SKIP_SETS="1")
+    if sort_only:
+        env["EMU_SORT_ONLY"] = "1"
     r = subprocess.run([str(emu_exe)] + sizes, capture_output=True, text=True, timeout=900, env=env)
     assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-4000:]
+
+
+@pytest.fixture(scope="module")
+def fiber_exe():
+    exe = Path("/tmp") / f"del3d_emu_fibers_{os.getuid()}"
+    csrc = HERE.parent / "del3d_amd" / "csrc"
+    subprocess.run(["python", str(HERE / "emu" / "build_emu.py"), str(exe),
+                    str(csrc / "capi.hip"), str(csrc / "predicates.hip"), str(csrc / "predicates2.hip"),
+                    str(csrc / "circumcenter.hip"), str(csrc / "batch.hip"), str(csrc / "server.hip"),
+                    str(csrc / "spatial_sort.hip"), str(csrc / "host_scalar.cpp")],
+                   check=True, env=dict(os.environ, EMU_SANITIZER="none", EMU_OPT="-O2"))
+    return exe
+
+
+def test_emulated_sort_device_rounds_fibers(fiber_exe):
+    """The sort's device-wide rounds (selects above 4,096 elements), the
+    LDS finish and subtree passes, at sizes up to 20,000 and on tie-heavy
+    sets, a permuted subset order, a bad index and eight sets in one call,
+    against the oracle: the kernels run as fibers (one OS thread, switched
+    at every barrier and wavefront operation), which keeps these sizes to
+    seconds; AddressSanitizer is off in this mode (its stack poisoning does
+    not follow switched stacks), the guards in the kernels are not."""
+    env = dict(os.environ, EMU_FIBERS="1", EMU_SORT_ONLY="1", EMU_SORT_CASES="1")
+    r = subprocess.run([str(fiber_exe), "2", "9", "65", "1100", "4681", "20000"], capture_output=True,
+                       text=True, timeout=900, env=env)
+    assert r.returncode == 0, r.stdout[-3000:] + r.stderr[-3000:]
+    assert r.stdout.count(" ok") >= 12 and "MISMATCH" not in r.stdout
 
 
 @pytest.fixture(scope="module")
@@ -44,7 +77,8 @@ def tsan_exe():
     csrc = HERE.parent / "del3d_amd" / "csrc"
     subprocess.run(["python", str(HERE / "emu" / "build_emu.py"), str(exe),
                     str(csrc / "capi.hip"), str(csrc / "predicates.hip"), str(csrc / "predicates2.hip"),
-                    str(csrc / "circumcenter.hip"), str(csrc / "batch.hip"), str(csrc / "server.hip"), str(csrc / "spatial_sort.hip")],
+                    str(csrc / "circumcenter.hip"), str(csrc / "batch.hip"), str(csrc / "server.hip"), str(csrc / "spatial_sort.hip"),
+                    str(csrc / "host_scalar.cpp")],
                    check=True,
                    env=dict(os.environ, EMU_SANITIZER="thread"))
     return exe

@@ -28,3 +28,27 @@ def test_reference_test_program_on_gpu_library(name):
     r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=600)
     say(r.stdout[-1500:])
     assert r.returncode == 0, r.stdout[-3000:] + r.stderr[-2000:]
+
+
+def test_delaunay_build_through_dropin_is_identical_and_fast():
+    """The reference's unchanged Delaunay driver on 10,000 rand48 points,
+    once on the reference's predicates and sort, once on this library's
+    (scalar predicates on the host, the spatial sort on the GPU): the same
+    66,368 cells in the same order, and a build time within about 2x of the
+    reference's (asserted at 3x, so that timing noise on a shared host does
+    not fail the run; the measured ratio is printed)."""
+    import json
+    ref, dropin = REF_BIN / "ref_delaunay_timing", REF_BIN / "dropin_delaunay_timing"
+    if not ref.exists() or not dropin.exists():
+        pytest.skip("timing drivers not built (need the reference sources at build time)")
+    require_gpu()
+    runs = {}
+    for name, exe in (("reference", ref), ("dropin", dropin)):
+        say("running", exe.name)
+        r = subprocess.run([str(exe), "10000", "3"], capture_output=True, text=True, timeout=300)
+        assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+        runs[name] = json.loads(r.stdout.strip().splitlines()[-1])
+    say(runs, "ratio", runs["dropin"]["build_s"] / runs["reference"]["build_s"])
+    assert runs["dropin"]["hash"] == runs["reference"]["hash"]
+    assert runs["dropin"]["cells"] == runs["reference"]["cells"] == 66368
+    assert runs["dropin"]["build_s"] <= 3.0 * runs["reference"]["build_s"]
