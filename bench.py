@@ -219,15 +219,18 @@ def run_batched(args) -> dict:
 
 
 def run_served(args) -> dict:
-    """Poisson stream of predicate jobs on resident point sets of 1,000 to
-    1,000,000 points (log-uniform; 30% lattice or quantised sets), each job
-    10 queries per point of its set (at most 2^22), follow-up queries on the
-    same resident sets, under a 200 GB HBM budget. Phase 1 submits every job
-    at once (sustained throughput); phase 2 offers half that rate as Poisson
+    """Poisson stream of jobs on resident point sets of 1,000 to 1,000,000
+    points (log-uniform; 30% lattice or quantised sets): a fifth of the jobs
+    are Hilbert sorts of their set, the others predicate batches of 10
+    queries per point of their set (at most 2^22), follow-ups on the same
+    resident sets, under a 200 GB HBM budget. Phase 1 submits every job at
+    once (sustained throughput); phase 2 offers half that rate as Poisson
     arrivals and reports p50 / p99 submit-to-result latency. Baseline: the
-    same jobs as plain per-request calls (upload the job's queries, one
-    batched call on the resident set, download the signs), one at a time.
-    Every job's signs are compared with the baseline's (tolerance 0)."""
+    same jobs as plain per-request calls on the same device-resident sets
+    (upload the job's queries, one batched call, download the signs; or one
+    del3d_spatial_sort and the download of the order), one at a time. Every
+    job's signs and orders are compared with the baseline's (tolerance 0).
+    Host buffers are pinned for both."""
     import threading
 
     import del3d_amd as D
@@ -241,64 +244,81 @@ def run_served(args) -> dict:
             sets.append(W.set_points(1 + (i % 2), int(n), rng))  # quantised sphere / lattice
         else:
             sets.append(W.set_points(0, int(n), rng))
-    # the jobs' host buffers are pinned (page-locked) for both the server and
-    # the per-request baseline, so copies are plain DMA in both
+    sets = [D._hip.pinned_copy(p) for p in sets]
     pools = []
     for p in sets:
         nq = min(10 * len(p), 1 << 22)
         pools.append((D._hip.pinned_copy(rng.integers(0, 2, size=nq).astype(np.uint8)),
                       D._hip.pinned_copy(rng.integers(0, len(p), size=(nq, 5)).astype(np.int32))))
     job_set = rng.integers(0, n_sets, size=n_jobs)
-    total_q = int(sum(len(pools[s][0]) for s in job_set))
+    job_sort = rng.random(n_jobs) < 0.2
+    total_q = int(sum(len(pools[s][0]) for s, srt in zip(job_set, job_sort) if not srt))
+    total_sorted = int(sum(len(sets[s]) for s, srt in zip(job_set, job_sort) if srt))
+
+    def outbuf(k):
+        s = job_set[k]
+        return D._hip.pinned_empty(len(sets[s]), np.int32) if job_sort[k] else \
+            D._hip.pinned_empty(len(pools[s][0]), np.int8)
 
     # baseline: one synchronous request at a time on device-resident points
     H = D._hip._lib()
     stream = D.Stream()
     dev_sets = [D.DeviceArray.from_host(p) for p in sets]
     qmax = max(len(p[0]) for p in pools)
+    nmax = int(max(sizes))
     dop, didx = D.DeviceArray((qmax,), np.uint8), D.DeviceArray((qmax, 5), np.int32)
     dout = D.DeviceArray((qmax,), np.int8)
+    ident = D.DeviceArray.from_host(np.arange(nmax, dtype=np.int32))
+    dord = D.DeviceArray((nmax,), np.int32)
     lib = D.lib()
-    base_out = []
+    base_out = [outbuf(k) for k in range(n_jobs)]
     t0 = time.perf_counter()
-    outs = [D._hip.pinned_empty(len(pools[s][0]), np.int8) for s in job_set]
     for k, s in enumerate(job_set):
-        op, idx = pools[s]
-        out = outs[k]
-        D._hip.check(H.hipMemcpyAsync(dop.ptr, op.ctypes.data, op.nbytes, 1, stream.handle), "H2D")
-        D._hip.check(H.hipMemcpyAsync(didx.ptr, idx.ctypes.data, idx.nbytes, 1, stream.handle), "H2D")
-        assert lib.del3d_predicates(dev_sets[s].ptr, len(sets[s]), dop.ptr, didx.ptr, len(op),
-                                    dout.ptr, stream.handle) == 0
-        D._hip.check(H.hipMemcpyAsync(out.ctypes.data, dout.ptr, out.nbytes, 2, stream.handle), "D2H")
+        out = base_out[k]
+        if job_sort[k]:
+            n = len(sets[s])
+            D._hip.check(H.hipMemcpyAsync(dord.ptr, ident.ptr, 4 * n, 3, stream.handle), "D2D")
+            assert lib.del3d_spatial_sort(dev_sets[s].ptr, n, dord.ptr, n, stream.handle) == 0
+            D._hip.check(H.hipMemcpyAsync(out.ctypes.data, dord.ptr, 4 * n, 2, stream.handle), "D2H")
+        else:
+            op, idx = pools[s]
+            D._hip.check(H.hipMemcpyAsync(dop.ptr, op.ctypes.data, op.nbytes, 1, stream.handle), "H2D")
+            D._hip.check(H.hipMemcpyAsync(didx.ptr, idx.ctypes.data, idx.nbytes, 1, stream.handle), "H2D")
+            assert lib.del3d_predicates(dev_sets[s].ptr, len(sets[s]), dop.ptr, didx.ptr, len(op),
+                                        dout.ptr, stream.handle) == 0
+            D._hip.check(H.hipMemcpyAsync(out.ctypes.data, dout.ptr, out.nbytes, 2, stream.handle), "D2H")
         stream.synchronize()
-        base_out.append(out)
     t_base = time.perf_counter() - t0
-    del dev_sets, dop, didx, dout
+    del dev_sets, dop, didx, dout, ident, dord
 
     srv = D.Server(200 << 30, slots=3, max_batch_queries=1 << 24)
     ids = [srv.put_points(p) for p in sets]
-    # warm-up: one job per slot size class
-    for s in range(3):
-        srv.wait(srv.submit(ids[s], *pools[s]))
+    outs = [outbuf(k) for k in range(n_jobs)]
+
+    def submit(k):
+        s = job_set[k]
+        return srv.submit_sort(ids[s], outs[k]) if job_sort[k] else srv.submit(ids[s], *pools[s], outs[k])
+
+    for k in range(3):  # warm-up: the slots' buffers
+        srv.wait(submit(k))
 
     def phase(rate):
         res, lat = [None] * n_jobs, [0.0] * n_jobs
         sub = [None] * n_jobs
-        done = threading.Event()
 
         def waiter():
             for k in range(n_jobs):
                 while sub[k] is None:
                     time.sleep(0.0001)
-                res[k], lat[k] = srv.wait(sub[k])
-            done.set()
+                r, lat[k] = srv.wait(sub[k])
+                res[k] = r.copy()
 
         th = threading.Thread(target=waiter)
         th.start()
         t_start = time.perf_counter()
         r = np.random.default_rng(11)
-        for k, s in enumerate(job_set):
-            sub[k] = srv.submit(ids[s], *pools[s])
+        for k in range(n_jobs):
+            sub[k] = submit(k)
             if rate:
                 time.sleep(r.exponential(1.0 / rate))
         th.join()
@@ -321,7 +341,7 @@ def run_served(args) -> dict:
     t0 = time.perf_counter()
     for k, s in enumerate(small_set):
         op, idx = small[k]
-        out = np.empty(q_small, np.int8)
+        out = D._hip.pinned_empty(q_small, np.int8)
         D._hip.check(H.hipMemcpyAsync(dop.ptr, op.ctypes.data, op.nbytes, 1, stream.handle), "H2D")
         D._hip.check(H.hipMemcpyAsync(didx.ptr, idx.ctypes.data, idx.nbytes, 1, stream.handle), "H2D")
         assert lib.del3d_predicates(dev_sets[s].ptr, len(sets[s]), dop.ptr, didx.ptr, q_small,
@@ -331,8 +351,9 @@ def run_served(args) -> dict:
         small_base.append(out)
     t_small_base = time.perf_counter() - t0
     del dev_sets, dop, didx, dout
+    small_out = [D._hip.pinned_empty(q_small, np.int8) for _ in range(n_small)]
     t0 = time.perf_counter()
-    sids = [srv.submit(ids[s], *small[k]) for k, s in enumerate(small_set)]
+    sids = [srv.submit(ids[s], *small[k], small_out[k]) for k, s in enumerate(small_set)]
     small_res = [srv.wait(j) for j in sids]
     t_small = time.perf_counter() - t0
     same_small = all(np.array_equal(r[0], b) for r, b in zip(small_res, small_base))
@@ -341,26 +362,29 @@ def run_served(args) -> dict:
     srv.close()
     same = all(np.array_equal(a, b) for a, b in zip(res1, base_out)) and \
         all(np.array_equal(a, b) for a, b in zip(res2, base_out))
-    return {"config": "served", "sets": n_sets, "jobs": n_jobs, "queries": total_q,
+    sort_lat = [l for l, srt in zip(lat2, job_sort) if srt]
+    return {"config": "served", "sets": n_sets, "jobs": n_jobs, "sort_jobs": int(job_sort.sum()),
+            "queries": total_q, "points_sorted": total_sorted,
             "points_resident": int(sizes.sum()),
+            "saturated_s": round(t1, 4), "per_request_s": round(t_base, 4),
+            "speedup_vs_per_request": round(t_base / t1, 3),
             "saturated_gpred_per_s": round(total_q / t1 / 1e9, 3),
             "per_request_gpred_per_s": round(total_q / t_base / 1e9, 3),
-            "speedup_vs_per_request": round(t_base / t1, 2),
             "poisson_rate_jobs_per_s": round(rate, 1),
             "poisson_gpred_per_s": round(total_q / t2 / 1e9, 3),
             "p50_ms": round(float(np.percentile(lat2, 50)), 3),
             "p99_ms": round(float(np.percentile(lat2, 99)), 3),
+            "sort_job_p50_ms": round(float(np.percentile(sort_lat, 50)), 3) if sort_lat else None,
             "saturated_p99_ms": round(float(np.percentile(lat1, 99)), 3),
             "per_request_mean_ms": round(1e3 * t_base / n_jobs, 3),
-            "same_signs_as_per_request": bool(same and same_small), "server": info,
+            "same_results_as_per_request": bool(same and same_small), "server": info,
             "small_jobs": {"jobs": n_small, "queries_per_job": q_small,
                            "served_jobs_per_s": round(n_small / t_small, 1),
                            "per_request_jobs_per_s": round(n_small / t_small_base, 1),
                            "speedup_vs_per_request": round(t_small_base / t_small, 2),
                            "saturated_p50_ms": round(float(np.percentile(small_lat, 50)), 3),
                            "saturated_p99_ms": round(float(np.percentile(small_lat, 99)), 3)},
-            "note": "predicate jobs only: sort jobs are not served (GPU sort disabled); "
-                    "latencies include the copies of each job's queries and signs between "
+            "note": "latencies include the copies of each job's queries and results between "
                     "pinned host buffers and the device"}
 
 

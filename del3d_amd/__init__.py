@@ -72,6 +72,8 @@ def lib() -> C.CDLL:
         L.del3d_server_drop_points.argtypes = [vp, i64]
         L.del3d_server_submit.argtypes = [vp, i64, vp, vp, i64, vp, i64p]
         L.del3d_server_wait.argtypes = [vp, i64, C.POINTER(C.c_double)]
+        L.del3d_server_submit_sort.argtypes = [vp, i64, vp, i64p]
+        L.del3d_release_cache.argtypes = []
         L.del3d_server_info.argtypes = [vp, i64p]
         L.del3d_server_destroy.argtypes = [vp]
         L.del3d_spatial_sort.argtypes = [vp, i64, vp, i64, vp]
@@ -281,7 +283,9 @@ class Server:
         self._jobs = {}
 
     def put_points(self, points) -> int:
-        xyz = np.ascontiguousarray(np.asarray(points, dtype=np.float64).reshape(-1, 3))
+        xyz = points if (isinstance(points, np.ndarray) and points.dtype == np.float64 and points.ndim == 2
+                         and points.shape[1] == 3 and points.flags.c_contiguous) \
+            else np.ascontiguousarray(np.asarray(points, dtype=np.float64).reshape(-1, 3))
         sid = C.c_int64()
         _check(lib().del3d_server_put_points(self.handle, xyz.ctypes.data, len(xyz), C.byref(sid)),
                "del3d_server_put_points")
@@ -292,23 +296,39 @@ class Server:
         _check(lib().del3d_server_drop_points(self.handle, int(set_id)), "del3d_server_drop_points")
         self._sets.pop(set_id, None)
 
-    def submit(self, set_id: int, op, idx) -> int:
+    def submit(self, set_id: int, op, idx, out=None) -> int:
+        """A predicate job; `out` (int8, e.g. pinned) receives the signs."""
         op = np.ascontiguousarray(op, dtype=np.uint8)
         idx = np.ascontiguousarray(np.asarray(idx, dtype=np.int32).reshape(-1, 5))
-        out = np.empty(len(op), dtype=np.int8)
+        if out is None:
+            out = np.empty(len(op), dtype=np.int8)
+        assert out.dtype == np.int8 and len(out) == len(op) and out.flags.c_contiguous
         jid = C.c_int64()
         _check(lib().del3d_server_submit(self.handle, int(set_id), op.ctypes.data, idx.ctypes.data,
                                          len(op), out.ctypes.data, C.byref(jid)), "del3d_server_submit")
         self._jobs[jid.value] = (op, idx, out)
         return jid.value
 
+    def submit_sort(self, set_id: int, out=None) -> int:
+        """A sort job: the spatial sort of the whole resident set; `out` (an
+        int32 array of the set's size, e.g. pinned) receives the order."""
+        n = len(self._sets[set_id])
+        if out is None:
+            out = np.empty(n, dtype=np.int32)
+        assert out.dtype == np.int32 and len(out) == n and out.flags.c_contiguous
+        jid = C.c_int64()
+        _check(lib().del3d_server_submit_sort(self.handle, int(set_id), out.ctypes.data, C.byref(jid)),
+               "del3d_server_submit_sort")
+        self._jobs[jid.value] = (out,)
+        return jid.value
+
     def wait(self, job_id: int):
-        """(signs, latency in ms) of a finished job."""
+        """(signs or sort order, latency in ms) of a finished job."""
         ms = C.c_double()
         rc = lib().del3d_server_wait(self.handle, int(job_id), C.byref(ms))
-        op, idx, out = self._jobs.pop(job_id)
+        bufs = self._jobs.pop(job_id)
         _check(rc, "del3d_server_wait")
-        return out, ms.value
+        return bufs[-1], ms.value
 
     def info(self) -> dict:
         v = (C.c_int64 * 8)()

@@ -39,20 +39,51 @@ int device_cu_count() {
 }
 
 // ---- scratch cache, pinned staging, fill -----------------------------------
+// Device scratch blocks are cached per (device, stream) after a call, so the
+// next call on that stream reuses them without hipMalloc. The cache holds at
+// most kCacheCapPerStream bytes per stream (a block given back beyond that is
+// freed), is emptied and retried once when hipMalloc runs out of memory, and
+// del3d_release_cache() empties it on request.
 namespace {
 struct CacheKey {
     int dev;
     hipStream_t s;
     bool operator<(const CacheKey& o) const { return dev != o.dev ? dev < o.dev : s < o.s; }
 };
+struct FreeList {
+    std::multimap<size_t, void*> blocks;  // size -> block
+    size_t bytes = 0;
+};
 std::mutex g_cache_mu;
-std::map<CacheKey, std::multimap<size_t, void*>> g_free;      // per stream: size -> block
+std::map<CacheKey, FreeList> g_free;                         // per stream
 std::map<CacheKey, std::pair<void*, size_t>> g_staging;      // per stream pinned buffer
+constexpr size_t kCacheCapPerStream = size_t(8) << 30;
 
 size_t round_block(size_t b) {
     size_t r = 256;
     while (r < b) r = r < (1u << 20) ? r * 2 : r + (1u << 20);
     return r;
+}
+
+// Frees the cached blocks (and, with staging, the pinned staging buffers) of
+// every stream of device `dev`. hipFree waits for the device, so no block is
+// freed while queued work may still use it. Caller holds g_cache_mu.
+void trim_cache_locked(int dev, bool staging_too) {
+    for (auto& kv : g_free) {
+        if (kv.first.dev != dev) continue;
+        for (auto& b : kv.second.blocks) (void)hipFree(b.second);
+        kv.second.blocks.clear();
+        kv.second.bytes = 0;
+    }
+    if (!staging_too) return;
+    for (auto it = g_staging.begin(); it != g_staging.end();) {
+        if (it->first.dev == dev) {
+            if (it->second.first) (void)hipHostFree(it->second.first);
+            it = g_staging.erase(it);
+        } else {
+            ++it;
+        }
+    }
 }
 }  // namespace
 
@@ -62,17 +93,28 @@ void* scratch_take(size_t bytes, hipStream_t s, size_t* got) {
     const size_t want = round_block(bytes);
     {
         std::lock_guard<std::mutex> lock(g_cache_mu);
-        auto& fl = g_free[CacheKey{dev, s}];
-        auto it = fl.lower_bound(want);
-        if (it != fl.end() && it->first <= 2 * want + (1u << 20)) {
+        FreeList& fl = g_free[CacheKey{dev, s}];
+        auto it = fl.blocks.lower_bound(want);
+        if (it != fl.blocks.end() && it->first <= 2 * want + (1u << 20)) {
             void* p = it->second;
             *got = it->first;
-            fl.erase(it);
+            fl.bytes -= it->first;
+            fl.blocks.erase(it);
             return p;
         }
     }
     void* p = nullptr;
-    D3_HIP(hipMalloc(&p, want));
+    hipError_t e = hipMalloc(&p, want);
+    if (e == hipErrorOutOfMemory) {  // give the cache back to the runtime and retry once
+        (void)hipGetLastError();
+        {
+            std::lock_guard<std::mutex> lock(g_cache_mu);
+            trim_cache_locked(dev, false);
+        }
+        p = nullptr;
+        e = hipMalloc(&p, want);
+    }
+    D3_HIP(e);
     *got = want;
     return p;
 }
@@ -81,7 +123,41 @@ void scratch_give(void* p, size_t bytes, hipStream_t s) {
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess) return;
     std::lock_guard<std::mutex> lock(g_cache_mu);
-    g_free[CacheKey{dev, s}].emplace(bytes, p);
+    FreeList& fl = g_free[CacheKey{dev, s}];
+    if (fl.bytes + bytes > kCacheCapPerStream) {
+        (void)hipFree(p);  // waits for the device: the block's last users have finished
+        return;
+    }
+    fl.blocks.emplace(bytes, p);
+    fl.bytes += bytes;
+}
+
+void release_stream_cache(hipStream_t s) {
+    int dev = 0;
+    D3_HIP(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lock(g_cache_mu);
+    auto it = g_free.find(CacheKey{dev, s});
+    if (it == g_free.end()) return;
+    for (auto& b : it->second.blocks) (void)hipFree(b.second);
+    g_free.erase(it);
+}
+
+bool host_is_pinned(const void* p) {
+    hipPointerAttribute_t a{};
+    const hipError_t e = hipPointerGetAttributes(&a, p);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();  // an unregistered pointer is not an error here
+        return false;
+    }
+    return a.type == hipMemoryTypeHost;
+}
+
+void release_cache_current_device() {
+    int dev = 0;
+    D3_HIP(hipGetDevice(&dev));
+    D3_HIP(hipDeviceSynchronize());
+    std::lock_guard<std::mutex> lock(g_cache_mu);
+    trim_cache_locked(dev, true);
 }
 
 static void* staging(size_t bytes, hipStream_t s) {
@@ -192,7 +268,14 @@ extern "C" {
 
 const char* del3d_last_error(void) { return last_error_slot().c_str(); }
 
-const char* del3d_version(void) { return "del3d_amd 0.1 (gfx950)"; }
+const char* del3d_version(void) { return "del3d_amd 0.2 (gfx950)"; }
+
+int del3d_release_cache(void) {
+    return guarded([&] {
+        release_cache_current_device();
+        return DEL3D_OK;
+    });
+}
 
 int del3d_predicates(const double* xyz, int64_t n_pts, const uint8_t* op, const int32_t* idx,
                      int64_t n_q, int8_t* sign_out, void* stream) {

@@ -51,9 +51,19 @@ struct HipError {
 // hipMalloc'd blocks and given back to that cache when the call ends. A block
 // given back may still be read by work queued on its stream, so it is only
 // handed out again to a later call on the same stream, which the stream
-// orders after that work. Nothing is freed to the runtime while in use.
+// orders after that work; a block freed to the runtime (cache over its cap,
+// out-of-memory retry, del3d_release_cache) goes through hipFree, which waits
+// for the device.
 void* scratch_take(size_t bytes, hipStream_t s, size_t* got);
 void scratch_give(void* p, size_t bytes, hipStream_t s);
+// Waits for the current device, then frees every cached scratch block and
+// pinned staging buffer of it (del3d_release_cache).
+void release_cache_current_device();
+// Frees the cached scratch blocks of one stream (hipFree waits for the device).
+void release_stream_cache(hipStream_t s);
+// Whether host memory p is page-locked (hipHostMalloc'd), so that the DMA
+// engines can copy it without staging.
+bool host_is_pinned(const void* p);
 
 class Scratch {
 public:

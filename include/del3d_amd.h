@@ -68,6 +68,17 @@ const char* del3d_last_error(void);
 const char* del3d_version(void);
 
 /*
+ * Device memory the library keeps between calls (no counterpart in the
+ * reference): scratch blocks are cached per stream after a call (at most
+ * 8 GiB per stream; a block beyond that is freed), so later calls on the
+ * stream reuse them, and pinned staging buffers serve the host-memory forms.
+ * A hipMalloc that runs out of memory empties the cache and is retried once.
+ * del3d_release_cache waits for the current device and frees all of it (the
+ * next call allocates again).
+ */
+int del3d_release_cache(void);
+
+/*
  * Batched predicates over device-resident data.
  *
  * Replaces, per query q, one call of
@@ -206,7 +217,7 @@ int del3d_pred_plan_destroy(del3d_pred_plan plan);
 
 /*
  * Serving loop (no counterpart in the reference): continuous batching of
- * predicate jobs on resident point sets, under an HBM budget.
+ * predicate and sort jobs on resident point sets, under an HBM budget.
  *
  *   del3d_server_create      budget in bytes for resident sets plus batch
  *                            buffers; n_slots batches in flight at once (1..8,
@@ -227,6 +238,14 @@ int del3d_pred_plan_destroy(del3d_pred_plan plan);
  *                            del3d_server_wait returns for the job. Returns at
  *                            once; jobs that arrive while batches run are
  *                            admitted into the next free slot.
+ *   del3d_server_submit_sort queues a sort job: the Hilbert spatial sort of the
+ *                            whole resident set (order 0..n-1, as
+ *                            del3d_spatial_sort sorts it); the permutation is
+ *                            written to the host array order_out (n_pts int32),
+ *                            which must stay valid until the job is waited for.
+ *                            A sort job runs as a batch of its own; its scratch
+ *                            (about 96 B per point + 64 MiB) counts against the
+ *                            budget while it runs.
  *   del3d_server_wait        blocks until the job finished; latency_ms (may be
  *                            NULL) receives submit-to-completion time. Returns
  *                            the job's status (DEL3D_ERR_NOMEM if its set or
@@ -235,7 +254,10 @@ int del3d_pred_plan_destroy(del3d_pred_plan plan);
  *                            resident sets, evictions, uploads, batches, jobs
  *                            finished, budget.
  *   del3d_server_destroy     lets queued jobs finish, then frees everything.
- * Each job's signs are exactly those of del3d_predicates on its set alone.
+ * Each job's signs are exactly those of del3d_predicates on its set alone,
+ * each sort job's permutation that of del3d_spatial_sort. Host arrays that
+ * are pinned (hipHostMalloc) are copied by DMA directly; pageable ones go
+ * through the serving loop's pinned staging.
  */
 typedef struct del3d_server_s* del3d_server;
 int del3d_server_create(int64_t hbm_budget_bytes, int32_t n_slots, int64_t max_batch_queries,
@@ -244,6 +266,8 @@ int del3d_server_put_points(del3d_server server, const double* xyz, int64_t n_pt
 int del3d_server_drop_points(del3d_server server, int64_t set_id);
 int del3d_server_submit(del3d_server server, int64_t set_id, const uint8_t* op, const int32_t* idx,
                         int64_t n_q, int8_t* sign_out, int64_t* job_id);
+int del3d_server_submit_sort(del3d_server server, int64_t set_id, int32_t* order_out,
+                             int64_t* job_id);
 int del3d_server_wait(del3d_server server, int64_t job_id, double* latency_ms);
 int del3d_server_info(del3d_server server, int64_t* info);
 int del3d_server_destroy(del3d_server server);

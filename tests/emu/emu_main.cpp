@@ -192,9 +192,33 @@ static int check_server() {
             }
         }
     };
+    // sort jobs beside them, on the two smallest sets (a sort's scratch takes
+    // most of the budget, so they wait for memory and evict)
+    std::vector<int32_t> sort_out[3], sort_want[3];
+    const int sort_set[3] = {0, 1, 0};
+    int64_t sort_id[3];
+    for (int k = 0; k < 3; ++k) {
+        const int64_t n = sizes[sort_set[k]];
+        sort_out[k].assign(n, -1);
+        sort_want[k].resize(n);
+        for (int64_t i = 0; i < n; ++i) sort_want[k][i] = (int32_t)i;
+        orc_spatial_sort(pts[sort_set[k]].data(), n, sort_want[k].data());
+        if (del3d_server_submit_sort(sv, ids[sort_set[k]], sort_out[k].data(), &sort_id[k]) != 0) {
+            std::printf("sort submit failed: %s\n", del3d_last_error());
+            ++fails;
+            sort_id[k] = -1;
+        }
+    }
     std::thread t0(client, 0), t1(client, 1);
     t0.join();
     t1.join();
+    for (int k = 0; k < 3; ++k) {
+        double ms = 0;
+        if (sort_id[k] < 0 || del3d_server_wait(sv, sort_id[k], &ms) != 0 || sort_out[k] != sort_want[k]) {
+            std::printf("server sort job %d MISMATCH (%s)\n", k, del3d_last_error());
+            ++fails;
+        }
+    }
     for (int k = 0; k < kJobs; ++k)
         if (jobs[k].out != jobs[k].want) {
             std::printf("server job %d (set %d) MISMATCH\n", k, jobs[k].set);

@@ -245,11 +245,17 @@ inline void enqueue(std::function<void()> f) {
 }  // namespace emu
 
 namespace emu {
+// The registry of pinned allocations (several host threads allocate and query).
 inline std::vector<std::pair<const char*, size_t>>& pinned() {
     static std::vector<std::pair<const char*, size_t>> v;
     return v;
 }
+inline std::mutex& pinned_mutex() {
+    static std::mutex m;
+    return m;
+}
 inline bool is_pinned(const void* p) {
+    std::lock_guard<std::mutex> lk(pinned_mutex());
     for (auto& r : pinned())
         if ((const char*)p >= r.first && (const char*)p < r.first + r.second) return true;
     return false;
@@ -297,10 +303,40 @@ inline hipError_t hipEventSynchronize(hipEvent_t) {
     emu::drain();
     return hipSuccess;
 }
-inline hipError_t hipStreamCreateWithFlags(hipStream_t* s, unsigned) { *s = nullptr; return hipSuccess; }
+// Streams are distinct handles (the library keys per-stream state on them);
+// the emulation orders all work in one queue anyway.
+inline hipError_t hipStreamCreateWithFlags(hipStream_t* s, unsigned) {
+    static std::atomic<uintptr_t> next{0x1000};
+    *s = reinterpret_cast<hipStream_t>(next.fetch_add(0x10));
+    return hipSuccess;
+}
+inline hipError_t hipStreamCreate(hipStream_t* s) { return hipStreamCreateWithFlags(s, 0); }
+enum hipMemoryType { hipMemoryTypeUnregistered = 0, hipMemoryTypeHost = 1, hipMemoryTypeDevice = 2 };
+struct hipPointerAttribute_t {
+    hipMemoryType type;
+    int device;
+    void* devicePointer;
+    void* hostPointer;
+    int isManaged;
+    unsigned allocationFlags;
+};
+// Pinned (hipHostMalloc'd) memory is host memory to the runtime; anything
+// else is unregistered (an error, as the runtime reports it).
+inline hipError_t hipPointerGetAttributes(hipPointerAttribute_t* a, const void* p) {
+    if (!emu::is_pinned(p)) return hipErrorInvalidValue;
+    a->type = hipMemoryTypeHost;
+    return hipSuccess;
+}
+inline hipError_t hipDeviceSynchronize() {
+    emu::drain();
+    return hipSuccess;
+}
 inline hipError_t hipHostMalloc(void** p, size_t n, unsigned) {
     hipError_t e = hipMallocAsync(p, n, nullptr);
-    if (e == hipSuccess) emu::pinned().emplace_back((const char*)*p, n);
+    if (e == hipSuccess) {
+        std::lock_guard<std::mutex> lk(emu::pinned_mutex());
+        emu::pinned().emplace_back((const char*)*p, n);
+    }
     return e;
 }
 inline hipError_t hipEventCreate(hipEvent_t* e) { *e = (void*)1; return hipSuccess; }
@@ -332,9 +368,12 @@ inline hipError_t hipSetDevice(int) { return hipSuccess; }
 #define amdgpu_waves_per_eu(...) unused
 inline hipError_t hipHostFree(void* p) {
     emu::drain();
-    auto& v = emu::pinned();
-    for (size_t i = 0; i < v.size(); ++i)
-        if (v[i].first == p) { v.erase(v.begin() + i); break; }
+    {
+        std::lock_guard<std::mutex> lk(emu::pinned_mutex());
+        auto& v = emu::pinned();
+        for (size_t i = 0; i < v.size(); ++i)
+            if (v[i].first == p) { v.erase(v.begin() + i); break; }
+    }
     std::free(p);
     return hipSuccess;
 }

@@ -1,7 +1,8 @@
-"""GPU checks of the serving loop: jobs arriving as a Poisson stream from two
-client threads, over resident point sets whose sizes together exceed the HBM
-budget (so sets are evicted and uploaded again, and batches wait for memory),
-each job's signs equal to the same queries run alone (tolerance 0)."""
+"""GPU checks of the serving loop: predicate and sort jobs arriving as a
+Poisson stream from two client threads, over resident point sets whose sizes
+together exceed the HBM budget (so sets are evicted and uploaded again, and
+batches wait for memory), each job's signs or sort order equal to the same
+work run alone (tolerance 0), with pinned and pageable host arrays."""
 import threading
 
 import numpy as np
@@ -100,3 +101,45 @@ def test_oversized_set_is_refused_and_others_run():
     got, _ = srv.wait(srv.submit(sid, op, idx))
     srv.close()
     assert np.array_equal(got, predicates_dev(small, op, idx)[0])
+
+
+def test_sort_and_predicate_jobs_mixed_equal_alone():
+    rng = np.random.default_rng(33)
+    sizes = [5000, 60000, 300000, 12000]
+    sets = [W.set_points(i % 3, n, rng) for i, n in enumerate(sizes)]
+    sorted_alone = [O.spatial_sort(p) for p in sets]
+    jobs = []
+    for k in range(40):
+        s = int(rng.integers(0, len(sets)))
+        if k % 3 == 0:
+            jobs.append(("sort", s, None, None))
+        else:
+            nq = int(2000 * 100 ** rng.random())
+            op = rng.integers(0, 3, size=nq).astype(np.uint8)
+            idx = rng.integers(0, len(sets[s]), size=(nq, 5)).astype(np.int32)
+            jobs.append(("pred", s, op, idx))
+    # 160 MB: the sets (9 MB), a 300,000-point sort's scratch (93 MB as
+    # budgeted) and the batch buffers (up to 31 MB per slot) do not all fit
+    # at once, so work waits for memory
+    srv = D.Server(160 << 20, slots=3, max_batch_queries=1 << 20)
+    # every other set from a pinned host array, the others pageable
+    ids = [srv.put_points(D._hip.pinned_copy(p) if i % 2 else p) for i, p in enumerate(sets)]
+    sub = []
+    for k, (kind, s, op, idx) in enumerate(jobs):
+        if kind == "sort":
+            out = D._hip.pinned_empty(len(sets[s]), np.int32) if k % 2 else None
+            sub.append(srv.submit_sort(ids[s], out))
+        else:
+            out = D._hip.pinned_empty(len(op), np.int8) if k % 2 else None
+            sub.append(srv.submit(ids[s], op, idx, out))
+    say("serving", len(jobs), "mixed jobs")
+    res = [srv.wait(j) for j in sub]
+    info = srv.info()
+    srv.close()
+    say("server", info)
+    for k, (kind, s, op, idx) in enumerate(jobs):
+        got = res[k][0]
+        if kind == "sort":
+            assert np.array_equal(got, sorted_alone[s]), k
+        else:
+            assert np.array_equal(got, predicates_dev(sets[s], op, idx)[0]), k
