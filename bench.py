@@ -138,6 +138,12 @@ def run_batched(args) -> dict:
     ms_graph = _events_ms(D, stream, lambda: plan.run(xyz, op, idx, out, stream, graph=True), args.steps)
 
     lib = D.lib()
+    if args.plan_only:
+        knobs = plan.knobs
+        plan.close()
+        return {"config": "batched", "sets": len(sizes), "queries": n_q,
+                "gpred_per_s_graph": round(n_q * args.steps / (ms_graph / 1e3) / 1e9, 3),
+                "ms_per_step_graph": round(ms_graph / args.steps, 3), "autotuned_knobs": knobs}
 
     def one_by_one():
         for s in range(len(sizes)):
@@ -889,6 +895,8 @@ def main() -> int:
     ap.add_argument("--headline-only", action="store_true",
                     help="only the large configuration, without the other configurations")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--plan-only", action="store_true",
+                    help="batched configuration: the plan's graph replays only (for profiling)")
     ap.add_argument("--no-sort", action="store_true",
                     help="headline without its sort half (for counter runs of the predicate kernel)")
     ap.add_argument("--no-check", action="store_true",

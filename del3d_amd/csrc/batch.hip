@@ -29,6 +29,9 @@ struct del3d_pred_plan_s {
     int64_t* d_q_off = nullptr;
     d3::PredTask* d_tasks = nullptr;
     size_t d_tasks_cap = 0;
+    uint32_t* d_bins = nullptr;  // per chunk, kBins+1 offsets of the XCD bins of its global tasks
+    size_t d_bins_cap = 0;
+    bool binned = false;
     uint32_t* queue = nullptr;
     uint32_t* wide = nullptr;
     d3::PredCounters* ctr = nullptr;
@@ -37,6 +40,7 @@ struct del3d_pred_plan_s {
     uint32_t queue_cap = 0;
     struct Chunk {
         uint32_t lds0, n_lds, glob0, n_glob;  // task ranges in d_tasks
+        uint32_t bins0;                       // the chunk's bin offsets in d_bins (when binned)
     };
     std::vector<Chunk> chunks;
     // graph cache
@@ -54,7 +58,7 @@ void default_knobs(int32_t* k) {
     k[1] = 32 * kLdsThreads;    // lds_task_queries
     k[2] = 16 * kGlobalThreads; // global_task_queries
     k[3] = 8;                   // global_blocks_per_cu
-    k[4] = 4;                   // exact_blocks_per_cu (unused by the fast kernels)
+    k[4] = 1;                   // global tasks in per-XCD bins (0: one shared list)
 }
 
 void drop_graph(del3d_pred_plan_s* p) {
@@ -67,54 +71,96 @@ void drop_graph(del3d_pred_plan_s* p) {
 }
 
 // Builds the task table from the offsets and the knobs, uploads it.
+//
+// LDS tasks are ordered largest first. Global tasks stay grouped by set (a
+// set's tail task next to its others), and a launch chunk takes whole sets
+// while they fit (a set larger than the room left is split at a task
+// boundary). With knobs[4] = 1, each chunk's sets are then dealt to kBins
+// bins, one per XCD, largest set first to the bin with the fewest queries,
+// so that each XCD works through its own sets and keeps them in its L2.
 void build_tasks(del3d_pred_plan_s* p, hipStream_t s) {
     const int64_t lds_max = std::min<int64_t>(p->knobs[0], kLdsPts);
-    const int64_t lds_q = std::max<int64_t>(p->knobs[1], kLdsThreads);
-    const int64_t glob_q = std::max<int64_t>(p->knobs[2], kGlobalThreads);
-    std::vector<PredTask> lds, glob;
+    // a task never exceeds a launch chunk
+    const int64_t lds_q = std::min<int64_t>(std::max<int64_t>(p->knobs[1], kLdsThreads), kChunk);
+    const int64_t glob_q = std::min<int64_t>(std::max<int64_t>(p->knobs[2], kGlobalThreads), kChunk);
+    const bool binned = p->knobs[4] != 0;
+    std::vector<PredTask> lds;
+    struct Group {
+        int64_t q;                   // queries of the group
+        std::vector<PredTask> tasks;  // tasks of one set, in query order
+    };
+    std::vector<Group> glob;
     for (int64_t si = 0; si < p->n_sets; ++si) {
         const int64_t n = p->pt_off[si + 1] - p->pt_off[si];
         const int64_t a = p->q_off[si], b = p->q_off[si + 1];
         if (b <= a) continue;
         const bool in_lds = n <= lds_max;
         const int64_t step = in_lds ? lds_q : glob_q;
+        if (!in_lds) glob.push_back(Group{b - a, {}});
         for (int64_t q = a; q < b; q += step) {
             PredTask t;
             t.q0 = q;
             t.q1 = std::min(b, q + step);
             t.set = (int32_t)si;
             t.pad = 0;
-            (in_lds ? lds : glob).push_back(t);
+            (in_lds ? lds : glob.back().tasks).push_back(t);
         }
     }
     auto bigger = [](const PredTask& x, const PredTask& y) {
         return (x.q1 - x.q0) != (y.q1 - y.q0) ? (x.q1 - x.q0) > (y.q1 - y.q0) : x.q0 < y.q0;
     };
     std::stable_sort(lds.begin(), lds.end(), bigger);
-    std::stable_sort(glob.begin(), glob.end(), bigger);
+    std::stable_sort(glob.begin(), glob.end(), [](const Group& x, const Group& y) { return x.q > y.q; });
     // launch chunks: at most kChunk queries each
     p->chunks.clear();
     std::vector<PredTask> all;
-    all.reserve(lds.size() + glob.size());
-    size_t il = 0, ig = 0;
+    std::vector<uint32_t> bins;
+    size_t il = 0, ig = 0, it = 0;  // next LDS task; next global group and its next task
     while (il < lds.size() || ig < glob.size()) {
         del3d_pred_plan_s::Chunk c{};
         int64_t q = 0;
-        std::vector<PredTask> cl, cg;
+        std::vector<PredTask> cl;
+        std::vector<Group> cg;
         while (il < lds.size() && q + (lds[il].q1 - lds[il].q0) <= (int64_t)kChunk) {
             q += lds[il].q1 - lds[il].q0;
             cl.push_back(lds[il++]);
         }
-        while (ig < glob.size() && q + (glob[ig].q1 - glob[ig].q0) <= (int64_t)kChunk) {
-            q += glob[ig].q1 - glob[ig].q0;
-            cg.push_back(glob[ig++]);
+        while (ig < glob.size()) {
+            const auto& gt = glob[ig].tasks;
+            Group part{0, {}};
+            while (it < gt.size() && q + (gt[it].q1 - gt[it].q0) <= (int64_t)kChunk) {
+                q += gt[it].q1 - gt[it].q0;
+                part.q += gt[it].q1 - gt[it].q0;
+                part.tasks.push_back(gt[it++]);
+            }
+            if (!part.tasks.empty()) cg.push_back(std::move(part));
+            if (it < gt.size()) break;  // the chunk is full
+            ++ig;
+            it = 0;
         }
         c.lds0 = (uint32_t)all.size();
         c.n_lds = (uint32_t)cl.size();
         all.insert(all.end(), cl.begin(), cl.end());
         c.glob0 = (uint32_t)all.size();
-        c.n_glob = (uint32_t)cg.size();
-        all.insert(all.end(), cg.begin(), cg.end());
+        c.bins0 = (uint32_t)bins.size();
+        if (binned) {
+            // cg is largest first: each group goes to the least-loaded bin
+            std::vector<std::vector<const Group*>> bin(kBins);
+            int64_t load[kBins] = {};
+            for (const auto& g : cg) {
+                const int b = (int)(std::min_element(load, load + kBins) - load);
+                load[b] += g.q;
+                bin[b].push_back(&g);
+            }
+            for (uint32_t b = 0; b < kBins; ++b) {
+                bins.push_back((uint32_t)(all.size() - c.glob0));
+                for (const Group* g : bin[b]) all.insert(all.end(), g->tasks.begin(), g->tasks.end());
+            }
+            bins.push_back((uint32_t)(all.size() - c.glob0));
+        } else {
+            for (const auto& g : cg) all.insert(all.end(), g.tasks.begin(), g.tasks.end());
+        }
+        c.n_glob = (uint32_t)(all.size() - c.glob0);
         p->chunks.push_back(c);
     }
     if (all.size() > p->d_tasks_cap) {
@@ -124,7 +170,16 @@ void build_tasks(del3d_pred_plan_s* p, hipStream_t s) {
         D3_HIP(hipMalloc((void**)&p->d_tasks, std::max<size_t>(1, all.size()) * sizeof(PredTask)));
         p->d_tasks_cap = all.size();
     }
+    if (bins.size() > p->d_bins_cap) {
+        if (p->d_bins) D3_HIP(hipFree(p->d_bins));
+        p->d_bins = nullptr;
+        p->d_bins_cap = 0;
+        D3_HIP(hipMalloc((void**)&p->d_bins, bins.size() * sizeof(uint32_t)));
+        p->d_bins_cap = bins.size();
+    }
     if (!all.empty()) copy_h2d(p->d_tasks, all.data(), all.size() * sizeof(PredTask), s);
+    if (!bins.empty()) copy_h2d(p->d_bins, bins.data(), bins.size() * sizeof(uint32_t), s);
+    p->binned = binned;
     drop_graph(p);
 }
 
@@ -165,6 +220,7 @@ void run_plan(del3d_pred_plan_s* p, const double* xyz, const uint8_t* op, const 
         sh.n_global_tasks = c.n_glob;
         sh.global_chunk = 0;
         sh.global_blocks_per_cu = (uint32_t)std::max(1, p->knobs[3]);
+        sh.global_bins = p->binned ? p->d_bins + c.bins0 : nullptr;
         sh.counters_cleared = first_chunk;  // the grid prepass zeroed them
         first_chunk = false;
         enqueue_batch(a, sh, s, p->chunks.size() == 1 ? ev : nullptr);
@@ -343,7 +399,11 @@ int del3d_pred_plan_autotune(del3d_pred_plan p, const double* xyz, const uint8_t
         std::vector<std::vector<int32_t>> cands;
         for (int32_t lmax : {kLdsPts, 2048, 0})
             for (int32_t lq : {8 * kLdsThreads, 32 * kLdsThreads, 128 * kLdsThreads})
-                for (int32_t gb : {4, 8}) cands.push_back({lmax, lq, 16 * kGlobalThreads, gb, base[4]});
+                for (int32_t gb : {4, 8}) cands.push_back({lmax, lq, 16 * kGlobalThreads, gb, 1});
+        // the global task size and the XCD bins, at the default routing
+        for (int32_t gq : {4 * kGlobalThreads, 64 * kGlobalThreads})
+            for (int32_t gb : {4, 8}) cands.push_back({kLdsPts, 32 * kLdsThreads, gq, gb, 1});
+        for (int32_t gb : {4, 8}) cands.push_back({kLdsPts, 32 * kLdsThreads, 16 * kGlobalThreads, gb, 0});
         float best = 1e30f;
         std::vector<int32_t> pick(base, base + DEL3D_PLAN_KNOBS);
         for (const auto& c : cands) {
@@ -378,6 +438,7 @@ int del3d_pred_plan_destroy(del3d_pred_plan p) {
     (void)hipFree(p->d_pt_off);
     (void)hipFree(p->d_q_off);
     (void)hipFree(p->d_tasks);
+    (void)hipFree(p->d_bins);
     (void)hipFree(p->queue);
     (void)hipFree(p->wide);
     (void)hipFree(p->ctr);

@@ -14,6 +14,11 @@ constexpr int kLdsPts = 5632;       // largest point set held in LDS (132 KB)
 constexpr int kPackPts = 33792;     // largest packed point set held in LDS (132 KB, 4 bytes a point)
 constexpr uint32_t kChunk = 1u << 26;  // queries per launch sequence (bounds the queues)
 constexpr int kStats = 6;           // decision counters (see del3d_predicates_stats)
+// XCDs of the chip. Workgroups are dispatched to the XCDs round-robin
+// (workgroup b runs on XCD b % 8), and each XCD has its own L2, so a plan
+// can give each XCD its own task list of whole point sets: the sets one XCD
+// works on at a time then stay in that XCD's L2 (bins, below).
+constexpr uint32_t kBins = 8;
 
 // Queries [q0, q1) of point set `set`; their indices are local to the set.
 struct PredTask {
@@ -25,6 +30,7 @@ struct PredCounters {
     uint32_t queued;        // entries in the exact queue
     uint32_t wide;          // entries in the wide queue
     uint32_t next_task[2];  // task cursors of the global [0] and LDS [1] kernels
+    uint32_t bin_next[kBins];  // task cursors of the global kernel's XCD bins
 };
 
 // A point stored as four floats (16 bytes, one aligned load): the compact
@@ -68,6 +74,10 @@ struct BatchArgs {
     // returns at once when the set's summary routes it to the other
     const uint32_t* pk;
     int32_t pack_route;
+    // global tasks in XCD bins (nullptr: one shared list): bin b holds tasks
+    // [bins[b], bins[b+1]); a workgroup takes tasks from bin blockIdx.x % kBins
+    // first, then from the others once its own is empty
+    const uint32_t* bins;
 };
 
 // How a batch is split between the two fast kernels.
@@ -76,6 +86,7 @@ struct BatchShape {
     uint32_t n_lds_tasks = 0, lds_chunk = 0, lds_blocks_per_cu = 1;
     const PredTask* global_tasks = nullptr;
     uint32_t n_global_tasks = 0, global_chunk = 0, global_blocks_per_cu = 8;
+    const uint32_t* global_bins = nullptr;  // kBins+1 offsets into global_tasks, or nullptr
     bool counters_cleared = false;  // the grid prepass already zeroed the counters
     bool packed = false;            // the LDS tasks run the packed-point kernel
     bool global_packed = false;     // the global tasks also run the global packed variant

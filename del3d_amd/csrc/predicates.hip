@@ -249,11 +249,30 @@ __device__ __forceinline__ void pred_batch_body(const BatchArgs& a, BatchLds<kSt
     int32_t loaded_set = -1;
     unsigned long long* stats = a.stats;
     uint32_t* next = &a.ctr->next_task[kLds ? 1 : 0];
+    // thread 0's position in the bin order (own bin first), when binned
+    const uint32_t home = blockIdx.x % kBins;
+    uint32_t bin_k = 0;
 
     for (;;) {
         __syncthreads();  // the previous task is done with every LDS array
         if (tid == 0) {
-            const uint32_t t = atomicAdd(next, 1u);
+            uint32_t t = 0xFFFFFFFFu;
+            if (!kLds && a.bins) {
+                // each bin's cursor only grows, so every task of a bin is
+                // handed out once; a bin found empty is not tried again
+                while (bin_k < kBins) {
+                    const uint32_t b = (home + bin_k) % kBins;
+                    const uint32_t lo = a.bins[b], hi = a.bins[b + 1];
+                    const uint32_t i = atomicAdd(&a.ctr->bin_next[b], 1u);
+                    if (lo <= hi && i < hi - lo) {
+                        t = lo + i;
+                        break;
+                    }
+                    ++bin_k;
+                }
+            } else {
+                t = atomicAdd(next, 1u);
+            }
             if (t < a.n_tasks) {
                 const PredTask tk = task_at(a, t);
                 int64_t np = a.n_pts;
@@ -756,6 +775,7 @@ void enqueue_batch(const BatchArgs& base, const BatchShape& sh, hipStream_t s, h
         a.tasks = sh.global_tasks;
         a.n_tasks = sh.n_global_tasks;
         a.chunk = sh.global_chunk;
+        a.bins = sh.global_tasks ? sh.global_bins : nullptr;
         const unsigned g =
             (unsigned)std::min<uint64_t>(sh.n_global_tasks, (uint64_t)cus * sh.global_blocks_per_cu);
         if (a.stats) pred_batch_global<true><<<g, kGlobalThreads, 0, s>>>(a);

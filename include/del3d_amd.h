@@ -193,7 +193,9 @@ int del3d_predicates_sets(const double* xyz, const int64_t* pt_off, int64_t n_se
  *                             LDS kernel (at most 5632; 0 = none),
  *                             [1] queries per LDS task, [2] queries per
  *                             global task, [3] global workgroups per CU,
- *                             [4] reserved.
+ *                             [4] 1 = each XCD takes the global tasks of
+ *                             its own share of the sets first (sets stay
+ *                             in that XCD's L2), 0 = one shared list.
  *   del3d_pred_plan_destroy   frees the plan; call it once its runs finished.
  * A plan holds one set of queues: runs of one plan must not overlap (use one
  * stream per plan, or one plan per stream).

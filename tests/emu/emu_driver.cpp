@@ -53,6 +53,28 @@ extern "C" int emu_predicates_sets(const double* xyz, const int64_t* pt_off, int
     return del3d_predicates_sets(xyz, pt_off, n_sets, op, idx, q_off, out, nullptr);
 }
 
+// A multi-set plan run with the given global task size and XCD binning, all
+// sets routed to the global kernel (task stealing between bins is exercised
+// when a bin runs dry before the others).
+extern "C" int emu_plan_sets(const double* xyz, const int64_t* pt_off, int64_t n_sets,
+                             const uint8_t* op, const int32_t* idx, const int64_t* q_off,
+                             int8_t* out, int glob_q, int binned) {
+    del3d_pred_plan p = nullptr;
+    int rc = del3d_pred_plan_create(pt_off, q_off, n_sets, &p);
+    if (rc) return rc;
+    int32_t knobs[DEL3D_PLAN_KNOBS];
+    del3d_pred_plan_get_knobs(p, knobs);
+    knobs[0] = 0;
+    knobs[2] = glob_q;
+    knobs[3] = 1;
+    knobs[4] = binned;
+    rc = del3d_pred_plan_set_knobs(p, knobs, DEL3D_PLAN_KNOBS);
+    int64_t st[6];
+    if (!rc) rc = del3d_pred_plan_run_stats(p, xyz, op, idx, out, nullptr, st);
+    del3d_pred_plan_destroy(p);
+    return rc;
+}
+
 extern "C" int emu_predicates2(const double* xyz, int64_t n_pts, const uint8_t* op,
                                const int32_t* idx, int64_t n_q, int8_t* out) {
     return del3d_predicates2_host(xyz, n_pts, op, idx, n_q, out);

@@ -52,6 +52,9 @@ extern "C" int emu_predicates_sets(const double* xyz, const int64_t* pt_off, int
                                    const uint8_t* op, const int32_t* idx, const int64_t* q_off,
                                    int8_t* out);
 
+extern "C" int emu_plan_sets(const double* xyz, const int64_t* pt_off, int64_t n_sets,
+                             const uint8_t* op, const int32_t* idx, const int64_t* q_off,
+                             int8_t* out, int glob_q, int binned);
 extern "C" int emu_predicates_noalloc(const double* xyz, int64_t n_pts, const uint8_t* op,
                                       const int32_t* idx, int64_t n_q, int8_t* out);
 
@@ -129,8 +132,18 @@ static int check_sets() {
     }
     std::vector<int8_t> got(want.size(), 9);
     const int rc = emu_predicates_sets(xyz.data(), po.data(), 5, op.data(), idx.data(), qo.data(), got.data());
-    const bool ok = rc == 0 && got == want;
+    bool ok = rc == 0 && got == want;
     std::printf("pred sets %s\n", ok ? "ok" : "MISMATCH");
+    // every set through the global kernel in tasks of 256 queries, with and
+    // without XCD bins (5 sets in 8 bins: most workgroups start on an empty bin)
+    for (int binned = 0; binned < 2; ++binned) {
+        std::vector<int8_t> g2(want.size(), 9);
+        const int rc2 = emu_plan_sets(xyz.data(), po.data(), 5, op.data(), idx.data(), qo.data(), g2.data(),
+                                      256, binned);
+        const bool ok2 = rc2 == 0 && g2 == want;
+        std::printf("pred sets plan binned=%d %s\n", binned, ok2 ? "ok" : "MISMATCH");
+        ok = ok && ok2;
+    }
     return ok ? 0 : 1;
 }
 // The serving loop: jobs from two client threads over four sets whose sizes

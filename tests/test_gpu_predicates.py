@@ -279,6 +279,32 @@ def test_sets_call_and_graph_replays():
     plan.close()
 
 
+def test_plan_xcd_bins_and_shared_list_agree():
+    """Global tasks in per-XCD bins (with stealing once a bin is empty) and
+    in one shared list, at several task sizes and occupancies, all routed to
+    the global kernel: every sign equals the single-set calls'."""
+    pts_l, op_l, idx_l = _mixed_sets(48, seed=12)
+    sizes = [len(p) for p in pts_l]
+    po = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
+    qo = np.concatenate([[0], np.cumsum([len(o) for o in op_l])]).astype(np.int64)
+    want = np.concatenate([predicates_dev(p, o, i)[0] for p, o, i in zip(pts_l, op_l, idx_l)])
+    xyz = D.DeviceArray.from_host(np.vstack(pts_l))
+    dop = D.DeviceArray.from_host(np.concatenate(op_l))
+    didx = D.DeviceArray.from_host(np.vstack(idx_l))
+    out = D.DeviceArray((int(qo[-1]),), np.int8)
+    plan = D.PredPlan(po, qo)
+    s = D.Stream()
+    for knobs in ([0, 32768, 256, 1, 1], [0, 32768, 256, 1, 0], [0, 32768, 4096, 8, 1],
+                  [5632, 32768, 1024, 4, 1], [0, 32768, 1 << 20, 2, 1]):
+        say("knobs", knobs)
+        plan.knobs = knobs
+        out.zero()
+        plan.run(xyz, dop, didx, out, s, graph=True)
+        s.synchronize()
+        assert np.array_equal(out.to_host(), want), knobs
+    plan.close()
+
+
 @pytest.mark.parametrize("kind", ["float_exact_lattice", "lattice_2^-120", "lattice_2^-135",
                                   "wide_integers", "random_doubles", "packed_lattice",
                                   "packed_lattice_2^-200"])
