@@ -635,7 +635,39 @@ __global__ __launch_bounds__(256) void k_rd_scan(const RTask* __restrict__ tasks
     if (threadIdx.x == 0) st[t].lim = S.L + (int64_t)carry;
 }
 
-// Round part 3 (one slot per thread, kSlices blocks per tile): the whole
+// Round part 3 (one slot per thread, kSlices blocks per tile): the slots of
+// the elements that are not less than the pivot, in order: the j-th of them
+// (from 0) at npos[L + j], as an offset from the task's first slot. The move
+// kernel's chains jump with it (below).
+constexpr int kSlices = kTile / kTileThreads;
+
+__global__ __launch_bounds__(kTileThreads) void k_rd_npos(const RTask* __restrict__ tasks, uint32_t ntasks,
+                                                          const RTile* __restrict__ tiles, uint32_t ntiles,
+                                                          const RState* __restrict__ st,
+                                                          const uint64_t* __restrict__ mask,
+                                                          const uint16_t* __restrict__ wloc,
+                                                          const uint32_t* __restrict__ toff,
+                                                          uint32_t* __restrict__ npos, int64_t n) {
+    const uint32_t tile = blockIdx.x / kSlices, slice = blockIdx.x % kSlices;
+    if (tile >= ntiles) return;
+    const RTile tl = tiles[tile];
+    if (tl.task < 0 || (uint32_t)tl.task >= ntasks) return;  // k_rd_count raised it
+    const RTask T = tasks[tl.task];
+    const RState S = st[tl.task];
+    if (!S.active || !rd_range_ok(S, n) || S.L < T.b || S.R >= T.e) return;  // k_rd_move raises it
+    const int lane = threadIdx.x & 63;
+    const int w = (int)slice * (kTileThreads / 64) + (threadIdx.x >> 6);
+    const int64_t p = tl.start + (int64_t)w * 64 + lane;
+    if (p >= tl.start + tl.len || p < S.L || p >= S.R) return;
+    const uint64_t gw = (uint64_t)tile * kTileWords + w;
+    const uint64_t m = mask[gw];
+    if ((m >> lane) & 1ull) return;
+    const int64_t rank = (int64_t)toff[tile] + wloc[gw] + __popcll(m & ((1ull << lane) - 1ull));
+    const int64_t j = (p - S.L) - rank;  // not-less elements in [L, p)
+    if (j >= 0 && S.L + j < S.R) npos[S.L + j] = (uint32_t)(p - T.b);
+}
+
+// Round part 4 (one slot per thread, kSlices blocks per tile): the whole
 // Lomuto round as a permutation written into the other buffer.
 //   - an element less than the pivot, at p, goes to L + rank(p), where
 //     rank(p) = number of such elements in [L, p) (order kept);
@@ -650,10 +682,13 @@ __global__ __launch_bounds__(256) void k_rd_scan(const RTask* __restrict__ tasks
 //   - what lands on lim goes to R, and the pivot to lim
 //     (src/spatial_sort.cpp:41-50).
 // rank() needs only the flag words and their prefixes, which are small and
-// stay in cache; every write of this kernel is to consecutive slots. The
-// chains are long only when nearly every element is less than the pivot;
-// one slot per thread spreads them over the whole device.
-constexpr int kSlices = kTile / kTileThreads;
+// stay in cache; every write of this kernel is to consecutive slots.
+// A chain is long when nearly every element is less than the pivot. Between
+// two consecutive not-less slots P = npos[L + j - 1] < s, every slot is less
+// and has j not-less slots below it, so each hop there is s <- s - j: after
+// a few single hops a chain jumps a whole such interval at once, to
+// s - j * ceil((s - P) / j), which is P itself (the chain ends on the element
+// there) or a slot below P with fewer not-less slots below it.
 
 __global__ __launch_bounds__(kTileThreads) void k_rd_move(Bufs bufs, const RTask* __restrict__ tasks,
                                                           uint32_t ntasks, const RTile* __restrict__ tiles,
@@ -662,6 +697,7 @@ __global__ __launch_bounds__(kTileThreads) void k_rd_move(Bufs bufs, const RTask
                                                           const uint64_t* __restrict__ mask,
                                                           const uint16_t* __restrict__ wloc,
                                                           const uint32_t* __restrict__ toff,
+                                                          const uint32_t* __restrict__ npos,
                                                           uint32_t* __restrict__ ring, uint32_t round,
                                                           uint32_t* __restrict__ err) {
     const uint32_t tile = blockIdx.x / kSlices, slice = blockIdx.x % kSlices;
@@ -704,7 +740,23 @@ __global__ __launch_bounds__(kTileThreads) void k_rd_move(Bufs bufs, const RTask
                         bad = true;
                         break;
                     }
-                    s = S.L + (int64_t)ts_off + ws + __popcll(ms & ((1ull << (os & 63)) - 1ull));
+                    const int64_t rank = (int64_t)ts_off + ws + __popcll(ms & ((1ull << (os & 63)) - 1ull));
+                    if (hops <= 3) {
+                        s = S.L + rank;
+                    } else {
+                        const int64_t j = (s - S.L) - rank;  // not-less slots below s
+                        if (j < 1 || j > span) {
+                            bad = true;
+                            break;
+                        }
+                        const int64_t P = T.b + (int64_t)npos[S.L + j - 1];
+                        if (P < S.L || P >= s) {
+                            bad = true;
+                            break;
+                        }
+                        s -= j * ((s - P + j - 1) / j);
+                        if (s == P) break;  // the not-less element at P
+                    }
                     if (s < S.L || s >= S.lim) {
                         bad = true;
                         break;
@@ -837,12 +889,92 @@ __device__ int lds_partition(const double* key, uint16_t* loc, uint16_t* lp, uin
     return lim;
 }
 
-// quickselect of src/spatial_sort.cpp:60-71 over local slots [l, r]
+// Position of the t-th (from 0) set bit of m; m has more than t set bits.
+__device__ inline int select_bit(uint64_t m, int t) {
+    int lo = 0, hi = 63;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (__popcll(m & ((2ull << mid) - 1ull)) >= t + 1) hi = mid;
+        else lo = mid + 1;
+    }
+    return lo;
+}
+
+// The rounds of a select on local slots [l, r] of at most 64 elements, in one
+// wavefront's registers: lane i holds slot l + i (its loc[] entry and key).
+// A round is the same Lomuto round as lds_partition, evaluated by pulls:
+// slot d < lim takes the (d - l)-th smaller element; slot lim the pivot;
+// a slot in (lim, r) keeps an element that is not less and otherwise takes
+// the end of the backward chain s <- l + rank(s) (found by pointer jumping:
+// the chain is shorter than 64 steps, so six doublings reach its end); slot
+// r takes the end of the chain from lim. Called by the 64 lanes of one
+// wavefront; loc[l .. r] is rewritten at the end.
+__device__ void wave_select(const double* key, uint16_t* loc, int l, int r, int nth, int up, uint32_t* err) {
+    const int lane = __lane_id();
+    const int m = r - l + 1;
+    if (m < 1 || m > 64 || nth < l || nth > r) {
+        if (lane == 0) raise_err(err, kErrLds);
+        return;
+    }
+    const bool in = lane < m;
+    uint32_t id = in ? loc[l + lane] : 0u;
+    double k = in ? key[id] : 0.0;
+    int a = 0, b = m - 1;
+    const int t = nth - l;
+    bool done = false;
+    for (int round = 0; round <= m && !done; ++round) {
+        const int pv = a + (b - a) / 2;
+        // the pivot to slot b
+        const uint32_t id_pv = __shfl(id, pv), id_b = __shfl(id, b);
+        const double k_pv = __shfl(k, pv), k_b = __shfl(k, b);
+        if (lane == b) {
+            id = id_pv;
+            k = k_pv;
+        } else if (lane == pv) {
+            id = id_b;
+            k = k_b;
+        }
+        const bool f = lane >= a && lane < b && key_less(k, k_pv, up);
+        const uint64_t mask = __ballot(f);
+        const int rank = __popcll(mask & ((1ull << lane) - 1ull));
+        const int lim = a + __popcll(mask);
+        int h = f ? a + rank : lane;  // backward chain; fixed at slots that are not less
+        for (int j = 0; j < 6; ++j) h = __shfl(h, h);
+        const int h_lim = __shfl(h, lim);
+        int src = lane;
+        if (lane >= a && lane <= b) {
+            if (lane < lim) src = select_bit(mask, lane - a);
+            else if (lane == lim) src = b;
+            else if (lane == b) src = h_lim;
+            else src = h;
+        }
+        id = __shfl(id, src);
+        k = __shfl(k, src);
+        if (lim == t) {
+            done = true;
+        } else {
+            if (t < lim) b = lim - 1;
+            else a = lim + 1;
+            if (a > b) break;
+        }
+    }
+    if (!done && lane == 0) raise_err(err, kErrLoop);
+    if (in) loc[l + lane] = (uint16_t)id;
+}
+
+// quickselect of src/spatial_sort.cpp:60-71 over local slots [l, r]; once the
+// range holds at most 64 elements, the first wavefront finishes it in
+// registers
 template <int NW>
 __device__ void lds_select(const double* key, uint16_t* loc, uint16_t* lp, uint16_t* tmp, uint32_t* s_cnt,
                            int l, int r, int nth, int up, uint32_t* err) {
     const int bound = r - l + 1;  // every round removes at least the pivot's slot
     for (int round = 0; round <= bound; ++round) {
+        if (r - l < 64) {
+            if (threadIdx.x < 64) wave_select(key, loc, l, r, nth, up, err);
+            __syncthreads();
+            return;
+        }
         const int pv = l + (r - l) / 2;
         const int pos = lds_partition<NW>(key, loc, lp, tmp, s_cnt, l, r, pv, up, err);
         if (pos == nth) return;
@@ -1295,7 +1427,7 @@ void sort_sets(const double* xyz, int64_t n_pts, const int64_t* poff, const int6
     RState* st[2] = {nullptr, nullptr};
     uint64_t* mask = nullptr;
     uint16_t* wloc = nullptr;
-    uint32_t *tcount = nullptr, *toff = nullptr, *ring = nullptr;
+    uint32_t *tcount = nullptr, *toff = nullptr, *ring = nullptr, *npos = nullptr;
     if (max_tasks) {
         st[0] = sc.get<RState>(max_tasks);
         st[1] = sc.get<RState>(max_tasks);
@@ -1303,6 +1435,7 @@ void sort_sets(const double* xyz, int64_t n_pts, const int64_t* poff, const int6
         tcount = sc.get<uint32_t>(max_tiles);
         toff = sc.get<uint32_t>(max_tiles);
         wloc = sc.get<uint16_t>(max_tiles * kTileWords);
+        npos = sc.get<uint32_t>(total);
         ring = sc.get<uint32_t>(kRing);
         fill_u32(ring, 0u, kRing, s);
     }
@@ -1329,8 +1462,11 @@ void sort_sets(const double* xyz, int64_t n_pts, const int64_t* poff, const int6
                 D3_LAUNCH_CHECK();
                 k_rd_scan<<<nt, 256, 0, s>>>(pt, nt, ntl, st[c], tcount, toff, err);
                 D3_LAUNCH_CHECK();
+                k_rd_npos<<<ntl * kSlices, kTileThreads, 0, s>>>(pt, nt, ptl, ntl, st[c], mask, wloc, toff, npos,
+                                                                 bufs.n);
+                D3_LAUNCH_CHECK();
                 k_rd_move<<<ntl * kSlices, kTileThreads, 0, s>>>(bufs, pt, nt, ptl, ntl, st[c], st[c ^ 1], mask, wloc,
-                                                                 toff, ring, round, err);
+                                                                 toff, npos, ring, round, err);
                 D3_LAUNCH_CHECK();
                 ++round;
                 if ((int64_t)round >= check_at) {
