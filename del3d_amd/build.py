@@ -15,7 +15,7 @@ HERE = Path(__file__).resolve().parent
 CSRC = HERE / "csrc"
 LIB = HERE / "libdel3d_amd.so"
 SOURCES = ["capi.hip", "predicates.hip", "predicates2.hip", "circumcenter.hip", "batch.hip", "server.hip",
-           "spatial_sort.hip", "host_scalar.cpp", "dropin.cpp"]
+           "spatial_sort.hip", "bridge.hip", "host_scalar.cpp", "dropin.cpp"]
 ARCH = os.environ.get("DEL3D_OFFLOAD_ARCH", "gfx950")
 
 COMMON = [

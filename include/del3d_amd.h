@@ -336,6 +336,84 @@ int del3d_predicates_mixed(const double* xyz, int64_t n_pts, const uint8_t* op,
                            const int32_t* idx, int64_t n_q, int8_t* out, void* stream);
 
 /*
+ * Conflict regions of an incremental Delaunay insertion, with the conflict
+ * tests batched per breadth-first wave: the driver bridge. Replaces the body
+ * of the reference driver's flood fill
+ *   Builder::find_conflicts(d, p, cells_out, facets_out)
+ *                              src/triangulation.cpp:643-689
+ * and its per-cell tests in_conflict / in_conflict_2 (:588-629). The region
+ * is found in waves: every untested neighbour of the cells found in conflict
+ * in the previous wave is tested, the tests of one wave over all regions of
+ * the call in one mixed predicate batch; then the reference's depth-first
+ * order is replayed over the answers, so the cells and facets come out
+ * exactly as the reference's fill lists them (its star rebuild depends on
+ * that order). The bridge reads the caller's cell tables in place and writes
+ * nothing to them: the caller sets the marks the rebuild expects (the
+ * region's cells InConflict; the neighbour across each listed facet
+ * OnBoundary).
+ *
+ * A view of one triangulation data structure (strided int32 tables, so the
+ * reference's Cell / Vertex arrays are read where they lie):
+ *   cell_v[c*cell_stride + k]  vertex k of cell c; cell_n[c*cell_stride + k]
+ *                              the neighbour opposite it (k < 4; dimension 2
+ *                              uses k < 3); cells in [0, n_cells)
+ *   vertex_point[u*vertex_stride] + point_base
+ *                              the point index of vertex u, in [0, n_pts) of
+ *                              the bridge's points; vertices in [0, n_vertices)
+ *   infinite                   the infinite vertex; dimension 2 or 3
+ * Many triangulations can share one bridge: concatenate their points and give
+ * each view its point_base.
+ *
+ *   del3d_bridge_create     over n_pts host points (3 doubles each, kept by
+ *                           pointer: the array must outlive the bridge;
+ *                           uploaded to the current device at the first GPU
+ *                           wave). A wave of fewer than gpu_min_queries tests
+ *                           is answered on the host with the kernels'
+ *                           per-query code (0 = every wave on the GPU;
+ *                           INT64_MAX = host only, which needs no device).
+ *   del3d_bridge_conflicts  n_regions regions at once: region r starts at
+ *                           start_cells[r] (the located cell, in conflict by
+ *                           construction, not tested) for the point index
+ *                           points[r]. Its cells go to
+ *                           cells_out[cells_off[r] .. cells_off[r+1]) and its
+ *                           facets, as (cell, local index) int32 pairs, to
+ *                           facets_out[2*facets_off[r] .. 2*facets_off[r+1]),
+ *                           both in the reference's order; cells_off and
+ *                           facets_off hold n_regions+1 entries. If a buffer
+ *                           is too small the call returns DEL3D_ERR_INVALID
+ *                           with the needed totals in cells_off[n_regions]
+ *                           and facets_off[n_regions]. A bad index in a view
+ *                           returns DEL3D_ERR_INVALID.
+ *   del3d_bridge_info       info[6]: waves, GPU batches, conflict tests,
+ *                           tests answered on the GPU, on the host, and
+ *                           follow-up tests (a hull orientation of 0 decided
+ *                           in the facet's plane).
+ *   del3d_bridge_destroy    frees the bridge.
+ * Calls on one bridge are serialized.
+ */
+typedef struct del3d_tds_view {
+    const int32_t* cell_v;
+    const int32_t* cell_n;
+    int64_t cell_stride;
+    int64_t n_cells;
+    const int32_t* vertex_point;
+    int64_t vertex_stride;
+    int64_t n_vertices;
+    int64_t point_base;
+    int32_t infinite;
+    int32_t dimension;
+} del3d_tds_view;
+typedef struct del3d_bridge_s* del3d_bridge;
+int del3d_bridge_create(const double* xyz, int64_t n_pts, int64_t gpu_min_queries,
+                        del3d_bridge* bridge_out);
+int del3d_bridge_conflicts(del3d_bridge bridge, int32_t n_regions, const del3d_tds_view* views,
+                           const int32_t* start_cells, const int32_t* points, int32_t* cells_out,
+                           int64_t cells_cap, int64_t* cells_off, int32_t* facets_out,
+                           int64_t facets_cap, int64_t* facets_off);
+int del3d_bridge_info(del3d_bridge bridge, int64_t* info);
+int del3d_bridge_destroy(del3d_bridge bridge);
+
+/*
  * Batched circumcentres of tetrahedra over device-resident data.
  *
  * Replaces, per cell c, one call of
@@ -366,9 +444,12 @@ int del3d_spatial_sort_host(const double* xyz, int64_t n_pts, int32_t* order, in
 
 /*
  * Scalar predicates for the C++ drop-in (see del3d_amd_dropin.hpp): one
- * query evaluated on the GPU, synchronously, on a per-thread stream. They
- * return the sign; on a device failure they print del3d_last_error() and
- * abort, since the reference's signatures have no error channel.
+ * query evaluated on the host with the kernels' own per-query code (the same
+ * filter, exact integer tiers and perturbation, compiled for the CPU with the
+ * same floating-point rules), since a device round trip would cost far more
+ * than the query. They return the sign; on an internal failure they print
+ * del3d_last_error() and abort, since the reference's signatures have no
+ * error channel.
  */
 int del3d_orient3d(const double* a, const double* b, const double* c, const double* d);
 int del3d_in_sphere(const double* a, const double* b, const double* c, const double* d,
@@ -390,7 +471,8 @@ int del3d_coplanar_in_circle(const double* p0, const double* p1, const double* p
 int del3d_coplanar_in_circle_unperturbed(const double* p0, const double* p1, const double* p2,
                                          const double* p);
 
-/* One circumcentre (lazy or exact), computed on the GPU; out: x, y, z. */
+/* One circumcentre (lazy or exact), computed on the host like the scalar
+ * predicates; out: x, y, z. */
 void del3d_circumcenter(const double* p0, const double* p1, const double* p2, const double* p3,
                         int lazy, double* out);
 

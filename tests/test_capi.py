@@ -98,3 +98,68 @@ def test_noalloc_refuses_sets_above_its_limit(lib):
 def test_version(lib):
     lib.del3d_version.restype = ctypes.c_char_p
     assert b"gfx950" in lib.del3d_version()
+
+
+class TdsView(ctypes.Structure):
+    _fields_ = [("cell_v", ctypes.c_void_p), ("cell_n", ctypes.c_void_p), ("cell_stride", ctypes.c_int64),
+                ("n_cells", ctypes.c_int64), ("vertex_point", ctypes.c_void_p),
+                ("vertex_stride", ctypes.c_int64), ("n_vertices", ctypes.c_int64),
+                ("point_base", ctypes.c_int64), ("infinite", ctypes.c_int32), ("dimension", ctypes.c_int32)]
+
+
+def test_bridge_argument_errors_without_device(lib):
+    """The bridge's host-only form needs no device; bad views and short
+    output buffers are errors, not reads outside the caller's tables."""
+    lib.del3d_last_error.restype = ctypes.c_char_p
+    vp, i64 = ctypes.c_void_p, ctypes.c_int64
+    lib.del3d_bridge_create.argtypes = [vp, i64, i64, ctypes.POINTER(vp)]
+    lib.del3d_bridge_conflicts.argtypes = [vp, ctypes.c_int32, ctypes.POINTER(TdsView), vp, vp, vp, i64, vp, vp,
+                                           i64, vp]
+    lib.del3d_bridge_info.argtypes = [vp, vp]
+    lib.del3d_bridge_destroy.argtypes = [vp]
+    br = vp()
+    assert lib.del3d_bridge_create(None, 5, 0, ctypes.byref(br)) == 1
+    pts = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1], [0.3, 0.3, 0.3], [0, 0, -5]], np.float64)
+    assert lib.del3d_bridge_create(pts.ctypes.data, 6, -1, ctypes.byref(br)) == 1
+    assert lib.del3d_bridge_create(pts.ctypes.data, 6, 2**63 - 1, ctypes.byref(br)) == 0
+    # two cells, each the other's neighbour across all four facets: cell 0
+    # over points 0..3 (the start), cell 1 over 0, 1, 2, 5, positively
+    # oriented, whose circumsphere (centre (0.5, 0.5, -2.5)) excludes point
+    # 4. The region of point 4 is cell 0 alone, with four facets, all
+    # towards cell 1.
+    lib.del3d_orient3d.argtypes = [vp] * 4
+    c1 = [0, 1, 2, 5]
+    if lib.del3d_orient3d(*[pts[k].ctypes.data for k in c1]) < 0:
+        c1 = [1, 0, 2, 5]
+    cv = np.array([0, 1, 2, 3] + c1, np.int32)
+    cn = np.array([1, 1, 1, 1, 0, 0, 0, 0], np.int32)
+    vpt = np.arange(6, dtype=np.int32)
+    view = TdsView(cv.ctypes.data, cn.ctypes.data, 4, 2, vpt.ctypes.data, 1, 6, 0, 99, 3)
+    starts = np.array([0], np.int32)
+    point = np.array([4], np.int32)
+    cells = np.zeros(8, np.int32)
+    facets = np.zeros(16, np.int32)
+    coff = np.zeros(2, np.int64)
+    foff = np.zeros(2, np.int64)
+    args = lambda v, ccap=8, fcap=8: (br, 1, ctypes.byref(v), starts.ctypes.data, point.ctypes.data,
+                                      cells.ctypes.data, ccap, coff.ctypes.data, facets.ctypes.data, fcap,
+                                      foff.ctypes.data)
+    assert lib.del3d_bridge_conflicts(*args(view)) == 0, lib.del3d_last_error()
+    assert list(coff) == [0, 1] and list(foff) == [0, 4] and cells[0] == 0
+    assert facets[:8].tolist() == [0, 0, 0, 1, 0, 2, 0, 3]
+    assert lib.del3d_bridge_conflicts(*args(view, fcap=2)) == 1      # short facet buffer
+    assert b"capacity" in lib.del3d_last_error() and foff[1] == 4
+    cn[0] = 7                                                          # neighbour outside the view
+    assert lib.del3d_bridge_conflicts(*args(view)) == 1
+    assert b"bad view" in lib.del3d_last_error()
+    cn[0] = 1
+    vpt[5] = 6                                                         # point outside the bridge's
+    assert lib.del3d_bridge_conflicts(*args(view)) == 1
+    assert b"bad view" in lib.del3d_last_error()
+    vpt[5] = 5
+    bad_dim = TdsView(cv.ctypes.data, cn.ctypes.data, 4, 2, vpt.ctypes.data, 1, 6, 0, 99, 4)
+    assert lib.del3d_bridge_conflicts(*args(bad_dim)) == 1
+    info = np.zeros(6, np.int64)
+    assert lib.del3d_bridge_info(br, info.ctypes.data) == 0
+    assert info[0] >= 1 and info[4] == info[2] and info[3] == 0       # all on the host
+    assert lib.del3d_bridge_destroy(br) == 0
