@@ -142,6 +142,14 @@ void release_stream_cache(hipStream_t s) {
     g_free.erase(it);
 }
 
+int64_t stream_cache_bytes(hipStream_t s) {
+    int dev = 0;
+    D3_HIP(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lock(g_cache_mu);
+    auto it = g_free.find(CacheKey{dev, s});
+    return it == g_free.end() ? 0 : (int64_t)it->second.bytes;
+}
+
 bool host_is_pinned(const void* p) {
     hipPointerAttribute_t a{};
     const hipError_t e = hipPointerGetAttributes(&a, p);

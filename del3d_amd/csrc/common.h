@@ -61,6 +61,8 @@ void scratch_give(void* p, size_t bytes, hipStream_t s);
 void release_cache_current_device();
 // Frees the cached scratch blocks of one stream (hipFree waits for the device).
 void release_stream_cache(hipStream_t s);
+// Bytes of scratch blocks cached for one stream of the current device.
+int64_t stream_cache_bytes(hipStream_t s);
 // Whether host memory p is page-locked (hipHostMalloc'd), so that the DMA
 // engines can copy it without staging.
 bool host_is_pinned(const void* p);

@@ -2,7 +2,7 @@
 //
 // Not part of the product library: built into del3d_amd/libpredcore_host.so
 // so the tests can run every tier of the GPU's arithmetic (filter, certified
-// small grid, 1-2 limb and any-width exact integers, perturbation walk) on
+// small grid, 1-4 limb and any-width exact integers, perturbation walk) on
 // the CPU and compare it with the oracle, with a switch that forces a chosen
 // tier on every query, which the GPU kernels only reach on rare inputs.
 #include <cstdint>
@@ -17,7 +17,7 @@ using namespace d3;
 namespace {
 
 // mode 0: the kernels' tier order; 1: skip T0/T1 (force exact integers,
-// 1-2 limbs where possible); 2: force the any-width tier.
+// 1-4 limbs in registers where possible); 2: force the any-width tier.
 int eval_one(uint8_t o, const P3 p_in[5], const int32_t id[5], int mode, int* tier_out) {
     P3 p[5];
     const int np = o == 0 ? 4 : 5;
@@ -28,6 +28,8 @@ int eval_one(uint8_t o, const P3 p_in[5], const int32_t id[5], int mode, int* ti
     auto exact_orient = [&](const P3* t) -> int {
         if (mode != 2 && L == 1) return orient_public_int<1>(t, g.q);
         if (mode != 2 && L == 2) return orient_public_int<2>(t, g.q);
+        if (mode != 2 && L == 3) return orient_public_int<3>(t, g.q);
+        if (mode != 2 && L == 4) return orient_public_int<4>(t, g.q);
         return -rt_orient_raw(t[0], t[1], t[2], t[3], g.q, L, ws.data());
     };
     if (o == 0) {
@@ -36,7 +38,7 @@ int eval_one(uint8_t o, const P3 p_in[5], const int32_t id[5], int mode, int* ti
             if (filter_decides(f, kOrientBound)) { *tier_out = 0; return -sgn(f.det); }
             if (t1_certify(p, 4, f.maxd, 15, -358)) { *tier_out = 1; return -sgn(f.det); }
         }
-        *tier_out = (mode != 2 && L <= 2) ? 2 : 3;
+        *tier_out = (mode != 2 && L <= 4) ? 2 : 3;
         return exact_orient(p);
     }
     int raw;
@@ -50,12 +52,9 @@ int eval_one(uint8_t o, const P3 p_in[5], const int32_t id[5], int mode, int* ti
         *tier_out = 1;
         cert = true;
         raw = -sgn(f.det);
-    } else if (mode != 2 && L <= 2) {
+    } else if (mode != 2 && L <= 4) {
         *tier_out = 2;
-        raw = L == 1 ? -int_insphere_raw<1>(to_lp3<1>(p[0], g.q), to_lp3<1>(p[1], g.q), to_lp3<1>(p[2], g.q),
-                                           to_lp3<1>(p[3], g.q), to_lp3<1>(p[4], g.q))
-                     : -int_insphere_raw<2>(to_lp3<2>(p[0], g.q), to_lp3<2>(p[1], g.q), to_lp3<2>(p[2], g.q),
-                                           to_lp3<2>(p[3], g.q), to_lp3<2>(p[4], g.q));
+        raw = -insphere_raw_regs(p, g.q, L);
     } else {
         *tier_out = 3;
         raw = -rt_insphere_raw(p[0], p[1], p[2], p[3], p[4], g.q, L, ws.data());

@@ -41,7 +41,7 @@ double* circumcenter_workspace() {
 
 // The tier order of the kernels' per-query path: the reference's filter
 // (src/predicates.cpp:125-201), then exact integers on the query's own
-// power-of-two grid (1-2 limbs unrolled, else any width), then for in_sphere
+// power-of-two grid (1-4 limbs unrolled, else any width), then for in_sphere
 // the symbolic perturbation (src/predicates.cpp:206-232), whose substituted
 // orient3d calls take the same route.
 int host_pred3(uint8_t op, const P3 pin[5], const int32_t id[5]) {
@@ -56,6 +56,8 @@ int host_pred3(uint8_t op, const P3 pin[5], const int32_t id[5]) {
         if (filter_decides(f, kOrientBound)) return -sgn(f.det);
         if (L == 1) return orient_public_int<1>(t, g.q);
         if (L == 2) return orient_public_int<2>(t, g.q);
+        if (L == 3) return orient_public_int<3>(t, g.q);
+        if (L == 4) return orient_public_int<4>(t, g.q);
         return -rt_orient_raw(t[0], t[1], t[2], t[3], g.q, L, ws);
     };
     if (op == DEL3D_OP_ORIENT3D) return orient_pub(p);
@@ -63,12 +65,8 @@ int host_pred3(uint8_t op, const P3 pin[5], const int32_t id[5]) {
     int s;
     if (filter_decides(f, kInSphereBound)) {
         s = -sgn(f.det);
-    } else if (L == 1) {
-        s = -int_insphere_raw<1>(to_lp3<1>(p[0], g.q), to_lp3<1>(p[1], g.q), to_lp3<1>(p[2], g.q),
-                                 to_lp3<1>(p[3], g.q), to_lp3<1>(p[4], g.q));
-    } else if (L == 2) {
-        s = -int_insphere_raw<2>(to_lp3<2>(p[0], g.q), to_lp3<2>(p[1], g.q), to_lp3<2>(p[2], g.q),
-                                 to_lp3<2>(p[3], g.q), to_lp3<2>(p[4], g.q));
+    } else if (L <= 4) {
+        s = -insphere_raw_regs(p, g.q, L);
     } else {
         s = -rt_insphere_raw(p[0], p[1], p[2], p[3], p[4], g.q, L, ws);
     }

@@ -666,6 +666,25 @@ D3_HD int int_insphere_raw(const LP3<L>& a, const LP3<L>& b, const LP3<L>& c, co
     return li_sign(det);
 }
 
+// The raw in_sphere sign in registers for 1-4 limbs per difference (the caller
+// checks L <= 4).
+D3_HD int insphere_raw_regs(const P3 p[5], int q, int L) {
+    switch (L) {
+    case 1:
+        return int_insphere_raw<1>(to_lp3<1>(p[0], q), to_lp3<1>(p[1], q), to_lp3<1>(p[2], q), to_lp3<1>(p[3], q),
+                                   to_lp3<1>(p[4], q));
+    case 2:
+        return int_insphere_raw<2>(to_lp3<2>(p[0], q), to_lp3<2>(p[1], q), to_lp3<2>(p[2], q), to_lp3<2>(p[3], q),
+                                   to_lp3<2>(p[4], q));
+    case 3:
+        return int_insphere_raw<3>(to_lp3<3>(p[0], q), to_lp3<3>(p[1], q), to_lp3<3>(p[2], q), to_lp3<3>(p[3], q),
+                                   to_lp3<3>(p[4], q));
+    default:
+        return int_insphere_raw<4>(to_lp3<4>(p[0], q), to_lp3<4>(p[1], q), to_lp3<4>(p[2], q), to_lp3<4>(p[3], q),
+                                   to_lp3<4>(p[4], q));
+    }
+}
+
 // ---- runtime-width exact arithmetic (T3), on a caller buffer ----------------
 // Values are two's complement arrays of n limbs. The buffer is a bump arena.
 

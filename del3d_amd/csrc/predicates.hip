@@ -24,8 +24,9 @@
 //  pred_exact         persistent grid over the queue: exact integer arithmetic
 //                     with 1-2 limbs per difference (T2), perturbation
 //                     included; wider exponent spans go to a second queue.
+//  pred_exact_mid     the wide queue's 3-4 limb queries, in registers.
 //  pred_wide          one query per workgroup, exact integers of any width in
-//                     LDS (T3).
+//                     LDS, the products of each level on separate lanes (T3).
 //
 // Why LDS for small sets: a query gathers 4-5 random points of 24 bytes, two
 // L2 requests each (a 16- and an 8-byte load), so with points in L2 the filter
@@ -460,7 +461,7 @@ void pred_batch_global_packed(BatchArgs a) {
 }
 
 // ---------------------------------------------------------------------------
-// Exact path, registers (1..2 limbs)
+// Exact path, registers (1..4 limbs)
 // ---------------------------------------------------------------------------
 template <int L>
 __device__ int exact_query(uint8_t o, const P3 p[5], const int32_t id[5], int q, bool& perturbed) {
@@ -536,14 +537,189 @@ __global__ __launch_bounds__(256) void pred_exact(BatchArgs a) {
     }
 }
 
-// ---------------------------------------------------------------------------
-// Exact path, any width, LDS (one query per workgroup, lane 0 computes)
-// ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kWideBlock) void pred_wide(BatchArgs a) {
-    __shared__ uint32_t ws[64 * kWideLimbsMax];
+// The wide queue's queries of 3 or 4 limbs per difference (exponent spans up
+// to 2^119: random doubles whose coordinates differ much in magnitude), in
+// registers like pred_exact; its own kernel, so that pred_exact keeps the
+// occupancy of the 1-2 limb forms. Wider queries are left to pred_wide.
+__global__ __launch_bounds__(256) void pred_exact_mid(BatchArgs a) {
     const uint32_t n = a.ctr->wide;
-    if (threadIdx.x != 0) return;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t i0 = blockIdx.x * blockDim.x + (threadIdx.x & ~63u); i0 < n; i0 += stride) {
+        const uint32_t i = i0 + __lane_id();
+        bool perturbed = false, done = false;
+        if (i < n) {
+            const uint32_t q = a.wide[i];
+            const uint8_t o = a.op[q];
+            const double* pts = points_of_query(a, q);
+            int32_t id[5];
+#pragma unroll
+            for (int k = 0; k < 5; ++k) id[k] = a.idx[(size_t)q * 5 + k];
+            P3 p[5];
+            const int np = (o == DEL3D_OP_ORIENT3D) ? 4 : 5;
+            for (int k = 0; k < np; ++k) p[k] = load_p3(pts, id[k]);
+            if (np == 4) p[4] = p[3];
+            const Grid g = grid_of(p, np);
+            const int L = limbs_for(g);
+            int s = 0;
+            if (L == 3) {
+                s = exact_query<3>(o, p, id, g.q, perturbed);
+                done = true;
+            } else if (L == 4) {
+                s = exact_query<4>(o, p, id, g.q, perturbed);
+                done = true;
+            }
+            if (done) a.out[q] = (int8_t)s;
+        }
+        if (a.stats) {
+            wave_count(done, &a.stats[2]);
+            wave_count(perturbed, &a.stats[4]);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Exact path, any width, LDS: one query per workgroup, its products spread
+// over the lanes
+// ---------------------------------------------------------------------------
+// The determinants' integer products form a short DAG (differences -> 2x2
+// minors -> 3x3 cofactors and lifts -> the 4x4 expansion), and the products of
+// one level are independent: each runs serially on its own lane, in its own
+// LDS slots, with a barrier between levels. The critical path is then one
+// product per level (about 9 L^2 limb multiplies for in_sphere, against 72 L^2
+// for the whole determinant on one lane). Every lane of the workgroup runs the
+// same control flow (the query, its limb count and the perturbation walk are
+// uniform), so the barriers inside are reached by all of them.
+
+// LDS limbs of one workgroup's workspace, per limb of L.
+constexpr int kCoopPerLimb = 3 + 12 + 12 + 12 + 8 + 24 * 3 + 24 * 3;  // 191
+
+struct CoopWs {
+    uint32_t* base;
+    int L;
+    __device__ uint32_t* ev() const { return base; }                       // 3L: the base point
+    __device__ uint32_t* diff(int i, int k) const { return base + (3 + 3 * i + k) * L; }  // 12L
+    __device__ uint32_t* minor(int k) const { return base + (15 + 2 * k) * L; }          // 6 x 2L
+    __device__ uint32_t* cof(int k) const { return base + (27 + 3 * k) * L; }            // 4 x 3L
+    __device__ uint32_t* lift(int k) const { return base + (39 + 2 * k) * L; }           // 4 x 2L
+    __device__ uint32_t* slot(int k) const { return base + (47 + 3 * k) * L; }           // 24 x 3L
+    __device__ uint32_t* tmp(int k) const { return base + (47 + 72 + 3 * k) * L; }       // 24 x 3L
+    // the four 5L products of the last level, and their temporaries, reuse
+    // the slot and temporary areas: 4 x 5L <= 72L each
+    __device__ uint32_t* wide_slot(int k) const { return base + (47 + 5 * k) * L; }
+    __device__ uint32_t* wide_tmp(int k) const { return base + (47 + 72 + 5 * k) * L; }
+};
+
+// Sign of |a-d, b-d, c-d| (the computed orient3d form), cooperatively.
+__device__ int coop_orient_raw(const P3& a, const P3& b, const P3& c, const P3& d, int q, const CoopWs& W,
+                               int* s_sign) {
+    const int t = threadIdx.x, L = W.L;
+    const P3* pt[3] = {&a, &b, &c};
+    if (t < 3) rt_to_limbs(t == 0 ? d.x : t == 1 ? d.y : d.z, q, W.ev() + t * L, L);
+    __syncthreads();
+    if (t < 9) {
+        const int i = t / 3, k = t % 3;
+        uint32_t* r = W.diff(i, k);
+        rt_to_limbs(k == 0 ? pt[i]->x : k == 1 ? pt[i]->y : pt[i]->z, q, r, L);
+        rt_addsub(r, r, W.ev() + k * L, L, 1);
+    }
+    __syncthreads();
+    // minors m_k = X(i) Y(j) - X(j) Y(i) for (i, j) = (b, c), (c, a), (a, b)
+    const int mi[3][2] = {{1, 2}, {2, 0}, {0, 1}};
+    if (t < 6) {
+        const int k = t >> 1, second = t & 1;
+        const int i = mi[k][second], j = mi[k][1 - second];
+        rt_mul(W.slot(t), W.diff(i, 0), L, W.diff(j, 1), L, W.tmp(t));
+    }
+    __syncthreads();
+    if (t < 3) rt_addsub(W.minor(t), W.slot(2 * t), W.slot(2 * t + 1), 2 * L, 1);
+    __syncthreads();
+    if (t < 3) rt_mul(W.slot(t), W.diff(t, 2), L, W.minor(t), 2 * L, W.tmp(t));
+    __syncthreads();
+    if (t == 0) {
+        uint32_t* det = W.slot(0);
+        rt_addsub(det, det, W.slot(1), 3 * L, 0);
+        rt_addsub(det, det, W.slot(2), 3 * L, 0);
+        *s_sign = rt_sign(det, 3 * L);
+    }
+    __syncthreads();
+    const int sign = *s_sign;
+    __syncthreads();  // s_sign is reused by the next call
+    return sign;
+}
+
+// Sign of the lifted determinant relative to e (the computed in_sphere form),
+// cooperatively; the same products as rt_insphere_raw (src/predicates.cpp:
+// 165-176's cofactors and lifts).
+__device__ int coop_insphere_raw(const P3& a, const P3& b, const P3& c, const P3& d, const P3& e, int q,
+                                 const CoopWs& W, int* s_sign) {
+    const int t = threadIdx.x, L = W.L;
+    const P3* pt[4] = {&a, &b, &c, &d};
+    if (t < 3) rt_to_limbs(t == 0 ? e.x : t == 1 ? e.y : e.z, q, W.ev() + t * L, L);
+    __syncthreads();
+    if (t < 12) {
+        const int i = t / 3, k = t % 3;
+        uint32_t* r = W.diff(i, k);
+        rt_to_limbs(k == 0 ? pt[i]->x : k == 1 ? pt[i]->y : pt[i]->z, q, r, L);
+        rt_addsub(r, r, W.ev() + k * L, L, 1);
+    }
+    __syncthreads();
+    // minors ab bc cd da ac bd: X(i) Y(j) - X(j) Y(i)
+    const int mp[6][2] = {{0, 1}, {1, 2}, {2, 3}, {3, 0}, {0, 2}, {1, 3}};
+    if (t < 12) {
+        const int k = t >> 1, second = t & 1;
+        const int i = mp[k][second], j = mp[k][1 - second];
+        rt_mul(W.slot(t), W.diff(i, 0), L, W.diff(j, 1), L, W.tmp(t));
+    }
+    __syncthreads();
+    if (t < 6) rt_addsub(W.minor(t), W.slot(2 * t), W.slot(2 * t + 1), 2 * L, 1);
+    __syncthreads();
+    // cofactors abc bcd cda dab: three terms z(point) * minor each; lifts:
+    // x^2 + y^2 + z^2 of each difference
+    const int cz_[4][3] = {{0, 1, 2}, {1, 2, 3}, {2, 3, 0}, {3, 0, 1}};
+    const int cm_[4][3] = {{1, 4, 0}, {2, 5, 1}, {3, 4, 2}, {0, 5, 3}};
+    const int cs_[4][3] = {{0, 1, 0}, {0, 1, 0}, {0, 0, 0}, {0, 0, 0}};  // 1 = subtract
+    if (t < 12) {
+        const int k = t / 3, j = t % 3;
+        rt_mul(W.slot(t), W.diff(cz_[k][j], 2), L, W.minor(cm_[k][j]), 2 * L, W.tmp(t));
+    } else if (t < 24) {
+        const int k = (t - 12) / 3, j = (t - 12) % 3;
+        rt_mul(W.slot(t), W.diff(k, j), L, W.diff(k, j), L, W.tmp(t));
+    }
+    __syncthreads();
+    if (t < 4) {
+        uint32_t* ck = W.cof(t);
+        for (int i = 0; i < 3 * L; ++i) ck[i] = 0;
+        for (int j = 0; j < 3; ++j) rt_addsub(ck, ck, W.slot(3 * t + j), 3 * L, cs_[t][j]);
+    } else if (t < 8) {
+        const int k = t - 4;
+        uint32_t* lk = W.lift(k);
+        for (int i = 0; i < 2 * L; ++i) lk[i] = 0;
+        for (int j = 0; j < 3; ++j) rt_addsub(lk, lk, W.slot(12 + 3 * k + j), 2 * L, 0);
+    }
+    __syncthreads();
+    // det = (dlift*abc - clift*dab) + (blift*cda - alift*bcd)
+    const int dl[4] = {3, 2, 1, 0}, dc[4] = {0, 3, 2, 1};
+    if (t < 4) rt_mul(W.wide_slot(t), W.lift(dl[t]), 2 * L, W.cof(dc[t]), 3 * L, W.wide_tmp(t));
+    __syncthreads();
+    if (t == 0) {
+        uint32_t* det = W.wide_slot(0);
+        rt_addsub(det, det, W.wide_slot(1), 5 * L, 1);
+        rt_addsub(det, det, W.wide_slot(2), 5 * L, 0);
+        rt_addsub(det, det, W.wide_slot(3), 5 * L, 1);
+        *s_sign = rt_sign(det, 5 * L);
+    }
+    __syncthreads();
+    const int sign = *s_sign;
+    __syncthreads();
+    return sign;
+}
+
+__global__ __launch_bounds__(kWideBlock) void pred_wide(BatchArgs a) {
+    __shared__ uint32_t ws[kCoopPerLimb * kWideLimbsMax];
+    __shared__ int s_sign;
+    const uint32_t n = a.ctr->wide;
     for (uint32_t i = blockIdx.x; i < n; i += gridDim.x) {
+        // every lane reads the same query: the control flow below is uniform
         const uint32_t q = a.wide[i];
         const uint8_t o = a.op[q];
         const double* pts = points_of_query(a, q);
@@ -555,24 +731,28 @@ __global__ __launch_bounds__(kWideBlock) void pred_wide(BatchArgs a) {
         if (np == 4) p[4] = p[3];
         const Grid g = grid_of(p, np);
         const int L = limbs_for(g);
+        if (L <= 4) continue;  // decided by pred_exact_mid
+        const CoopWs W{ws, L};
         int s;
         if (o == DEL3D_OP_ORIENT3D) {
-            s = -rt_orient_raw(p[0], p[1], p[2], p[3], g.q, L, ws);
+            s = -coop_orient_raw(p[0], p[1], p[2], p[3], g.q, W, &s_sign);
         } else {
-            s = -rt_insphere_raw(p[0], p[1], p[2], p[3], p[4], g.q, L, ws);
+            s = -coop_insphere_raw(p[0], p[1], p[2], p[3], p[4], g.q, W, &s_sign);
             if (s == 0 && o == DEL3D_OP_IN_SPHERE) {
-                if (a.stats) atomicAdd(&a.stats[4], 1ull);
+                if (a.stats && threadIdx.x == 0) atomicAdd(&a.stats[4], 1ull);
                 s = perturbation_walk(p, id, [&](int k) {
                     P3 t[4];
                     subst_tet(p, k, t);
                     const Filt f = orient_filter(t[0], t[1], t[2], t[3]);
                     if (filter_decides(f, kOrientBound)) return -sgn(f.det);
-                    return -rt_orient_raw(t[0], t[1], t[2], t[3], g.q, L, ws);
+                    return -coop_orient_raw(t[0], t[1], t[2], t[3], g.q, W, &s_sign);
                 });
             }
         }
-        a.out[q] = (int8_t)s;
-        if (a.stats) atomicAdd(&a.stats[3], 1ull);
+        if (threadIdx.x == 0) {
+            a.out[q] = (int8_t)s;
+            if (a.stats) atomicAdd(&a.stats[3], 1ull);
+        }
     }
 }
 
@@ -793,6 +973,8 @@ void enqueue_batch(const BatchArgs& base, const BatchShape& sh, hipStream_t s, h
         D3_HIP(hipEventRecord(ev[2], s));
     }
     pred_exact<<<cus * 2, 256, 0, s>>>(base);
+    D3_LAUNCH_CHECK();
+    pred_exact_mid<<<cus, 256, 0, s>>>(base);
     D3_LAUNCH_CHECK();
     if (ev) {
         D3_HIP(hipEventRecord(ev[3], s));

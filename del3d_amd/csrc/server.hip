@@ -102,6 +102,10 @@ struct Slot {
     int64_t cap_q = 0, cap_jobs = 0, cap_tasks = 0, cap_order = 0;
     int64_t bytes = 0;       // device bytes held by the arenas
     int64_t sort_bytes = 0;  // budget reserved for a running sort's scratch
+    // scratch blocks the slot's stream keeps cached after its sorts (the next
+    // sort reuses them without hipMalloc); counted in the budget, freed when
+    // memory is short and the slot is idle
+    int64_t cache_bytes = 0;
     bool busy = false;
     int64_t q_run = 0, n_tasks = 0, n_lds = 0;  // the batch in flight
     std::vector<std::shared_ptr<Job>> jobs;
@@ -136,6 +140,10 @@ struct del3d_server_s {
 
 namespace d3 {
 namespace {
+
+constexpr int64_t kTaskQ = 16 * kGlobalThreads;  // queries per task (both kernels)
+
+int64_t tasks_for(int64_t n_q) { return (n_q + kTaskQ - 1) / kTaskQ; }
 
 // Device scratch budgeted for a spatial sort of n points: two record
 // buffers (56 B per point), the shuffle tables (about 32 B per point) and the
@@ -184,6 +192,15 @@ int64_t slot_bytes(int64_t q, int64_t j, int64_t t, int64_t o) {
     return q * (1 + 20 + 1 + 4 + 4) + (int64_t)sizeof(PredCounters) + (int64_t)table_bytes(j, t) + 4 * o;
 }
 
+// Frees the scratch blocks an idle slot's stream keeps cached (hipFree waits
+// for the device) and returns their bytes. Caller holds sv->mu.
+void slot_release_cache(del3d_server_s* sv, Slot& sl) {
+    if (sl.cache_bytes == 0) return;
+    release_stream_cache(sl.s);
+    sv->used -= sl.cache_bytes;
+    sl.cache_bytes = 0;
+}
+
 // Evicts resident sets no running job or upload uses, least recently used
 // first, then frees the arenas of idle slots (a slot being filled
 // reallocates its own), until `need` more bytes fit the budget. Returns
@@ -202,9 +219,10 @@ bool make_room(del3d_server_s* sv, int64_t need) {
         if (victim < 0) {
             Slot* idle = nullptr;
             for (auto& sl : sv->slots)
-                if (!sl->busy && sl->bytes > 0) idle = sl.get();
+                if (!sl->busy && (sl->bytes > 0 || sl->cache_bytes > 0)) idle = sl.get();
             if (!idle) return false;
             slot_release(sv, *idle);
+            slot_release_cache(sv, *idle);
             continue;
         }
         ResidentSet& r = sv->sets[victim];
@@ -249,10 +267,17 @@ void upload_set(const double* host, double* dev, int64_t n, hipStream_t s) {
 // of o entries. Caller holds sv->mu.
 void slot_reserve(del3d_server_s* sv, Slot& sl, int64_t q, int64_t j, int64_t t, int64_t o) {
     if (q <= sl.cap_q && j <= sl.cap_jobs && t <= sl.cap_tasks && o <= sl.cap_order) return;
-    q = std::max(q, sl.cap_q);
-    j = std::max(j, sl.cap_jobs);
-    t = std::max(t, sl.cap_tasks);
-    o = std::max(o, sl.cap_order);
+    // Reallocating frees the old arenas, and hipFree waits for the whole
+    // device, stalling the other slots: grow each capacity that must grow to
+    // at least twice its old value, so a slot reallocates a few times only.
+    auto grow = [](int64_t want, int64_t cap, int64_t limit) {
+        if (want <= cap) return cap;
+        return std::max(want, std::min(2 * cap, limit));
+    };
+    q = grow(q, sl.cap_q, std::max(q, sv->max_batch_q));
+    j = grow(j, sl.cap_jobs, std::max<int64_t>(j, 4096));
+    t = grow(t, sl.cap_tasks, std::max(t, tasks_for(sv->max_batch_q) + 4096));
+    o = grow(o, sl.cap_order, std::max<int64_t>(o, int64_t(1) << 31));
     const int64_t want = slot_bytes(q, j, t, o);
     slot_release(sv, sl);
     if (!make_room(sv, want))
@@ -307,9 +332,6 @@ uint8_t* slot_stage(Slot& sl, int64_t bytes) {
     return sl.h_stage;
 }
 
-constexpr int64_t kTaskQ = 16 * kGlobalThreads;  // queries per task (both kernels)
-
-int64_t tasks_for(int64_t n_q) { return (n_q + kTaskQ - 1) / kTaskQ; }
 
 void finish_job(del3d_server_s* sv, const std::shared_ptr<Job>& j, int rc, const char* err) {
     j->t_done = Clock::now();
@@ -412,9 +434,8 @@ void run_sort_job(Slot* sl, const double* dev_pts, int64_t n) {
     } else {
         copy_d2h(j->order, sl->d_order, (size_t)n * 4, s);
     }
-    // the sort's scratch is cached on this stream: give it back (its bytes
-    // were reserved in the budget for this job only)
-    release_stream_cache(s);
+    // the sort's scratch stays cached on this stream for the slot's next
+    // sort; slot_run moves its bytes from the job's reservation to the slot
 }
 
 // Runs a slot's batch: uploads the sets it needs (outside the lock), then
@@ -457,7 +478,17 @@ void slot_run(del3d_server_s* sv, Slot* sl) {
     });
     std::string err = rc ? last_error_slot() : std::string();
     if (rc) (void)hipStreamSynchronize(sl->s);  // let whatever was enqueued drain
+    const bool was_sort = sl->jobs[0]->kind == kJobSort;
+    int64_t cached = 0;
+    if (was_sort) (void)guarded([&] {
+        cached = stream_cache_bytes(sl->s);
+        return DEL3D_OK;
+    });
     std::lock_guard<std::mutex> lock(sv->mu);
+    if (was_sort) {  // the job's reservation ends; the blocks its stream keeps stay counted
+        sv->used += cached - sl->cache_bytes;
+        sl->cache_bytes = cached;
+    }
     if (rc) {
         // a failed upload leaves the set unusable: drop its device copy
         for (int64_t id : sl->uploads) {
@@ -577,7 +608,9 @@ bool launch_batch(del3d_server_s* sv, Slot& sl, std::unique_lock<std::mutex>& lo
         if (sort) {
             // the scratch first: make_room may free idle slots' arenas, this
             // one's included, which slot_reserve then allocates
-            const int64_t need = sort_scratch_bytes(n_sort);
+            // the blocks the slot's stream already caches serve this sort
+            // too: reserve only what may come on top of them
+            const int64_t need = std::max<int64_t>(0, sort_scratch_bytes(n_sort) - sl.cache_bytes);
             if (!make_room(sv, need))
                 throw HipError{set_error(DEL3D_ERR_NOMEM, "a sort of %lld points needs %lld bytes of scratch "
                                          "beyond the HBM budget", (long long)n_sort, (long long)need)};
@@ -902,6 +935,10 @@ int del3d_server_destroy(del3d_server sv) {
     for (auto& s : sv->slots) {
         if (s->worker.joinable()) s->worker.join();
         slot_release(sv, *s);
+        (void)guarded([&] {
+            slot_release_cache(sv, *s);
+            return DEL3D_OK;
+        });
         (void)hipEventDestroy(s->done);
         (void)hipStreamDestroy(s->s);
     }

@@ -107,8 +107,8 @@ int del3d_predicates(const double* xyz, int64_t n_pts, const uint8_t* op, const 
  * stream, how the queries were decided (host array of 6 int64):
  *   [0] by the FP64 semi-static filter
  *   [1] by the certified small-integer re-evaluation in registers
- *   [2] by exact integer arithmetic in registers (1-2 limbs per difference)
- *   [3] by exact integer arithmetic in LDS (wide exponent spans)
+ *   [2] by exact integer arithmetic in registers (1-4 limbs per difference)
+ *   [3] by exact integer arithmetic in LDS (exponent spans above 2^119)
  *   [4] in_sphere queries that needed the symbolic perturbation
  *   [5] queries with a point index outside the point set or an unknown op
  *       code (sign 0 written; the reference's behaviour for such a pointer is

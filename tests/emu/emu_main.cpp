@@ -425,6 +425,17 @@ int main(int argc, char** argv) {
                 wide.push_back(std::ldexp(m + 0.5, (int)(rnd() % 400) - 200));
             }
         fails += check_predicates("wide", wide, 1500);
+        // spans of 2^60 .. 2^119: 3-4 limbs per difference (the register
+        // kernel for the wide queue), with near-degenerate queries from
+        // near-coplanar and near-cospherical points
+        std::vector<double> mid;
+        for (int i = 0; i < 80; ++i) {
+            const double u = (double)(rnd() % 1000003) / 1000003.0, v = (double)(rnd() % 999983) / 999983.0;
+            const double s = std::ldexp(1.0, (int)(rnd() % 60) - 40);
+            if (i % 2) mid.insert(mid.end(), {u * s, v * s, (u + v) * s});            // on a plane
+            else mid.insert(mid.end(), {std::cos(6.3 * u) * s, std::sin(6.3 * u) * s, v * 1e-3});
+        }
+        fails += check_predicates("mid", mid, 3000);
         std::vector<double> q48;
         for (int i = 0; i < 200; ++i)
             for (int k = 0; k < 3; ++k) q48.push_back((double)(rnd() >> 16) * 3.552713678800501e-15);
