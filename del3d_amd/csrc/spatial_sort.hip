@@ -1068,22 +1068,84 @@ __global__ __launch_bounds__(256) void k_lds_select(Bufs bufs, const FTask* __re
 // ---------------------------------------------------------------------------
 constexpr int kSub = 1024;
 constexpr int kStack = 64;
+constexpr int kSmall = 64;       // nodes of at most this many elements: one lane each
+constexpr int kSmallCap = 128;   // > kSub / 9: small nodes have more than 8 elements, disjoint
 
 struct Leaf {
     int64_t lo, hi;  // [lo, hi)
     int32_t x, ux, uy, uz;
 };
 
+// hilbert_split of src/spatial_sort.cpp:74-79 on local slots [b, e), by one
+// lane alone: the quickselect of :60-71 with the Lomuto partition of :40-51,
+// step for step, on loc[] with the keys read through it.
+__device__ int lane_split(const double* key, uint16_t* loc, int b, int e, int up, uint32_t* err) {
+    if (b >= e) return b;
+    const int mid = b + (e - b) / 2;
+    int l = b, r = e - 1;
+    if (l == r) return mid;
+    for (int guard = 0; guard <= e - b; ++guard) {
+        const int pv = l + (r - l) / 2;
+        uint16_t t = loc[pv];
+        loc[pv] = loc[r];
+        loc[r] = t;
+        const double pk = key[loc[r]];
+        int res = l;
+        for (int i = l; i < r; ++i) {
+            const uint16_t li = loc[i];
+            if (key_less(key[li], pk, up)) {
+                loc[i] = loc[res];
+                loc[res] = li;
+                ++res;
+            }
+        }
+        t = loc[r];
+        loc[r] = loc[res];
+        loc[res] = t;
+        if (res == mid) return mid;
+        if (mid < res) r = res - 1;
+        else l = res + 1;
+    }
+    raise_err(err, kErrLoop);
+    return mid;
+}
+
+// A node of the recursion (src/spatial_sort.cpp:95-117) in one word: slots
+// [b, e) (11 bits each), the split axis x and the three direction flags.
+__device__ inline uint32_t pack_node(int b, int e, int x, int ux, int uy, int uz) {
+    return (uint32_t)b | ((uint32_t)e << 11) | ((uint32_t)x << 22) | ((uint32_t)(ux != 0) << 24) |
+           ((uint32_t)(uy != 0) << 25) | ((uint32_t)(uz != 0) << 26);
+}
+
+// The eight children of a node split at m0..m8 (src/spatial_sort.cpp:109-116).
+__device__ inline void child_nodes(int ch[8][6], const int m[9], int x, int y, int z, int ux, int uy, int uz) {
+    const int c[8][6] = {{m[0], m[1], z, uz, ux, uy},   {m[1], m[2], y, uy, uz, ux},
+                         {m[2], m[3], y, uy, uz, ux},   {m[3], m[4], x, ux, !uy, !uz},
+                         {m[4], m[5], x, ux, !uy, !uz}, {m[5], m[6], y, !uy, uz, !ux},
+                         {m[6], m[7], y, !uy, uz, !ux}, {m[7], m[8], z, !uz, !ux, uy}};
+    for (int i = 0; i < 8; ++i)
+        for (int f = 0; f < 6; ++f) ch[i][f] = c[i][f];
+}
+
+// Nodes above kSmall elements are split by the whole wavefront (lds_split);
+// their children of more than 8 and at most kSmall elements are set aside,
+// and once the stack is empty each lane takes its share of them and runs the
+// reference's sequential splits on its own node (disjoint slot ranges, so the
+// lanes never touch the same slots, and the order in which nodes are
+// processed does not matter). A node of at most 64 elements has children of
+// at most 8 (64 -> 32 -> 16 -> 8), so its seven splits finish it.
 __global__ __launch_bounds__(64) void k_subtree(Bufs bufs, const Leaf* __restrict__ leaves, uint32_t nleaves,
                                                 int32_t* __restrict__ order, uint32_t* __restrict__ err) {
     __shared__ double s_key[3][kSub];
     __shared__ uint16_t s_loc[kSub], s_lp[kSub], s_tmp[kSub];
     __shared__ int32_t s_stack[kStack][6];
+    __shared__ uint32_t s_small[kSmallCap];
     __shared__ uint32_t s_cnt[1];
+    __shared__ int s_nsmall;
     for (uint32_t li = blockIdx.x; li < nleaves; li += gridDim.x) {
         const Leaf nd = leaves[li];
         const int64_t m = nd.hi - nd.lo;
-        if (m < 1 || m > kSub || nd.lo < 0 || nd.hi > bufs.n) {
+        if (m < 1 || m > kSub || nd.lo < 0 || nd.hi > bufs.n || nd.x < 0 || nd.x > 2) {
             if (threadIdx.x == 0) raise_err(err, kErrLds);
             continue;
         }
@@ -1095,39 +1157,47 @@ __global__ __launch_bounds__(64) void k_subtree(Bufs bufs, const Leaf* __restric
             s_key[2][i] = bufs.z[0][nd.lo + i];
             s_loc[i] = (uint16_t)i;
         }
+        int sp = 0;  // the same in every lane
         if (threadIdx.x == 0) {
-            s_stack[0][0] = 0;
-            s_stack[0][1] = mi;
-            s_stack[0][2] = nd.x;
-            s_stack[0][3] = nd.ux;
-            s_stack[0][4] = nd.uy;
-            s_stack[0][5] = nd.uz;
+            s_nsmall = 0;
+            if (mi > kSmall) {
+                s_stack[0][0] = 0;
+                s_stack[0][1] = mi;
+                s_stack[0][2] = nd.x;
+                s_stack[0][3] = nd.ux;
+                s_stack[0][4] = nd.uy;
+                s_stack[0][5] = nd.uz;
+            } else if (mi > 8) {
+                s_small[0] = pack_node(0, mi, nd.x, nd.ux, nd.uy, nd.uz);
+                s_nsmall = 1;
+            }
         }
+        if (mi > kSmall) sp = 1;
         __syncthreads();
-        int sp = 1;  // the same in every lane
         for (int guard = 0; sp > 0 && guard <= mi; ++guard) {  // at most one pop per node
             --sp;
             const int b = s_stack[sp][0], e = s_stack[sp][1];
             const int x = s_stack[sp][2], ux = s_stack[sp][3], uy = s_stack[sp][4], uz = s_stack[sp][5];
             __syncthreads();
-            if (e - b <= 8 || b < 0 || e > mi || x < 0 || x > 2) continue;
+            if (e - b <= kSmall || b < 0 || e > mi || x < 0 || x > 2) {
+                if (threadIdx.x == 0) raise_err(err, kErrLds);  // only nodes above kSmall are stacked
+                continue;
+            }
             const int y = (x + 1) % 3, z = (x + 2) % 3;
-            const int m0 = b, m8 = e;
-            const int m4 = lds_split<1>(s_key[x], s_loc, s_lp, s_tmp, s_cnt, m0, m8, ux, err);
-            const int m2 = lds_split<1>(s_key[y], s_loc, s_lp, s_tmp, s_cnt, m0, m4, uy, err);
-            const int m1 = lds_split<1>(s_key[z], s_loc, s_lp, s_tmp, s_cnt, m0, m2, uz, err);
-            const int m3 = lds_split<1>(s_key[z], s_loc, s_lp, s_tmp, s_cnt, m2, m4, !uz, err);
-            const int m6 = lds_split<1>(s_key[y], s_loc, s_lp, s_tmp, s_cnt, m4, m8, !uy, err);
-            const int m5 = lds_split<1>(s_key[z], s_loc, s_lp, s_tmp, s_cnt, m4, m6, uz, err);
-            const int m7 = lds_split<1>(s_key[z], s_loc, s_lp, s_tmp, s_cnt, m6, m8, !uz, err);
-            // the recursion of src/spatial_sort.cpp:109-116 (disjoint ranges,
-            // so the order in which they are processed does not matter)
-            const int ch[8][6] = {{m0, m1, z, uz, ux, uy},   {m1, m2, y, uy, uz, ux},
-                                  {m2, m3, y, uy, uz, ux},   {m3, m4, x, ux, !uy, !uz},
-                                  {m4, m5, x, ux, !uy, !uz}, {m5, m6, y, !uy, uz, !ux},
-                                  {m6, m7, y, !uy, uz, !ux}, {m7, m8, z, !uz, !ux, uy}};
+            int mm[9];
+            mm[0] = b;
+            mm[8] = e;
+            mm[4] = lds_split<1>(s_key[x], s_loc, s_lp, s_tmp, s_cnt, mm[0], mm[8], ux, err);
+            mm[2] = lds_split<1>(s_key[y], s_loc, s_lp, s_tmp, s_cnt, mm[0], mm[4], uy, err);
+            mm[1] = lds_split<1>(s_key[z], s_loc, s_lp, s_tmp, s_cnt, mm[0], mm[2], uz, err);
+            mm[3] = lds_split<1>(s_key[z], s_loc, s_lp, s_tmp, s_cnt, mm[2], mm[4], !uz, err);
+            mm[6] = lds_split<1>(s_key[y], s_loc, s_lp, s_tmp, s_cnt, mm[4], mm[8], !uy, err);
+            mm[5] = lds_split<1>(s_key[z], s_loc, s_lp, s_tmp, s_cnt, mm[4], mm[6], uz, err);
+            mm[7] = lds_split<1>(s_key[z], s_loc, s_lp, s_tmp, s_cnt, mm[6], mm[8], !uz, err);
+            int ch[8][6];
+            child_nodes(ch, mm, x, y, z, ux, uy, uz);
             int add = 0;
-            for (int c = 0; c < 8; ++c) add += (ch[c][1] - ch[c][0] > 8) ? 1 : 0;
+            for (int c = 0; c < 8; ++c) add += (ch[c][1] - ch[c][0] > kSmall) ? 1 : 0;
             if (sp + add > kStack) {
                 if (threadIdx.x == 0) raise_err(err, kErrLds);
                 break;
@@ -1135,7 +1205,14 @@ __global__ __launch_bounds__(64) void k_subtree(Bufs bufs, const Leaf* __restric
             if (threadIdx.x == 0) {
                 int w = sp;
                 for (int c = 0; c < 8; ++c) {
-                    if (ch[c][1] - ch[c][0] <= 8) continue;
+                    const int sz = ch[c][1] - ch[c][0];
+                    if (sz <= 8) continue;
+                    if (sz <= kSmall) {
+                        if (s_nsmall < kSmallCap) s_small[s_nsmall++] = pack_node(ch[c][0], ch[c][1], ch[c][2], ch[c][3],
+                                                                                  ch[c][4], ch[c][5]);
+                        else raise_err(err, kErrLds);
+                        continue;
+                    }
                     for (int f = 0; f < 6; ++f) s_stack[w][f] = ch[c][f];
                     ++w;
                 }
@@ -1144,6 +1221,26 @@ __global__ __launch_bounds__(64) void k_subtree(Bufs bufs, const Leaf* __restric
             __syncthreads();
         }
         if (sp > 0 && threadIdx.x == 0) raise_err(err, kErrLoop);
+        __syncthreads();
+        // the small nodes, one lane each
+        const int ns = s_nsmall;
+        for (int k = threadIdx.x; k < ns && k < kSmallCap; k += 64) {
+            const uint32_t w = s_small[k];
+            const int b = (int)(w & 2047u), e = (int)((w >> 11) & 2047u), x = (int)((w >> 22) & 3u);
+            const int ux = (int)((w >> 24) & 1u), uy = (int)((w >> 25) & 1u), uz = (int)((w >> 26) & 1u);
+            if (b < 0 || e > mi || e - b <= 8 || e - b > kSmall || x > 2) {
+                raise_err(err, kErrLds);
+                continue;
+            }
+            const int y = (x + 1) % 3, z = (x + 2) % 3;
+            const int m4 = lane_split(s_key[x], s_loc, b, e, ux, err);
+            const int m2 = lane_split(s_key[y], s_loc, b, m4, uy, err);
+            lane_split(s_key[z], s_loc, b, m2, uz, err);
+            lane_split(s_key[z], s_loc, m2, m4, !uz, err);
+            const int m6 = lane_split(s_key[y], s_loc, m4, e, !uy, err);
+            lane_split(s_key[z], s_loc, m4, m6, uz, err);
+            lane_split(s_key[z], s_loc, m6, e, !uz, err);
+        }
         __syncthreads();
         for (int i = threadIdx.x; i < mi; i += 64) {
             const int k = s_loc[i];

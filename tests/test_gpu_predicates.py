@@ -399,14 +399,26 @@ def test_packed_kernel_point_formats(kind):
         assert st["perturbed"] > 1000
 
 
-@pytest.mark.parametrize("kind", ["lattice32", "random9000"])
+@pytest.mark.parametrize("kind", ["lattice32", "lattice34", "random9000"])
 def test_more_queries_than_one_launch_sequence(kind):
     """2^26 + 2^18 queries in one call, above the 2^26 of one launch sequence
     (so two sequences with their own queues and counters), on the 32^3
-    lattice (packed LDS kernel) and on 9,000 random points (the global kernel
-    beside it): equal to the same queries in calls of 2^24."""
+    lattice (packed LDS kernel), the 34^3 lattice (above the packed LDS
+    kernel's capacity: the global kernel's packed variant) and on 9,000
+    random points (the global kernel beside it): equal to the same queries in
+    calls of 2^24."""
     if kind == "lattice32":
         pts, op, idx = W.c2(1 << 24, seed=2)
+    elif kind == "lattice34":
+        m = 34
+        g = np.arange(m, dtype=np.float64)
+        pts = np.stack(np.meshgrid(g, g, g, indexing="ij"), -1).reshape(-1, 3)
+        rng = np.random.default_rng(34)
+        n = 1 << 24
+        op = rng.integers(0, 3, size=n).astype(np.uint8)
+        base = rng.integers(0, m - 2, size=(n, 3))
+        ijk = base[:, None, :] + rng.integers(0, 3, size=(n, 5, 3))   # local cells: mostly degenerate
+        idx = ((ijk[..., 0] * m + ijk[..., 1]) * m + ijk[..., 2]).astype(np.int32)
     else:
         rng = np.random.default_rng(33)
         pts = rng.random((9000, 3))

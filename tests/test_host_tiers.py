@@ -193,3 +193,44 @@ def test_packed_decision_matches_oracle(kind):
                                    out.ctypes.data_as(C.c_void_p))
     assert ok == 1
     assert np.array_equal(out, O.pred_batch(pts, op, idx, "oracle"))
+
+
+@pytest.mark.parametrize("g,packable", [(-206, True), (-207, False), (171, True), (172, False)])
+def test_packable_range_edges(g, packable):
+    """Span-255 sets at the edges of the packable range (grid 2^-206 .. 2^171):
+    random integer points in [-255, 255] with an 8^3 unit lattice block,
+    scaled by 2^g. At -206 the in_sphere products of the lattice's unit
+    differences are multiples of 2^-1030 (subnormal); at 171 the largest reach
+    2^900. Inside the range the packed decision equals the oracle and the
+    big-integer oracle on local-cell (mostly degenerate) and random queries;
+    one step outside, the set is not packable."""
+    rng = np.random.default_rng(1000 + g)
+    lat = O.lattice(8, 8, 8) - 4.0
+    ints = rng.integers(-255, 256, size=(1500, 3)).astype(np.float64)
+    ints[0] = [255.0, -255.0, 1.0]  # the full span, and an odd value: the grid is exactly 2^g
+    pts = np.ascontiguousarray(np.vstack([lat, ints]) * 2.0 ** g)
+    ok, _, back = _pack(pts)
+    assert ok == packable
+    if not ok:
+        return
+    assert np.array_equal(pts, back)
+    gmin, mexp = _set_grid(pts)
+    assert gmin == g and mexp - gmin == 7
+    n = 20000
+    op = rng.integers(0, 3, size=n).astype(np.uint8)
+    idx = rng.integers(0, len(pts), size=(n, 5)).astype(np.int32)
+    base = rng.integers(0, 6, size=(n // 2, 3))
+    ijk = base[:, None, :] + rng.integers(0, 3, size=(n // 2, 5, 3))
+    idx[: n // 2] = ((ijk[..., 0] * 8 + ijk[..., 1]) * 8 + ijk[..., 2]).astype(np.int32)
+    out = np.zeros(n, np.int8)
+    lib = C.CDLL(str(LIB))
+    assert lib.predcore_packed_batch(pts.ctypes.data_as(C.c_void_p), C.c_int64(len(pts)), C.c_int32(gmin),
+                                     C.c_int32(mexp), op.ctypes.data_as(C.c_void_p),
+                                     idx.ctypes.data_as(C.c_void_p), C.c_int64(n),
+                                     out.ctypes.data_as(C.c_void_p)) == 1
+    want = O.pred_batch(pts, op, idx, "oracle")
+    assert np.array_equal(out, want)
+    assert np.array_equal(out, O.pred_batch(pts, op, idx, "int"))
+    # the local cells are degenerate: exact zeros of orient3d and of the
+    # unperturbed in_sphere among them
+    assert np.count_nonzero((out[: n // 2] == 0) & (op[: n // 2] != 1)) > 500
