@@ -526,13 +526,35 @@ __global__ __launch_bounds__(256) void k_rd_init(const RTask* __restrict__ tasks
     }
 }
 
-// Round part 1 (per tile): copies the margins the previous round left in
-// buffer 1, then flags the tile's elements that are less than the pivot (one
-// bit each), counts them, and records each 64-element word's exclusive
-// prefix inside the tile. Wavefront w takes words 16w .. 16w+15.
+// The tiles a round works on (one thread per tile of the phase): those that
+// intersect the current range [L, R] of an active select, appended in any
+// order to list[] (count in *cnt, zeroed by the previous round's k_rd_count).
+// The ranges only shrink, so a round's list is contained in the previous
+// one's, which covers the margins that round finalized as well.
+__global__ __launch_bounds__(256) void k_rd_list(const RTile* __restrict__ tiles, uint32_t ntiles,
+                                                 uint32_t ntasks, const RState* __restrict__ st,
+                                                 uint32_t* __restrict__ list, uint32_t* __restrict__ cnt) {
+    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < ntiles; t += gridDim.x * blockDim.x) {
+        const RTile tl = tiles[t];
+        if (tl.task < 0 || (uint32_t)tl.task >= ntasks) continue;  // k_rd_count raises it
+        const RState S = st[tl.task];
+        if (!S.active || tl.start > S.R || tl.start + tl.len <= S.L) continue;
+        const uint32_t k = atomicAdd(cnt, 1u);
+        if (k < ntiles) list[k] = t;
+    }
+}
+
+// Round part 1 (per listed tile, grid-stride): copies the margins the
+// previous round left in buffer 1, then flags the tile's elements that are
+// less than the pivot (one bit each), counts them, and records each
+// 64-element word's exclusive prefix inside the tile. Wavefront w takes
+// words 16w .. 16w+15. Block 0 also clears the next round's counters.
 __global__ __launch_bounds__(kTileThreads) void k_rd_count(Bufs bufs, const RTask* __restrict__ tasks,
                                                            uint32_t ntasks, const RTile* __restrict__ tiles,
                                                            uint32_t ntiles, const RState* __restrict__ st,
+                                                           const uint32_t* __restrict__ list,
+                                                           const uint32_t* __restrict__ list_cnt,
+                                                           uint32_t* __restrict__ next_list_cnt,
                                                            uint64_t* __restrict__ mask,
                                                            uint16_t* __restrict__ wloc,
                                                            uint32_t* __restrict__ tcount,
@@ -541,76 +563,98 @@ __global__ __launch_bounds__(kTileThreads) void k_rd_count(Bufs bufs, const RTas
     constexpr int kWordsPerWave = kTileWords / (kTileThreads / 64);
     __shared__ uint32_t s_cnt[kTileThreads / 64];
     __shared__ uint16_t s_wl[kTileWords];
-    const uint32_t tile = blockIdx.x;
-    if (tile == 0 && threadIdx.x == 0) ring[(round + 1) % kRing] = 0;
-    if (tile >= ntiles) return;
-    const RTile tl = tiles[tile];
-    if (tl.task < 0 || (uint32_t)tl.task >= ntasks || tl.len < 0 || tl.len > kTile || tl.start < 0 ||
-        tl.start + tl.len > bufs.n) {
-        if (threadIdx.x == 0) raise_err(err, kErrRound);
-        return;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        ring[(round + 1) % kRing] = 0;
+        if (next_list_cnt) *next_list_cnt = 0;
     }
-    const RTask T = tasks[tl.task];
-    const RState S = st[tl.task];
-    const int64_t t0 = tl.start, t1 = tl.start + tl.len;
-    if (S.mpar) {
-        for (int m = 0; m < 2; ++m) {
-            const int64_t mlo = m ? S.m1lo : S.m0lo, mhi = m ? S.m1hi : S.m0hi;
-            const int64_t lo = mlo > t0 ? mlo : t0, hi = mhi < t1 ? mhi : t1;
-            if (lo < hi && (lo < 0 || hi > bufs.n)) {
-                if (threadIdx.x == 0) raise_err(err, kErrRound);
-                continue;
+    const uint32_t n = min(*list_cnt, ntiles);
+    for (uint32_t li = blockIdx.x; li < n; li += gridDim.x) {
+        __syncthreads();  // s_cnt / s_wl of the previous tile are read
+        const uint32_t tile = list[li];
+        if (tile >= ntiles) {
+            if (threadIdx.x == 0) raise_err(err, kErrRound);
+            continue;
+        }
+        const RTile tl = tiles[tile];
+        if (tl.task < 0 || (uint32_t)tl.task >= ntasks || tl.len < 0 || tl.len > kTile || tl.start < 0 ||
+            tl.start + tl.len > bufs.n) {
+            if (threadIdx.x == 0) raise_err(err, kErrRound);
+            continue;
+        }
+        const RTask T = tasks[tl.task];
+        const RState S = st[tl.task];
+        const int64_t t0 = tl.start, t1 = tl.start + tl.len;
+        if (S.mpar) {
+            for (int m = 0; m < 2; ++m) {
+                const int64_t mlo = m ? S.m1lo : S.m0lo, mhi = m ? S.m1hi : S.m0hi;
+                const int64_t lo = mlo > t0 ? mlo : t0, hi = mhi < t1 ? mhi : t1;
+                if (lo < hi && (lo < 0 || hi > bufs.n)) {
+                    if (threadIdx.x == 0) raise_err(err, kErrRound);
+                    continue;
+                }
+                for (int64_t p = lo + threadIdx.x; p < hi; p += kTileThreads) copy_rec(bufs, 1, 0, p, p);
             }
-            for (int64_t p = lo + threadIdx.x; p < hi; p += kTileThreads) copy_rec(bufs, 1, 0, p, p);
         }
-    }
-    if (!S.active) {
-        if (threadIdx.x == 0) tcount[tile] = 0;
-        return;
-    }
-    if (!rd_range_ok(S, bufs.n) || (S.par & ~1)) {
+        if (!S.active) {
+            if (threadIdx.x == 0) tcount[tile] = 0;
+            continue;
+        }
+        if (!rd_range_ok(S, bufs.n) || (S.par & ~1)) {
+            if (threadIdx.x == 0) {
+                raise_err(err, kErrRound);
+                tcount[tile] = 0;
+            }
+            continue;
+        }
+        const double* key = axis_of(bufs, S.par, T.axis);
+        const int64_t pv = S.L + (S.R - S.L) / 2;
+        const double pk = key[pv];
+        const double kr = key[S.R];
+        const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+        // every key of the thread's words first, so their loads are in flight
+        // together (the mask stores below could otherwise order them)
+        double kv[kWordsPerWave];
+#pragma unroll
+        for (int it = 0; it < kWordsPerWave; ++it) {
+            const int64_t p = t0 + (int64_t)(wave * kWordsPerWave + it) * 64 + lane;
+            kv[it] = (p < t1 && p >= S.L && p < S.R) ? (p == pv ? kr : key[p]) : 0.0;
+        }
+        uint32_t c = 0;
+#pragma unroll
+        for (int it = 0; it < kWordsPerWave; ++it) {
+            const int w = wave * kWordsPerWave + it;
+            const int64_t p = t0 + (int64_t)w * 64 + lane;
+            const bool f = p < t1 && p >= S.L && p < S.R && key_less(kv[it], pk, T.up);
+            const uint64_t m = __ballot(f);
+            if (lane == 0) {
+                mask[(uint64_t)tile * kTileWords + w] = m;
+                s_wl[w] = (uint16_t)c;
+            }
+            c += (uint32_t)__popcll(m);
+        }
+        if (lane == 0) s_cnt[wave] = c;
+        __syncthreads();
+        if (threadIdx.x < kTileWords) {
+            const int w = threadIdx.x, ww = w / kWordsPerWave;
+            uint32_t before = 0;
+            for (int i = 0; i < ww; ++i) before += s_cnt[i];
+            wloc[(uint64_t)tile * kTileWords + w] = (uint16_t)(before + s_wl[w]);
+        }
         if (threadIdx.x == 0) {
-            raise_err(err, kErrRound);
-            tcount[tile] = 0;
+            uint32_t sum = 0;
+            for (int i = 0; i < kTileThreads / 64; ++i) sum += s_cnt[i];
+            tcount[tile] = sum;
         }
-        return;
-    }
-    const double* key = axis_of(bufs, S.par, T.axis);
-    const int64_t pv = S.L + (S.R - S.L) / 2;
-    const double pk = key[pv];
-    const double kr = key[S.R];
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    uint32_t c = 0;
-    for (int it = 0; it < kWordsPerWave; ++it) {
-        const int w = wave * kWordsPerWave + it;
-        const int64_t p = t0 + (int64_t)w * 64 + lane;
-        bool f = false;
-        if (p < t1 && p >= S.L && p < S.R) f = key_less(p == pv ? kr : key[p], pk, T.up);
-        const uint64_t m = __ballot(f);
-        if (lane == 0) {
-            mask[(uint64_t)tile * kTileWords + w] = m;
-            s_wl[w] = (uint16_t)c;
-        }
-        c += (uint32_t)__popcll(m);
-    }
-    if (lane == 0) s_cnt[wave] = c;
-    __syncthreads();
-    if (threadIdx.x < kTileWords) {
-        const int w = threadIdx.x, ww = w / kWordsPerWave;
-        uint32_t before = 0;
-        for (int i = 0; i < ww; ++i) before += s_cnt[i];
-        wloc[(uint64_t)tile * kTileWords + w] = (uint16_t)(before + s_wl[w]);
-    }
-    if (threadIdx.x == 0) {
-        uint32_t sum = 0;
-        for (int i = 0; i < kTileThreads / 64; ++i) sum += s_cnt[i];
-        tcount[tile] = sum;
     }
 }
 
-// Round part 2 (per task): exclusive prefix of the task's tile counts, lim.
+// Round part 2 (per task): exclusive prefix of the counts of the task's tiles
+// that intersect [L, R], and lim. An inactive task's state is carried to the
+// next round's buffer here (with its margins, copied by this round's
+// k_rd_count, cleared), since no block of k_rd_move handles it.
 __global__ __launch_bounds__(256) void k_rd_scan(const RTask* __restrict__ tasks, uint32_t ntasks,
                                                  uint32_t ntiles, RState* __restrict__ st,
+                                                 RState* __restrict__ st_next,
                                                  const uint32_t* __restrict__ tcount,
                                                  uint32_t* __restrict__ toff, uint32_t* __restrict__ err) {
     __shared__ uint32_t s_wave[4];
@@ -618,18 +662,28 @@ __global__ __launch_bounds__(256) void k_rd_scan(const RTask* __restrict__ tasks
     if (t >= ntasks) return;
     const RTask T = tasks[t];
     const RState S = st[t];
-    if (!S.active) return;
-    if (T.tile0 < 0 || T.ntiles < 0 || T.tile0 + T.ntiles > (int64_t)ntiles) {
+    if (!S.active) {
+        if (threadIdx.x == 0) {
+            RState N = S;
+            N.mpar = 0;
+            N.m0lo = N.m0hi = N.m1lo = N.m1hi = 0;
+            st_next[t] = N;
+        }
+        return;
+    }
+    if (T.tile0 < 0 || T.ntiles < 0 || T.tile0 + T.ntiles > (int64_t)ntiles || S.L < T.b || S.R >= T.e ||
+        S.L > S.R) {
         if (threadIdx.x == 0) raise_err(err, kErrRound);
         return;
     }
+    const int64_t tb = (S.L - T.b) / kTile, te = (S.R - T.b) / kTile + 1;  // the tiles of [L, R]
     uint32_t carry = 0;
-    for (int64_t base = 0; base < T.ntiles; base += blockDim.x) {
+    for (int64_t base = tb; base < te; base += blockDim.x) {
         const int64_t i = base + threadIdx.x;
-        const uint32_t v = i < T.ntiles ? tcount[T.tile0 + i] : 0u;
+        const uint32_t v = i < te ? tcount[T.tile0 + i] : 0u;
         uint32_t total;
         const uint32_t ex = wg_exclusive(v, s_wave, &total);
-        if (i < T.ntiles) toff[T.tile0 + i] = carry + ex;
+        if (i < te) toff[T.tile0 + i] = carry + ex;
         carry += total;
     }
     if (threadIdx.x == 0) st[t].lim = S.L + (int64_t)carry;
@@ -644,27 +698,32 @@ constexpr int kSlices = kTile / kTileThreads;
 __global__ __launch_bounds__(kTileThreads) void k_rd_npos(const RTask* __restrict__ tasks, uint32_t ntasks,
                                                           const RTile* __restrict__ tiles, uint32_t ntiles,
                                                           const RState* __restrict__ st,
+                                                          const uint32_t* __restrict__ list,
+                                                          const uint32_t* __restrict__ list_cnt,
                                                           const uint64_t* __restrict__ mask,
                                                           const uint16_t* __restrict__ wloc,
                                                           const uint32_t* __restrict__ toff,
                                                           uint32_t* __restrict__ npos, int64_t n) {
-    const uint32_t tile = blockIdx.x / kSlices, slice = blockIdx.x % kSlices;
-    if (tile >= ntiles) return;
-    const RTile tl = tiles[tile];
-    if (tl.task < 0 || (uint32_t)tl.task >= ntasks) return;  // k_rd_count raised it
-    const RTask T = tasks[tl.task];
-    const RState S = st[tl.task];
-    if (!S.active || !rd_range_ok(S, n) || S.L < T.b || S.R >= T.e) return;  // k_rd_move raises it
-    const int lane = threadIdx.x & 63;
-    const int w = (int)slice * (kTileThreads / 64) + (threadIdx.x >> 6);
-    const int64_t p = tl.start + (int64_t)w * 64 + lane;
-    if (p >= tl.start + tl.len || p < S.L || p >= S.R) return;
-    const uint64_t gw = (uint64_t)tile * kTileWords + w;
-    const uint64_t m = mask[gw];
-    if ((m >> lane) & 1ull) return;
-    const int64_t rank = (int64_t)toff[tile] + wloc[gw] + __popcll(m & ((1ull << lane) - 1ull));
-    const int64_t j = (p - S.L) - rank;  // not-less elements in [L, p)
-    if (j >= 0 && S.L + j < S.R) npos[S.L + j] = (uint32_t)(p - T.b);
+    const uint32_t units = min(*list_cnt, ntiles) * kSlices;
+    for (uint32_t u = blockIdx.x; u < units; u += gridDim.x) {
+        const uint32_t tile = list[u / kSlices], slice = u % kSlices;
+        if (tile >= ntiles) continue;  // k_rd_count raised it
+        const RTile tl = tiles[tile];
+        if (tl.task < 0 || (uint32_t)tl.task >= ntasks) continue;  // k_rd_count raised it
+        const RTask T = tasks[tl.task];
+        const RState S = st[tl.task];
+        if (!S.active || !rd_range_ok(S, n) || S.L < T.b || S.R >= T.e) continue;  // k_rd_move raises it
+        const int lane = threadIdx.x & 63;
+        const int w = (int)slice * (kTileThreads / 64) + (threadIdx.x >> 6);
+        const int64_t p = tl.start + (int64_t)w * 64 + lane;
+        if (p >= tl.start + tl.len || p < S.L || p >= S.R) continue;
+        const uint64_t gw = (uint64_t)tile * kTileWords + w;
+        const uint64_t m = mask[gw];
+        if ((m >> lane) & 1ull) continue;
+        const int64_t rank = (int64_t)toff[tile] + wloc[gw] + __popcll(m & ((1ull << lane) - 1ull));
+        const int64_t j = (p - S.L) - rank;  // not-less elements in [L, p)
+        if (j >= 0 && S.L + j < S.R) npos[S.L + j] = (uint32_t)(p - T.b);
+    }
 }
 
 // Round part 4 (one slot per thread, kSlices blocks per tile): the whole
@@ -694,24 +753,34 @@ __global__ __launch_bounds__(kTileThreads) void k_rd_move(Bufs bufs, const RTask
                                                           uint32_t ntasks, const RTile* __restrict__ tiles,
                                                           uint32_t ntiles, const RState* __restrict__ st,
                                                           RState* __restrict__ st_next,
+                                                          const uint32_t* __restrict__ list,
+                                                          const uint32_t* __restrict__ list_cnt,
                                                           const uint64_t* __restrict__ mask,
                                                           const uint16_t* __restrict__ wloc,
                                                           const uint32_t* __restrict__ toff,
                                                           const uint32_t* __restrict__ npos,
                                                           uint32_t* __restrict__ ring, uint32_t round,
                                                           uint32_t* __restrict__ err) {
-    const uint32_t tile = blockIdx.x / kSlices, slice = blockIdx.x % kSlices;
-    if (tile >= ntiles) return;
+  const uint32_t units = min(*list_cnt, ntiles) * kSlices;
+  for (uint32_t u = blockIdx.x; u < units; u += gridDim.x) {
+    const uint32_t tile = list[u / kSlices], slice = u % kSlices;
+    if (tile >= ntiles) continue;  // k_rd_count raised it
     const RTile tl = tiles[tile];
-    if (tl.task < 0 || (uint32_t)tl.task >= ntasks) return;  // k_rd_count raised it
+    if (tl.task < 0 || (uint32_t)tl.task >= ntasks) continue;  // k_rd_count raised it
     const RTask T = tasks[tl.task];
     const RState S = st[tl.task];
-    const bool first = (int64_t)tile == T.tile0 && slice == 0;
-    const bool ok = S.active && rd_range_ok(S, bufs.n) && !(S.par & ~1) && S.lim >= S.L && S.lim <= S.R &&
+    if (!S.active) continue;  // listed tiles belong to active selects; k_rd_scan carries the others
+    const bool ok = rd_range_ok(S, bufs.n) && !(S.par & ~1) && S.lim >= S.L && S.lim <= S.R &&
                     T.tile0 >= 0 && T.tile0 + T.ntiles <= (int64_t)ntiles && S.L >= T.b && S.R < T.e;
-    if (S.active && !ok && first && threadIdx.x == 0) raise_err(err, kErrRound);
+    // the block of the tile holding L, slice 0, finishes the round for the
+    // select: the pivot's move and the next state
+    const bool first = ok && slice == 0 && (int64_t)tile == T.tile0 + (S.L - T.b) / kTile;
+    if (!ok) {
+        if (threadIdx.x == 0) raise_err(err, kErrRound);
+        continue;
+    }
     const int64_t pv = S.L + (S.R - S.L) / 2;
-    if (ok) {
+    {
         const int src = S.par, dst = S.par ^ 1;
         const int lane = threadIdx.x & 63;
         const int w = (int)slice * (kTileThreads / 64) + (threadIdx.x >> 6);
@@ -775,7 +844,7 @@ __global__ __launch_bounds__(kTileThreads) void k_rd_move(Bufs bufs, const RTask
         }
         if (first && threadIdx.x == 0) copy_rec(bufs, src, dst, pv, S.lim);  // the pivot
     }
-    if (!(first && threadIdx.x == 0)) return;
+    if (!(first && threadIdx.x == 0)) continue;
     RState N = S;
     N.mpar = 0;
     N.m0lo = N.m0hi = N.m1lo = N.m1hi = 0;
@@ -812,10 +881,9 @@ __global__ __launch_bounds__(kTileThreads) void k_rd_move(Bufs bufs, const RTask
                 N.m1lo = N.m1hi = 0;
             }
         }
-    } else if (S.active) {
-        N.active = 0;  // an invalid state stops (the error is raised)
     }
     st_next[tl.task] = N;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -1525,6 +1593,8 @@ void sort_sets(const double* xyz, int64_t n_pts, const int64_t* poff, const int6
     uint64_t* mask = nullptr;
     uint16_t* wloc = nullptr;
     uint32_t *tcount = nullptr, *toff = nullptr, *ring = nullptr, *npos = nullptr;
+    uint32_t* alist[3] = {nullptr, nullptr, nullptr};  // the tiles of rounds r, r-1 and r-2 (r mod 3)
+    uint32_t* acnt = nullptr;
     if (max_tasks) {
         st[0] = sc.get<RState>(max_tasks);
         st[1] = sc.get<RState>(max_tasks);
@@ -1535,6 +1605,8 @@ void sort_sets(const double* xyz, int64_t n_pts, const int64_t* poff, const int6
         npos = sc.get<uint32_t>(total);
         ring = sc.get<uint32_t>(kRing);
         fill_u32(ring, 0u, kRing, s);
+        for (auto& l : alist) l = sc.get<uint32_t>(max_tiles);
+        acnt = sc.get<uint32_t>(3);
     }
     for (const Phase& ph : P.phases) {
         const RState* final_state = st[0];
@@ -1544,6 +1616,15 @@ void sort_sets(const double* xyz, int64_t n_pts, const int64_t* poff, const int6
             const uint32_t nt = (uint32_t)ph.ntasks, ntl = (uint32_t)ph.ntiles;
             k_rd_init<<<grid_for(nt, 256, cus), 256, 0, s>>>(pt, nt, st[0]);
             D3_LAUNCH_CHECK();
+            fill_u32(acnt, 0u, 3, s);
+            // Each round works on the tiles that intersect a live range
+            // (k_rd_list): its count over the previous round's tiles (which
+            // hold the margins that round finalized), its moves over its own;
+            // fixed grids stride over the lists, so a round's cost follows the
+            // live ranges, not the phase's original size.
+            const unsigned g_count = (unsigned)std::min<uint64_t>(ntl, (uint64_t)cus * 8);
+            const unsigned g_move = (unsigned)std::min<uint64_t>((uint64_t)ntl * kSlices, (uint64_t)cus * 64);
+            const unsigned g_list = grid_for(ntl, 256, cus);
             // A select's range shrinks by at least one slot per round; the
             // expected count is ~1.5 log2(m / kFin) rounds. Check every two
             // rounds once that many have run.
@@ -1554,16 +1635,20 @@ void sort_sets(const double* xyz, int64_t n_pts, const int64_t* poff, const int6
             uint32_t round = 0;
             for (;;) {
                 const int c = round & 1;
-                k_rd_count<<<ntl, kTileThreads, 0, s>>>(bufs, pt, nt, ptl, ntl, st[c], mask, wloc, tcount, ring,
-                                                        round, err);
+                const int lr = (int)(round % 3), lp = round ? (int)((round + 2) % 3) : lr;
+                k_rd_list<<<g_list, 256, 0, s>>>(ptl, ntl, nt, st[c], alist[lr], acnt + lr);
                 D3_LAUNCH_CHECK();
-                k_rd_scan<<<nt, 256, 0, s>>>(pt, nt, ntl, st[c], tcount, toff, err);
+                k_rd_count<<<g_count, kTileThreads, 0, s>>>(bufs, pt, nt, ptl, ntl, st[c], alist[lp], acnt + lp,
+                                                            acnt + (round + 1) % 3, mask, wloc, tcount, ring, round,
+                                                            err);
                 D3_LAUNCH_CHECK();
-                k_rd_npos<<<ntl * kSlices, kTileThreads, 0, s>>>(pt, nt, ptl, ntl, st[c], mask, wloc, toff, npos,
-                                                                 bufs.n);
+                k_rd_scan<<<nt, 256, 0, s>>>(pt, nt, ntl, st[c], st[c ^ 1], tcount, toff, err);
                 D3_LAUNCH_CHECK();
-                k_rd_move<<<ntl * kSlices, kTileThreads, 0, s>>>(bufs, pt, nt, ptl, ntl, st[c], st[c ^ 1], mask, wloc,
-                                                                 toff, npos, ring, round, err);
+                k_rd_npos<<<g_move, kTileThreads, 0, s>>>(pt, nt, ptl, ntl, st[c], alist[lr], acnt + lr, mask, wloc,
+                                                          toff, npos, bufs.n);
+                D3_LAUNCH_CHECK();
+                k_rd_move<<<g_move, kTileThreads, 0, s>>>(bufs, pt, nt, ptl, ntl, st[c], st[c ^ 1], alist[lr],
+                                                          acnt + lr, mask, wloc, toff, npos, ring, round, err);
                 D3_LAUNCH_CHECK();
                 ++round;
                 if ((int64_t)round >= check_at) {
@@ -1578,9 +1663,10 @@ void sort_sets(const double* xyz, int64_t n_pts, const int64_t* poff, const int6
                     throw HipError{DEL3D_ERR_INTERNAL};
                 }
             }
-            // the margins of the last round
-            k_rd_count<<<ntl, kTileThreads, 0, s>>>(bufs, pt, nt, ptl, ntl, st[round & 1], mask, wloc, tcount,
-                                                    ring, round, err);
+            // the margins of the last round, over its tiles
+            const int last = (int)((round + 2) % 3);
+            k_rd_count<<<g_count, kTileThreads, 0, s>>>(bufs, pt, nt, ptl, ntl, st[round & 1], alist[last],
+                                                        acnt + last, nullptr, mask, wloc, tcount, ring, round, err);
             D3_LAUNCH_CHECK();
             final_state = st[round & 1];
         }
