@@ -1132,6 +1132,297 @@ __global__ __launch_bounds__(256) void k_lds_select(Bufs bufs, const FTask* __re
 }
 
 // ---------------------------------------------------------------------------
+// Mid-size selects (kFin < m <= kWgMax): every round in one workgroup
+// ---------------------------------------------------------------------------
+// The same Lomuto rounds as the device-wide kernels (flags, ranks, the slots
+// of the not-less elements, the moves with their backward chains), each pass
+// a loop of the workgroup's 1024 threads over the live range, separated by
+// barriers: no launch, state round trip or host check per round. The flag
+// words of the range and their prefix counts stay in LDS, so a chain's hops
+// read LDS. Once the range holds at most kFin elements the select finishes
+// in LDS like k_lds_select, in the same workgroup.
+constexpr int kWgThreads = 1024;
+constexpr int kWgWaves = kWgThreads / 64;
+constexpr int kWgWords = 4096;                         // flag words in LDS
+constexpr int64_t kWgMax = (int64_t)kWgWords * 64;    // 262,144 elements
+
+struct WTask {           // a mid-size select of one phase (host-built)
+    int64_t b, e, nth;
+    int32_t axis, up;
+};
+
+struct WgLds {
+    union {
+        struct {
+            uint64_t mask[kWgWords];  // word w: slots L + 64w .. L + 64w + 63, bit set = less
+            uint32_t pre[kWgWords];   // exclusive prefix of the words' counts
+        } r;
+        struct {
+            double key[kFin];
+            uint16_t loc[kFin], lp[kFin], tmp[kFin];
+        } f;
+    } u;
+    uint32_t wave[kWgWaves];
+    uint32_t cnt[kWgWaves];
+    int64_t st[4];  // L, R, lim, next action (broadcast by thread 0)
+};
+
+// Copies [lo, hi) of every field from buffer `from` to buffer 0, by the workgroup.
+__device__ inline void wg_copy_to0(const Bufs& b, int from, int64_t lo, int64_t hi) {
+    if (from == 0) return;
+    for (int64_t p = lo + threadIdx.x; p < hi; p += kWgThreads) copy_rec(b, from, 0, p, p);
+}
+
+__global__ __launch_bounds__(kWgThreads) void k_wg_select(Bufs bufs, const WTask* __restrict__ wt, uint32_t nwt,
+                                                          uint32_t* __restrict__ npos, uint32_t* __restrict__ err) {
+    __shared__ WgLds S;
+    const uint32_t ti = blockIdx.x;
+    if (ti >= nwt) return;
+    const WTask T = wt[ti];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    if (T.b < 0 || T.e > bufs.n || T.e - T.b <= kFin || T.e - T.b > kWgMax || T.nth < T.b || T.nth >= T.e ||
+        T.axis < 0 || T.axis > 2) {
+        if (tid == 0) raise_err(err, kErrRound);
+        return;
+    }
+    int64_t L = T.b, R = T.e - 1;
+    int par = 0;
+    bool finish = false;  // [L, R] left for the LDS finish
+    const int64_t bound = T.e - T.b;
+    int64_t round = 0;
+    for (; round <= bound; ++round) {
+        const double* key = axis_of(bufs, par, T.axis);
+        const int64_t pv = L + (R - L) / 2;
+        const double pk = key[pv];
+        const double kr = key[R];
+        const int nw = (int)((R - L + 63) / 64);  // words over [L, R): slot R holds the pivot
+        // flags and per-word counts, four words of a wavefront per step so
+        // their key loads are in flight together
+        for (int w0 = wave; w0 < nw; w0 += 4 * kWgWaves) {
+            double kv[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const int64_t p = L + (int64_t)(w0 + k * kWgWaves) * 64 + lane;
+                kv[k] = p < R ? (p == pv ? kr : key[p]) : 0.0;
+            }
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const int w = w0 + k * kWgWaves;
+                const int64_t p = L + (int64_t)w * 64 + lane;
+                const bool f = p < R && key_less(kv[k], pk, T.up);
+                const uint64_t m = __ballot(f);
+                if (lane == 0 && w < nw) {
+                    S.u.r.mask[w] = m;
+                    S.u.r.pre[w] = (uint32_t)__popcll(m);
+                }
+            }
+        }
+        __syncthreads();
+        // exclusive prefix of the counts: four words per thread
+        {
+            constexpr int kPer = kWgWords / kWgThreads;
+            uint32_t c[kPer], sum = 0;
+#pragma unroll
+            for (int k = 0; k < kPer; ++k) {
+                const int w = tid * kPer + k;
+                c[k] = w < nw ? S.u.r.pre[w] : 0u;
+                sum += c[k];
+            }
+            uint32_t total;
+            uint32_t run = wg_exclusive(sum, S.wave, &total);
+#pragma unroll
+            for (int k = 0; k < kPer; ++k) {
+                const int w = tid * kPer + k;
+                if (w < nw) S.u.r.pre[w] = run;
+                run += c[k];
+            }
+            if (tid == 0) S.st[2] = L + (int64_t)total;
+        }
+        __syncthreads();
+        const int64_t lim = S.st[2];
+        if (lim < L || lim > R) {
+            if (tid == 0) raise_err(err, kErrRound);
+            return;
+        }
+        auto rank_of = [&](int64_t q, uint64_t* mword) {
+            const int64_t o = q - L;
+            const uint64_t m = S.u.r.mask[o >> 6];
+            *mword = m;
+            return (int64_t)S.u.r.pre[o >> 6] + __popcll(m & ((1ull << (o & 63)) - 1ull));
+        };
+        // the slots of the not-less elements, in order (offsets from T.b)
+        for (int64_t p = L + tid; p < R; p += kWgThreads) {
+            uint64_t m;
+            const int64_t rk = rank_of(p, &m);
+            if ((m >> ((p - L) & 63)) & 1ull) continue;
+            const int64_t j = (p - L) - rk;
+            if (j >= 0 && L + j < R) npos[L + j] = (uint32_t)(p - T.b);
+        }
+        __syncthreads();
+        // the round's permutation into the other buffer (see k_rd_move): the
+        // direct moves first, four slots per thread per step with their
+        // loads issued together (a less element to the front, a not-less one
+        // at or above lim in place), then the slots whose element is replaced
+        // by the end of a backward chain
+        const int src = par, dst = par ^ 1;
+        const int64_t span = R - L;
+        for (int64_t p0 = L + tid; p0 < R; p0 += 4 * kWgThreads) {
+            int64_t from[4], to[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const int64_t p = p0 + (int64_t)k * kWgThreads;
+                from[k] = -1;
+                to[k] = -1;
+                if (p >= R) continue;
+                uint64_t m;
+                const int64_t rk = rank_of(p, &m);
+                const bool f = (m >> ((p - L) & 63)) & 1ull;
+                if (f) {
+                    const int64_t d = L + rk;
+                    if (d >= L && d < lim) {
+                        from[k] = p == pv ? R : p;
+                        to[k] = d;
+                    } else {
+                        raise_err(err, kErrRound);
+                    }
+                } else if (p >= lim) {
+                    from[k] = p == pv ? R : p;  // the slot pv holds what was at R
+                    to[k] = p == lim ? R : p;
+                }
+            }
+            double vx[4], vy[4], vz[4];
+            int32_t vi[4];
+            const double *sx = bx(bufs, src), *sy = by(bufs, src), *sz = bz(bufs, src);
+            const int32_t* si = bid(bufs, src);
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const int64_t q = from[k] < 0 ? L : from[k];
+                vx[k] = sx[q];
+                vy[k] = sy[q];
+                vz[k] = sz[q];
+                vi[k] = si[q];
+            }
+            double *tx = bx(bufs, dst), *ty = by(bufs, dst), *tz = bz(bufs, dst);
+            int32_t* tdi = bid(bufs, dst);
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                if (to[k] < 0) continue;
+                tx[to[k]] = vx[k];
+                ty[to[k]] = vy[k];
+                tz[to[k]] = vz[k];
+                tdi[to[k]] = vi[k];
+            }
+        }
+        for (int64_t p = lim + tid; p < R; p += kWgThreads) {
+            uint64_t m;
+            const int64_t rk = rank_of(p, &m);
+            const bool f = (m >> ((p - L) & 63)) & 1ull;
+            if (!f) continue;  // moved in place above
+            int64_t s = p;
+            bool less = f;
+            int64_t rs = rk;
+            int64_t hops = 0;
+            bool bad = false;
+            while (less) {
+                if (++hops > span) {
+                    bad = true;
+                    break;
+                }
+                if (hops <= 3) {
+                    s = L + rs;
+                } else {
+                    const int64_t j = (s - L) - rs;  // not-less slots below s
+                    if (j < 1 || j > span) {
+                        bad = true;
+                        break;
+                    }
+                    const int64_t P = T.b + (int64_t)npos[L + j - 1];
+                    if (P < L || P >= s) {
+                        bad = true;
+                        break;
+                    }
+                    s -= j * ((s - P + j - 1) / j);
+                    if (s == P) break;
+                }
+                if (s < L || s >= lim) {
+                    bad = true;
+                    break;
+                }
+                uint64_t ms;
+                rs = rank_of(s, &ms);
+                less = (ms >> ((s - L) & 63)) & 1ull;
+            }
+            if (bad) raise_err(err, hops > span ? kErrLoop : kErrRound);
+            else copy_rec(bufs, src, dst, s == pv ? R : s, p == lim ? R : p);
+        }
+        if (tid == 0) copy_rec(bufs, src, dst, pv, lim);  // the pivot
+        __syncthreads();
+        // the next range (src/spatial_sort.cpp:65-69); what leaves it is final
+        int64_t nL = L, nR = R;
+        int action;  // 0: all of [L, R] final; 1: more rounds; 2: the LDS finish
+        if (lim == T.nth) {
+            action = 0;
+        } else {
+            if (T.nth < lim) nR = lim - 1;
+            else nL = lim + 1;
+            const int64_t size = nR - nL + 1;
+            action = size > kFin ? 1 : (size >= 2 ? 2 : 0);
+        }
+        if (action == 0) {
+            wg_copy_to0(bufs, dst, L, R + 1);
+            L = R + 1;
+            par = 0;
+            break;
+        }
+        wg_copy_to0(bufs, dst, L, nL);
+        wg_copy_to0(bufs, dst, nR + 1, R + 1);
+        L = nL;
+        R = nR;
+        par = dst;
+        __syncthreads();  // every read of this round's flag words is done
+        if (action == 2) {
+            finish = true;
+            break;
+        }
+    }
+    if (round > bound) {
+        if (tid == 0) raise_err(err, kErrLoop);
+        return;
+    }
+    if (!finish) return;
+    // the LDS finish (as k_lds_select): keys of [L, R] from buffer par, the
+    // quickselect on a 16-bit arrangement, then every field permuted into buffer 0
+    const int mi = (int)(R - L + 1);
+    const double* key = axis_of(bufs, par, T.axis);
+    for (int i = tid; i < mi; i += kWgThreads) {
+        S.u.f.key[i] = key[L + i];
+        S.u.f.loc[i] = (uint16_t)i;
+    }
+    __syncthreads();
+    lds_select<kWgWaves>(S.u.f.key, S.u.f.loc, S.u.f.lp, S.u.f.tmp, S.cnt, 0, mi - 1, (int)(T.nth - L), T.up, err);
+    for (int f = 0; f < 3; ++f) {
+        const double* fsrc = axis_of(bufs, par, f);
+        double* fdst = f == 0 ? bufs.x[0] : (f == 1 ? bufs.y[0] : bufs.z[0]);
+        __syncthreads();
+        for (int i = tid; i < mi; i += kWgThreads) S.u.f.key[i] = fsrc[L + i];
+        __syncthreads();
+        for (int i = tid; i < mi; i += kWgThreads) {
+            const int k = S.u.f.loc[i];
+            if (k < mi) fdst[L + i] = S.u.f.key[k];
+        }
+    }
+    int32_t* s_id = reinterpret_cast<int32_t*>(S.u.f.key);
+    __syncthreads();
+    const int32_t* isrc = bid(bufs, par);
+    for (int i = tid; i < mi; i += kWgThreads) s_id[i] = isrc[L + i];
+    __syncthreads();
+    for (int i = tid; i < mi; i += kWgThreads) {
+        const int k = S.u.f.loc[i];
+        if (k < mi) bufs.id[0][L + i] = s_id[k];
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Whole Hilbert subtrees of at most kSub elements, one wavefront each
 // ---------------------------------------------------------------------------
 constexpr int kSub = 1024;
@@ -1424,6 +1715,7 @@ struct Phase {
     size_t task0 = 0, ntasks = 0;   // RTask range
     size_t tile0 = 0, ntiles = 0;   // RTile range
     size_t fin0 = 0, nfin = 0;      // FTask range
+    size_t wtask0 = 0, nwtasks = 0; // WTask range
     int64_t max_big = 0;            // largest big select
 };
 
@@ -1431,21 +1723,40 @@ struct Plan {
     std::vector<RTask> tasks;
     std::vector<RTile> tiles;
     std::vector<FTask> fins;
+    std::vector<WTask> wtasks;
     std::vector<Leaf> leaves;
     std::vector<Phase> phases;
 };
 
-void add_phase(Plan& P, const std::vector<Split>& splits) {
+// A phase's selects of more than kFin elements run device-wide rounds, all
+// of them together; when none of them exceeds kWgMax and there are at least
+// wg_min of them (enough to give every CU one) they run instead one workgroup
+// each (k_wg_select). A phase that has a larger select anyway pays for the
+// device-wide rounds' latency, which the smaller selects then share; a
+// phase of few selects would leave most of the chip idle in workgroups.
+void add_phase(Plan& P, const std::vector<Split>& splits, int64_t wg_min) {
     Phase ph;
     ph.task0 = P.tasks.size();
     ph.tile0 = P.tiles.size();
     ph.fin0 = P.fins.size();
+    ph.wtask0 = P.wtasks.size();
+    bool wg = true;
+    int64_t mid = 0;
+    for (const Split& sp : splits) {
+        wg = wg && sp.e - sp.b <= kWgMax;
+        mid += sp.e - sp.b > kFin ? 1 : 0;
+    }
+    wg = wg && mid >= wg_min;
     for (const Split& sp : splits) {
         const int64_t m = sp.e - sp.b;
         if (m < 2) continue;  // nth_element returns at once (src/spatial_sort.cpp:62-64)
         const int64_t nth = sp.b + m / 2;
         if (m <= kFin) {
             P.fins.push_back(FTask{sp.b, sp.e - 1, nth, sp.axis, sp.up, -1, 0});
+            continue;
+        }
+        if (wg) {
+            P.wtasks.push_back(WTask{sp.b, sp.e, nth, sp.axis, sp.up});
             continue;
         }
         RTask t;
@@ -1466,12 +1777,13 @@ void add_phase(Plan& P, const std::vector<Split>& splits) {
     ph.ntasks = P.tasks.size() - ph.task0;
     ph.ntiles = P.tiles.size() - ph.tile0;
     ph.nfin = P.fins.size() - ph.fin0;
-    if (ph.ntasks || ph.nfin) P.phases.push_back(ph);
+    ph.nwtasks = P.wtasks.size() - ph.wtask0;
+    if (ph.ntasks || ph.nfin || ph.nwtasks) P.phases.push_back(ph);
 }
 
 // The data-independent recursion plan of every set
 // (src/spatial_sort.cpp:95-137).
-Plan make_plan(const int64_t* off, int64_t n_sets) {
+Plan make_plan(const int64_t* off, int64_t n_sets, int64_t wg_min) {
     Plan P;
     std::vector<Node> level;
     auto place = [&](const Node& nd, std::vector<Node>& next) {
@@ -1517,9 +1829,9 @@ Plan make_plan(const int64_t* off, int64_t n_sets) {
             place(Node{m6, m7, y, !uy, uz, !ux}, next);
             place(Node{m7, m8, z, !uz, !ux, uy}, next);
         }
-        add_phase(P, px);
-        add_phase(P, py);
-        add_phase(P, pz);
+        add_phase(P, px, wg_min);
+        add_phase(P, py, wg_min);
+        add_phase(P, pz, wg_min);
         level.swap(next);
     }
     return P;
@@ -1546,13 +1858,14 @@ void sort_sets(const double* xyz, int64_t n_pts, const int64_t* poff, const int6
     const Shuffle T = build_shuffle(K, s, sc, err);
 
     // the plan and the set tables, uploaded in one copy
-    const Plan P = make_plan(off, n_sets);
+    const Plan P = make_plan(off, n_sets, cus);
     const size_t b_off = 0;
     const size_t b_poff = b_off + align_up_bytes<int64_t>(n_sets + 1);
     const size_t b_tasks = b_poff + align_up_bytes<int64_t>(n_sets + 1);
     const size_t b_tiles = b_tasks + align_up_bytes<RTask>(P.tasks.size());
     const size_t b_fins = b_tiles + align_up_bytes<RTile>(P.tiles.size());
-    const size_t b_leaves = b_fins + align_up_bytes<FTask>(P.fins.size());
+    const size_t b_wtasks = b_fins + align_up_bytes<FTask>(P.fins.size());
+    const size_t b_leaves = b_wtasks + align_up_bytes<WTask>(P.wtasks.size());
     const size_t b_end = b_leaves + align_up_bytes<Leaf>(P.leaves.size());
     std::vector<char> blob(b_end, 0);
     std::memcpy(blob.data() + b_off, off, (n_sets + 1) * sizeof(int64_t));
@@ -1560,6 +1873,8 @@ void sort_sets(const double* xyz, int64_t n_pts, const int64_t* poff, const int6
     if (!P.tasks.empty()) std::memcpy(blob.data() + b_tasks, P.tasks.data(), P.tasks.size() * sizeof(RTask));
     if (!P.tiles.empty()) std::memcpy(blob.data() + b_tiles, P.tiles.data(), P.tiles.size() * sizeof(RTile));
     if (!P.fins.empty()) std::memcpy(blob.data() + b_fins, P.fins.data(), P.fins.size() * sizeof(FTask));
+    if (!P.wtasks.empty())
+        std::memcpy(blob.data() + b_wtasks, P.wtasks.data(), P.wtasks.size() * sizeof(WTask));
     if (!P.leaves.empty())
         std::memcpy(blob.data() + b_leaves, P.leaves.data(), P.leaves.size() * sizeof(Leaf));
     char* dblob = sc.get<char>(b_end);
@@ -1569,6 +1884,7 @@ void sort_sets(const double* xyz, int64_t n_pts, const int64_t* poff, const int6
     const RTask* dtasks = reinterpret_cast<const RTask*>(dblob + b_tasks);
     const RTile* dtiles = reinterpret_cast<const RTile*>(dblob + b_tiles);
     const FTask* dfins = reinterpret_cast<const FTask*>(dblob + b_fins);
+    const WTask* dwtasks = reinterpret_cast<const WTask*>(dblob + b_wtasks);
     const Leaf* dleaves = reinterpret_cast<const Leaf*>(dblob + b_leaves);
 
     Bufs bufs;
@@ -1595,6 +1911,7 @@ void sort_sets(const double* xyz, int64_t n_pts, const int64_t* poff, const int6
     uint32_t *tcount = nullptr, *toff = nullptr, *ring = nullptr, *npos = nullptr;
     uint32_t* alist[3] = {nullptr, nullptr, nullptr};  // the tiles of rounds r, r-1 and r-2 (r mod 3)
     uint32_t* acnt = nullptr;
+    if (!P.wtasks.empty()) npos = sc.get<uint32_t>(total);
     if (max_tasks) {
         st[0] = sc.get<RState>(max_tasks);
         st[1] = sc.get<RState>(max_tasks);
@@ -1602,7 +1919,7 @@ void sort_sets(const double* xyz, int64_t n_pts, const int64_t* poff, const int6
         tcount = sc.get<uint32_t>(max_tiles);
         toff = sc.get<uint32_t>(max_tiles);
         wloc = sc.get<uint16_t>(max_tiles * kTileWords);
-        npos = sc.get<uint32_t>(total);
+        if (!npos) npos = sc.get<uint32_t>(total);
         ring = sc.get<uint32_t>(kRing);
         fill_u32(ring, 0u, kRing, s);
         for (auto& l : alist) l = sc.get<uint32_t>(max_tiles);
@@ -1610,6 +1927,11 @@ void sort_sets(const double* xyz, int64_t n_pts, const int64_t* poff, const int6
     }
     for (const Phase& ph : P.phases) {
         const RState* final_state = st[0];
+        if (ph.nwtasks) {
+            k_wg_select<<<(unsigned)ph.nwtasks, kWgThreads, 0, s>>>(bufs, dwtasks + ph.wtask0,
+                                                                   (uint32_t)ph.nwtasks, npos, err);
+            D3_LAUNCH_CHECK();
+        }
         if (ph.ntasks) {
             const RTask* pt = dtasks + ph.task0;
             const RTile* ptl = dtiles + ph.tile0;

@@ -100,6 +100,31 @@ def test_sets_match_individual_sorts():
         assert np.array_equal(got[off[i]:off[i + 1]], O.spatial_sort(sets[i])), n
 
 
+def test_many_mid_size_sets_vs_oracle():
+    """300 sets of 4,097 to 40,000 points in one call: every phase has at
+    least one select per CU, none above 262,144 elements, so each select runs
+    all its rounds in one workgroup (the workgroup tier); sets of tie-heavy
+    grids among them. Each set against the oracle."""
+    rng = np.random.default_rng(81)
+    sizes = rng.integers(4097, 40001, size=300).tolist()
+    sets = []
+    for i, n in enumerate(sizes):
+        p = O.random_cube(n, 100 + i)
+        if i % 3 == 0:
+            p = np.floor(p * 16) / 16  # ties on every axis
+        sets.append(p)
+    off = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
+    order = np.concatenate([np.arange(n, dtype=np.int32) for n in sizes])
+    xyz = D.DeviceArray.from_host(np.vstack(sets))
+    d = D.DeviceArray.from_host(order)
+    say("sort 300 mid-size sets", sum(sizes), "points")
+    D.spatial_sort_sets(xyz, off, d)
+    D.synchronize()
+    got = d.to_host()
+    for i, n in enumerate(sizes):
+        assert np.array_equal(got[off[i]:off[i + 1]], O.spatial_sort(sets[i])), (i, n)
+
+
 def test_ten_million_points_vs_oracle():
     pts = W.c3_points(10_000_000, seed=3)
     say("sort 1e7")
