@@ -630,7 +630,8 @@ def run_c1_trace(args) -> dict:
     replayed: every exact predicate call it makes (recorded from the
     reference's own objects by oracle/_ref/trace_recorder), one mixed batch
     per step, against the recorded results; the reference's time for the same
-    calls, one core, beside it."""
+    calls, one core, beside it. Then the Hilbert sort of the same points on
+    the device against the reference's permutation."""
     import subprocess
     import tempfile
     import del3d_amd as D
@@ -664,6 +665,26 @@ def run_c1_trace(args) -> dict:
         t0 = time.perf_counter()
         O.pred_batch(pts, op[m3], i3, "ref")
         line["cpu_reference_s_3d_calls"] = round(time.perf_counter() - t0, 4)
+    # the Hilbert sort of the same 10,000 points (the build's first step,
+    # src/triangulation.cpp:68), on the device, against the compiled
+    # reference's permutation (the oracle's where it is not built)
+    dord = D.DeviceArray((len(pts),), np.int32)
+    ident = D.DeviceArray.from_host(np.arange(len(pts), dtype=np.int32))
+    H = D._hip._lib()
+
+    def sort_once():
+        D._hip.check(H.hipMemcpyAsync(dord.ptr, ident.ptr, 4 * len(pts), 3, stream.handle), "D2D")
+        D.spatial_sort_device(xyz, dord, stream)
+
+    for _ in range(args.warmup + 1):
+        sort_once()
+    sms = _events_ms(D, stream, sort_once, args.steps)
+    got_order = dord.to_host()
+    t0 = time.perf_counter()
+    want_order = O.spatial_sort(np.ascontiguousarray(pts), None, "ref" if O.have_ref() else "oracle")
+    line["sort_ms"] = round(sms / args.steps, 4)
+    line["sort_cpu_ms"] = round((time.perf_counter() - t0) * 1e3, 3)
+    line["sort_bit_exact"] = bool(np.array_equal(got_order, want_order))
     return line
 
 

@@ -83,12 +83,14 @@ struct del3d_bridge_s {
     int64_t gpu_min = 0;
     double* d_xyz = nullptr;  // uploaded at the first wave that goes to the GPU
     size_t cap = 0;           // queries the wave buffers hold
-    uint8_t* h_op = nullptr;
-    int32_t* h_idx = nullptr;
+    // a wave's records in one block, copied in one transfer: the index
+    // records (20 B per query) followed by the op bytes
+    uint8_t* h_rec = nullptr;
     int8_t* h_out = nullptr;
-    uint8_t* d_op = nullptr;
-    int32_t* d_idx = nullptr;
+    uint8_t* d_rec = nullptr;
     int8_t* d_out = nullptr;
+    uint32_t* d_wide = nullptr;  // the one-pass kernel's wide queue
+    d3::PredCounters* d_ctr = nullptr;
     int64_t info[8] = {};
 };
 
@@ -96,18 +98,18 @@ namespace d3 {
 namespace {
 
 void free_wave_buffers(del3d_bridge_s* b) {
-    if (b->h_op) (void)hipHostFree(b->h_op);
-    if (b->h_idx) (void)hipHostFree(b->h_idx);
+    if (b->h_rec) (void)hipHostFree(b->h_rec);
     if (b->h_out) (void)hipHostFree(b->h_out);
-    if (b->d_op) (void)hipFree(b->d_op);
-    if (b->d_idx) (void)hipFree(b->d_idx);
+    if (b->d_rec) (void)hipFree(b->d_rec);
     if (b->d_out) (void)hipFree(b->d_out);
-    b->h_op = nullptr;
-    b->h_idx = nullptr;
+    if (b->d_wide) (void)hipFree(b->d_wide);
+    if (b->d_ctr) (void)hipFree(b->d_ctr);
+    b->h_rec = nullptr;
     b->h_out = nullptr;
-    b->d_op = nullptr;
-    b->d_idx = nullptr;
+    b->d_rec = nullptr;
     b->d_out = nullptr;
+    b->d_wide = nullptr;
+    b->d_ctr = nullptr;
     b->cap = 0;
 }
 
@@ -117,12 +119,12 @@ void reserve_wave(del3d_bridge_s* b, size_t n) {
     while (cap < n) cap *= 2;
     D3_HIP(hipStreamSynchronize(b->s));
     free_wave_buffers(b);
-    D3_HIP(hipHostMalloc((void**)&b->h_op, cap, hipHostMallocDefault));
-    D3_HIP(hipHostMalloc((void**)&b->h_idx, cap * 20, hipHostMallocDefault));
+    D3_HIP(hipHostMalloc((void**)&b->h_rec, cap * 21, hipHostMallocDefault));
     D3_HIP(hipHostMalloc((void**)&b->h_out, cap, hipHostMallocDefault));
-    D3_HIP(hipMalloc((void**)&b->d_op, cap));
-    D3_HIP(hipMalloc((void**)&b->d_idx, cap * 20));
+    D3_HIP(hipMalloc((void**)&b->d_rec, cap * 21));
     D3_HIP(hipMalloc((void**)&b->d_out, cap));
+    D3_HIP(hipMalloc((void**)&b->d_wide, cap * 4));
+    D3_HIP(hipMalloc((void**)&b->d_ctr, sizeof(PredCounters)));
     b->cap = cap;
 }
 
@@ -158,17 +160,23 @@ void answer(del3d_bridge_s* b, const std::vector<uint8_t>& op, const std::vector
         copy_h2d(b->d_xyz, b->h_xyz, (size_t)b->n_pts * 24, b->s);
     }
     reserve_wave(b, n);
-    std::memcpy(b->h_op, op.data(), n);
-    std::memcpy(b->h_idx, idx.data(), n * 20);
+    std::memcpy(b->h_rec, idx.data(), n * 20);
+    std::memcpy(b->h_rec + n * 20, op.data(), n);
     bool has3 = false, has2 = false;
     for (uint8_t o : op) (o < DEL3D_MIXED_OP2_BASE ? has3 : has2) = true;
-    D3_HIP(hipMemcpyAsync(b->d_op, b->h_op, n, hipMemcpyHostToDevice, b->s));
-    D3_HIP(hipMemcpyAsync(b->d_idx, b->h_idx, n * 20, hipMemcpyHostToDevice, b->s));
-    // the 3D pass writes every query (0 for coplanar op bytes); the coplanar
-    // pass then writes only its own ops
-    if (has3 || !has2) launch_predicates(b->d_xyz, b->n_pts, b->d_op, b->d_idx, (int64_t)n, b->d_out, b->s, nullptr);
-    if (has2) launch_predicates2(b->d_xyz, b->n_pts, b->d_op, b->d_idx, (int64_t)n, b->d_out, b->s,
-                                 DEL3D_MIXED_OP2_BASE);
+    D3_HIP(hipMemcpyAsync(b->d_rec, b->h_rec, n * 21, hipMemcpyHostToDevice, b->s));
+    const int32_t* d_idx = reinterpret_cast<const int32_t*>(b->d_rec);
+    const uint8_t* d_op = b->d_rec + n * 20;
+    if (!has2) {
+        // 3D tests only (every wave of a 3D triangulation but the hull's
+        // rare follow-ups): the one-pass kernel
+        launch_predicates_small(b->d_xyz, b->n_pts, d_op, d_idx, (int64_t)n, b->d_out, b->d_ctr, b->d_wide, b->s);
+    } else {
+        // the 3D pass writes every query (0 for coplanar op bytes); the
+        // coplanar pass then writes only its own ops
+        if (has3) launch_predicates(b->d_xyz, b->n_pts, d_op, d_idx, (int64_t)n, b->d_out, b->s, nullptr);
+        launch_predicates2(b->d_xyz, b->n_pts, d_op, d_idx, (int64_t)n, b->d_out, b->s, DEL3D_MIXED_OP2_BASE);
+    }
     D3_HIP(hipMemcpyAsync(b->h_out, b->d_out, n, hipMemcpyDeviceToHost, b->s));
     D3_HIP(hipStreamSynchronize(b->s));
     std::memcpy(out.data(), b->h_out, n);

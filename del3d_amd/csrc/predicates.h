@@ -128,6 +128,13 @@ constexpr size_t kNoAllocScratchBytes = size_t(32) << 20;
 void launch_predicates_noalloc(const double* xyz, int64_t n_pts, const uint8_t* op, const int32_t* idx,
                                int64_t n_q, int8_t* out, hipStream_t s);
 
+// Small batch of 3D queries (op codes DEL3D_OP_*) in one pass, each lane all
+// tiers (no set summary, no exact queue); queries wider than 4 limbs go to
+// the wide queue `wide` (capacity n_q) and pred_wide. ctr is cleared first.
+// Same results as launch_predicates.
+void launch_predicates_small(const double* xyz, int64_t n_pts, const uint8_t* op, const int32_t* idx,
+                             int64_t n_q, int8_t* out, PredCounters* ctr, uint32_t* wide, hipStream_t s);
+
 // Coplanar batch (predicates2.hip): op codes DEL3D_OP2_* + op_base, idx 5
 // per query (the op's points in argument order), out = the predicate's
 // result. Queries with an op byte below op_base are left untouched (another

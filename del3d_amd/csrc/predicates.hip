@@ -578,6 +578,54 @@ __global__ __launch_bounds__(256) void pred_exact_mid(BatchArgs a) {
 }
 
 // ---------------------------------------------------------------------------
+// Small batches of 3D queries in one pass
+// ---------------------------------------------------------------------------
+// One lane per query runs every tier the batch pipeline runs (the filter, the
+// per-coordinate certificate, exact integers of 1-4 limbs in registers, the
+// perturbation); the rare wider query goes to the wide queue for pred_wide.
+// No grid summary pass, no exact queue: for batches of a few thousand
+// queries, where the pipeline's launches cost more than its work (a driver's
+// conflict-test waves).
+__global__ __launch_bounds__(256) void pred_small(BatchArgs a) {
+    const uint32_t np32 = (uint32_t)min(a.n_pts, (int64_t)0x80000000);
+    const SetGrid none{-100000, 100000};  // no set summary: the per-coordinate certificate only
+    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < a.n_q;
+         q += (int64_t)gridDim.x * blockDim.x) {
+        const uint8_t o = a.op[q];
+        int32_t id[5];
+#pragma unroll
+        for (int k = 0; k < 5; ++k) id[k] = a.idx[5 * q + k];
+        const int nchk = o == DEL3D_OP_ORIENT3D ? 4 : 5;
+        bool valid = o <= DEL3D_OP_IN_SPHERE_UNPERTURBED;
+#pragma unroll
+        for (int k = 0; k < 5; ++k) valid = valid && (k >= nchk || (uint32_t)id[k] < np32);
+        if (!valid) {
+            a.out[q] = 0;
+            continue;
+        }
+        int tier = kTierFilter;
+        bool perturbed = false;
+        int sign = fast_decide<false>(GlobalPts{a.xyz, nullptr, false}, o, id, none, false, tier, perturbed);
+        if (sign == 2) {
+            P3 p[5];
+            for (int k = 0; k < nchk; ++k) p[k] = load_p3(a.xyz, id[k]);
+            if (nchk == 4) p[4] = p[3];
+            const Grid g = grid_of(p, nchk);
+            const int L = limbs_for(g);
+            if (L == 1) sign = exact_query<1>(o, p, id, g.q, perturbed);
+            else if (L == 2) sign = exact_query<2>(o, p, id, g.q, perturbed);
+            else if (L == 3) sign = exact_query<3>(o, p, id, g.q, perturbed);
+            else if (L == 4) sign = exact_query<4>(o, p, id, g.q, perturbed);
+            else {
+                a.wide[atomicAdd(&a.ctr->wide, 1u)] = (uint32_t)q;  // pred_wide writes it
+                continue;
+            }
+        }
+        a.out[q] = (int8_t)sign;
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Exact path, any width, LDS: one query per workgroup, its products spread
 // over the lanes
 // ---------------------------------------------------------------------------
@@ -1089,6 +1137,28 @@ void launch_predicates(const double* xyz, int64_t n_pts, const uint8_t* op, cons
     // global memory by the global one's packed variant for larger sets
     b.pk = (r.packed || r.global_packed) ? sc.get<uint32_t>((size_t)n_pts) : nullptr;
     run_predicates(b, r, xyz, n_pts, op, idx, n_q, out, s, stats_dev, ev, ev2);
+}
+
+void launch_predicates_small(const double* xyz, int64_t n_pts, const uint8_t* op, const int32_t* idx,
+                             int64_t n_q, int8_t* out, PredCounters* ctr, uint32_t* wide, hipStream_t s) {
+    if (n_q <= 0) return;
+    BatchArgs a{};
+    a.xyz = xyz;
+    a.n_pts = n_pts;
+    a.op = op;
+    a.idx = idx;
+    a.out = out;
+    a.n_q = n_q;
+    a.wide = wide;
+    a.ctr = ctr;
+    k_clear_counters<<<1, 64, 0, s>>>(ctr);
+    D3_LAUNCH_CHECK();
+    const unsigned g = (unsigned)std::min<int64_t>((n_q + 255) / 256, (int64_t)device_cu_count() * 8);
+    pred_small<<<g, 256, 0, s>>>(a);
+    D3_LAUNCH_CHECK();
+    const unsigned gw = (unsigned)std::min<int64_t>(n_q, 64);
+    pred_wide<<<gw, kWideBlock, 0, s>>>(a);
+    D3_LAUNCH_CHECK();
 }
 
 // ---------------------------------------------------------------------------

@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 
 #include "../../include/del3d_amd.h"
+#include "../../del3d_amd/csrc/predicates.h"
 
 extern "C" int emu_spatial_sort(const double* xyz, int64_t n_pts, int32_t* order, int64_t n_order) {
     return del3d_spatial_sort_host(xyz, n_pts, order, n_order);
@@ -91,5 +92,25 @@ extern "C" int emu_predicates_noalloc(const double* xyz, int64_t n_pts, const ui
     // stream-ordered, so wait for it (the deferred mode runs kernels only here)
     const int rc = del3d_predicates_noalloc(xyz, n_pts, op, idx, n_q, out, nullptr);
     hipStreamSynchronize(nullptr);
+    return rc;
+}
+
+// The one-pass small-batch kernel (launch_predicates_small, the driver
+// bridge's 3D waves) with its wide-tier tail.
+extern "C" int emu_predicates_small(const double* xyz, int64_t n_pts, const uint8_t* op, const int32_t* idx,
+                                    int64_t n_q, int8_t* out) {
+    d3::PredCounters* ctr = nullptr;
+    uint32_t* wide = nullptr;
+    if (hipMalloc((void**)&ctr, sizeof(d3::PredCounters)) != hipSuccess) return 1;
+    if (hipMalloc((void**)&wide, (size_t)n_q * 4 + 4) != hipSuccess) return 1;
+    int rc = 0;
+    try {
+        d3::launch_predicates_small(xyz, n_pts, op, idx, n_q, out, ctr, wide, nullptr);
+        if (hipDeviceSynchronize() != hipSuccess) rc = 1;
+    } catch (...) {
+        rc = 1;
+    }
+    (void)hipFree(ctr);
+    (void)hipFree(wide);
     return rc;
 }

@@ -57,6 +57,40 @@ extern "C" int emu_plan_sets(const double* xyz, const int64_t* pt_off, int64_t n
                              int8_t* out, int glob_q, int binned);
 extern "C" int emu_predicates_noalloc(const double* xyz, int64_t n_pts, const uint8_t* op,
                                       const int32_t* idx, int64_t n_q, int8_t* out);
+extern "C" int emu_predicates_small(const double* xyz, int64_t n_pts, const uint8_t* op,
+                                    const int32_t* idx, int64_t n_q, int8_t* out);
+
+// The one-pass small-batch kernel against the integer oracle.
+static int check_small(const char* name, const std::vector<double>& xyz, int64_t nq) {
+    const int64_t n = (int64_t)xyz.size() / 3;
+    std::vector<uint8_t> op(nq);
+    std::vector<int32_t> idx(5 * nq);
+    for (int64_t q = 0; q < nq; ++q) {
+        op[q] = (uint8_t)(rnd() % 3);
+        for (int k = 0; k < 5; ++k) idx[5 * q + k] = (int32_t)(rnd() % n);
+        if (q % 7 == 0) idx[5 * q + 4] = idx[5 * q];
+        if (q % 97 == 0) idx[5 * q + 2] = (int32_t)n;  // an index outside the set: sign 0
+        if (q % 89 == 0) op[q] = 9;                     // an unknown op: sign 0
+    }
+    std::vector<int8_t> got(nq, 7), want(nq);
+    const int rc = emu_predicates_small(xyz.data(), n, op.data(), idx.data(), nq, got.data());
+    for (int64_t q = 0; q < nq; ++q) {
+        bool bad = op[q] > 2;
+        for (int k = 0; k < (op[q] == 0 ? 4 : 5); ++k) bad = bad || idx[5 * q + k] >= n;
+        if (bad) {
+            op[q] = 0;  // the oracle's sign is replaced below
+            for (int k = 0; k < 5; ++k) idx[5 * q + k] = 0;
+        }
+    }
+    orc_int_pred_batch(xyz.data(), n, op.data(), idx.data(), nq, want.data());
+    bool ok = rc == 0;
+    for (int64_t q = 0; q < nq && ok; ++q) {
+        const bool bad = q % 97 == 0 || q % 89 == 0;
+        ok = bad ? got[q] == 0 : got[q] == want[q];
+    }
+    std::printf("small %s %s\n", name, ok ? "ok" : "MISMATCH");
+    return ok ? 0 : 1;
+}
 
 // The allocation-free form against the integer oracle, on consecutive calls
 // sharing its fixed scratch (the second on a smaller set).
@@ -458,6 +492,10 @@ int main(int argc, char** argv) {
                 }
         fails += check_predicates("lattice34 (global packed)", lat34, 3000);
         fails += check_predicates("lattice34+1000 (float copy)", lat34f, 3000);
+        fails += check_small("lattice", lat, 3000);
+        fails += check_small("wide", wide, 1500);
+        fails += check_small("mid", mid, 3000);
+        fails += check_small("grid48", q48, 2000);
         fails += check_noalloc("lattice34 (global packed)", lat34, 4000);
         fails += check_noalloc("lattice20 (packed)", lat20, 3000);
         fails += check_noalloc("grid48 (LDS)", q48, 2000);
