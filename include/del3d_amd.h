@@ -365,9 +365,10 @@ int del3d_predicates_mixed(const double* xyz, int64_t n_pts, const uint8_t* op,
  * each view its point_base.
  *
  *   del3d_bridge_create     over n_pts host points (3 doubles each, kept by
- *                           pointer: the array must outlive the bridge;
- *                           uploaded to the current device at the first GPU
- *                           wave). A wave of fewer than gpu_min_queries tests
+ *                           pointer: the array must outlive the bridge and
+ *                           must not change while it lives, since it is
+ *                           uploaded to the current device once, at the
+ *                           first GPU wave). A wave of fewer than gpu_min_queries tests
  *                           is answered on the host with the kernels'
  *                           per-query code (0 = every wave on the GPU;
  *                           INT64_MAX = host only, which needs no device).
