@@ -71,12 +71,72 @@ __device__ inline void wave_count(bool flag, unsigned long long* slot) {
     if (m && __lane_id() == __ffsll((long long)m) - 1) atomicAdd(slot, (unsigned long long)__popcll(m));
 }
 
+// Orders this wavefront's LDS accesses (a buffer written by some lanes and
+// read by others) without a workgroup barrier: a wavefront's LDS
+// instructions execute in order, so this only stops the compiler from
+// moving them.
+__device__ inline void wave_lds_order() { __builtin_amdgcn_wave_barrier(); }
+
+// A wavefront's entries for the exact queue, gathered in its own LDS buffer
+// of kCap entries and appended to the queue in blocks of at least kCap/2:
+// an atomic on the queue's counter costs about as much as a query's whole
+// filter when every wavefront of the device adds to that one address
+// (device-wide atomics are done in memory, one at a time), so one per block
+// instead of one per wavefront step. flush() before the wavefront ends.
+template <int kCap>
+struct WaveQueue {
+    uint32_t* buf;  // kCap entries in LDS
+    uint32_t n;     // entries held (the same in every lane)
+    __device__ void flush(uint32_t* counter, uint32_t* list) {
+        if (n == 0) return;
+        wave_lds_order();
+        const int lane = __lane_id();
+        uint32_t base = 0;
+        if (lane == 0) base = atomicAdd(counter, n);
+        base = __builtin_amdgcn_readfirstlane(__shfl(base, 0));
+        for (uint32_t i = (uint32_t)lane; i < n; i += 64) list[base + i] = buf[i];
+        wave_lds_order();  // the reads come before the next entries' writes
+        n = 0;
+    }
+    __device__ void push(bool want, uint32_t q, uint32_t* counter, uint32_t* list) {
+        const uint64_t m = __ballot(want);
+        if (m == 0) return;
+        if (want) buf[n + (uint32_t)__popcll(m & lanemask_lt())] = q;
+        n += (uint32_t)__popcll(m);  // n < kCap/2 before, so at most kCap/2 + 64
+        if (n >= (uint32_t)kCap / 2) flush(counter, list);
+    }
+};
+
+// A wavefront's statistics counts (stats slots 0, 1, 4, 5), added to the
+// device counters once, by flush().
+struct WaveStats {
+    uint64_t c[4] = {0, 0, 0, 0};
+    __device__ void count(int k, bool flag) { c[k] += (uint64_t)__popcll(__ballot(flag)); }
+    __device__ void flush(unsigned long long* stats) {
+        const int slot[4] = {0, 1, 4, 5};
+        if (__lane_id() == 0) {
+#pragma unroll
+            for (int k = 0; k < 4; ++k)
+                if (c[k]) atomicAdd(&stats[slot[k]], (unsigned long long)c[k]);
+        }
+    }
+};
+
 // A value every lane holds equally, moved to scalar registers.
 __device__ inline int64_t uniform64(int64_t v) {
     const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)(uint64_t)v);
     const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)((uint64_t)v >> 32));
     return (int64_t)(((uint64_t)hi << 32) | lo);
 }
+
+// Pointers the compiler knows are in global memory, so that loads through
+// them are global_load rather than flat_load (a flat load also counts as an
+// LDS access, and every wait for LDS data would wait for it too).
+#if defined(__HIP_DEVICE_COMPILE__)
+#define D3_GLOBAL __attribute__((address_space(1)))
+#else
+#define D3_GLOBAL
+#endif
 
 // Point fetch from global memory (set base already applied) or from LDS.
 // use32 (uniform per task): the set's float copy, one 16-byte load per point
@@ -87,10 +147,11 @@ struct GlobalPts {
     bool use32;
     __device__ P3 operator()(int32_t i) const {
         if (use32) {
-            const F4 v = p32[i];
+            const F4 v = *(const D3_GLOBAL F4*)(p32 + i);
             return P3{(double)v.x, (double)v.y, (double)v.z};
         }
-        return load_p3(p, i);
+        const D3_GLOBAL double* q = (const D3_GLOBAL double*)(p + 3 * (int64_t)i);
+        return P3{q[0], q[1], q[2]};
     }
 };
 struct LdsPts {
@@ -203,43 +264,153 @@ __device__ inline PredTask task_at(const BatchArgs& a, uint32_t t) {
     return r;
 }
 
-// Body of the fast kernels. kMode: where the points come from (kGlobal:
-// gathers from global memory; kLdsF64: the workgroup loads each task's point
-// set into LDS first; kLdsPacked: the same with the set's packed copy;
-// kGlobalPacked: gathers of the packed copy's words from global memory).
-// Workgroups take tasks dynamically (one atomic per task).
+// Where a fast kernel's points come from. kGlobal: gathers from global
+// memory; kLdsF64: the workgroup loads each task's point set into LDS first;
+// kLdsPacked: the same with the set's packed copy; kGlobalPacked: gathers of
+// the packed copy's words from global memory.
 enum { kGlobal = 0, kLdsF64 = 1, kLdsPacked = 2, kGlobalPacked = 3 };
 
-template <bool kStats, int kMode>
-struct BatchLds {
-    static constexpr bool kLds = kMode == kLdsF64 || kMode == kLdsPacked;
-    static constexpr int kT = kLds ? kLdsThreads : kGlobalThreads;  // queries per tile
-    int32_t idx[kT * 5];
-    uint8_t op[kT];
-    uint16_t perm[kT];
-    uint32_t wave_ins[kT / 256];  // in_sphere count of each wavefront, one byte each
+// The per-task constants every query of a task shares.
+struct TaskView {
+    uint32_t np32;        // indices below this are inside the set
+    SetGrid sg;           // the set's grid summary
+    bool cert_first;      // set-level T1 certificate first (set_certifiable)
+    const double* setpts; // the set's points in global memory
+    const F4* set_pts32;  // their float copy, or nullptr
+    bool use32;
+    const double* s_pts;  // the LDS copy (kLdsF64)
+    const uint32_t* s_pk; // the LDS packed copy (kLdsPacked)
+};
+
+// One query: index and op checks, the fast tiers, then the sign or a queue
+// entry for the exact tiers. Every lane of the wavefront calls it together
+// (act: the lane holds a query).
+template <bool kStats, int kMode, class Queue>
+__device__ __forceinline__ void decide_query(const BatchArgs& a, const TaskView& tv, bool act, int64_t q,
+                                             uint8_t o, const int32_t id[5], Queue& wq, WaveStats& ws) {
+    const int nchk = (o == DEL3D_OP_ORIENT3D) ? 4 : 5;
+    bool valid = o <= DEL3D_OP_IN_SPHERE_UNPERTURBED;
+#pragma unroll
+    for (int k = 0; k < 5; ++k) valid = valid && (k >= nchk || (uint32_t)id[k] < tv.np32);
+
+    int sign = 0, tier = kTierFilter;
+    bool perturbed = false;
+    if (act && valid) {
+        if (kMode == kLdsF64)
+            sign = fast_decide<true>(LdsPts{tv.s_pts}, o, id, tv.sg, tv.cert_first, tier, perturbed);
+        else if (kMode == kLdsPacked) {
+            uint32_t w[5];
+#pragma unroll
+            for (int k = 0; k < 5; ++k) w[k] = k < nchk ? tv.s_pk[id[k]] : 0u;
+            sign = packed_decide(w, o, id, tier, perturbed);
+        } else if (kMode == kGlobalPacked) {
+            uint32_t w[5];
+#pragma unroll
+            for (int k = 0; k < 5; ++k) w[k] = k < nchk ? a.pk[id[k]] : 0u;
+            sign = packed_decide(w, o, id, tier, perturbed);
+        } else {
+            sign = fast_decide<false>(GlobalPts{tv.setpts, tv.set_pts32, tv.use32}, o, id, tv.sg,
+                                      tv.cert_first, tier, perturbed);
+        }
+    }
+    const bool queued = act && valid && sign == 2;
+    if (act && !queued) a.out[q] = (int8_t)sign;
+    wq.push(queued, (uint32_t)q, &a.ctr->queued, a.queue);
+    if (kStats) {
+        ws.count(0, act && valid && !queued && tier == kTierFilter);
+        ws.count(1, act && valid && !queued && tier == kTierCert);
+        ws.count(2, act && perturbed);
+        ws.count(3, act && !valid);
+    }
+}
+
+// Takes the next task for thread 0 and publishes it in s_task: {q0, q1, set,
+// points of the set}, q0 = -1 when none is left.
+__device__ inline void take_task(const BatchArgs& a, uint32_t* next, int64_t* s_task, uint32_t home,
+                                 uint32_t& bin_k, bool binned) {
+    uint32_t t = 0xFFFFFFFFu;
+    if (binned) {
+        // each bin's cursor only grows, so every task of a bin is handed out
+        // once; a bin found empty is not tried again
+        while (bin_k < kBins) {
+            const uint32_t b = (home + bin_k) % kBins;
+            const uint32_t lo = a.bins[b], hi = a.bins[b + 1];
+            const uint32_t i = atomicAdd(&a.ctr->bin_next[b], 1u);
+            if (lo <= hi && i < hi - lo) {
+                t = lo + i;
+                break;
+            }
+            ++bin_k;
+        }
+    } else {
+        t = atomicAdd(next, 1u);
+    }
+    if (t < a.n_tasks) {
+        const PredTask tk = task_at(a, t);
+        int64_t np = a.n_pts;
+        if (a.set_ptr) np = a.set_np[tk.set];
+        else if (a.pt_off) np = a.pt_off[tk.set + 1] - a.pt_off[tk.set];
+        s_task[0] = tk.q0;
+        s_task[1] = tk.q1;
+        s_task[2] = tk.set;
+        s_task[3] = np;
+    } else {
+        s_task[0] = -1;
+    }
+}
+
+// The per-task constants of task (set, np); np already capped to what the
+// kernel's copy of the set holds.
+template <int kMode>
+__device__ __forceinline__ TaskView task_view(const BatchArgs& a, int32_t set, int64_t np, const double* s_pts,
+                                              const uint32_t* s_pk) {
+    TaskView tv;
+    // the same bound for 32-bit compares: an index as uint32 is below np
+    // exactly when it is below min(np, 2^31) (negative ones never are)
+    tv.np32 = (uint32_t)min(np, (int64_t)0x80000000);
+    tv.setpts = set_points(a, set);
+    // no summary: an impossible grid, so only the per-coordinate test runs
+    tv.sg = SetGrid{-100000, 100000};
+    if (a.grid) {
+        tv.sg.gmin = __builtin_amdgcn_readfirstlane(a.grid[2 * set]);
+        tv.sg.mexp = __builtin_amdgcn_readfirstlane(a.grid[2 * set + 1]);
+    }
+    tv.cert_first = set_certifiable(tv.sg);
+    tv.use32 = kMode == kGlobal && a.pts32 != nullptr && set_f32_exact(tv.sg);
+    tv.set_pts32 = tv.use32 ? a.pts32 + (a.pt_off ? a.pt_off[set] : 0) : nullptr;
+    tv.s_pts = s_pts;
+    tv.s_pk = s_pk;
+    return tv;
+}
+
+// ---------------------------------------------------------------------------
+// Global-gather kernels: 256-thread workgroups, several per CU, 256-query
+// tiles staged in LDS and regrouped by op type across the workgroup.
+// ---------------------------------------------------------------------------
+constexpr int kGlobalQBuf = 1024;  // exact-queue buffer entries per wavefront
+struct GlobalLds {
+    uint32_t qbuf[kGlobalThreads / 64][kGlobalQBuf];
+    int32_t idx[kGlobalThreads * 5];
+    uint8_t op[kGlobalThreads];
+    uint16_t perm[kGlobalThreads];
+    uint32_t wave_ins;  // in_sphere count of each wavefront, one byte each
     int64_t task[4];
-    double pts[kMode == kLdsF64 ? 3 * kLdsPts : 1];
-    uint32_t pk[kMode == kLdsPacked ? kPackPts : 1];
 };
 
 template <bool kStats, int kMode>
-__device__ __forceinline__ void pred_batch_body(const BatchArgs& a, BatchLds<kStats, kMode>& L) {
-    constexpr bool kLds = BatchLds<kStats, kMode>::kLds;
-    constexpr bool kPacked = kMode == kLdsPacked || kMode == kGlobalPacked;
-    constexpr int kT = BatchLds<kStats, kMode>::kT;
+__device__ __forceinline__ void pred_global_body(const BatchArgs& a, GlobalLds& L) {
+    constexpr bool kPacked = kMode == kGlobalPacked;
+    constexpr int kT = kGlobalThreads;
     constexpr int kW = kT / 64;
     int32_t* s_idx = L.idx;
     uint8_t* s_op = L.op;
     uint16_t* s_perm = L.perm;
-    uint8_t* s_wave_ins = reinterpret_cast<uint8_t*>(L.wave_ins);
+    uint8_t* s_wave_ins = reinterpret_cast<uint8_t*>(&L.wave_ins);
     int64_t* s_task = L.task;
-    double* s_pts = L.pts;
-    uint32_t* s_pk = L.pk;
 
     // a single set run by both a packed kernel and the global one: the set's
     // summary picks one, the other returns at once (a.grid holds the summary)
-    if (kMode != kLdsF64 && a.pack_route) {
+    if (a.pack_route) {
         const bool packable = set_packable(SetGrid{a.grid[0], a.grid[1]});
         if (packable != kPacked) return;
     }
@@ -247,83 +418,22 @@ __device__ __forceinline__ void pred_batch_body(const BatchArgs& a, BatchLds<kSt
     const int tid = threadIdx.x;
     const int wave = (int)__builtin_amdgcn_readfirstlane((uint32_t)tid >> 6);
     const int lane = __lane_id();
-    int32_t loaded_set = -1;
-    unsigned long long* stats = a.stats;
-    uint32_t* next = &a.ctr->next_task[kLds ? 1 : 0];
     // thread 0's position in the bin order (own bin first), when binned
     const uint32_t home = blockIdx.x % kBins;
     uint32_t bin_k = 0;
+    WaveQueue<kGlobalQBuf> wq{L.qbuf[wave], 0};
+    WaveStats ws;
 
     for (;;) {
         __syncthreads();  // the previous task is done with every LDS array
-        if (tid == 0) {
-            uint32_t t = 0xFFFFFFFFu;
-            if (!kLds && a.bins) {
-                // each bin's cursor only grows, so every task of a bin is
-                // handed out once; a bin found empty is not tried again
-                while (bin_k < kBins) {
-                    const uint32_t b = (home + bin_k) % kBins;
-                    const uint32_t lo = a.bins[b], hi = a.bins[b + 1];
-                    const uint32_t i = atomicAdd(&a.ctr->bin_next[b], 1u);
-                    if (lo <= hi && i < hi - lo) {
-                        t = lo + i;
-                        break;
-                    }
-                    ++bin_k;
-                }
-            } else {
-                t = atomicAdd(next, 1u);
-            }
-            if (t < a.n_tasks) {
-                const PredTask tk = task_at(a, t);
-                int64_t np = a.n_pts;
-                if (a.set_ptr) np = a.set_np[tk.set];
-                else if (a.pt_off) np = a.pt_off[tk.set + 1] - a.pt_off[tk.set];
-                s_task[0] = tk.q0;
-                s_task[1] = tk.q1;
-                s_task[2] = tk.set;
-                s_task[3] = np;
-            } else {
-                s_task[0] = -1;
-            }
-        }
+        if (tid == 0) take_task(a, &a.ctr->next_task[0], s_task, home, bin_k, a.bins != nullptr);
         __syncthreads();
         // the task is the same for every lane: keep it in scalar registers
         const int64_t q0 = uniform64(s_task[0]);
         if (q0 < 0) break;
         const int64_t q1 = uniform64(s_task[1]);
         const int32_t set = (int32_t)uniform64(s_task[2]);
-        // points addressable by this task's indices (the LDS copy holds at most
-        // kLdsPts, the packed one kPackPts)
-        const int64_t np = kMode == kLdsF64      ? min(uniform64(s_task[3]), (int64_t)kLdsPts)
-                           : kMode == kLdsPacked ? min(uniform64(s_task[3]), (int64_t)kPackPts)
-                                                 : uniform64(s_task[3]);
-        // the same bound for 32-bit compares: an index as uint32 is below np
-        // exactly when it is below min(np, 2^31) (negative ones never are)
-        const uint32_t np32 = (uint32_t)min(np, (int64_t)0x80000000);
-        const double* setpts = set_points(a, set);
-        // no summary: an impossible grid, so only the per-coordinate test runs
-        SetGrid sg{-100000, 100000};
-        if (a.grid) {
-            sg.gmin = __builtin_amdgcn_readfirstlane(a.grid[2 * set]);
-            sg.mexp = __builtin_amdgcn_readfirstlane(a.grid[2 * set + 1]);
-        }
-        const bool cert_first = set_certifiable(sg);
-        const bool use32 = kMode == kGlobal && a.pts32 != nullptr && set_f32_exact(sg);
-        const F4* set_pts32 = use32 ? a.pts32 + (a.pt_off ? a.pt_off[set] : 0) : nullptr;
-        if (kMode == kLdsF64 && set != loaded_set) {
-            // np <= kLdsPts is guaranteed by the host, which routes larger sets
-            // to the global kernel
-            const int n3 = 3 * (int)np;
-            for (int i = tid; i < n3; i += kT) s_pts[i] = setpts[i];
-            loaded_set = set;
-        }
-        // the packed kernel runs one set of at most kPackPts points, which the
-        // host and the pack_route test above guarantee
-        if (kMode == kLdsPacked && set != loaded_set) {
-            for (int i = tid; i < (int)np; i += kT) s_pk[i] = a.pk[i];
-            loaded_set = set;
-        }
+        const TaskView tv = task_view<kMode>(a, set, uniform64(s_task[3]), nullptr, nullptr);
 
         // Each thread's share of a tile's records (dwords tid + k*kT of the
         // 5*kT, and op byte tid) is loaded one tile ahead into registers, so
@@ -365,19 +475,13 @@ __device__ __forceinline__ void pred_batch_body(const BatchArgs& a, BatchLds<kSt
 #pragma unroll
             for (int k = 0; k < 5; ++k) s_idx[tid + k * kT] = pf_idx[k];
             if (base + kT < q1) prefetch(base + kT);
-            // sums of the byte counts, four wavefronts per dot product: all of
-            // them, and those of the wavefronts before this one
-            uint32_t ins_before = 0, ins_total = 0;
-#pragma unroll
-            for (int j = 0; j < kW / 4; ++j) {
-                const uint32_t d = L.wave_ins[j];
-                const int below = wave - 4 * j;  // bytes 0 .. below-1 of d come before
-                const uint32_t m = below >= 4 ? 0x01010101u
-                                   : below <= 0 ? 0u
-                                                : (0x01010101u >> (8 * (4 - below)));
-                ins_total = __builtin_amdgcn_udot4(d, 0x01010101u, ins_total, false);
-                ins_before = __builtin_amdgcn_udot4(d, m, ins_before, false);
-            }
+            // sums of the byte counts: all of them, and those of the
+            // wavefronts before this one
+            const uint32_t d = L.wave_ins;
+            const uint32_t m = wave >= 4 ? 0x01010101u : wave <= 0 ? 0u : (0x01010101u >> (8 * (4 - wave)));
+            const uint32_t ins_total = __builtin_amdgcn_udot4(d, 0x01010101u, 0u, false);
+            const uint32_t ins_before = __builtin_amdgcn_udot4(d, m, 0u, false);
+            static_assert(kW == 4, "one dword of byte counts");
             const uint32_t lt = (uint32_t)__popcll(ins_mask & lanemask_lt());
             const uint32_t pos = is_ins ? ins_before + lt
                                         : ins_total + ((uint32_t)tid - (ins_before + lt));
@@ -387,77 +491,225 @@ __device__ __forceinline__ void pred_batch_body(const BatchArgs& a, BatchLds<kSt
 
             const bool act = (uint32_t)tid < cnt;
             const int lq = act ? s_perm[tid] : 0;
-            const int64_t q = base + lq;
-            const uint8_t o = s_op[lq];
             int32_t id[5];
 #pragma unroll
             for (int k = 0; k < 5; ++k) id[k] = s_idx[lq * 5 + k];
-            const int nchk = (o == DEL3D_OP_ORIENT3D) ? 4 : 5;
-            bool valid = o <= DEL3D_OP_IN_SPHERE_UNPERTURBED;
-#pragma unroll
-            for (int k = 0; k < 5; ++k) valid = valid && (k >= nchk || (uint32_t)id[k] < np32);
-
-            int sign = 0, tier = kTierFilter;
-            bool perturbed = false;
-            if (act && valid) {
-                if (kMode == kLdsF64)
-                    sign = fast_decide<true>(LdsPts{s_pts}, o, id, sg, cert_first, tier, perturbed);
-                else if (kMode == kLdsPacked) {
-                    uint32_t w[5];
-#pragma unroll
-                    for (int k = 0; k < 5; ++k) w[k] = k < nchk ? s_pk[id[k]] : 0u;
-                    sign = packed_decide(w, o, id, tier, perturbed);
-                }
-                else if (kMode == kGlobalPacked) {
-                    uint32_t w[5];
-#pragma unroll
-                    for (int k = 0; k < 5; ++k) w[k] = k < nchk ? a.pk[id[k]] : 0u;
-                    sign = packed_decide(w, o, id, tier, perturbed);
-                } else {
-                    sign = fast_decide<false>(GlobalPts{setpts, set_pts32, use32}, o, id, sg, cert_first,
-                                              tier, perturbed);
-                }
-            }
-            const bool queued = act && valid && sign == 2;
-            if (act && !queued) a.out[q] = (int8_t)sign;
-            wave_push(queued, (uint32_t)q, &a.ctr->queued, a.queue);
-            if (kStats) {
-                wave_count(act && valid && !queued && tier == kTierFilter, &stats[0]);
-                wave_count(act && valid && !queued && tier == kTierCert, &stats[1]);
-                wave_count(act && perturbed, &stats[4]);
-                wave_count(act && !valid, &stats[5]);
-            }
+            decide_query<kStats, kMode>(a, tv, act, base + lq, s_op[lq], id, wq, ws);
         }
     }
+    wq.flush(&a.ctr->queued, a.queue);
+    if (kStats) ws.flush(a.stats);
 }
 
-// The two fast kernels. The global one is held to 128 VGPRs (four
-// wavefronts per SIMD) to hide gather latency; the LDS one gets that bound
-// from its 1024-thread workgroup.
+// ---------------------------------------------------------------------------
+// LDS kernels: one 1024-thread workgroup per CU holds a task's point set in
+// LDS; its 16 wavefronts then run independently, with no barrier until the
+// task ends. A wavefront takes chunks of 64*kq consecutive queries (kq of
+// them per lane, 1 <= kq <= 4) from a counter in LDS, regroups each chunk so
+// that in_sphere-type queries fill whole 64-query packs first and orient3d
+// queries the packs after (at most one pack of a chunk mixes both), and runs
+// the packs. The regrouping goes through a per-wavefront table in LDS: the
+// position in the regrouped order of each query is its rank among those of
+// its kind, and the table maps positions back to queries (9 bits) and their
+// op (2 bits; 3 = an unknown op byte). Loads run ahead of the arithmetic: a
+// pack's index records are loaded while the pack before it is decided, and
+// a chunk's op bytes while the chunk before it runs.
+// ---------------------------------------------------------------------------
+constexpr int kLdsWaves = kLdsThreads / 64;
+constexpr int kChunkMax = 256;
+constexpr int kChunkCols = kChunkMax / 64;
+constexpr int kLdsQBuf = 256;  // exact-queue buffer entries per wavefront
+
+template <int kMode>
+struct WaveLds {
+    double pts[kMode == kLdsF64 ? 3 * kLdsPts : 1];
+    uint32_t pk[kMode == kLdsPacked ? kPackPts : 1];
+    uint32_t qbuf[kLdsWaves][kLdsQBuf];
+    uint16_t sel[kLdsWaves][kChunkMax];
+    int64_t task[4];
+    uint32_t next_chunk;
+};
+
+template <bool kStats, int kMode>
+__device__ __forceinline__ void pred_lds_body(const BatchArgs& a, WaveLds<kMode>& L) {
+    constexpr int kCap = kMode == kLdsF64 ? kLdsPts : kPackPts;
+    if (kMode == kLdsPacked && a.pack_route) {
+        if (!set_packable(SetGrid{a.grid[0], a.grid[1]})) return;
+    }
+    const int tid = threadIdx.x;
+    const int wave = (int)__builtin_amdgcn_readfirstlane((uint32_t)tid >> 6);
+    const int lane = __lane_id();
+    uint16_t* sel = L.sel[wave];
+    int32_t loaded_set = -1;
+    uint32_t bin_k = 0;
+    WaveQueue<kLdsQBuf> wq{L.qbuf[wave], 0};
+    WaveStats ws;
+
+    for (;;) {
+        __syncthreads();  // every wavefront is done with the previous task
+        if (tid == 0) {
+            take_task(a, &a.ctr->next_task[1], L.task, 0, bin_k, false);
+            L.next_chunk = 0;
+        }
+        __syncthreads();
+        const int64_t q0 = uniform64(L.task[0]);
+        if (q0 < 0) break;
+        const int64_t q1 = uniform64(L.task[1]);
+        const int32_t set = (int32_t)uniform64(L.task[2]);
+        // the host routes only sets of at most kCap points here
+        const int64_t np = min(uniform64(L.task[3]), (int64_t)kCap);
+        if (set != loaded_set) {
+            if (kMode == kLdsF64) {
+                const double* src = set_points(a, set);
+                const int n3 = 3 * (int)np;
+                for (int i = tid; i < n3; i += kLdsThreads) L.pts[i] = src[i];
+            } else {
+                for (int i = tid; i < (int)np; i += kLdsThreads) L.pk[i] = a.pk[i];
+            }
+            loaded_set = set;
+            __syncthreads();
+        }
+        const TaskView tv = task_view<kMode>(a, set, np, L.pts, L.pk);
+
+        // queries per lane of a chunk: the most that still gives every
+        // wavefront two chunks of the task
+        const int64_t len = q1 - q0;
+        int kq = kChunkCols;
+        while (kq > 1 && len < (int64_t)kLdsWaves * 64 * kq * 2) kq >>= 1;
+        const int csz = 64 * kq;
+
+        // the first query of this wavefront's next chunk, q1 when none is left
+        auto claim = [&]() -> int64_t {
+            uint32_t c = 0;
+            if (lane == 0) c = atomicAdd(&L.next_chunk, 1u);
+            c = __builtin_amdgcn_readfirstlane(__shfl(c, 0));
+            return min(q0 + (int64_t)c * csz, q1);
+        };
+        // The loads below run ahead of the arithmetic and are issued
+        // unconditionally (inactive lanes read a query of the task), so that
+        // every pack issues the same loads in the same order and a pack's
+        // wait for its records does not also wait for the loads after them.
+        // the op bytes of the chunk at b: query 64k + lane in ops[k]
+        uint8_t ops[kChunkCols];
+        auto load_ops = [&](int64_t b) {
+#pragma unroll
+            for (int k = 0; k < kChunkCols; ++k) ops[k] = a.op[min(b + 64 * k + lane, q1 - 1)];
+        };
+        // the table of the chunk at b, from ops; returns its query count
+        auto regroup = [&](int64_t b) -> int {
+            const int cnt = (int)min((int64_t)csz, q1 - b);
+            uint64_t insm[kChunkCols];
+            int n_ins = 0;
+#pragma unroll
+            for (int k = 0; k < kChunkCols; ++k) {
+                insm[k] = 0;
+                if (k < kq) {
+                    insm[k] = __ballot(64 * k + lane < cnt && ops[k] != DEL3D_OP_ORIENT3D);
+                    n_ins += __popcll(insm[k]);
+                }
+            }
+            int before = 0;  // in_sphere-type queries of the columns before k
+#pragma unroll
+            for (int k = 0; k < kChunkCols; ++k) {
+                if (k < kq) {
+                    const int j = 64 * k + lane;
+                    const int lt = before + __popcll(insm[k] & lanemask_lt());
+                    const bool is_ins = (insm[k] >> lane) & 1ull;
+                    const int pos = is_ins ? lt : n_ins + (j - lt);
+                    const uint32_t ob = ops[k] > DEL3D_OP_IN_SPHERE_UNPERTURBED ? 3u : (uint32_t)ops[k];
+                    if (j < cnt) sel[pos] = (uint16_t)((uint32_t)j | (ob << 9));
+                    before += __popcll(insm[k]);
+                }
+            }
+            wave_lds_order();
+            return cnt;
+        };
+        // pack k of the chunk at b (cnt queries): the lane's table entry,
+        // and the loads of its query's index record
+        auto read_pack = [&](int64_t b, int k, int cnt, uint32_t& s, int32_t (&id)[5]) {
+            const bool act = 64 * k + lane < cnt;
+            s = act ? sel[64 * k + lane] : 0u;
+            const int32_t* rec = a.idx + (b + (int64_t)(s & 511u)) * 5;
+#pragma unroll
+            for (int m = 0; m < 5; ++m) id[m] = rec[m];
+        };
+
+        int64_t cb = claim();  // the chunk of the pack being decided
+        if (cb >= q1) continue;
+        load_ops(cb);
+        int ccnt = regroup(cb);
+        int64_t nb = claim();  // the chunk after it
+        load_ops(nb);
+        int k = 0;
+        uint32_t s;
+        int32_t id[5];
+        read_pack(cb, 0, ccnt, s, id);
+        for (;;) {
+            // the pack after this one: its loads go out before this one's work
+            int64_t xb = cb;
+            int xk = k + 1, xcnt = ccnt;
+            bool more = true;
+            if (64 * xk >= ccnt) {
+                if (nb < q1) {
+                    // this chunk's table is read: the next chunk's replaces it
+                    xb = nb;
+                    xk = 0;
+                    xcnt = regroup(nb);
+                    nb = claim();
+                } else {
+                    more = false;
+                    xk = k;  // (loads of this pack again, unused)
+                }
+            }
+            uint32_t ns;
+            int32_t nid[5];
+            read_pack(xb, xk, xcnt, ns, nid);
+            load_ops(nb);
+
+            const bool act = 64 * k + lane < ccnt;
+            const uint32_t ob = s >> 9;
+            decide_query<kStats, kMode>(a, tv, act, cb + (int64_t)(s & 511u),
+                                        ob == 3u ? (uint8_t)0xFF : (uint8_t)ob, id, wq, ws);
+            if (!more) break;
+            cb = xb;
+            k = xk;
+            ccnt = xcnt;
+            s = ns;
+#pragma unroll
+            for (int m = 0; m < 5; ++m) id[m] = nid[m];
+        }
+    }
+    wq.flush(&a.ctr->queued, a.queue);
+    if (kStats) ws.flush(a.stats);
+}
+
 template <bool kStats>
 __global__ __launch_bounds__(kLdsThreads) void pred_batch_lds(BatchArgs a) {
-    __shared__ BatchLds<kStats, kLdsF64> L;
-    pred_batch_body<kStats, kLdsF64>(a, L);
+    __shared__ WaveLds<kLdsF64> L;
+    pred_lds_body<kStats, kLdsF64>(a, L);
 }
 
 template <bool kStats>
 __global__ __launch_bounds__(kLdsThreads) void pred_batch_packed(BatchArgs a) {
-    __shared__ BatchLds<kStats, kLdsPacked> L;
-    pred_batch_body<kStats, kLdsPacked>(a, L);
+    __shared__ WaveLds<kLdsPacked> L;
+    pred_lds_body<kStats, kLdsPacked>(a, L);
 }
 
+// The global kernels are held to 128 VGPRs (four wavefronts per SIMD) to
+// hide gather latency; the LDS ones get that bound from their 1024-thread
+// workgroup.
 template <bool kStats>
 __global__ __launch_bounds__(kGlobalThreads) __attribute__((amdgpu_waves_per_eu(4)))
 void pred_batch_global(BatchArgs a) {
-    __shared__ BatchLds<kStats, kGlobal> L;
-    pred_batch_body<kStats, kGlobal>(a, L);
+    __shared__ GlobalLds L;
+    pred_global_body<kStats, kGlobal>(a, L);
 }
 
 template <bool kStats>
 __global__ __launch_bounds__(kGlobalThreads) __attribute__((amdgpu_waves_per_eu(4)))
 void pred_batch_global_packed(BatchArgs a) {
-    __shared__ BatchLds<kStats, kGlobalPacked> L;
-    pred_batch_body<kStats, kGlobalPacked>(a, L);
+    __shared__ GlobalLds L;
+    pred_global_body<kStats, kGlobalPacked>(a, L);
 }
 
 // ---------------------------------------------------------------------------

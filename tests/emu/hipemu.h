@@ -356,6 +356,12 @@ inline hipError_t hipGraphLaunch(hipGraphExec_t, hipStream_t) { return hipErrorI
 inline hipError_t hipGraphExecDestroy(hipGraphExec_t) { return hipSuccess; }
 inline hipError_t hipGraphDestroy(hipGraph_t) { return hipSuccess; }
 inline uint32_t __builtin_amdgcn_readfirstlane(uint32_t v) { return v; }
+// on the GPU a compiler-only fence for a wavefront's LDS accesses, which the
+// hardware executes in order; here the lanes are threads, so a wavefront barrier
+inline void __builtin_amdgcn_wave_barrier() {
+    emu::Wave& w = cur_wave();
+    w.bar->arrive_and_wait();
+}
 // c + the dot product of the four unsigned bytes of a and b
 inline uint32_t __builtin_amdgcn_udot4(uint32_t a, uint32_t b, uint32_t c, bool) {
     for (int i = 0; i < 32; i += 8) c += ((a >> i) & 0xFFu) * ((b >> i) & 0xFFu);
