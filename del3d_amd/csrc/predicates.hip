@@ -129,15 +129,6 @@ __device__ inline int64_t uniform64(int64_t v) {
     return (int64_t)(((uint64_t)hi << 32) | lo);
 }
 
-// Pointers the compiler knows are in global memory, so that loads through
-// them are global_load rather than flat_load (a flat load also counts as an
-// LDS access, and every wait for LDS data would wait for it too).
-#if defined(__HIP_DEVICE_COMPILE__)
-#define D3_GLOBAL __attribute__((address_space(1)))
-#else
-#define D3_GLOBAL
-#endif
-
 // Point fetch from global memory (set base already applied) or from LDS.
 // use32 (uniform per task): the set's float copy, one 16-byte load per point
 // instead of a 16- and an 8-byte load; the values are the same doubles.
@@ -147,11 +138,12 @@ struct GlobalPts {
     bool use32;
     __device__ P3 operator()(int32_t i) const {
         if (use32) {
-            const F4 v = *(const D3_GLOBAL F4*)(p32 + i);
+            const F4 v = p32[i];
             return P3{(double)v.x, (double)v.y, (double)v.z};
         }
-        const D3_GLOBAL double* q = (const D3_GLOBAL double*)(p + 3 * (int64_t)i);
-        return P3{q[0], q[1], q[2]};
+        // (a flat load: measured faster here than the global form, 0.49 vs
+        // 0.55 ms per 2^24 in_sphere over an L2-resident set)
+        return load_p3(p, i);
     }
 };
 struct LdsPts {
