@@ -1429,6 +1429,7 @@ constexpr int kSub = 1024;
 constexpr int kStack = 64;
 constexpr int kSmall = 64;       // nodes of at most this many elements: one lane each
 constexpr int kSmallCap = 128;   // > kSub / 9: small nodes have more than 8 elements, disjoint
+constexpr int kSubSmall = 320;   // the small form's capacity
 
 struct Leaf {
     int64_t lo, hi;  // [lo, hi)
@@ -1493,10 +1494,17 @@ __device__ inline void child_nodes(int ch[8][6], const int m[9], int x, int y, i
 // lanes never touch the same slots, and the order in which nodes are
 // processed does not matter). A node of at most 64 elements has children of
 // at most 8 (64 -> 32 -> 16 -> 8), so its seven splits finish it.
+//
+// kCap (at most kSub) sizes the LDS arrays: a plan whose leaves are all small
+// (every leaf of a 10^7-point set has at most 268 elements) runs the small
+// form, whose 12 KB of LDS let the VGPRs set the occupancy (3 wavefronts per
+// SIMD) instead of the LDS (1 per SIMD at kSub).
+template <int kCap>
 __global__ __launch_bounds__(64) void k_subtree(Bufs bufs, const Leaf* __restrict__ leaves, uint32_t nleaves,
                                                 int32_t* __restrict__ order, uint32_t* __restrict__ err) {
-    __shared__ double s_key[3][kSub];
-    __shared__ uint16_t s_loc[kSub], s_lp[kSub], s_tmp[kSub];
+    static_assert(kCap <= kSub && kCap / 9 < kSmallCap, "subtree capacity");
+    __shared__ double s_key[3][kCap];
+    __shared__ uint16_t s_loc[kCap], s_lp[kCap], s_tmp[kCap];
     __shared__ int32_t s_stack[kStack][6];
     __shared__ uint32_t s_small[kSmallCap];
     __shared__ uint32_t s_cnt[1];
@@ -1504,7 +1512,7 @@ __global__ __launch_bounds__(64) void k_subtree(Bufs bufs, const Leaf* __restric
     for (uint32_t li = blockIdx.x; li < nleaves; li += gridDim.x) {
         const Leaf nd = leaves[li];
         const int64_t m = nd.hi - nd.lo;
-        if (m < 1 || m > kSub || nd.lo < 0 || nd.hi > bufs.n || nd.x < 0 || nd.x > 2) {
+        if (m < 1 || m > kCap || nd.lo < 0 || nd.hi > bufs.n || nd.x < 0 || nd.x > 2) {
             if (threadIdx.x == 0) raise_err(err, kErrLds);
             continue;
         }
@@ -1726,6 +1734,7 @@ struct Plan {
     std::vector<WTask> wtasks;
     std::vector<Leaf> leaves;
     std::vector<Phase> phases;
+    int64_t max_leaf = 0;  // elements of the largest leaf
 };
 
 // A phase's selects of more than kFin elements run device-wide rounds, all
@@ -1789,7 +1798,10 @@ Plan make_plan(const int64_t* off, int64_t n_sets, int64_t wg_min) {
     auto place = [&](const Node& nd, std::vector<Node>& next) {
         const int64_t m = nd.hi - nd.lo;
         if (m <= 8) return;  // kHilbertLimit: left as it is
-        if (m <= kSub) P.leaves.push_back(Leaf{nd.lo, nd.hi, nd.x, nd.ux, nd.uy, nd.uz});
+        if (m <= kSub) {
+            P.leaves.push_back(Leaf{nd.lo, nd.hi, nd.x, nd.ux, nd.uy, nd.uz});
+            P.max_leaf = std::max(P.max_leaf, m);
+        }
         else next.push_back(nd);
     };
     for (int64_t si = 0; si < n_sets; ++si) {
@@ -2002,7 +2014,10 @@ void sort_sets(const double* xyz, int64_t n_pts, const int64_t* poff, const int6
     D3_LAUNCH_CHECK();
     if (!P.leaves.empty()) {
         const unsigned g = (unsigned)std::min<size_t>(P.leaves.size(), (size_t)cus * 16);
-        k_subtree<<<g, 64, 0, s>>>(bufs, dleaves, (uint32_t)P.leaves.size(), order, err);
+        if (P.max_leaf <= kSubSmall)
+            k_subtree<kSubSmall><<<g, 64, 0, s>>>(bufs, dleaves, (uint32_t)P.leaves.size(), order, err);
+        else
+            k_subtree<kSub><<<g, 64, 0, s>>>(bufs, dleaves, (uint32_t)P.leaves.size(), order, err);
         D3_LAUNCH_CHECK();
     }
     uint32_t herr = 0;

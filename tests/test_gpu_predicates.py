@@ -210,6 +210,48 @@ def test_out_of_range_index_gives_zero():
     assert got[2] == O.pred_batch(pts, op[2:3], idx[2:3], "oracle")[0]
 
 
+def _sphere_points(radius):
+    """Every integer point on the sphere x^2 + y^2 + z^2 = radius^2."""
+    x = np.arange(-radius, radius + 1)
+    X, Y = np.meshgrid(x, x, indexing="ij")
+    r = radius * radius - X * X - Y * Y
+    ok = r >= 0
+    Z = np.sqrt(np.where(ok, r, 0)).astype(np.int64)
+    ok &= Z * Z == r
+    neg = ok & (Z > 0)
+    return np.concatenate([np.stack([X[ok], Y[ok], Z[ok]], 1),
+                           np.stack([X[neg], Y[neg], -Z[neg]], 1)]).astype(np.float64)
+
+
+@pytest.mark.parametrize("radius,kernel", [(625, "lds"), (975, "global")])
+def test_exact_queue_buffers_with_bad_entries(radius, kernel):
+    """Five points of an integer sphere are exactly cospherical and their
+    differences exceed the certified grid, so every in_sphere goes through the
+    exact queue: each wavefront's queue buffer fills and is appended many
+    times, in the LDS kernel (3,750 points) and the global one (9,750).
+    Orient3d queries, unknown op codes and out-of-range indices are mixed in,
+    so the regrouped packs mix kinds; the bad ones give 0 and are counted."""
+    pts = _sphere_points(radius)
+    rng = np.random.default_rng(radius)
+    n = 1 << 17
+    idx = rng.integers(0, len(pts), size=(n, 5)).astype(np.int32)
+    op = np.where(rng.random(n) < 0.8, 1, rng.integers(0, 3, size=n)).astype(np.uint8)
+    bad_op = rng.random(n) < 0.01
+    op[bad_op] = rng.integers(3, 256, size=int(bad_op.sum()))
+    bad_idx = (rng.random(n) < 0.01) & ~bad_op
+    rows = np.nonzero(bad_idx)[0]
+    idx[rows, rng.integers(0, 4, size=len(rows))] = len(pts) + rng.integers(0, 1000, size=len(rows))
+    say("exact queue", kernel, len(pts), "points")
+    got, st = predicates_dev(pts, op, idx, stats=True)
+    bad = bad_op | bad_idx
+    assert st["bad_index"] == int(bad.sum())
+    assert np.all(got[bad] == 0)
+    ok = ~bad
+    want = O.pred_batch(pts, op[ok], idx[ok], "oracle")
+    assert np.array_equal(got[ok], want)
+    assert st["exact_regs"] + st["exact_wide"] > n // 2
+
+
 def _mixed_sets(n_sets, seed, max_pts=12000):
     """Sets of log-uniform size (uniform cube, quantised sphere, jittered
     lattice), 4 queries per point, set-local indices."""
