@@ -1071,12 +1071,17 @@ struct FTask {           // a select finished in LDS
 };
 
 // One workgroup per select: keys of [L, R] into LDS, the quickselect, then
-// every field of the records permuted into buffer 0.
+// every field of the records permuted into buffer 0. kCap (at most kFin)
+// sizes the LDS arrays: a phase whose selects are all known to have at most
+// 2048 or 1024 elements runs a smaller form (28 or 14 KB: five or eleven
+// workgroups per CU instead of two).
+template <int kCap>
 __global__ __launch_bounds__(256) void k_lds_select(Bufs bufs, const FTask* __restrict__ ft, uint32_t nft,
                                                     const RState* __restrict__ st_big, uint32_t nbig,
                                                     uint32_t* __restrict__ err) {
-    __shared__ double s_key[kFin];
-    __shared__ uint16_t s_loc[kFin], s_lp[kFin], s_tmp[kFin];
+    static_assert(kCap <= kFin, "select capacity");
+    __shared__ double s_key[kCap];
+    __shared__ uint16_t s_loc[kCap], s_lp[kCap], s_tmp[kCap];
     __shared__ uint32_t s_cnt[4];
     const uint32_t ti = blockIdx.x;
     if (ti >= nft) return;
@@ -1095,7 +1100,7 @@ __global__ __launch_bounds__(256) void k_lds_select(Bufs bufs, const FTask* __re
         par = S.par;
     }
     const int64_t m = R - L + 1;
-    if (m < 2 || m > kFin || L < 0 || R >= bufs.n || T.nth < L || T.nth > R || (par & ~1)) {
+    if (m < 2 || m > kCap || L < 0 || R >= bufs.n || T.nth < L || T.nth > R || (par & ~1)) {
         if (threadIdx.x == 0) raise_err(err, kErrLds);
         return;
     }
@@ -1725,6 +1730,7 @@ struct Phase {
     size_t fin0 = 0, nfin = 0;      // FTask range
     size_t wtask0 = 0, nwtasks = 0; // WTask range
     int64_t max_big = 0;            // largest big select
+    int64_t max_fin = 0;            // largest select finished in LDS (kFin for a big one's finish)
 };
 
 struct Plan {
@@ -1762,6 +1768,7 @@ void add_phase(Plan& P, const std::vector<Split>& splits, int64_t wg_min) {
         const int64_t nth = sp.b + m / 2;
         if (m <= kFin) {
             P.fins.push_back(FTask{sp.b, sp.e - 1, nth, sp.axis, sp.up, -1, 0});
+            ph.max_fin = std::max(ph.max_fin, m);
             continue;
         }
         if (wg) {
@@ -1781,6 +1788,7 @@ void add_phase(Plan& P, const std::vector<Split>& splits, int64_t wg_min) {
         t.ntiles = (int64_t)(P.tiles.size() - ph.tile0) - t.tile0;
         P.tasks.push_back(t);
         P.fins.push_back(FTask{0, 0, nth, sp.axis, sp.up, local, 0});
+        ph.max_fin = kFin;  // its remaining range, known on the device only
         ph.max_big = std::max(ph.max_big, m);
     }
     ph.ntasks = P.tasks.size() - ph.task0;
@@ -2005,8 +2013,15 @@ void sort_sets(const double* xyz, int64_t n_pts, const int64_t* poff, const int6
             final_state = st[round & 1];
         }
         if (ph.nfin) {
-            k_lds_select<<<(unsigned)ph.nfin, 256, 0, s>>>(bufs, dfins + ph.fin0, (uint32_t)ph.nfin, final_state,
-                                                           (uint32_t)ph.ntasks, err);
+            if (ph.max_fin <= 1024)
+                k_lds_select<1024><<<(unsigned)ph.nfin, 256, 0, s>>>(bufs, dfins + ph.fin0, (uint32_t)ph.nfin,
+                                                                     final_state, (uint32_t)ph.ntasks, err);
+            else if (ph.max_fin <= 2048)
+                k_lds_select<2048><<<(unsigned)ph.nfin, 256, 0, s>>>(bufs, dfins + ph.fin0, (uint32_t)ph.nfin,
+                                                                     final_state, (uint32_t)ph.ntasks, err);
+            else
+                k_lds_select<kFin><<<(unsigned)ph.nfin, 256, 0, s>>>(bufs, dfins + ph.fin0, (uint32_t)ph.nfin,
+                                                                     final_state, (uint32_t)ph.ntasks, err);
             D3_LAUNCH_CHECK();
         }
     }
