@@ -1264,7 +1264,7 @@ void enqueue_batch(const BatchArgs& base, const BatchShape& sh, hipStream_t s, h
         D3_HIP(hipEventRecord(ev[1], s));
         D3_HIP(hipEventRecord(ev[2], s));
     }
-    pred_exact<<<cus * 2, 256, 0, s>>>(base);
+    pred_exact<<<cus * 3, 256, 0, s>>>(base);  // 3 wavefronts per SIMD: its VGPR limit
     D3_LAUNCH_CHECK();
     pred_exact_mid<<<cus, 256, 0, s>>>(base);
     D3_LAUNCH_CHECK();
