@@ -1266,7 +1266,7 @@ void enqueue_batch(const BatchArgs& base, const BatchShape& sh, hipStream_t s, h
     }
     pred_exact<<<cus * 3, 256, 0, s>>>(base);  // 3 wavefronts per SIMD: its VGPR limit
     D3_LAUNCH_CHECK();
-    pred_exact_mid<<<cus, 256, 0, s>>>(base);
+    pred_exact_mid<<<cus * 2, 256, 0, s>>>(base);  // its VGPR limit: 2 wavefronts per SIMD
     D3_LAUNCH_CHECK();
     if (ev) {
         D3_HIP(hipEventRecord(ev[3], s));
