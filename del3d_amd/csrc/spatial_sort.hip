@@ -177,19 +177,25 @@ __global__ __launch_bounds__(1024) void k_scan_one(uint32_t* __restrict__ a, uin
 // draw indices with u >= thr, in draw order, into cand[0 .. cap)
 __global__ __launch_bounds__(kScanThreads) void k_cand_write(const uint32_t* __restrict__ u, uint64_t count,
                                                              uint32_t thr, const uint32_t* __restrict__ blk,
-                                                             uint32_t* __restrict__ cand, uint32_t cap,
+                                                             uint32_t* __restrict__ cand,
+                                                             uint32_t* __restrict__ cand_u, uint32_t cap,
                                                              uint32_t* __restrict__ err) {
     __shared__ uint32_t s_wave[kScanThreads / 64];
     const uint64_t base = (uint64_t)blockIdx.x * kScanTile;
     uint32_t off = blk[blockIdx.x];
     for (int i = 0; i < kScanItems; ++i) {
         const uint64_t t = base + (uint64_t)i * kScanThreads + threadIdx.x;
-        const bool f = t < count && u[t] >= thr;
+        const uint32_t ut = t < count ? u[t] : 0u;
+        const bool f = t < count && ut >= thr;
         uint32_t total;
         const uint32_t ex = wg_exclusive(f ? 1u : 0u, s_wave, &total);
         if (f) {
-            if (off + ex < cap) cand[off + ex] = (uint32_t)t;
-            else raise_err(err, kErrShuffle);
+            if (off + ex < cap) {
+                cand[off + ex] = (uint32_t)t;
+                cand_u[off + ex] = ut;
+            } else {
+                raise_err(err, kErrShuffle);
+            }
         }
         off += total;
     }
@@ -200,11 +206,16 @@ __global__ __launch_bounds__(kScanThreads) void k_cand_write(const uint32_t* __r
 // earlier rejections is found by iterating prefix -> decisions -> prefix to
 // its fixed point (lane i is final after at most i+1 iterations).
 // out[0] = number of rejections; rej[] = rejected draw indices, ascending.
-__global__ __launch_bounds__(64) void k_resolve(const uint32_t* __restrict__ u, uint64_t count,
+// The candidates and their draws (cand, cand_u) are staged through LDS in
+// blocks of kResBlock, so the walk waits for memory once per block rather
+// than once per chunk of 64.
+constexpr uint32_t kResBlock = 2048;
+__global__ __launch_bounds__(64) void k_resolve(const uint32_t* __restrict__ cand_u, uint64_t count,
                                                 const uint32_t* __restrict__ cand,
                                                 const uint32_t* __restrict__ ncand_p, uint32_t cap,
                                                 uint32_t last_step, uint32_t* __restrict__ rej,
                                                 uint32_t* __restrict__ out, uint32_t* __restrict__ err) {
+    __shared__ uint32_t s_t[kResBlock], s_u[kResBlock];
     uint32_t ncand = *ncand_p;
     if (ncand > cap) {
         if (threadIdx.x == 0) raise_err(err, kErrShuffle);
@@ -214,32 +225,41 @@ __global__ __launch_bounds__(64) void k_resolve(const uint32_t* __restrict__ u, 
     const uint64_t lt = (1ull << lane) - 1ull;
     uint32_t s_in = 0;
     bool finished = false;
-    for (uint32_t base = 0; base < ncand && !finished; base += 64) {
-        const uint32_t c = base + lane;
-        const bool valid = c < ncand;
-        const uint32_t t = valid ? cand[c] : 0u;
-        const bool t_ok = t < count;
-        const uint32_t uv = (valid && t_ok) ? u[t] : 0u;
-        uint32_t r = 0;
-        uint64_t m = 0;
-        bool in_range = false;
-        for (int it = 0; it < 66; ++it) {
-            const uint32_t k = t + 1u - (s_in + r);
-            in_range = valid && t_ok && k <= last_step;
-            bool rj = false;
-            if (in_range) rj = (uv / (0x80000000u / (k + 1u))) > k;
-            m = __ballot(rj);
-            const uint32_t rn = (uint32_t)__popcll(m & lt);
-            const bool changed = rn != r;
-            r = rn;
-            if (__ballot(changed) == 0) break;
+    for (uint32_t blk = 0; blk < ncand && !finished; blk += kResBlock) {
+        const uint32_t nb = min(kResBlock, ncand - blk);
+        __syncthreads();  // the previous block's reads are done
+        for (uint32_t i = (uint32_t)lane; i < nb; i += 64) {
+            s_t[i] = cand[blk + i];
+            s_u[i] = cand_u[blk + i];
         }
-        if ((m >> lane) & 1ull) {
-            if (s_in + r < cap) rej[s_in + r] = t;
-            else raise_err(err, kErrShuffle);
+        __syncthreads();
+        for (uint32_t base = 0; base < nb && !finished; base += 64) {
+            const uint32_t c = base + lane;
+            const bool valid = c < nb;
+            const uint32_t t = valid ? s_t[c] : 0u;
+            const bool t_ok = t < count;
+            const uint32_t uv = (valid && t_ok) ? s_u[c] : 0u;
+            uint32_t r = 0;
+            uint64_t m = 0;
+            bool in_range = false;
+            for (int it = 0; it < 66; ++it) {
+                const uint32_t k = t + 1u - (s_in + r);
+                in_range = valid && t_ok && k <= last_step;
+                bool rj = false;
+                if (in_range) rj = (uv / (0x80000000u / (k + 1u))) > k;
+                m = __ballot(rj);
+                const uint32_t rn = (uint32_t)__popcll(m & lt);
+                const bool changed = rn != r;
+                r = rn;
+                if (__ballot(changed) == 0) break;
+            }
+            if ((m >> lane) & 1ull) {
+                if (s_in + r < cap) rej[s_in + r] = t;
+                else raise_err(err, kErrShuffle);
+            }
+            s_in += (uint32_t)__popcll(m);
+            if (__ballot(valid && !in_range) != 0) finished = true;
         }
-        s_in += (uint32_t)__popcll(m);
-        if (__ballot(valid && !in_range) != 0) finished = true;
     }
     if (lane == 0) out[0] = s_in;
 }
@@ -1665,10 +1685,11 @@ Shuffle build_shuffle(uint64_t K, hipStream_t s, Scratch& sc, uint32_t* err) {
         copy_d2h(&ncand, meta, 4, s);
         const uint32_t cap = ncand + 1;
         uint32_t* cand = sc.get<uint32_t>(cap);
+        uint32_t* cand_u = sc.get<uint32_t>(cap);
         uint32_t* rej = sc.get<uint32_t>(cap);
-        k_cand_write<<<(unsigned)nblk, kScanThreads, 0, s>>>(u, D, thr, blk, cand, cap, err);
+        k_cand_write<<<(unsigned)nblk, kScanThreads, 0, s>>>(u, D, thr, blk, cand, cand_u, cap, err);
         D3_LAUNCH_CHECK();
-        k_resolve<<<1, 64, 0, s>>>(u, D, cand, meta, cap, last_step, rej, meta + 1, err);
+        k_resolve<<<1, 64, 0, s>>>(cand_u, D, cand, meta, cap, last_step, rej, meta + 1, err);
         D3_LAUNCH_CHECK();
         uint32_t nrej = 0;
         copy_d2h(&nrej, meta + 1, 4, s);
