@@ -5,22 +5,26 @@
 //
 //  pred_batch_lds     sets of at most kLdsPts points: one 1024-thread
 //                     workgroup per CU holds the whole set in LDS (up to
-//                     132 KB) and walks its queries in 1024-query tiles.
+//                     132 KB); its 16 wavefronts then take chunks of the
+//                     task's queries independently, each regrouping its
+//                     chunk through its own table in LDS.
 //  pred_batch_packed  single sets of kLdsPts+1 .. kPackPts points on a small
 //                     integer grid (pack_point): the same, with the set's
 //                     packed copy (4 bytes a point) in LDS and every query
 //                     decided on the grid (packed_decide).
 //  pred_batch_global  larger sets: 256-thread workgroups gather points from
-//                     global memory (L2 / Infinity Cache / HBM); for a
-//                     single packable set, pred_batch_global_packed gathers
-//                     its 4-byte words and decides on the grid.
-//     All: the tile's index records are staged through LDS with coalesced
-//     dword loads, the tile is regrouped so in_sphere and orient3d queries
-//     occupy separate wavefronts, and each lane runs, in registers, the FP64
-//     filter in the reference's operation order, then for undecided queries
-//     the certified small-grid tier (T1) and, for exactly cospherical
-//     in_sphere queries, the whole symbolic perturbation. Whatever is left is
-//     appended (one atomic per wavefront) to a compacted queue.
+//                     global memory (L2 / Infinity Cache / HBM), with each
+//                     256-query tile's index records staged through LDS with
+//                     coalesced dword loads and regrouped across the
+//                     workgroup; for a single packable set,
+//                     pred_batch_global_packed gathers its 4-byte words and
+//                     decides on the grid.
+//     All: in_sphere and orient3d queries occupy separate wavefronts as far
+//     as possible, and each lane runs, in registers, the FP64 filter in the
+//     reference's operation order, then for undecided queries the certified
+//     small-grid tier (T1) and, for exactly cospherical in_sphere queries,
+//     the whole symbolic perturbation. Whatever is left goes to a compacted
+//     queue, through a per-wavefront buffer in LDS (WaveQueue).
 //  pred_exact         persistent grid over the queue: exact integer arithmetic
 //                     with 1-2 limbs per difference (T2), perturbation
 //                     included; wider exponent spans go to a second queue.
@@ -53,18 +57,6 @@ constexpr int kWideBlock = 64;
 // is counted in stats[5], so a bad index can never make the device read
 // outside the point array.
 __device__ inline uint64_t lanemask_lt() { return __lanemask_lt(); }
-
-// Appends `q` for every lane with `want`; one atomic per wavefront.
-__device__ inline void wave_push(bool want, uint32_t q, uint32_t* counter, uint32_t* list) {
-    const uint64_t m = __ballot(want);
-    if (m == 0) return;
-    const int lane = __lane_id();
-    const int leader = __ffsll((long long)m) - 1;
-    uint32_t base = 0;
-    if (lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(m));
-    base = __shfl(base, leader);
-    if (want) list[base + (uint32_t)__popcll(m & lanemask_lt())] = q;
-}
 
 __device__ inline void wave_count(bool flag, unsigned long long* slot) {
     const uint64_t m = __ballot(flag);
