@@ -1454,7 +1454,7 @@ constexpr int kSub = 1024;
 constexpr int kStack = 64;
 constexpr int kSmall = 64;       // nodes of at most this many elements: one lane each
 constexpr int kSmallCap = 128;   // > kSub / 9: small nodes have more than 8 elements, disjoint
-constexpr int kSubSmall = 320;   // the small form's capacity
+constexpr int kSubCaps[3] = {320, 640, kSub};  // the forms' capacities, one per leaf class
 
 struct Leaf {
     int64_t lo, hi;  // [lo, hi)
@@ -1520,10 +1520,11 @@ __device__ inline void child_nodes(int ch[8][6], const int m[9], int x, int y, i
 // processed does not matter). A node of at most 64 elements has children of
 // at most 8 (64 -> 32 -> 16 -> 8), so its seven splits finish it.
 //
-// kCap (at most kSub) sizes the LDS arrays: a plan whose leaves are all small
-// (every leaf of a 10^7-point set has at most 268 elements) runs the small
-// form, whose 12 KB of LDS let the VGPRs set the occupancy (3 wavefronts per
-// SIMD) instead of the LDS (1 per SIMD at kSub).
+// kCap (at most kSub) sizes the LDS arrays. The plan sorts its leaves into
+// size classes and each class runs the smallest form that holds it: the
+// 320-element form's 12 KB of LDS allow 3-4 wavefronts per SIMD, the
+// 640-element one's 21 KB about 2, the 1,024-element one's 33 KB 1 (every
+// leaf of a 10^7-point set has at most 268 elements).
 template <int kCap>
 __global__ __launch_bounds__(64) void k_subtree(Bufs bufs, const Leaf* __restrict__ leaves, uint32_t nleaves,
                                                 int32_t* __restrict__ order, uint32_t* __restrict__ err) {
@@ -1759,9 +1760,9 @@ struct Plan {
     std::vector<RTile> tiles;
     std::vector<FTask> fins;
     std::vector<WTask> wtasks;
-    std::vector<Leaf> leaves;
+    std::vector<Leaf> leaves;       // by class: leaves[class_end[c-1] .. class_end[c]) fit kSubCaps[c]
     std::vector<Phase> phases;
-    int64_t max_leaf = 0;  // elements of the largest leaf
+    size_t class_end[3] = {0, 0, 0};
 };
 
 // A phase's selects of more than kFin elements run device-wide rounds, all
@@ -1827,10 +1828,7 @@ Plan make_plan(const int64_t* off, int64_t n_sets, int64_t wg_min) {
     auto place = [&](const Node& nd, std::vector<Node>& next) {
         const int64_t m = nd.hi - nd.lo;
         if (m <= 8) return;  // kHilbertLimit: left as it is
-        if (m <= kSub) {
-            P.leaves.push_back(Leaf{nd.lo, nd.hi, nd.x, nd.ux, nd.uy, nd.uz});
-            P.max_leaf = std::max(P.max_leaf, m);
-        }
+        if (m <= kSub) P.leaves.push_back(Leaf{nd.lo, nd.hi, nd.x, nd.ux, nd.uy, nd.uz});
         else next.push_back(nd);
     };
     for (int64_t si = 0; si < n_sets; ++si) {
@@ -1874,6 +1872,18 @@ Plan make_plan(const int64_t* off, int64_t n_sets, int64_t wg_min) {
         add_phase(P, py, wg_min);
         add_phase(P, pz, wg_min);
         level.swap(next);
+    }
+    // leaves by size class (disjoint ranges: their order does not matter)
+    auto cls = [](const Leaf& l) {
+        const int64_t m = l.hi - l.lo;
+        return m <= kSubCaps[0] ? 0 : (m <= kSubCaps[1] ? 1 : 2);
+    };
+    std::stable_sort(P.leaves.begin(), P.leaves.end(),
+                     [&](const Leaf& a, const Leaf& b) { return cls(a) < cls(b); });
+    for (int c = 0; c < 3; ++c) {
+        size_t e = c ? P.class_end[c - 1] : 0;
+        while (e < P.leaves.size() && cls(P.leaves[e]) == c) ++e;
+        P.class_end[c] = e;
     }
     return P;
 }
@@ -2048,12 +2058,13 @@ void sort_sets(const double* xyz, int64_t n_pts, const int64_t* poff, const int6
     }
     k_write_order<<<grid_for(total, 256, cus), 256, 0, s>>>(bufs.id[0], total, order);
     D3_LAUNCH_CHECK();
-    if (!P.leaves.empty()) {
-        const unsigned g = (unsigned)std::min<size_t>(P.leaves.size(), (size_t)cus * 16);
-        if (P.max_leaf <= kSubSmall)
-            k_subtree<kSubSmall><<<g, 64, 0, s>>>(bufs, dleaves, (uint32_t)P.leaves.size(), order, err);
-        else
-            k_subtree<kSub><<<g, 64, 0, s>>>(bufs, dleaves, (uint32_t)P.leaves.size(), order, err);
+    for (int c = 0; c < 3; ++c) {
+        const size_t l0 = c ? P.class_end[c - 1] : 0, nl = P.class_end[c] - l0;
+        if (!nl) continue;
+        const unsigned g = (unsigned)std::min<size_t>(nl, (size_t)cus * 16);
+        if (c == 0) k_subtree<kSubCaps[0]><<<g, 64, 0, s>>>(bufs, dleaves + l0, (uint32_t)nl, order, err);
+        else if (c == 1) k_subtree<kSubCaps[1]><<<g, 64, 0, s>>>(bufs, dleaves + l0, (uint32_t)nl, order, err);
+        else k_subtree<kSubCaps[2]><<<g, 64, 0, s>>>(bufs, dleaves + l0, (uint32_t)nl, order, err);
         D3_LAUNCH_CHECK();
     }
     uint32_t herr = 0;
