@@ -750,9 +750,12 @@ __global__ __launch_bounds__(256) void pred_exact(BatchArgs a) {
             for (int k = 0; k < 5; ++k) id[k] = a.idx[(size_t)q * 5 + k];
             P3 p[5];
             const int np = (o == DEL3D_OP_ORIENT3D) ? 4 : 5;
-            for (int k = 0; k < np; ++k) p[k] = load_p3(pts, id[k]);
-            if (np == 4) p[4] = p[3];
-            const Grid g = grid_of(p, np);
+            // all five slots with static indices (an orient3d query repeats
+            // its last point, which leaves the grid unchanged), so p[] stays
+            // in registers
+#pragma unroll
+            for (int k = 0; k < 5; ++k) p[k] = load_p3(pts, id[k < np ? k : 3]);
+            const Grid g = grid_of(p, 5);
             const int L = limbs_for(g);
             int s = 0;
             done = true;
@@ -792,9 +795,12 @@ __global__ __launch_bounds__(256) void pred_exact_mid(BatchArgs a) {
             for (int k = 0; k < 5; ++k) id[k] = a.idx[(size_t)q * 5 + k];
             P3 p[5];
             const int np = (o == DEL3D_OP_ORIENT3D) ? 4 : 5;
-            for (int k = 0; k < np; ++k) p[k] = load_p3(pts, id[k]);
-            if (np == 4) p[4] = p[3];
-            const Grid g = grid_of(p, np);
+            // all five slots with static indices (an orient3d query repeats
+            // its last point, which leaves the grid unchanged), so p[] stays
+            // in registers
+#pragma unroll
+            for (int k = 0; k < 5; ++k) p[k] = load_p3(pts, id[k < np ? k : 3]);
+            const Grid g = grid_of(p, 5);
             const int L = limbs_for(g);
             int s = 0;
             if (L == 3) {
@@ -844,9 +850,9 @@ __global__ __launch_bounds__(256) void pred_small(BatchArgs a) {
         int sign = fast_decide<false>(GlobalPts{a.xyz, nullptr, false}, o, id, none, false, tier, perturbed);
         if (sign == 2) {
             P3 p[5];
-            for (int k = 0; k < nchk; ++k) p[k] = load_p3(a.xyz, id[k]);
-            if (nchk == 4) p[4] = p[3];
-            const Grid g = grid_of(p, nchk);
+#pragma unroll
+            for (int k = 0; k < 5; ++k) p[k] = load_p3(a.xyz, id[k < nchk ? k : 3]);
+            const Grid g = grid_of(p, 5);  // the repeated point leaves the grid unchanged
             const int L = limbs_for(g);
             if (L == 1) sign = exact_query<1>(o, p, id, g.q, perturbed);
             else if (L == 2) sign = exact_query<2>(o, p, id, g.q, perturbed);
@@ -1256,7 +1262,7 @@ void enqueue_batch(const BatchArgs& base, const BatchShape& sh, hipStream_t s, h
         D3_HIP(hipEventRecord(ev[1], s));
         D3_HIP(hipEventRecord(ev[2], s));
     }
-    pred_exact<<<cus * 3, 256, 0, s>>>(base);  // 3 wavefronts per SIMD: its VGPR limit
+    pred_exact<<<cus * 2, 256, 0, s>>>(base);  // 2 wavefronts per SIMD: its VGPR limit
     D3_LAUNCH_CHECK();
     pred_exact_mid<<<cus * 2, 256, 0, s>>>(base);  // its VGPR limit: 2 wavefronts per SIMD
     D3_LAUNCH_CHECK();
