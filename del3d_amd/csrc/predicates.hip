@@ -1360,10 +1360,27 @@ static void run_predicates(const PredScratch& b, const PredRoute& r, const doubl
     }
 }
 
+// Calls of at most this many queries (without statistics, timing events or
+// a forced path) run the one-pass kernel: no grid-summary prepass and no
+// separate exact launches, whose fixed cost (about 25 us a call) outweighs
+// the pipeline's faster fast path for small calls. Measured per call, the
+// pipeline -> the one-pass kernel: on 10^4 random points, 2^12 queries
+// 42 -> 21 us, 2^16 50 -> 29 us, 2^18 54 -> 38 us, 2^20 92 -> 64 us; on the
+// 8^3 integer lattice (mostly certificate and exact work), 2^12 30 -> 12 us,
+// 2^18 44 -> 19 us, but 2^20 46 -> 49 us and 2^21 50 -> 85 us.
+constexpr int64_t kSmallBatch = int64_t(1) << 19;
+
 void launch_predicates(const double* xyz, int64_t n_pts, const uint8_t* op, const int32_t* idx,
                        int64_t n_q, int8_t* out, hipStream_t s, unsigned long long* stats_dev,
                        hipEvent_t* ev, int force_path, hipEvent_t* ev2) {
     if (n_q <= 0) return;
+    if (n_q <= kSmallBatch && !stats_dev && !ev && !ev2 && force_path == 0) {
+        Scratch sc(s);
+        PredCounters* ctr = sc.get<PredCounters>(1);
+        uint32_t* wide = sc.get<uint32_t>((size_t)n_q);
+        launch_predicates_small(xyz, n_pts, op, idx, n_q, out, ctr, wide, s);
+        return;
+    }
     const PredRoute r = route_for(n_pts, force_path);
     PredScratch b;
     b.chunk = (uint32_t)std::min<int64_t>(n_q, kChunk);

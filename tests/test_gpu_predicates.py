@@ -134,14 +134,20 @@ def test_hard_cases_vs_int_oracle(name):
 
 
 def test_tails_and_empty():
+    """Ragged sizes through both forms a single call can take: the one-pass
+    kernel (a plain call of at most 2^19 queries) and the pipeline (the
+    statistics form, and a plain call above 2^19)."""
     pts = O.lattice(3, 3, 3)
-    for n in (0, 1, 63, 255, 256, 257, 1000):
+    for n in (0, 1, 63, 255, 256, 257, 1000, (1 << 19) + 1):
         rng = np.random.default_rng(n)
         idx = rng.integers(0, len(pts), size=(n, 5)).astype(np.int32)
         op = rng.integers(0, 3, size=n).astype(np.uint8)
         say("tail", n)
+        want = O.pred_batch(pts, op, idx, "oracle")
         got, _ = predicates_dev(pts, op, idx)
-        assert np.array_equal(got, O.pred_batch(pts, op, idx, "oracle"))
+        assert np.array_equal(got, want)
+        got, _ = predicates_dev(pts, op, idx, stats=True)
+        assert np.array_equal(got, want)
 
 
 def test_host_memory_form():
