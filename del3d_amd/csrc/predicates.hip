@@ -140,7 +140,12 @@ struct GlobalPts {
 };
 struct LdsPts {
     const double* p;  // points to the __shared__ array
-    __device__ P3 operator()(int32_t i) const { return P3{p[3 * i], p[3 * i + 1], p[3 * i + 2]}; }
+    // i < kLdsPts < 2^24: the byte offset from the full-rate 24-bit multiply
+    __device__ P3 operator()(int32_t i) const {
+        const double* q = reinterpret_cast<const double*>(reinterpret_cast<const char*>(p) +
+                                                          __umul24((uint32_t)i, 24u));
+        return P3{q[0], q[1], q[2]};
+    }
 };
 
 // The fast tiers of one query: T0 filter, T1 certificate, in-register
@@ -574,10 +579,16 @@ __device__ __forceinline__ void pred_lds_body(const BatchArgs& a, WaveLds<kMode>
         // every pack issues the same loads in the same order and a pack's
         // wait for its records does not also wait for the loads after them.
         // the op bytes of the chunk at b: query 64k + lane in ops[k]
+        // (a uniform base and 32-bit lane offsets: no 64-bit address
+        // arithmetic per lane; past the task's end every lane reads its last
+        // byte)
         uint8_t ops[kChunkCols];
         auto load_ops = [&](int64_t b) {
+            const int64_t bb = min(b, q1 - 1);
+            const uint8_t* ob = a.op + bb;
+            const uint32_t lim = (uint32_t)min(q1 - 1 - bb, (int64_t)(kChunkMax - 1));
 #pragma unroll
-            for (int k = 0; k < kChunkCols; ++k) ops[k] = a.op[min(b + 64 * k + lane, q1 - 1)];
+            for (int k = 0; k < kChunkCols; ++k) ops[k] = ob[min((uint32_t)(64 * k + lane), lim)];
         };
         // the table of the chunk at b, from ops; returns its query count
         auto regroup = [&](int64_t b) -> int {
@@ -613,7 +624,7 @@ __device__ __forceinline__ void pred_lds_body(const BatchArgs& a, WaveLds<kMode>
         auto read_pack = [&](int64_t b, int k, int cnt, uint32_t& s, int32_t (&id)[5]) {
             const bool act = 64 * k + lane < cnt;
             s = act ? sel[64 * k + lane] : 0u;
-            const int32_t* rec = a.idx + (b + (int64_t)(s & 511u)) * 5;
+            const int32_t* rec = a.idx + b * 5 + __umul24(s & 511u, 5u);
 #pragma unroll
             for (int m = 0; m < 5; ++m) id[m] = rec[m];
         };

@@ -140,6 +140,7 @@ inline int __lane_id() { return (int)(threadIdx.x & 63u); }
 inline uint64_t __lanemask_lt() { return (1ull << __lane_id()) - 1ull; }
 inline int __popcll(uint64_t v) { return __builtin_popcountll(v); }
 inline int __ffsll(long long v) { return __builtin_ffsll(v); }
+inline uint32_t __umul24(uint32_t a, uint32_t b) { return (a & 0xFFFFFFu) * (b & 0xFFFFFFu); }
 
 inline emu::Wave& cur_wave() { return emu::tl_block->waves[threadIdx.x >> 6]; }
 
