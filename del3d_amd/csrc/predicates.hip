@@ -1074,11 +1074,16 @@ __global__ __launch_bounds__(256) void k_grid_init(int32_t* grid, uint32_t n_set
         reinterpret_cast<uint32_t*>(ctr)[threadIdx.x] = 0u;
 }
 
-// Each workgroup takes one contiguous range of the global point numbering
-// (set s owns [first[s], first[s+1])), so a thread's points change set only
-// at set boundaries: it accumulates its current set's minimum and maximum in
-// registers, flushes them with atomics when the set changes, and at the end
-// thread 0 merges the workgroup's runs of equal sets (one atomic pair per run).
+// Each workgroup takes one contiguous range of the sets' coordinates, read as
+// a flat array of doubles (set s owns doubles [3*first[s], 3*first[s+1]); the
+// summary does not depend on which coordinate a double is). A thread's
+// doubles change set only at set boundaries: it accumulates its current set's
+// minimum and maximum in registers, flushes them with atomics when the set
+// changes, and at the end thread 0 merges the workgroup's runs of equal sets
+// (one atomic pair per run). A wavefront's loads are consecutive doubles, and
+// each thread issues kGridUnroll of them before it uses any.
+constexpr int kGridUnroll = 4;
+
 __device__ inline void grid_flush(int32_t* grid, int32_t set, int32_t lo, int32_t hi) {
     if (lo <= hi) {
         atomicMin(&grid[2 * set], lo);
@@ -1086,45 +1091,61 @@ __device__ inline void grid_flush(int32_t* grid, int32_t set, int32_t lo, int32_
     }
 }
 
+__device__ inline void grid_acc(double c, int32_t& lo, int32_t& hi) {
+    const Bits d = decompose(c);
+    if (d.m) {
+        lo = min(lo, d.e + __builtin_ctzll(d.m));
+        hi = max(hi, d.e + 63 - __builtin_clzll(d.m));
+    }
+}
+
 __global__ __launch_bounds__(256) void k_grid_points(BatchArgs a, uint32_t n_sets, int64_t total,
                                                      const int64_t* __restrict__ first,
                                                      int32_t* __restrict__ grid) {
     __shared__ int32_t s_lo[256], s_hi[256], s_set[256];
-    const int64_t per = (total + gridDim.x - 1) / gridDim.x;
-    const int64_t b0 = (int64_t)blockIdx.x * per, b1 = min(total, b0 + per);
+    constexpr int64_t kStep = 256 * kGridUnroll;
+    const int64_t total_d = 3 * total;
+    const int64_t per = ((total_d + gridDim.x - 1) / gridDim.x + kStep - 1) / kStep * kStep;
+    const int64_t b0 = (int64_t)blockIdx.x * per, b1 = min(total_d, b0 + per);
     int32_t cur = -1, lo = 0x7FFFFFFF, hi = -0x7FFFFFFF;
-    int64_t cur_end = 0, cur_first = 0;
+    int64_t cur_end = 0, cur_first = 0;  // in doubles
     const double* pts = nullptr;
-    for (int64_t g = b0 + threadIdx.x; g < b1; g += blockDim.x) {
-        if (cur < 0 || g >= cur_end) {
-            grid_flush(grid, cur, lo, hi);
-            lo = 0x7FFFFFFF;
-            hi = -0x7FFFFFFF;
-            if (first) {
-                uint32_t l = cur < 0 ? 0 : (uint32_t)cur, h = n_sets;  // last s with first[s] <= g
-                while (h - l > 1) {
-                    const uint32_t mid = (l + h) >> 1;
-                    if (first[mid] <= g) l = mid;
-                    else h = mid;
-                }
-                cur = (int32_t)l;
-                cur_first = first[l];
-                cur_end = first[l + 1];
-            } else {
-                cur = 0;
-                cur_first = 0;
-                cur_end = total;
-            }
-            pts = set_points(a, cur);
-        }
-        const int64_t local = g - cur_first;
+    for (int64_t d0 = b0 + threadIdx.x; d0 < b1; d0 += kStep) {
+        if (cur >= 0 && d0 + 256 * (kGridUnroll - 1) < min(cur_end, b1)) {
+            // all of this step's doubles lie in the current set
+            double v[kGridUnroll];
 #pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            const Bits d = decompose(pts[3 * local + c]);
-            if (d.m) {
-                lo = min(lo, d.e + __builtin_ctzll(d.m));
-                hi = max(hi, d.e + 63 - __builtin_clzll(d.m));
+            for (int k = 0; k < kGridUnroll; ++k) v[k] = pts[d0 + 256 * k - cur_first];
+#pragma unroll
+            for (int k = 0; k < kGridUnroll; ++k) grid_acc(v[k], lo, hi);
+            continue;
+        }
+        for (int k = 0; k < kGridUnroll; ++k) {
+            const int64_t d = d0 + 256 * k;
+            if (d >= b1) break;
+            if (cur < 0 || d >= cur_end) {
+                grid_flush(grid, cur, lo, hi);
+                lo = 0x7FFFFFFF;
+                hi = -0x7FFFFFFF;
+                if (first) {
+                    const int64_t g = d / 3;
+                    uint32_t l = cur < 0 ? 0 : (uint32_t)cur, h = n_sets;  // last s with first[s] <= g
+                    while (h - l > 1) {
+                        const uint32_t mid = (l + h) >> 1;
+                        if (first[mid] <= g) l = mid;
+                        else h = mid;
+                    }
+                    cur = (int32_t)l;
+                    cur_first = 3 * first[l];
+                    cur_end = 3 * first[l + 1];
+                } else {
+                    cur = 0;
+                    cur_first = 0;
+                    cur_end = total_d;
+                }
+                pts = set_points(a, cur);
             }
+            grid_acc(pts[d - cur_first], lo, hi);
         }
     }
     s_lo[threadIdx.x] = lo;
@@ -1184,20 +1205,32 @@ __global__ __launch_bounds__(256) void k_points_f32(const double* __restrict__ x
             return;
         }
     }
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
-         i += (int64_t)gridDim.x * blockDim.x) {
-        uint32_t set = 0;
-        if (pt_off) {  // last s with pt_off[s] <= i
-            uint32_t lo = 0, hi = n_sets;
+    // Each workgroup takes one contiguous range of points, so a thread's set
+    // changes only at set boundaries (looked up then, from the previous set
+    // on); the points of sets that are not exact in float are skipped whole.
+    const int64_t per = ((total + gridDim.x - 1) / gridDim.x + 255) / 256 * 256;
+    const int64_t b0 = (int64_t)blockIdx.x * per, b1 = min(total, b0 + per);
+    int64_t cur_end = pt_off ? 0 : total;
+    uint32_t set = 0;
+    bool exact = pt_off ? false : set_f32_exact(SetGrid{grid[0], grid[1]});
+    for (int64_t i = b0 + threadIdx.x; i < b1; i += 256) {
+        if (i >= cur_end) {  // pt_off != nullptr here: last s with pt_off[s] <= i
+            uint32_t lo = set, hi = n_sets;
             while (hi - lo > 1) {
                 const uint32_t mid = (lo + hi) >> 1;
                 if (pt_off[mid] <= i) lo = mid;
                 else hi = mid;
             }
             set = lo;
+            cur_end = pt_off[set + 1];
+            exact = set_f32_exact(SetGrid{grid[2 * set], grid[2 * set + 1]});
         }
-        if (set_f32_exact(SetGrid{grid[2 * set], grid[2 * set + 1]}))
-            out[i] = F4{(float)xyz[3 * i], (float)xyz[3 * i + 1], (float)xyz[3 * i + 2], 0.0f};
+        if (!exact) {
+            // the last of this thread's points that precedes the set's end
+            i += (cur_end - 1 - i) / 256 * 256;
+            continue;
+        }
+        out[i] = F4{(float)xyz[3 * i], (float)xyz[3 * i + 1], (float)xyz[3 * i + 2], 0.0f};
     }
 }
 
@@ -1216,7 +1249,8 @@ void enqueue_set_grids(const BatchArgs& a, uint32_t n_sets, int64_t total_pts, i
     D3_LAUNCH_CHECK();
     if (total_pts <= 0) return;
     const int cus = device_cu_count();
-    const unsigned g = (unsigned)std::min<int64_t>((total_pts + 1023) / 1024, (int64_t)cus * 4);
+    const int64_t step = 256 * kGridUnroll;
+    const unsigned g = (unsigned)std::min<int64_t>((3 * total_pts + step - 1) / step, (int64_t)cus * 8);
     k_grid_points<<<g, 256, 0, s>>>(a, n_sets, total_pts, a.set_ptr ? a.pt_first : a.pt_off, grid);
     D3_LAUNCH_CHECK();
 }

@@ -353,6 +353,43 @@ def test_plan_xcd_bins_and_shared_list_agree():
     plan.close()
 
 
+def test_plan_prepass_across_set_boundaries():
+    """The plan's prepasses (each set's grid summary and the float copy of
+    the sets exact in float) read contiguous ranges that cross set
+    boundaries, with empty sets, one-point sets and sets whose coordinates
+    start at odd doubles: every sign equals the oracle's and the single-set
+    calls'."""
+    rng = np.random.default_rng(31)
+    kinds = [(0, "cube"), (5, "quant"), (0, "cube"), (1, "cube"), (341, "quant"), (342, "cube"),
+             (700, "lattice"), (0, "quant"), (6000, "quant"), (3, "lattice"), (9001, "cube"),
+             (40000, "lattice"), (70001, "cube"), (2, "quant"), (20000, "quant")]
+    pts_l, op_l, idx_l = [], [], []
+    for n, kind in kinds:
+        if kind == "cube":
+            p = rng.random((n, 3))
+        elif kind == "quant":  # sphere points on a 2^-10 grid: exact in float
+            v = rng.normal(size=(n, 3))
+            p = np.round(v / np.maximum(np.linalg.norm(v, axis=1, keepdims=True), 1e-300) * 1024) / 1024
+        else:  # integer lattice, jittered on a 2^-10 grid: exact in float
+            m = max(2, int(np.ceil(n ** (1 / 3))))
+            g = np.stack(np.meshgrid(*[np.arange(m)] * 3, indexing="ij"), -1).reshape(-1, 3)[:n]
+            p = g + np.round(rng.random(g.shape) * 8) / 1024
+        nq = 3 * n
+        pts_l.append(np.asarray(p, dtype=np.float64).reshape(n, 3))
+        op_l.append(rng.integers(0, 3, size=nq).astype(np.uint8))
+        idx_l.append(rng.integers(0, max(n, 1), size=(nq, 5)).astype(np.int32))
+    say("prepass sets", [n for n, _ in kinds])
+    got = _plan_signs(pts_l, op_l, idx_l, graph=True, replays=2)
+    assert np.array_equal(got[0], got[1])
+    off = 0
+    for p, o, i in zip(pts_l, op_l, idx_l):
+        nq = len(o)
+        if nq:
+            assert np.array_equal(got[0][off:off + nq], O.pred_batch(p, o, i, "oracle")), len(p)
+            assert np.array_equal(got[0][off:off + nq], predicates_dev(p, o, i)[0]), len(p)
+        off += nq
+
+
 @pytest.mark.parametrize("kind", ["float_exact_lattice", "lattice_2^-120", "lattice_2^-135",
                                   "wide_integers", "random_doubles", "packed_lattice",
                                   "packed_lattice_2^-200"])
