@@ -36,7 +36,7 @@ struct del3d_pred_plan_s {
     uint32_t* wide = nullptr;
     d3::PredCounters* ctr = nullptr;
     int32_t* grid = nullptr;  // per set SetGrid summary, recomputed by every run
-    d3::F4* pts32 = nullptr;  // float copy of the float-exact sets, rewritten by every run
+    d3::F4* pts32 = nullptr;  // float copy of the points (used for float-exact sets), rewritten by every run
     uint32_t queue_cap = 0;
     struct Chunk {
         uint32_t lds0, n_lds, glob0, n_glob;  // task ranges in d_tasks
@@ -190,8 +190,8 @@ void run_plan(del3d_pred_plan_s* p, const double* xyz, const uint8_t* op, const 
         g.xyz = xyz;
         g.pt_off = p->d_pt_off;
         g.ctr = p->ctr;
-        enqueue_set_grids(g, (uint32_t)p->n_sets, p->n_pts, p->grid, s);
-        enqueue_points_f32(xyz, p->d_pt_off, (uint32_t)p->n_sets, p->n_pts, p->grid, p->pts32, s);
+        // the set summaries and, in the same pass, the float copy of the points
+        enqueue_set_grids(g, (uint32_t)p->n_sets, p->n_pts, p->grid, s, p->pts32);
     }
     bool first_chunk = true;
     for (const auto& c : p->chunks) {

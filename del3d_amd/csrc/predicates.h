@@ -100,8 +100,10 @@ struct BatchShape {
 // set only), the packed copy instead when the set is packable.
 void enqueue_points_f32(const double* xyz, const int64_t* pt_off, uint32_t n_sets, int64_t total,
                         const int32_t* grid, F4* out, hipStream_t s, uint32_t* pk = nullptr);
+// With out32 (point numbering a.pt_off, n_pts entries), the same pass also
+// writes the float copy of every point (read only for sets exact in float).
 void enqueue_set_grids(const BatchArgs& a, uint32_t n_sets, int64_t total_pts, int32_t* grid,
-                       hipStream_t s);
+                       hipStream_t s, F4* out32 = nullptr);
 
 // Enqueues the whole pipeline of one batch on s (no host synchronization;
 // capturable into a graph). ev (6 events) brackets fast / exact / wide.

@@ -1074,14 +1074,16 @@ __global__ __launch_bounds__(256) void k_grid_init(int32_t* grid, uint32_t n_set
         reinterpret_cast<uint32_t*>(ctr)[threadIdx.x] = 0u;
 }
 
-// Each workgroup takes one contiguous range of the sets' coordinates, read as
-// a flat array of doubles (set s owns doubles [3*first[s], 3*first[s+1]); the
-// summary does not depend on which coordinate a double is). A thread's
-// doubles change set only at set boundaries: it accumulates its current set's
-// minimum and maximum in registers, flushes them with atomics when the set
-// changes, and at the end thread 0 merges the workgroup's runs of equal sets
-// (one atomic pair per run). A wavefront's loads are consecutive doubles, and
-// each thread issues kGridUnroll of them before it uses any.
+// Each workgroup takes one contiguous range of the global point numbering
+// (set s owns [first[s], first[s+1])), so a thread's points change set only
+// at set boundaries: it accumulates its current set's minimum and maximum in
+// registers, flushes them with atomics when the set changes, and at the end
+// thread 0 merges the workgroup's runs of equal sets (one atomic pair per
+// run). A thread issues the loads of kGridUnroll points before it uses any.
+// With out32 (plans: points numbered by a.pt_off), the same pass writes the
+// float copy of every point: the kernels read it only for a set whose
+// summary says its coordinates are exact in float, and there it equals the
+// doubles.
 constexpr int kGridUnroll = 4;
 
 __device__ inline void grid_flush(int32_t* grid, int32_t set, int32_t lo, int32_t hi) {
@@ -1101,34 +1103,42 @@ __device__ inline void grid_acc(double c, int32_t& lo, int32_t& hi) {
 
 __global__ __launch_bounds__(256) void k_grid_points(BatchArgs a, uint32_t n_sets, int64_t total,
                                                      const int64_t* __restrict__ first,
-                                                     int32_t* __restrict__ grid) {
+                                                     int32_t* __restrict__ grid, F4* __restrict__ out32) {
     __shared__ int32_t s_lo[256], s_hi[256], s_set[256];
     constexpr int64_t kStep = 256 * kGridUnroll;
-    const int64_t total_d = 3 * total;
-    const int64_t per = ((total_d + gridDim.x - 1) / gridDim.x + kStep - 1) / kStep * kStep;
-    const int64_t b0 = (int64_t)blockIdx.x * per, b1 = min(total_d, b0 + per);
+    const int64_t per = ((total + gridDim.x - 1) / gridDim.x + kStep - 1) / kStep * kStep;
+    const int64_t b0 = (int64_t)blockIdx.x * per, b1 = min(total, b0 + per);
     int32_t cur = -1, lo = 0x7FFFFFFF, hi = -0x7FFFFFFF;
-    int64_t cur_end = 0, cur_first = 0;  // in doubles
+    int64_t cur_end = 0, cur_first = 0;
     const double* pts = nullptr;
-    for (int64_t d0 = b0 + threadIdx.x; d0 < b1; d0 += kStep) {
-        if (cur >= 0 && d0 + 256 * (kGridUnroll - 1) < min(cur_end, b1)) {
-            // all of this step's doubles lie in the current set
-            double v[kGridUnroll];
+    for (int64_t g0 = b0 + threadIdx.x; g0 < b1; g0 += kStep) {
+        if (cur >= 0 && g0 + 256 * (kGridUnroll - 1) < min(cur_end, b1)) {
+            // all of this step's points lie in the current set
+            double v[kGridUnroll][3];
 #pragma unroll
-            for (int k = 0; k < kGridUnroll; ++k) v[k] = pts[d0 + 256 * k - cur_first];
+            for (int k = 0; k < kGridUnroll; ++k) {
+                const double* p = pts + 3 * (g0 + 256 * k - cur_first);
+                v[k][0] = p[0];
+                v[k][1] = p[1];
+                v[k][2] = p[2];
+            }
 #pragma unroll
-            for (int k = 0; k < kGridUnroll; ++k) grid_acc(v[k], lo, hi);
+            for (int k = 0; k < kGridUnroll; ++k) {
+                grid_acc(v[k][0], lo, hi);
+                grid_acc(v[k][1], lo, hi);
+                grid_acc(v[k][2], lo, hi);
+                if (out32) out32[g0 + 256 * k] = F4{(float)v[k][0], (float)v[k][1], (float)v[k][2], 0.0f};
+            }
             continue;
         }
         for (int k = 0; k < kGridUnroll; ++k) {
-            const int64_t d = d0 + 256 * k;
-            if (d >= b1) break;
-            if (cur < 0 || d >= cur_end) {
+            const int64_t g = g0 + 256 * k;
+            if (g >= b1) break;
+            if (cur < 0 || g >= cur_end) {
                 grid_flush(grid, cur, lo, hi);
                 lo = 0x7FFFFFFF;
                 hi = -0x7FFFFFFF;
                 if (first) {
-                    const int64_t g = d / 3;
                     uint32_t l = cur < 0 ? 0 : (uint32_t)cur, h = n_sets;  // last s with first[s] <= g
                     while (h - l > 1) {
                         const uint32_t mid = (l + h) >> 1;
@@ -1136,16 +1146,21 @@ __global__ __launch_bounds__(256) void k_grid_points(BatchArgs a, uint32_t n_set
                         else h = mid;
                     }
                     cur = (int32_t)l;
-                    cur_first = 3 * first[l];
-                    cur_end = 3 * first[l + 1];
+                    cur_first = first[l];
+                    cur_end = first[l + 1];
                 } else {
                     cur = 0;
                     cur_first = 0;
-                    cur_end = total_d;
+                    cur_end = total;
                 }
                 pts = set_points(a, cur);
             }
-            grid_acc(pts[d - cur_first], lo, hi);
+            const double* p = pts + 3 * (g - cur_first);
+            const double x = p[0], y = p[1], z = p[2];
+            grid_acc(x, lo, hi);
+            grid_acc(y, lo, hi);
+            grid_acc(z, lo, hi);
+            if (out32) out32[g] = F4{(float)x, (float)y, (float)z, 0.0f};
         }
     }
     s_lo[threadIdx.x] = lo;
@@ -1244,14 +1259,15 @@ void enqueue_points_f32(const double* xyz, const int64_t* pt_off, uint32_t n_set
 }
 
 void enqueue_set_grids(const BatchArgs& a, uint32_t n_sets, int64_t total_pts, int32_t* grid,
-                       hipStream_t s) {
+                       hipStream_t s, F4* out32) {
     k_grid_init<<<(n_sets + 255) / 256, 256, 0, s>>>(grid, n_sets, a.ctr);
     D3_LAUNCH_CHECK();
     if (total_pts <= 0) return;
     const int cus = device_cu_count();
     const int64_t step = 256 * kGridUnroll;
-    const unsigned g = (unsigned)std::min<int64_t>((3 * total_pts + step - 1) / step, (int64_t)cus * 8);
-    k_grid_points<<<g, 256, 0, s>>>(a, n_sets, total_pts, a.set_ptr ? a.pt_first : a.pt_off, grid);
+    const unsigned g = (unsigned)std::min<int64_t>((total_pts + step - 1) / step, (int64_t)cus * 8);
+    k_grid_points<<<g, 256, 0, s>>>(a, n_sets, total_pts, a.set_ptr ? a.pt_first : a.pt_off, grid,
+                                    a.set_ptr ? nullptr : out32);
     D3_LAUNCH_CHECK();
 }
 
