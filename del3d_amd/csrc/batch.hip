@@ -7,8 +7,8 @@
 // kernel (the whole set is loaded into one CU's LDS per task, so tasks are
 // large), larger sets to the global-gather kernel. Tasks are ordered largest
 // first, and workgroups take them dynamically, so the tail is short. Task
-// lists are split into launch chunks of at most kChunk queries (the exact
-// queues' capacity). Running a plan only enqueues kernels: no allocation, no
+// lists are split into launch chunks of at most kPlanChunk queries (the exact
+// queues' capacity, allocated with the plan: 8 bytes per query of a chunk). Running a plan only enqueues kernels: no allocation, no
 // copy, no host synchronization, so a run can be captured into a HIP graph;
 // the plan keeps one instantiated graph per (buffers, stream) and replays it.
 #include <algorithm>
@@ -53,6 +53,12 @@ struct del3d_pred_plan_s {
 namespace d3 {
 namespace {
 
+// Queries per launch sequence of a plan. Each sequence ends with the exact
+// tiers' launches and the fast kernel's tail, so fewer, larger ones are
+// faster: 2^28 (2 GB of queues) puts the 1.1 G queries of 4096 sets of
+// 10^2..2*10^5 points in 5 sequences instead of 17.
+constexpr int64_t kPlanChunk = int64_t(1) << 28;
+
 void default_knobs(int32_t* k) {
     k[0] = kLdsPts;             // lds_max_pts: largest set routed to the LDS kernel
     k[1] = 32 * kLdsThreads;    // lds_task_queries
@@ -81,8 +87,8 @@ void drop_graph(del3d_pred_plan_s* p) {
 void build_tasks(del3d_pred_plan_s* p, hipStream_t s) {
     const int64_t lds_max = std::min<int64_t>(p->knobs[0], kLdsPts);
     // a task never exceeds a launch chunk
-    const int64_t lds_q = std::min<int64_t>(std::max<int64_t>(p->knobs[1], kLdsThreads), kChunk);
-    const int64_t glob_q = std::min<int64_t>(std::max<int64_t>(p->knobs[2], kGlobalThreads), kChunk);
+    const int64_t lds_q = std::min<int64_t>(std::max<int64_t>(p->knobs[1], kLdsThreads), kPlanChunk);
+    const int64_t glob_q = std::min<int64_t>(std::max<int64_t>(p->knobs[2], kGlobalThreads), kPlanChunk);
     const bool binned = p->knobs[4] != 0;
     std::vector<PredTask> lds;
     struct Group {
@@ -111,7 +117,7 @@ void build_tasks(del3d_pred_plan_s* p, hipStream_t s) {
     };
     std::stable_sort(lds.begin(), lds.end(), bigger);
     std::stable_sort(glob.begin(), glob.end(), [](const Group& x, const Group& y) { return x.q > y.q; });
-    // launch chunks: at most kChunk queries each
+    // launch chunks: at most kPlanChunk queries each
     p->chunks.clear();
     std::vector<PredTask> all;
     std::vector<uint32_t> bins;
@@ -121,14 +127,14 @@ void build_tasks(del3d_pred_plan_s* p, hipStream_t s) {
         int64_t q = 0;
         std::vector<PredTask> cl;
         std::vector<Group> cg;
-        while (il < lds.size() && q + (lds[il].q1 - lds[il].q0) <= (int64_t)kChunk) {
+        while (il < lds.size() && q + (lds[il].q1 - lds[il].q0) <= kPlanChunk) {
             q += lds[il].q1 - lds[il].q0;
             cl.push_back(lds[il++]);
         }
         while (ig < glob.size()) {
             const auto& gt = glob[ig].tasks;
             Group part{0, {}};
-            while (it < gt.size() && q + (gt[it].q1 - gt[it].q0) <= (int64_t)kChunk) {
+            while (it < gt.size() && q + (gt[it].q1 - gt[it].q0) <= kPlanChunk) {
                 q += gt[it].q1 - gt[it].q0;
                 part.q += gt[it].q1 - gt[it].q0;
                 part.tasks.push_back(gt[it++]);
@@ -288,7 +294,7 @@ int del3d_pred_plan_create(const int64_t* pt_off, const int64_t* q_off, int64_t 
         D3_HIP(hipMalloc((void**)&p->d_q_off, (n_sets + 1) * 8));
         copy_h2d(p->d_pt_off, pt_off, (n_sets + 1) * 8, s);
         copy_h2d(p->d_q_off, q_off, (n_sets + 1) * 8, s);
-        p->queue_cap = (uint32_t)std::min<int64_t>(std::max<int64_t>(p->n_q, 1), kChunk);
+        p->queue_cap = (uint32_t)std::min<int64_t>(std::max<int64_t>(p->n_q, 1), kPlanChunk);
         D3_HIP(hipMalloc((void**)&p->queue, (size_t)p->queue_cap * 4));
         D3_HIP(hipMalloc((void**)&p->wide, (size_t)p->queue_cap * 4));
         D3_HIP(hipMalloc((void**)&p->ctr, sizeof(PredCounters)));

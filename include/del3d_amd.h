@@ -177,7 +177,10 @@ int del3d_predicates_sets(const double* xyz, const int64_t* pt_off, int64_t n_se
  *
  *   del3d_pred_plan_create    layout as in del3d_predicates_sets; allocates
  *                             the plan's device tables and queues on the
- *                             current device.
+ *                             current device (besides the tables, 16 bytes
+ *                             per point and 8 per query, up to 2^28 queries:
+ *                             a run goes in launch sequences of at most
+ *                             2^28 queries).
  *   del3d_pred_plan_run       enqueues the batch on `stream`.
  *   del3d_pred_plan_run_stats as run, then synchronizes and returns the
  *                             6 counters of del3d_predicates_stats.

@@ -484,6 +484,44 @@ def test_packed_kernel_point_formats(kind):
         assert st["perturbed"] > 1000
 
 
+def test_plan_more_queries_than_one_launch_sequence():
+    """A plan of 2^28 + 2^20 queries, above the 2^28 of one of a plan's launch
+    sequences: an LDS set first, then the global sets largest first, so the
+    second global set is split between the two sequences. Each set's queries
+    are copies of one 2^24 batch: every sign, in a run and a graph replay,
+    equals that batch's single-set signs, which a sample checks against the
+    oracle."""
+    rng = np.random.default_rng(35)
+    m = 34
+    g = np.arange(m, dtype=np.float64)
+    lat = np.stack(np.meshgrid(g, g, g, indexing="ij"), -1).reshape(-1, 3)
+    n = 1 << 24
+    base = rng.integers(0, m - 2, size=(n, 3))
+    ijk = base[:, None, :] + rng.integers(0, 3, size=(n, 5, 3))  # local cells: mostly degenerate
+    sets = [(rng.random((4000, 3)), None, 1), (lat, ((ijk[..., 0] * m + ijk[..., 1]) * m + ijk[..., 2]), 8),
+            (rng.random((9000, 3)), None, 7)]
+    pts_l, op_l, idx_l, want = [], [], [], []
+    for pts, idx, copies in sets:
+        op = rng.integers(0, 3, size=n).astype(np.uint8)
+        if idx is None:
+            idx = rng.integers(0, len(pts), size=(n, 5))
+        idx = np.ascontiguousarray(idx, dtype=np.int32)
+        one, _ = predicates_dev(pts, op, idx)
+        s = rng.choice(n, 1 << 12, replace=False)
+        assert np.array_equal(one[s], O.pred_batch(pts, op[s], idx[s], "oracle"))
+        tail = (1 << 20) if copies == 7 else 0
+        pts_l.append(pts)
+        op_l.append(np.concatenate([op] * copies + [op[:tail]]))
+        idx_l.append(np.concatenate([idx] * copies + [idx[:tail]]))
+        want.append(np.concatenate([one] * copies + [one[:tail]]))
+    want = np.concatenate(want)
+    assert len(want) == (1 << 28) + (1 << 20)
+    say("plan of two launch sequences", len(want), "queries")
+    got = _plan_signs(pts_l, op_l, idx_l, graph=True, replays=2)
+    for r in got:
+        assert np.array_equal(r, want)
+
+
 @pytest.mark.parametrize("kind", ["lattice32", "lattice34", "random9000"])
 def test_more_queries_than_one_launch_sequence(kind):
     """2^26 + 2^18 queries in one call, above the 2^26 of one launch sequence
