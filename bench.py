@@ -5,8 +5,8 @@ Workload (the large configuration): 10^7 points (even indices N(0,1)^3,
 odd indices uniform in [-1,1)^3, seed 3; del3d_amd/workloads.py), the Hilbert
 spatial sort of all of them, and 2^28 random orient3d / in_sphere queries
 (half each) over their indices, generated on the device. A step is one pass
-of the predicate half over all 2^28 device-resident queries (four calls of
-2^26, the library's launch-sequence size); the sort half is timed beside it,
+of the predicate half over all 2^28 device-resident queries (one call: one
+launch sequence of the library); the sort half is timed beside it,
 one del3d_spatial_sort call of all 10^7 points per step.
 
 Prints one JSON line. value = whole-job Gpred/s = queries * steps / timed
@@ -715,7 +715,7 @@ def run_headline(args) -> dict:
     sys.path.insert(0, str(ROOT / "tests"))
     import oracle_lib as O
     n_pts, n_q, seed = 10_000_000, 1 << 28, 3
-    chunk = 1 << 26
+    chunk = 1 << 28  # queries per call (one launch sequence)
     t0 = time.perf_counter()
     pts = W.c3_points(n_pts, seed=seed)
     stream = D.Stream()
@@ -763,7 +763,7 @@ def run_headline(args) -> dict:
     value = n_q * args.steps / (total_ms / 1e3) / 1e9
     kern_avg = float(np.mean(kern_ms))
     bytes_launch = PRED_BYTES_INSPHERE * n_ins + PRED_BYTES_ORIENT * (n_q - n_ins)
-    bytes_launch //= n_q // chunk  # per 2^26-query launch
+    bytes_launch //= n_q // chunk  # per launch
     achieved = bytes_launch / (kern_avg / 1e3) / 1e9
     gceil = gather_ceiling(D, stream, xyz, n_pts, op, idx, n_q, chunk)
     gceil["frac"] = round(gceil["ms_per_launch"] / kern_avg, 4)

@@ -348,8 +348,8 @@ int del3d_predicates_ev(const double* xyz, int64_t n_pts, const uint8_t* op, con
         const int rc = check_pred_args(xyz, n_pts, op, idx, n_q, sign_out);
         if (rc) return rc;
         if (!ev_begin || !ev_end) return set_error(DEL3D_ERR_INVALID, "null event");
-        if (n_q > (int64_t)kChunk)
-            return set_error(DEL3D_ERR_INVALID, "the event form takes at most 2^26 queries per call");
+        if (n_q > (int64_t)kPredChunk)
+            return set_error(DEL3D_ERR_INVALID, "the event form takes at most 2^28 queries per call");
         hipEvent_t ev2[2] = {(hipEvent_t)ev_begin, (hipEvent_t)ev_end};
         if (n_q == 0) {
             D3_HIP(hipEventRecord(ev2[0], (hipStream_t)stream));

@@ -12,7 +12,12 @@ constexpr int kLdsThreads = 1024;   // workgroup of the LDS kernel (one per CU)
 constexpr int kGlobalThreads = 256; // workgroup of the global-gather kernel
 constexpr int kLdsPts = 5632;       // largest point set held in LDS (132 KB)
 constexpr int kPackPts = 33792;     // largest packed point set held in LDS (132 KB, 4 bytes a point)
-constexpr uint32_t kChunk = 1u << 26;  // queries per launch sequence (bounds the queues)
+constexpr uint32_t kChunk = 1u << 26;  // queries per launch sequence of coplanar, circumcentre and served batches
+// 3D queries per launch sequence of one call (bounds its exact queues, 8 B per
+// query): each sequence ends with the exact tiers' launches (about 100 us of
+// latency-bound work however few queries they hold) and a grid pass, so a
+// large call runs best as few sequences as memory allows.
+constexpr uint32_t kPredChunk = 1u << 28;
 constexpr int kStats = 6;           // decision counters (see del3d_predicates_stats)
 // XCDs of the chip. Workgroups are dispatched to the XCDs round-robin
 // (workgroup b runs on XCD b % 8), and each XCD has its own L2, so a plan

@@ -1444,7 +1444,7 @@ void launch_predicates(const double* xyz, int64_t n_pts, const uint8_t* op, cons
     }
     const PredRoute r = route_for(n_pts, force_path);
     PredScratch b;
-    b.chunk = (uint32_t)std::min<int64_t>(n_q, kChunk);
+    b.chunk = (uint32_t)std::min<int64_t>(n_q, kPredChunk);
     Scratch sc(s);
     b.queue = sc.get<uint32_t>(b.chunk);
     b.wide = sc.get<uint32_t>(b.chunk);

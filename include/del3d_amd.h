@@ -131,7 +131,7 @@ int del3d_predicates_timed(const double* xyz, int64_t n_pts, const uint8_t* op,
 
 /*
  * Measurement helper (no counterpart in the reference): as del3d_predicates
- * (at most 2^26 queries), and records the caller's events (hipEvent_t passed
+ * (at most 2^28 queries: one launch sequence), and records the caller's events (hipEvent_t passed
  * as void*) on `stream` immediately before and after the fast-path kernel,
  * without synchronizing, so a timed loop can bracket every launch.
  */

@@ -524,7 +524,7 @@ def test_plan_more_queries_than_one_launch_sequence():
 
 @pytest.mark.parametrize("kind", ["lattice32", "lattice34", "random9000"])
 def test_more_queries_than_one_launch_sequence(kind):
-    """2^26 + 2^18 queries in one call, above the 2^26 of one launch sequence
+    """2^28 + 2^18 queries in one call, above the 2^28 of one launch sequence
     (so two sequences with their own queues and counters), on the 32^3
     lattice (packed LDS kernel), the 34^3 lattice (above the packed LDS
     kernel's capacity: the global kernel's packed variant) and on 9,000
@@ -550,9 +550,9 @@ def test_more_queries_than_one_launch_sequence(kind):
     idx = np.ascontiguousarray(idx, dtype=np.int32)
     one, _ = predicates_dev(pts, op, idx)
     tail = 1 << 18
-    say("two launch sequences", kind, 4 * len(op) + tail, "queries")
-    got, _ = predicates_dev(pts, np.concatenate([op] * 4 + [op[:tail]]),
-                            np.concatenate([idx] * 4 + [idx[:tail]]))
-    assert np.array_equal(got, np.concatenate([one] * 4 + [one[:tail]]))
+    say("two launch sequences", kind, 16 * len(op) + tail, "queries")
+    got, _ = predicates_dev(pts, np.concatenate([op] * 16 + [op[:tail]]),
+                            np.concatenate([idx] * 16 + [idx[:tail]]))
+    assert np.array_equal(got, np.concatenate([one] * 16 + [one[:tail]]))
     s = np.random.default_rng(1).choice(len(op), 1 << 14, replace=False)
     assert np.array_equal(one[s], O.pred_batch(pts, op[s], idx[s], "oracle"))
