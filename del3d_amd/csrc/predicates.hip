@@ -297,9 +297,16 @@ __device__ __forceinline__ void decide_query(const BatchArgs& a, const TaskView&
 #pragma unroll
             for (int k = 0; k < 5; ++k) w[k] = k < nchk ? a.pk[id[k]] : 0u;
             sign = packed_decide(w, o, id, tier, perturbed);
-        } else {
+        } else if (tv.cert_first) {
             sign = fast_decide<false>(GlobalPts{tv.setpts, tv.set_pts32, tv.use32}, o, id, tv.sg,
                                       tv.cert_first, tier, perturbed);
+        } else {
+            // a set the grid summary cannot certify (arbitrary doubles): the
+            // filter rarely fails, so T1 gathers the points again rather than
+            // keeping them live across the filter, which would spill a
+            // coordinate to scratch on every query (the branch is uniform)
+            sign = fast_decide<true>(GlobalPts{tv.setpts, tv.set_pts32, tv.use32}, o, id, tv.sg,
+                                     false, tier, perturbed);
         }
     }
     const bool queued = act && valid && sign == 2;
