@@ -1086,11 +1086,11 @@ __global__ __launch_bounds__(256) void k_grid_init(int32_t* grid, uint32_t n_set
 // at set boundaries: it accumulates its current set's minimum and maximum in
 // registers, flushes them with atomics when the set changes, and at the end
 // thread 0 merges the workgroup's runs of equal sets (one atomic pair per
-// run). A thread issues the loads of kGridUnroll points before it uses any.
-// With out32 (plans: points numbered by a.pt_off), the same pass writes the
-// float copy of every point: the kernels read it only for a set whose
-// summary says its coordinates are exact in float, and there it equals the
-// doubles.
+// run). A thread issues kGridUnroll loads before it uses any: doubles of the
+// flat coordinate array, or, with out32 (plans: points numbered by a.pt_off),
+// whole points, so that the same pass writes the float copy of every point:
+// the kernels read it only for a set whose summary says its coordinates are
+// exact in float, and there it equals the doubles.
 constexpr int kGridUnroll = 4;
 
 __device__ inline void grid_flush(int32_t* grid, int32_t set, int32_t lo, int32_t hi) {
@@ -1113,11 +1113,57 @@ __global__ __launch_bounds__(256) void k_grid_points(BatchArgs a, uint32_t n_set
                                                      int32_t* __restrict__ grid, F4* __restrict__ out32) {
     __shared__ int32_t s_lo[256], s_hi[256], s_set[256];
     constexpr int64_t kStep = 256 * kGridUnroll;
-    const int64_t per = ((total + gridDim.x - 1) / gridDim.x + kStep - 1) / kStep * kStep;
-    const int64_t b0 = (int64_t)blockIdx.x * per, b1 = min(total, b0 + per);
     int32_t cur = -1, lo = 0x7FFFFFFF, hi = -0x7FFFFFFF;
     int64_t cur_end = 0, cur_first = 0;
     const double* pts = nullptr;
+    if (!out32) {
+        // summaries only: the coordinates as a flat array of doubles (the
+        // summary does not depend on which coordinate a double is), so a
+        // wavefront's loads are consecutive doubles; set s owns doubles
+        // [3*first[s], 3*first[s+1])
+        const int64_t total_d = 3 * total;
+        const int64_t per_d = ((total_d + gridDim.x - 1) / gridDim.x + kStep - 1) / kStep * kStep;
+        const int64_t d_b0 = (int64_t)blockIdx.x * per_d, d_b1 = min(total_d, d_b0 + per_d);
+        for (int64_t d0 = d_b0 + threadIdx.x; d0 < d_b1; d0 += kStep) {
+            if (cur >= 0 && d0 + 256 * (kGridUnroll - 1) < min(cur_end, d_b1)) {
+                double v[kGridUnroll];
+#pragma unroll
+                for (int k = 0; k < kGridUnroll; ++k) v[k] = pts[d0 + 256 * k - cur_first];
+#pragma unroll
+                for (int k = 0; k < kGridUnroll; ++k) grid_acc(v[k], lo, hi);
+                continue;
+            }
+            for (int k = 0; k < kGridUnroll; ++k) {
+                const int64_t d = d0 + 256 * k;
+                if (d >= d_b1) break;
+                if (cur < 0 || d >= cur_end) {
+                    grid_flush(grid, cur, lo, hi);
+                    lo = 0x7FFFFFFF;
+                    hi = -0x7FFFFFFF;
+                    if (first) {
+                        const int64_t g = d / 3;
+                        uint32_t l = cur < 0 ? 0 : (uint32_t)cur, h = n_sets;  // last s with first[s] <= g
+                        while (h - l > 1) {
+                            const uint32_t mid = (l + h) >> 1;
+                            if (first[mid] <= g) l = mid;
+                            else h = mid;
+                        }
+                        cur = (int32_t)l;
+                        cur_first = 3 * first[l];
+                        cur_end = 3 * first[l + 1];
+                    } else {
+                        cur = 0;
+                        cur_first = 0;
+                        cur_end = total_d;
+                    }
+                    pts = set_points(a, cur);
+                }
+                grid_acc(pts[d - cur_first], lo, hi);
+            }
+        }
+    }
+    const int64_t per = ((total + gridDim.x - 1) / gridDim.x + kStep - 1) / kStep * kStep;
+    const int64_t b0 = out32 ? (int64_t)blockIdx.x * per : total, b1 = min(total, b0 + per);
     for (int64_t g0 = b0 + threadIdx.x; g0 < b1; g0 += kStep) {
         if (cur >= 0 && g0 + 256 * (kGridUnroll - 1) < min(cur_end, b1)) {
             // all of this step's points lie in the current set
@@ -1272,7 +1318,8 @@ void enqueue_set_grids(const BatchArgs& a, uint32_t n_sets, int64_t total_pts, i
     if (total_pts <= 0) return;
     const int cus = device_cu_count();
     const int64_t step = 256 * kGridUnroll;
-    const unsigned g = (unsigned)std::min<int64_t>((total_pts + step - 1) / step, (int64_t)cus * 8);
+    const int64_t units = out32 && !a.set_ptr ? total_pts : 3 * total_pts;  // points or doubles
+    const unsigned g = (unsigned)std::min<int64_t>((units + step - 1) / step, (int64_t)cus * 8);
     k_grid_points<<<g, 256, 0, s>>>(a, n_sets, total_pts, a.set_ptr ? a.pt_first : a.pt_off, grid,
                                     a.set_ptr ? nullptr : out32);
     D3_LAUNCH_CHECK();
