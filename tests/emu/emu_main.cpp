@@ -442,6 +442,15 @@ static int check_sort_cases() {
     return fails;
 }
 
+// Query counts of the predicate checks, divided by EMU_QUERY_DIV (the
+// ThreadSanitizer run, where every emulated lane is an OS thread, uses 3:
+// still several tiles per workgroup).
+static int nq(int n) {
+    const char* d = std::getenv("EMU_QUERY_DIV");
+    const int k = d ? std::atoi(d) : 1;
+    return k > 1 ? n / k : n;
+}
+
 int main(int argc, char** argv) {
     int fails = 0;
     // the serving loop alone (the ThreadSanitizer run)
@@ -451,14 +460,14 @@ int main(int argc, char** argv) {
         for (int i = 0; i < 5; ++i)
             for (int j = 0; j < 5; ++j)
                 for (int k = 0; k < 5; ++k) lat.insert(lat.end(), {(double)i, (double)j, (double)k});
-        fails += check_predicates("lattice", lat, 3000);
+        fails += check_predicates("lattice", lat, nq(3000));
         std::vector<double> wide;
         for (int i = 0; i < 60; ++i)
             for (int k = 0; k < 3; ++k) {
                 const double m = (double)(int64_t)(rnd() % 2000001) - 1000000.0;
                 wide.push_back(std::ldexp(m + 0.5, (int)(rnd() % 400) - 200));
             }
-        fails += check_predicates("wide", wide, 1500);
+        fails += check_predicates("wide", wide, nq(1500));
         // spans of 2^60 .. 2^119: 3-4 limbs per difference (the register
         // kernel for the wide queue), with near-degenerate queries from
         // near-coplanar and near-cospherical points
@@ -469,17 +478,17 @@ int main(int argc, char** argv) {
             if (i % 2) mid.insert(mid.end(), {u * s, v * s, (u + v) * s});            // on a plane
             else mid.insert(mid.end(), {std::cos(6.3 * u) * s, std::sin(6.3 * u) * s, v * 1e-3});
         }
-        fails += check_predicates("mid", mid, 3000);
+        fails += check_predicates("mid", mid, nq(3000));
         std::vector<double> q48;
         for (int i = 0; i < 200; ++i)
             for (int k = 0; k < 3; ++k) q48.push_back((double)(rnd() >> 16) * 3.552713678800501e-15);
-        fails += check_predicates("grid48", q48, 2000);
+        fails += check_predicates("grid48", q48, nq(2000));
         // above the LDS threshold, packable: the packed LDS kernel
         std::vector<double> lat20;
         for (int i = 0; i < 20; ++i)
             for (int j = 0; j < 20; ++j)
                 for (int k = 0; k < 20; ++k) lat20.insert(lat20.end(), {i * 0.5, j * 0.25, k - 10.0});
-        fails += check_predicates("lattice20 (packed)", lat20, 3000);
+        fails += check_predicates("lattice20 (packed)", lat20, nq(3000));
         // above the packed kernel's capacity, packable: the global kernel's
         // packed variant; moved by 1000, exact in float but too wide to pack:
         // the global kernel's float copy
@@ -490,17 +499,17 @@ int main(int argc, char** argv) {
                     lat34.insert(lat34.end(), {i * 0.5, j * 0.25, k - 10.0});
                     lat34f.insert(lat34f.end(), {i * 0.5 + 1000.0, j * 0.25, k - 10.0});
                 }
-        fails += check_predicates("lattice34 (global packed)", lat34, 3000);
-        fails += check_predicates("lattice34+1000 (float copy)", lat34f, 3000);
-        fails += check_small("lattice", lat, 3000);
-        fails += check_small("wide", wide, 1500);
-        fails += check_small("mid", mid, 3000);
-        fails += check_small("grid48", q48, 2000);
-        fails += check_noalloc("lattice34 (global packed)", lat34, 4000);
-        fails += check_noalloc("lattice20 (packed)", lat20, 3000);
-        fails += check_noalloc("grid48 (LDS)", q48, 2000);
-        fails += check_paths("lattice", lat, 5000);
-        fails += check_paths("wide", wide, 2500);
+        fails += check_predicates("lattice34 (global packed)", lat34, nq(3000));
+        fails += check_predicates("lattice34+1000 (float copy)", lat34f, nq(3000));
+        fails += check_small("lattice", lat, nq(3000));
+        fails += check_small("wide", wide, nq(1500));
+        fails += check_small("mid", mid, nq(3000));
+        fails += check_small("grid48", q48, nq(2000));
+        fails += check_noalloc("lattice34 (global packed)", lat34, nq(4000));
+        fails += check_noalloc("lattice20 (packed)", lat20, nq(3000));
+        fails += check_noalloc("grid48 (LDS)", q48, nq(2000));
+        fails += check_paths("lattice", lat, nq(5000));
+        fails += check_paths("wide", wide, nq(2500));
         fails += check_sets();
         fails += check_coplanar();
         fails += check_circumcenters();

@@ -86,10 +86,11 @@ def tsan_exe():
 
 def test_emulated_predicates_under_thread_sanitizer(tsan_exe):
     """The 3D predicate pipeline (LDS, packed and global kernels over several
-    tiles per workgroup, plans) under ThreadSanitizer: a missing barrier
+    tiles per workgroup, plans; a third of the other runs' queries) under
+    ThreadSanitizer: a missing barrier
     between a tile's LDS writes and another thread's reads is a data race
     between the emulated threads."""
-    env = dict(os.environ, EMU_PRED_ONLY="1", TSAN_OPTIONS="halt_on_error=1")
+    env = dict(os.environ, EMU_PRED_ONLY="1", EMU_QUERY_DIV="3", TSAN_OPTIONS="halt_on_error=1")
     r = subprocess.run([str(tsan_exe)], capture_output=True, text=True, timeout=900, env=env)
     assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-4000:]
     assert "pred lattice20 (packed) ok" in r.stdout and "MISMATCH" not in r.stdout
